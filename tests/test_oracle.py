@@ -1,0 +1,199 @@
+"""Pins the CPU restatement (oracle/bfo.c) before it is trusted as the checker.
+
+Each check is independent of the oracle's own formulas: a reference-held
+table, a library hash, a published RFC vector, a brute-force definition from
+the reference's tests, or a mathematical invariant.
+"""
+import hashlib
+import pathlib
+
+import numpy as np
+import pytest
+
+GOLDEN = pathlib.Path(__file__).parent / "golden"
+
+
+def rnd(O, n, seed):
+    return O.random_vec(seed, n)
+
+
+def test_polyval_nibble_pow_table_kat(oracle):
+    """p3/src/polyval/portable.rs:169-376: T[i][k][j] = from_canonical(j<<4k)^(2^(2^(i+1)))."""
+    O = oracle
+    vals = [int(l, 16) for l in (GOLDEN / "polyval_nibble_pow_table.txt").read_text().split("\n")
+            if l and not l.startswith("#")]
+    assert len(vals) == 3072
+    base = np.array([[(j << (4 * k)) & (2**64 - 1), (j << (4 * k)) >> 64]
+                     for k in range(32) for j in range(16)], dtype=np.uint64)
+    x = O.from_canonical(base)
+    for i in range(6):
+        for _ in range(2 ** (i + 1) - (2 ** i if i else 0)):
+            x = O.mul(x, x)
+        got = [O.to_int(e) for e in x]
+        assert got == vals[i * 512:(i + 1) * 512], f"table block {i}"
+
+
+def test_polyval_one_and_constants(oracle):
+    O = oracle
+    one = O.from_canonical(O.from_int(1))
+    assert O.to_int(one) == 0xC2000000000000000000000000000001
+    # basis(i) = from_canonical(1 << i) (polyval.rs:167-172) round-trips
+    for i in range(128):
+        assert O.to_int(O.to_canonical(O.from_canonical(O.from_int(1 << i)))) == 1 << i
+
+
+@pytest.mark.parametrize("impl", [0, 1])
+def test_polyval_mul_matches_textbook(oracle, impl):
+    O = oracle
+    a, b = rnd(O, 2000, 11), rnd(O, 2000, 12)
+    assert (O.mul(a, b, impl) == O.mul(a, b, 2)).all()
+
+
+def test_polyval_canonical_product_identity(oracle):
+    """to_canonical(from_canonical(x) * from_canonical(y)) == x*y mod P."""
+    O = oracle
+    x, y = rnd(O, 500, 13), rnd(O, 500, 14)
+    lhs = O.to_canonical(O.mul(O.from_canonical(x), O.from_canonical(y)))
+    assert (lhs == O.polymul_mod(x, y)).all()
+
+
+def test_polyval_field_axioms_and_inverse(oracle):
+    """polyval.rs:265-295 identities in characteristic 2."""
+    O = oracle
+    x, y, z = rnd(O, 64, 15), rnd(O, 64, 16), rnd(O, 64, 17)
+    one = np.repeat(O.from_canonical(O.from_int(1)), 64, axis=0)
+    assert (O.mul(x, one) == x).all()
+    assert (O.mul(x, y) == O.mul(y, x)).all()
+    assert (O.mul(x, O.mul(y, z)) == O.mul(O.mul(x, y), z)).all()
+    assert (O.mul(x, y ^ z) == (O.mul(x, y) ^ O.mul(x, z))).all()
+    assert (O.mul(x, O.inv(x)) == one).all()
+
+
+def test_keccak_kats(oracle):
+    O = oracle
+    assert O.keccak256(b"").hex() == "c5d2460186f7233c927e7db2dcc703c0e500b653ca82273b7bfad8045d85a470"
+    rng = np.random.default_rng(3)
+    for n in [0, 1, 16, 32, 64, 135, 136, 137, 272, 1000]:
+        msg = bytes(rng.integers(0, 256, n, dtype=np.uint8))
+        assert O.sha3_256(msg) == hashlib.sha3_256(msg).digest()
+
+
+def test_chacha20_rfc8439_block(oracle):
+    """RFC 8439 section 2.3.2 test vector pins the ChaCha core."""
+    O = oracle
+    key = [int.from_bytes(bytes(range(4 * i, 4 * i + 4)), "little") for i in range(8)]
+    nonce = bytes.fromhex("000000090000004a00000000")
+    cn = [1] + [int.from_bytes(nonce[4 * i:4 * i + 4], "little") for i in range(3)]
+    out = O.chacha_block(20, key, cn)
+    assert out[:4] == [0xE4E7F110, 0x15593BD1, 0x1FDD0F50, 0xC47120A3]
+    assert out[12:] == [0xD19C12B5, 0xB94E16DE, 0xE883D0CB, 0x4E3C50A2]
+
+
+@pytest.mark.parametrize("log_n", range(0, 9))
+def test_ntt_matches_definition(oracle, log_n):
+    """p3/src/field/binary.rs:194-203 (forward vs explicit novel-basis matrix)."""
+    O = oracle
+    d = rnd(O, 1 << log_n, 20 + log_n)
+    assert (O.ntt_forward(log_n, d) == O.ntt_forward_definition(log_n, d)).all()
+
+
+def test_ntt_round_trip(oracle):
+    """p3/src/field/binary.rs:205-217."""
+    O = oracle
+    for log_n in range(0, 10):
+        for width in (1, 3, 8):
+            a = rnd(O, (1 << log_n) * width, log_n * 10 + width)
+            b = O.ntt_forward(log_n, a, width)
+            assert (O.ntt_backward(log_n, b, width) == a).all()
+
+
+def test_ntt_last_layer_closed_form(oracle):
+    """twiddles[log_n-1][j] = from_canonical(2j) since W_0 is the identity."""
+    O = oracle
+    L = 9
+    tw = O.ntt_twiddles(L)
+    last = tw[(1 << (L - 1)) - 1:]
+    exp = O.from_canonical(np.array([[2 * j, 0] for j in range(1 << (L - 1))], dtype=np.uint64))
+    assert (last == exp).all()
+
+
+def _bitrev(v, bits):
+    return int(format(v, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+@pytest.mark.parametrize("log2_c,d", [(0, 3), (1, 4), (2, 3), (1, 6)])
+def test_rs_encode_closed_form(oracle, log2_c, d):
+    """codeword[v] = P(point bitrev(v)), P = novel-basis poly of bitrev(data), via the brute-force matrix."""
+    O = oracle
+    N, L = 1 << d, log2_c + d
+    data = rnd(O, N, 30 + d)
+    coeffs = np.zeros(((1 << L), 2), dtype=np.uint64)
+    coeffs[:N] = data[[_bitrev(j, d) for j in range(N)]]
+    ev = O.ntt_forward_definition(L, coeffs)
+    cw = O.rs_encode(log2_c, d, data)
+    assert (cw == ev[[_bitrev(v, L) for v in range(1 << L)]]).all()
+
+
+def test_num_queries(oracle):
+    """binary_reed_solomon.rs:55-59: c=2 -> 309, c=4 -> 189, c=1 -> 0 (saturating cast)."""
+    assert oracle.num_queries(128, 1) == 309
+    assert oracle.num_queries(128, 2) == 189
+    assert oracle.num_queries(128, 0) == 0
+
+
+def test_mle_and_eq(oracle):
+    """multilinear.rs:312-332 definitions restated: eval = sum eq(x, b) f(b)."""
+    O = oracle
+    for n in range(0, 7):
+        f, x = rnd(O, 1 << n, 40 + n), rnd(O, n, 50 + n)
+        eq = O.eq_table(x)
+        acc = 0
+        for k in range(1 << n):
+            acc ^= O.to_int(O.mul(eq[k:k + 1], f[k:k + 1]))
+        assert O.to_int(O.mle_evaluate(f, x)) == acc
+
+
+@pytest.mark.parametrize("log2_c", [0, 1, 2])
+def test_basefold_prove_verify_round_trip(oracle, log2_c):
+    """poly-commit/src/basefold.rs:480-490 (num_vars 0..9), plus tampering."""
+    O = oracle
+    for n in range(0, 10):
+        e, p = rnd(O, 1 << n, 60 + n), rnd(O, n, 70 + n)
+        root, claim, proof = O.basefold_prove(128, log2_c, e, p)
+        assert O.basefold_verify(128, log2_c, root, p, claim, proof) == 0
+        assert (claim == O.mle_evaluate(e, p)).all()
+        if n > 0:
+            bad_claim = claim.copy()
+            bad_claim[0, 0] ^= 1
+            assert O.basefold_verify(128, log2_c, root, p, bad_claim, proof) != 0
+        if O.num_queries(128, log2_c) and n > 0:
+            bad = bytearray(proof)
+            bad[-3] ^= 0x10
+            assert O.basefold_verify(128, log2_c, root, p, claim, bytes(bad)) == 3
+
+
+def test_proof_size_formula(oracle):
+    """bincode size of BasefoldProof at 2^n, rate 1/2 matches the wire-format arithmetic."""
+    O = oracle
+    for n, log2_c in [(6, 1), (8, 2)]:
+        e, p = rnd(O, 1 << n, 1), rnd(O, n, 2)
+        _, _, proof = O.basefold_prove(128, log2_c, e, p)
+        q = O.num_queries(128, log2_c)
+        L = n + log2_c
+        per = 24 + 8 + 32 * L + 8 + sum(16 + 8 + 32 * (log2_c + i) for i in range(n))
+        size = 8 + n * 40 + 24 + 8 + 32 * n + 8 + q * per
+        assert len(proof) == size
+
+
+def test_product_sumcheck_invariants(oracle):
+    """Degree-3 product sumcheck: p(0)+p(1)=claim is asserted inside; the final
+    subclaim must equal f(x) g(x) eq(r, x) computed independently."""
+    O = oracle
+    for n in range(1, 8):
+        f, g, r = rnd(O, 1 << n, 80 + n), rnd(O, 1 << n, 90 + n), rnd(O, n, 95 + n)
+        claim, polys, chal, fin, sub = O.product_sumcheck(f, g, r)
+        x = chal[::-1].copy()  # challenge of round k fixes variable n-1-k
+        fx, gx = O.mle_evaluate(f, x), O.mle_evaluate(g, x)
+        assert (fin[0:1] == fx).all() and (fin[1:2] == gx).all()
+        eq = O.mle_evaluate(O.eq_table(r), x)
+        assert (O.mul(O.mul(fx, gx), eq) == sub).all()
