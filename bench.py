@@ -1,0 +1,175 @@
+"""Benchmark of the Basefold commit+open hot path on one MI355X.
+
+Workload: one multilinear polynomial with 2^24 evaluations over POLYVAL
+GF(2^128), rate 1/4 (log_blowup 2), uniform random evaluations from
+StdRng seed 0, random opening point from seed 1.  One step = commit (RS
+encode via the additive NTT + Keccak Merkle tree) + the harness transcript
+prefix + open (interleaved fold/sumcheck rounds, pi-tree commitments, 189
+queries) with the evaluations already resident in HBM.  The claim is
+evaluated once before timing, as the reference bench excludes `eval`
+(poly-commit/benches/poly-commit.rs:147-158).
+
+Prints one JSON line.  `value` = evaluations committed+opened per second.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LOG_N = 24
+LOG2_C = 2
+LAMBDA = 128
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n: int, log2_c: int) -> dict:
+    """Compulsory HBM bytes per commit+open and per-kernel-family units.
+
+    Each persistent or inter-round array is read once and written once; the
+    leaf hash of a 16-byte element reads 16 B and writes a 32-byte digest, a
+    node reads two digests and writes one."""
+    N, c = 1 << n, 1 << log2_c
+    M = N * c
+    S, H = 16, 32
+    commit = S * N + S * M + H * (M - 1)
+    opn = 1.5 * S * (2 * M - 2 * c) + H * (M - c - n) + 2 * S * (2 * N - 2) + S * N / 2
+    return {"commit": commit, "open": opn, "total": commit + opn}
+
+
+def cpu_baseline(args) -> dict:
+    """The CPU restatement (oracle/) with OpenMP on this host's cores, on a
+    bounded sample: commit+open of 2^SAMPLE evaluations at the same rate."""
+    import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    O.set_threads(cores)
+    n = args.cpu_log_n
+    ev, pt = O.random_vec(0, 1 << n), O.random_vec(1, n)
+    O.basefold_prove(LAMBDA, LOG2_C, ev, pt)  # warm-up
+    times = []
+    for _ in range(args.cpu_samples):
+        t0 = time.perf_counter()
+        O.basefold_prove(LAMBDA, LOG2_C, ev, pt)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    return {"value": (1 << n) / t, "unit": "elems/s", "cores": cores, "kind": "port",
+            "sample": f"commit+open of 2^{n} evaluations at rate 1/4 (median of {len(times)}, "
+                      f"C restatement of the reference CPU path, OpenMP {cores} threads, PCLMUL)"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log-n", type=int, default=LOG_N)
+    ap.add_argument("--cpu-log-n", type=int, default=20)
+    ap.add_argument("--cpu-samples", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(each proof is independent; sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from __graft_entry__ import load_package
+    bfd = load_package()
+
+    n = args.log_n
+    N = 1 << n
+    ctx = bfd.Context(0)
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(LAMBDA, LOG2_C, n)), ctx)
+    evals = bfd.random_vec(0, N)
+    point = bfd.random_vec(1, n)
+    dev = torch.from_numpy(evals.view(np.int64).reshape(-1)).to("cuda:0")
+    torch.cuda.synchronize()
+    ptr = dev.data_ptr()
+    cd0 = pcs.commit_device(ptr, N)
+    claim = pcs.eval(cd0, point)
+    root0 = cd0.comm
+    del cd0
+
+    def step():
+        cd = pcs.commit_device(ptr, N)
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(cd.comm)
+        ch.observe_ext_slice(claim)
+        proof = pcs.open(cd, [bfd.PolyEvals(claim, point)], ch)
+        return cd, proof
+
+    for _ in range(args.warmup):
+        cd, proof = step()
+        assert cd.comm == root0
+        del cd
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ctx.kernel_times(enable=True)  # reset and start per-kernel timing
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cd, proof = step()
+        del cd
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ktimes = ctx.kernel_times(enable=False)
+    ms_per_step = 1e3 * t / args.steps
+
+    # dominant kernel family and its roofline position
+    name, (kms, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
+    M = N << LOG2_C
+    c = 1 << LOG2_C
+    per_step_units = {
+        # Keccak node hashing: main tree M-1 nodes + pi trees sum(L_k/2 - 1)
+        "k_nodes": (M - 1) + sum((M >> k) // 2 - 1 for k in range(n)),
+        "k_leaves_elem": M,
+        "k_leaves_pair": sum((M >> k) // 2 for k in range(n)),
+    }
+    bytes_per_unit = {"k_nodes": 96, "k_leaves_elem": 48, "k_leaves_pair": 64}
+    if name in per_step_units:
+        alg_bytes = per_step_units[name] * bytes_per_unit[name] * args.steps
+    elif name == "k_ntt_layer":
+        alg_bytes = (n - 1) * 2 * 16 * M * args.steps  # each layer reads and writes the codeword
+    else:
+        alg_bytes = None
+    achieved = alg_bytes / (kms * 1e-3) / 1e9 if alg_bytes else None
+    traffic = None
+    ab = algorithmic_bytes(n, LOG2_C)
+
+    result = {
+        "metric": "Basefold commit+open field-elems/s (2^24 evals, rate 1/4)",
+        "value": N / (t / args.steps),
+        "unit": "elems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "gf2_128",
+        "data": "synthetic (ChaCha12 StdRng seed 0 evaluations, seed 1 point)",
+        "config": {"workload": "basefold_commit_open_plain", "log_n": n, "log2_c": LOG2_C,
+                   "lambda": LAMBDA, "num_queries": pcs.num_queries, "width": 1},
+        "hbm_gbs_algorithmic": ab["total"] / (t / args.steps) / 1e9,
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic, "avg_launch_ms": kms / launches, "launches_per_step": launches / args.steps},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(ktimes.items())},
+        "proof_bytes": len(proof),
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+/*
+ * bfd.h — C ABI of the MI355X Basefold prover (POLYVAL GF(2^128), binary
+ * Reed-Solomon code, Keccak-256 Merkle trees).
+ *
+ * Drop-in boundary for the reference's Rust traits (han0110/learn-zkp):
+ *   PolyCommitScheme<F, E>            poly-commit/src/lib.rs:20-51
+ *   Basefold::{commit, eval, open}    poly-commit/src/basefold.rs:112-232
+ *   BinaryReedSolomonCode::new        poly-commit/src/basefold/code/binary_reed_solomon.rs:19-27
+ *   GenericChallenger<_, Keccak256>   p3/src/challenger.rs:24-90
+ *   prove_sumcheck                    sumcheck/src/lib.rs:56-79
+ *
+ * Conventions
+ *   - A field element is 16 bytes: the little-endian u128 Montgomery value of
+ *     the reference's `Polyval` (p3/src/polyval.rs:30-34), i.e. its bincode
+ *     bytes.  Digests are 32 bytes.
+ *   - Every function returns 0 on success or a BFD_ERR_* code; the message of
+ *     the last failure on the calling thread is bfd_last_error().  Nothing
+ *     unwinds across the ABI.  (The reference panics where these return
+ *     errors; commit/open never fail on valid input.)
+ *   - Host buffers belong to the caller.  Handles own their device memory.
+ *     bfd_open does not consume the commitment (the reference borrows
+ *     CommitmentData immutably), so a commitment can be opened repeatedly.
+ *   - A context is internally synchronised; calls on one context serialise.
+ *   - Proofs are returned as the bincode bytes of BasefoldProof
+ *     (basefold.rs:49-66) and freed with bfd_proof_free.
+ */
+#ifndef BFD_H
+#define BFD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BFD_OK 0
+#define BFD_ERR_ARG 1        /* invalid argument or unsupported shape */
+#define BFD_ERR_HIP 2        /* HIP runtime / kernel failure, or no GPU */
+#define BFD_ERR_BUDGET 3     /* admission refused: HBM budget exceeded */
+#define BFD_ERR_STATE 4      /* handle used in the wrong state */
+
+typedef struct bfd_ctx bfd_ctx;
+typedef struct bfd_code bfd_code;
+typedef struct bfd_commitment bfd_commitment;
+typedef struct bfd_challenger bfd_challenger;
+
+const char *bfd_last_error(void);
+
+/* ---- context: one GPU, one stream, an HBM budget for resident commitments ---- */
+int bfd_ctx_create(int device, uint64_t hbm_budget_bytes, bfd_ctx **out);
+int bfd_ctx_destroy(bfd_ctx *ctx);
+int bfd_ctx_synchronize(bfd_ctx *ctx);
+/* bytes currently held by commitments of this context */
+uint64_t bfd_ctx_resident_bytes(const bfd_ctx *ctx);
+
+/* ---- code: BinaryReedSolomonCode::new(lambda, log2_c, d)  (binary_reed_solomon.rs:19-27)
+ * Builds the fold/NTT twiddle table on the device (setup, like AddtiveNtt::new). */
+int bfd_code_create(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d, bfd_code **out);
+int bfd_code_destroy(bfd_code *code);
+/* RandomFoldableCode::num_queries (binary_reed_solomon.rs:55-59) */
+uint64_t bfd_code_num_queries(const bfd_code *code);
+
+/* ---- challenger: GenericChallenger<Polyval, Keccak256Hash>  (p3/src/challenger.rs) ---- */
+int bfd_challenger_create(bfd_challenger **out);            /* ::keccak256(), :45-49 */
+int bfd_challenger_clone(const bfd_challenger *c, bfd_challenger **out);
+int bfd_challenger_destroy(bfd_challenger *c);
+int bfd_challenger_observe(bfd_challenger *c, const uint8_t *bytes, size_t len); /* CanObserve, :51-60 */
+int bfd_challenger_sample(bfd_challenger *c, uint8_t out[16]);                     /* CanSample, :62-74 */
+int bfd_challenger_sample_bits(bfd_challenger *c, uint32_t bits, uint64_t *out);   /* :76-83 */
+
+/* ---- PolyCommitScheme ---- */
+/* Basefold::commit (basefold.rs:112-120): evals is the height x width
+ * row-major matrix of evaluations (width 1 supported), height = 2^d. */
+int bfd_commit(bfd_ctx *ctx, const bfd_code *code, const uint8_t *evals, size_t height, size_t width,
+               bfd_commitment **out, uint8_t root[32]);
+/* Same, with the evaluations already resident in device memory (not copied
+ * out of; the commitment keeps its own copy of the data). */
+int bfd_commit_device(bfd_ctx *ctx, const bfd_code *code, const void *d_evals, size_t height, size_t width,
+                      bfd_commitment **out, uint8_t root[32]);
+int bfd_commitment_root(const bfd_commitment *comm, uint8_t root[32]);
+int bfd_commitment_destroy(bfd_commitment *comm);
+
+/* Basefold::eval (basefold.rs:126-131): values[col] = MLE(col)(point), point has d elements */
+int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uint8_t *values);
+
+/* Basefold::open (basefold.rs:133-232) for one PolyEvals {point, values}.
+ * The challenger is advanced in place (the reference takes it by value:
+ * pass a clone to keep the original).  *proof is allocated by the library. */
+int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
+             const uint8_t *values, bfd_challenger *challenger, uint8_t **proof, size_t *proof_len);
+void bfd_proof_free(uint8_t *proof);
+
+/* ---- degree-3 product sumcheck  sum_b f(b) g(b) eq(r, b)  (no reference
+ * counterpart; driver semantics of prove_sumcheck, sumcheck/src/lib.rs:56-103):
+ * observes the claim, then per round observes [c0, c2, c3] and samples the
+ * challenge; tables are folded on the top variable first.
+ * Outputs: claim, n x 3 compressed round polys, n challenges (round order),
+ * final [f, g] evaluations and the final subclaim. */
+int bfd_sumcheck_product(bfd_ctx *ctx, uint32_t n, const uint8_t *f, const uint8_t *g, const uint8_t *r,
+                         bfd_challenger *challenger, uint8_t claim[16], uint8_t *round_polys,
+                         uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim[16]);
+
+/* ---- input generation: Polyval::random_vec(n, StdRng::seed_from_u64(seed))
+ * (p3/src/field/from_uniform_bytes.rs:5-27) ---- */
+int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out);
+
+/* ---- measurement helpers ---- */
+/* Times `iters` dependent multiplies (variant 0: LDS comb, 1: register
+ * Karatsuba), Keccak-f permutations (2) on n lanes, or one copy of n
+ * 16-byte elements (3); returns milliseconds. */
+int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms);
+/* Per-kernel timing of the last commit/open on this context: fills up to
+ * `cap` (name, total ms, launches) records; returns the count in *count. */
+int bfd_ctx_kernel_times(bfd_ctx *ctx, int enable, char *names, size_t name_len, double *ms,
+                         uint32_t *launches, size_t cap, size_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

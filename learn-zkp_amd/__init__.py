@@ -1,0 +1,331 @@
+"""MI355X Basefold prover over POLYVAL GF(2^128): Python mirror of the
+reference's ``PolyCommitScheme`` / sumcheck prover interface over the C ABI
+in ``include/bfd.h`` (library ``libbfd.so`` built next to this file).
+
+Names follow the reference (han0110/learn-zkp):
+
+* ``BinaryReedSolomonCode(lambda_, log2_c, d)`` -- binary_reed_solomon.rs:19-27
+* ``Basefold.setup(BasefoldConfig(code))``      -- poly-commit/src/basefold.rs:104-110
+* ``Basefold.commit / data / eval / open``      -- basefold.rs:112-232
+* ``GenericChallenger.keccak256()``             -- p3/src/challenger.rs:45-49
+* ``prove_sumcheck_product``                    -- degree-3 product sumcheck
+                                                   driven like sumcheck/src/lib.rs:56-79
+
+Field elements are numpy ``uint64`` arrays of shape ``(n, 2)`` (low word
+first): the 16-byte little-endian Montgomery value of ``Polyval``.
+
+There is no CPU fallback: if the library is missing, or there is no GPU, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _DIR / "libbfd.so"
+HEADER = _DIR.parent / "include" / "bfd.h"
+
+_lib = None
+u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class BfdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bfd error {code}: {msg}")
+        self.code = code
+
+
+def build() -> None:
+    """Compile libbfd.so for gfx950 with the package Makefile (always rebuilds)."""
+    subprocess.run(["make", "-C", str(_DIR), "-B", "-j8"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise BfdError(-1, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.bfd_last_error.restype = ctypes.c_char_p
+        L.bfd_ctx_resident_bytes.restype = ctypes.c_uint64
+        L.bfd_ctx_resident_bytes.argtypes = [ctypes.c_void_p]
+        L.bfd_code_num_queries.restype = ctypes.c_uint64
+        L.bfd_code_num_queries.argtypes = [ctypes.c_void_p]
+        for name in ("bfd_ctx_destroy", "bfd_ctx_synchronize", "bfd_code_destroy",
+                     "bfd_commitment_destroy", "bfd_challenger_destroy"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        L.bfd_ctx_create.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+        L.bfd_code_create.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_void_p)]
+        L.bfd_challenger_create.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        L.bfd_challenger_clone.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.bfd_challenger_observe.argtypes = [ctypes.c_void_p, u8p, ctypes.c_size_t]
+        L.bfd_challenger_sample.argtypes = [ctypes.c_void_p, u8p]
+        L.bfd_challenger_sample_bits.argtypes = [ctypes.c_void_p, ctypes.c_uint32,
+                                                 ctypes.POINTER(ctypes.c_uint64)]
+        L.bfd_commit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, u8p, ctypes.c_size_t, ctypes.c_size_t,
+                                 ctypes.POINTER(ctypes.c_void_p), u8p]
+        L.bfd_commit_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), u8p]
+        L.bfd_commitment_root.argtypes = [ctypes.c_void_p, u8p]
+        L.bfd_eval.argtypes = [ctypes.c_void_p, ctypes.c_void_p, u8p, u8p]
+        L.bfd_open.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u8p, u8p, ctypes.c_void_p,
+                               ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+        L.bfd_proof_free.argtypes = [u8p]
+        L.bfd_proof_free.restype = None
+        L.bfd_sumcheck_product.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u8p, u8p, u8p, ctypes.c_void_p,
+                                           u8p, u8p, u8p, u8p, u8p]
+        L.bfd_random_vec.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u8p]
+        L.bfd_microbench.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double)]
+        L.bfd_ctx_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32),
+                                           ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise BfdError(rc, lib().bfd_last_error().decode(errors="replace"))
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(u8p)
+
+
+def _elems(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 2)
+
+
+def random_vec(seed: int, n: int) -> np.ndarray:
+    """Polyval::random_vec(n, StdRng::seed_from_u64(seed)) (from_uniform_bytes.rs:5-27)."""
+    out = np.zeros((n, 2), dtype=np.uint64)
+    _check(lib().bfd_random_vec(ctypes.c_uint64(seed), n, _p(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+class Context:
+    """One GPU, one stream, an HBM budget for resident commitments (0 = none)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().bfd_ctx_create(device, hbm_budget, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bfd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().bfd_ctx_synchronize(self.h))
+
+    @property
+    def resident_bytes(self) -> int:
+        return int(lib().bfd_ctx_resident_bytes(self.h))
+
+    def microbench(self, what: int, n: int, iters: int) -> float:
+        ms = ctypes.c_double()
+        _check(lib().bfd_microbench(self.h, what, n, iters, ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_times(self, enable: bool = True) -> dict:
+        """Returns {name: (total_ms, launches)} collected since the last call
+        and (re)enables or disables collection."""
+        cap, nl = 64, 64
+        names = ctypes.create_string_buffer(cap * nl)
+        ms = (ctypes.c_double * cap)()
+        ln = (ctypes.c_uint32 * cap)()
+        cnt = ctypes.c_size_t()
+        _check(lib().bfd_ctx_kernel_times(self.h, int(enable), names, nl, ms, ln, cap, ctypes.byref(cnt)))
+        out = {}
+        for i in range(cnt.value):
+            nm = names.raw[i * nl:(i + 1) * nl].split(b"\0", 1)[0].decode()
+            out[nm] = (ms[i], ln[i])
+        return out
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class GenericChallenger:
+    """GenericChallenger<Polyval, Keccak256Hash> (p3/src/challenger.rs:24-90)."""
+
+    def __init__(self, _h=None):
+        if _h is None:
+            _h = ctypes.c_void_p()
+            _check(lib().bfd_challenger_create(ctypes.byref(_h)))
+        self.h = _h
+
+    @classmethod
+    def keccak256(cls) -> "GenericChallenger":
+        return cls()
+
+    def clone(self) -> "GenericChallenger":
+        h = ctypes.c_void_p()
+        _check(lib().bfd_challenger_clone(self.h, ctypes.byref(h)))
+        return GenericChallenger(h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bfd_challenger_destroy(self.h)
+            self.h = None
+
+    def observe(self, data: bytes) -> None:
+        buf = (ctypes.c_uint8 * max(1, len(data))).from_buffer_copy(bytes(data) or b"\0")
+        _check(lib().bfd_challenger_observe(self.h, buf, len(data)))
+
+    def observe_ext_slice(self, elems) -> None:
+        self.observe(_elems(elems).tobytes())
+
+    def sample_ext_element(self) -> np.ndarray:
+        out = np.zeros((1, 2), dtype=np.uint64)
+        _check(lib().bfd_challenger_sample(self.h, _p(out)))
+        return out
+
+    def sample_bits(self, bits: int) -> int:
+        v = ctypes.c_uint64()
+        _check(lib().bfd_challenger_sample_bits(self.h, bits, ctypes.byref(v)))
+        return int(v.value)
+
+
+@dataclass
+class BinaryReedSolomonCode:
+    lambda_: int
+    log2_c: int
+    d: int
+
+
+@dataclass
+class BasefoldConfig:
+    code: BinaryReedSolomonCode
+
+
+@dataclass
+class PolyEvals:
+    values: np.ndarray
+    point: np.ndarray
+
+
+@dataclass
+class BasefoldCommitmentData:
+    comm: bytes
+    handle: ctypes.c_void_p
+    data: np.ndarray
+    _owner: "Basefold" = field(repr=False, default=None)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().bfd_commitment_destroy(self.handle)
+            self.handle = None
+
+
+class Basefold:
+    """PolyCommitScheme<Polyval, Polyval> for Basefold over the binary RS code."""
+
+    def __init__(self, config: BasefoldConfig, ctx: Optional[Context] = None):
+        self.config = config
+        self.ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        c = config.code
+        _check(lib().bfd_code_create(self.ctx.h, c.lambda_, c.log2_c, c.d, ctypes.byref(h)))
+        self.code_h = h
+
+    @classmethod
+    def setup(cls, config: BasefoldConfig, ctx: Optional[Context] = None) -> "Basefold":
+        return cls(config, ctx)
+
+    def __del__(self):
+        if getattr(self, "code_h", None):
+            lib().bfd_code_destroy(self.code_h)
+            self.code_h = None
+
+    @property
+    def num_queries(self) -> int:
+        return int(lib().bfd_code_num_queries(self.code_h))
+
+    def commit(self, data) -> BasefoldCommitmentData:
+        data = _elems(data)
+        root = (ctypes.c_uint8 * 32)()
+        h = ctypes.c_void_p()
+        _check(lib().bfd_commit(self.ctx.h, self.code_h, _p(data), len(data), 1, ctypes.byref(h), root))
+        return BasefoldCommitmentData(bytes(root), h, data, self)
+
+    def commit_device(self, ptr: int, height: int) -> BasefoldCommitmentData:
+        """Commit evaluations already resident in device memory at ``ptr``."""
+        root = (ctypes.c_uint8 * 32)()
+        h = ctypes.c_void_p()
+        _check(lib().bfd_commit_device(self.ctx.h, self.code_h, ctypes.c_void_p(ptr), height, 1,
+                                       ctypes.byref(h), root))
+        return BasefoldCommitmentData(bytes(root), h, None, self)
+
+    def data(self, comm_data: BasefoldCommitmentData) -> np.ndarray:
+        return comm_data.data
+
+    def eval(self, comm_data: BasefoldCommitmentData, point) -> np.ndarray:
+        point = _elems(point)
+        out = np.zeros((1, 2), dtype=np.uint64)
+        _check(lib().bfd_eval(self.ctx.h, comm_data.handle, _p(point), _p(out)))
+        return out
+
+    def open(self, comm_data: BasefoldCommitmentData, evals: Sequence[PolyEvals],
+             challenger: GenericChallenger) -> bytes:
+        if not evals:
+            return b""
+        point, values = _elems(evals[0].point), _elems(evals[0].values)
+        proof = u8p()
+        plen = ctypes.c_size_t()
+        _check(lib().bfd_open(self.ctx.h, self.code_h, comm_data.handle, _p(point), _p(values),
+                              challenger.h, ctypes.byref(proof), ctypes.byref(plen)))
+        try:
+            return ctypes.string_at(proof, plen.value)
+        finally:
+            lib().bfd_proof_free(proof)
+
+
+def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional[Context] = None):
+    """Degree-3 product sumcheck sum_b f(b) g(b) eq(r, b).
+
+    Returns (claim, compressed round polys (n, 3, 2), challenges (n, 2),
+    final [f, g] (2, 2), subclaim)."""
+    ctx = ctx or default_context()
+    f, g, r = _elems(f), _elems(g), _elems(r)
+    n = len(r)
+    claim = np.zeros((1, 2), dtype=np.uint64)
+    polys = np.zeros((max(1, 3 * n), 2), dtype=np.uint64)
+    chal = np.zeros((max(1, n), 2), dtype=np.uint64)
+    fin = np.zeros((2, 2), dtype=np.uint64)
+    sub = np.zeros((1, 2), dtype=np.uint64)
+    rr = r if n else np.zeros((1, 2), dtype=np.uint64)
+    _check(lib().bfd_sumcheck_product(ctx.h, n, _p(f), _p(g), _p(rr), challenger.h, _p(claim), _p(polys),
+                                      _p(chal), _p(fin), _p(sub)))
+    return claim, polys[:3 * n].reshape(n, 3, 2), chal[:n], fin, sub
+
+
+def header_symbols() -> List[str]:
+    """Names of the functions declared in include/bfd.h."""
+    import re
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(bfd_[a-z_0-9]+)\s*\(", text, re.M)))

@@ -1,0 +1,562 @@
+// Host runtime and C ABI of the MI355X Basefold prover.
+//
+// The host drives one HIP stream per context.  Bulk work (encode, Merkle
+// hashing, folds, sumcheck reductions, query gathers) runs as kernels; the
+// Fiat-Shamir transcript stays on the host and needs one 48-byte read-back
+// per round (the pi-tree root and the sumcheck partial sums).
+#include "internal.hpp"
+
+using namespace bfd::rt;
+
+namespace {
+
+// Basefold::open for width 1 (basefold.rs:133-232) with the interleaved
+// folding sumcheck (:335-369) and the Eval prover (eval.rs, eval_impr.rs).
+void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const uint8_t *point_b,
+             const uint8_t *values_b, Challenger &ch, std::vector<uint8_t> &proof) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    set_device(ctx);
+    hipStream_t st = ctx->stream;
+    const int n = (int)cm->n, d = n, b = (int)cm->b, L = b + d;
+    const size_t M = cm->M, c = (size_t)1 << b;
+    std::vector<gf> r(n), rinv(n);
+    for (int i = 0; i < n; i++) {
+        r[i] = load_elem(point_b + 16 * i);
+        if (gf_eq(r[i], gf_zero()) || gf_eq(r[i], gf_one()))
+            fail(BFD_ERR_ARG, "point coordinates 0 and 1 are not invertible in the Eval prover");
+        rinv[i] = hinv(r[i]);                            // EvalImpr::new, eval_impr.rs:20-27
+    }
+    const gf claim = load_elem(values_b);
+    const int hn = n > 0 ? n - 1 : 0;                    // HalfEq::new, eval_impr.rs:69-79
+    // correcting_factor(i) = prod_{j=i}^{n-2} (1 - r_j)^-1  (eval_impr.rs:81-88)
+    std::vector<gf> cf(n + 1, gf_one());
+    for (int j = hn - 1; j >= 0; j--) cf[j] = hmul(cf[j + 1], hinv(gf_add(gf_one(), r[j])));
+
+    // device scratch for the whole opening
+    const size_t half_eq_n = (size_t)1 << hn;
+    DevBuf half_eq(16 * half_eq_n, st);
+    if (n > 0) device_eq(ctx, r.data(), hn, half_eq.as<gf>());
+    DevBuf poly(16 * (cm->N > 1 ? cm->N / 2 : 1), st);
+    // pi_k for k >= 1 (pi_0 is the codeword): sum_{k=1..d} M >> k = M - c
+    std::vector<size_t> pi_off(d + 1, 0), tree_off(d + 1, 0);
+    for (int k = 1; k <= d; k++) pi_off[k] = (k == 1 ? 0 : pi_off[k - 1] + (M >> (k - 1)));
+    DevBuf pis(16 * (M - c + 1), st);
+    size_t tree_total = 0;
+    for (int k = 0; k < d; k++) {
+        tree_off[k] = tree_total;
+        tree_total += (M >> k) - 1;
+    }
+    DevBuf trees(32 * (tree_total ? tree_total : 1), st);
+    const unsigned kMaxPartials = 512;
+    DevBuf partials(16 * kMaxPartials, st);
+    ctx->pinned.ensure(32 + 16 * kMaxPartials + 64);
+    uint8_t *pin = (uint8_t *)ctx->pinned.p;
+
+    auto pi_ptr = [&](int k) -> gf * { return k == 0 ? cm->cw.as<gf>() : pis.as<gf>() + pi_off[k]; };
+    auto tree_ptr = [&](int k) -> uint64_t * { return trees.as<uint64_t>() + 4 * tree_off[k]; };
+
+    std::vector<std::array<gf, 2>> comp(d);
+    std::vector<std::array<uint8_t, 32>> pi_roots(d);
+    gf subclaim = claim, inner_sub = gf_zero(), s_delta = gf_one();
+    const gf *poly_cur = cm->data.as<gf>();
+
+    for (int k = 0; k < d; k++) {
+        const int i = d - 1 - k;
+        const size_t len = M >> k, half = len / 2;
+        gf *pik = pi_ptr(k);
+        uint64_t *tk = tree_ptr(k);
+        // pi tree over rows (pi[j], pi[j+half])  (basefold.rs:350-355)
+        ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half); });
+        tree_nodes(ctx, tk, half);
+        hip_check(hipMemcpyAsync(pin, tk + 4 * (2 * half - 2), 32, hipMemcpyDeviceToHost, st), "copy pi root");
+        // inner product of the halved eq table with the low half of poly (eval_impr.rs:187-197)
+        unsigned nb = 0;
+        const size_t ph = (size_t)1 << i;
+        ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, partials.as<gf>(), &nb); });
+        hip_check(hipMemcpyAsync(pin + 32, partials.p, 16 * nb, hipMemcpyDeviceToHost, st), "copy partials");
+        hip_check(hipStreamSynchronize(st), "round sync");
+        memcpy(pi_roots[k].data(), pin, 32);
+        gf S = gf_zero();
+        for (unsigned t = 0; t < nb; t++) S = gf_add(S, load_elem(pin + 32 + 16 * t));
+
+        ch.observe(pi_roots[k].data(), 32);                          // basefold.rs:356
+        const gf sub = (i == n - 1) ? subclaim : inner_sub;          // eval.rs:90-94
+        const gf coeff0 = hmul(cf[i], S);
+        const gf eval1 = hmul(gf_add(sub, hmul(gf_add(gf_one(), r[i]), coeff0)), rinv[i]);  // eval_impr.rs:41-43
+        const gf coeff1 = gf_add(eval1, coeff0);
+        // delta (eval_impr.rs:45-58): scalar = eq(r[i+1..], x[i+1..]); in
+        // characteristic 2 the factors are [s (1 + r_i), s]
+        const gf d0 = hmul(s_delta, gf_add(gf_one(), r[i]));
+        const gf d1 = s_delta;
+        const gf rp0 = hmul(coeff0, d0), rp1 = gf_add(hmul(coeff0, d1), hmul(coeff1, d0)), rp2 = hmul(coeff1, d1);
+        comp[k] = {rp0, rp2};                                        // eval.rs:64-66
+        ch.observe_elem(rp0);                                        // sumcheck/src/lib.rs:94
+        ch.observe_elem(rp2);
+        const gf x = ch.sample();                                    // lib.rs:96
+        inner_sub = gf_add(coeff0, hmul(coeff1, x));                 // eval.rs:110
+        subclaim = gf_add(rp0, hmul(x, gf_add(rp1, hmul(x, rp2))));  // lib.rs:100 (horner)
+        s_delta = hmul(s_delta, gf_add(gf_one(), gf_add(r[i], x)));
+        // fold pi (binary_reed_solomon.rs:88-96) and fix the top variable of poly
+        gf *pnext = pi_ptr(k + 1);
+        const gf *ftl = code->ft.as<gf>() + code->ft_off[i];
+        ctx->timer.run(st, "k_fold", [&] { return launch::fold(st, pik, pnext, ftl, half, x); });
+        gf *pout = poly.as<gf>();
+        ctx->timer.run(st, "k_fix", [&] { return launch::fix(st, poly_cur, pout, ph, x); });
+        poly_cur = pout;
+    }
+    // final_poly (basefold.rs:194-196)
+    hip_check(hipMemcpyAsync(pin, poly_cur, 16, hipMemcpyDeviceToHost, st), "copy final");
+    hip_check(hipStreamSynchronize(st), "final sync");
+    ctx->timer.collect();
+    const gf final_poly = load_elem(pin);
+    ch.observe_elem(final_poly);
+
+    // queries (basefold.rs:200-222)
+    const size_t q = code->q;
+    std::vector<uint64_t> idx(q);
+    for (size_t t = 0; t < q; t++) idx[t] = ch.sample_bits(L);
+    // gather plan
+    std::vector<uint64_t> g_main_dig, g_pi_dig, g_cw, g_pi;
+    auto layer_off = [](size_t leaves, int t) { return 2 * leaves - 2 * (leaves >> t); };
+    for (size_t t = 0; t < q; t++) {
+        g_cw.push_back(idx[t]);
+        for (int u = 0; u < L; u++) g_main_dig.push_back(layer_off(M, u) + ((idx[t] >> u) ^ 1));
+        for (int k = 0; k < d; k++) {
+            const int i = d - 1 - k;
+            const size_t ni = (size_t)1 << (b + i);
+            const size_t sib = ((idx[t] / ni) & 1) ^ 1, row = idx[t] % ni;
+            if (k == 0) g_cw.push_back(row + sib * ni);
+            else g_pi.push_back(pi_off[k] + row + sib * ni);
+            for (int u = 0; u < b + i; u++) g_pi_dig.push_back(tree_off[k] + layer_off(ni, u) + ((row >> u) ^ 1));
+        }
+    }
+    auto gather = [&](const std::vector<uint64_t> &ix, bool digest, const void *base, std::vector<uint8_t> &host) {
+        const size_t nrec = ix.size(), rb = digest ? 32 : 16;
+        host.resize(nrec * rb);
+        if (!nrec) return;
+        DevBuf dix(8 * nrec, st), dout(rb * nrec, st);
+        hip_check(hipMemcpyAsync(dix.p, ix.data(), 8 * nrec, hipMemcpyHostToDevice, st), "copy gather idx");
+        if (digest)
+            ctx->timer.run(st, "k_gather", [&] { return launch::gather_digest(st, (const uint64_t *)base, dix.as<uint64_t>(), dout.as<uint64_t>(), nrec); });
+        else
+            ctx->timer.run(st, "k_gather", [&] { return launch::gather_elem(st, (const gf *)base, dix.as<uint64_t>(), dout.as<gf>(), nrec); });
+        hip_check(hipMemcpyAsync(host.data(), dout.p, rb * nrec, hipMemcpyDeviceToHost, st), "copy gather");
+        hip_check(hipStreamSynchronize(st), "gather sync");
+    };
+    std::vector<uint8_t> h_main_dig, h_pi_dig, h_cw, h_pi;
+    gather(g_main_dig, true, cm->tree.p, h_main_dig);
+    gather(g_pi_dig, true, trees.p, h_pi_dig);
+    gather(g_cw, false, cm->cw.p, h_cw);
+    gather(g_pi, false, pis.p, h_pi);
+    ctx->timer.collect();
+
+    // bincode BasefoldProof (basefold.rs:49-66)
+    proof.clear();
+    put_u64(proof, (uint64_t)d);
+    for (int k = 0; k < d; k++) {
+        put_u64(proof, 2);
+        put_elem(proof, comp[k][0]);
+        put_elem(proof, comp[k][1]);
+    }
+    put_u64(proof, 1);
+    put_elem(proof, final_poly);
+    put_u64(proof, (uint64_t)d);
+    for (int k = 0; k < d; k++) put_bytes(proof, pi_roots[k].data(), 32);
+    put_u64(proof, (uint64_t)q);
+    size_t pm = 0, pp = 0, pc = 0, pv = 0;
+    for (size_t t = 0; t < q; t++) {
+        put_u64(proof, 1);
+        put_bytes(proof, &h_cw[16 * pc++], 16);
+        put_u64(proof, (uint64_t)L);
+        put_bytes(proof, &h_main_dig[32 * pm], 32 * (size_t)L);
+        pm += L;
+        put_u64(proof, (uint64_t)d);
+        for (int k = 0; k < d; k++) {
+            const int i = d - 1 - k;
+            if (k == 0) put_bytes(proof, &h_cw[16 * pc++], 16);
+            else put_bytes(proof, &h_pi[16 * pv++], 16);
+            put_u64(proof, (uint64_t)(b + i));
+            put_bytes(proof, &h_pi_dig[32 * pp], 32 * (size_t)(b + i));
+            pp += b + i;
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *bfd_last_error(void) { return g_err.c_str(); }
+
+int bfd_ctx_create(int device, uint64_t hbm_budget_bytes, bfd_ctx **out) {
+    return guarded([&] {
+        if (!out) fail(BFD_ERR_ARG, "null out");
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) fail(BFD_ERR_HIP, "no such HIP device");
+        auto ctx = std::make_unique<bfd_ctx>();
+        ctx->device = device;
+        ctx->budget = hbm_budget_bytes;
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+        *out = ctx.release();
+    });
+}
+
+int bfd_ctx_destroy(bfd_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int bfd_ctx_synchronize(bfd_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) fail(BFD_ERR_ARG, "null ctx");
+        set_device(ctx);
+        hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+    });
+}
+
+uint64_t bfd_ctx_resident_bytes(const bfd_ctx *ctx) { return ctx ? ctx->resident : 0; }
+
+int bfd_code_create(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d, bfd_code **out) {
+    return guarded([&] {
+        if (!ctx || !out) fail(BFD_ERR_ARG, "null argument");
+        if (log2_c > 8 || d > 32 || log2_c + d > 34) fail(BFD_ERR_ARG, "code too large");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        auto code = std::make_unique<bfd_code>();
+        code->ctx = ctx;
+        code->lambda = lambda;
+        code->log2_c = log2_c;
+        code->d = d;
+        code->q = num_queries(lambda, log2_c);
+        const int L = (int)(log2_c + d);
+        // level s table (fold order) = twiddles[b+s][bitrev_{b+s}(j)], j < 2^(b+s)
+        // = XOR over bits p of j of basis[b+s][b+s-1-p]
+        auto basis = ntt_basis(L);
+        code->ft_off.assign(d + 1, 0);
+        size_t total = 0;
+        for (uint32_t s = 0; s < d; s++) {
+            code->ft_off[s] = total;
+            total += (size_t)1 << (log2_c + s);
+        }
+        code->ft_off[d] = total;
+        code->ft = DevBuf(16 * (total ? total : 1), st);
+        std::vector<gf> bt;
+        std::vector<size_t> bt_off(d + 1, 0);
+        for (uint32_t s = 0; s < d; s++) {
+            const int K = (int)(log2_c + s);
+            bt_off[s] = bt.size();
+            for (int p = 0; p < K; p++) bt.push_back(basis[K][K - 1 - p]);
+        }
+        DevBuf dbt(16 * (bt.size() ? bt.size() : 1), st);
+        if (!bt.empty())
+            hip_check(hipMemcpyAsync(dbt.p, bt.data(), 16 * bt.size(), hipMemcpyHostToDevice, st), "copy basis");
+        for (uint32_t s = 0; s < d; s++) {
+            const int K = (int)(log2_c + s);
+            hip_check(launch::xor_span(st, code->ft.as<gf>() + code->ft_off[s], dbt.as<gf>() + bt_off[s], K,
+                                       (size_t)1 << K),
+                      "xor_span");
+        }
+        hip_check(hipStreamSynchronize(st), "code setup");
+        *out = code.release();
+    });
+}
+
+int bfd_code_destroy(bfd_code *code) {
+    return guarded([&] {
+        if (!code) return;
+        std::lock_guard<std::recursive_mutex> lk(code->ctx->mu);
+        set_device(code->ctx);
+        code->ft.release();
+        (void)hipStreamSynchronize(code->ctx->stream);
+        delete code;
+    });
+}
+
+uint64_t bfd_code_num_queries(const bfd_code *code) { return code ? code->q : 0; }
+
+int bfd_challenger_create(bfd_challenger **out) {
+    return guarded([&] {
+        if (!out) fail(BFD_ERR_ARG, "null out");
+        *out = new bfd_challenger();
+    });
+}
+int bfd_challenger_clone(const bfd_challenger *c, bfd_challenger **out) {
+    return guarded([&] {
+        if (!c || !out) fail(BFD_ERR_ARG, "null argument");
+        *out = new bfd_challenger(*c);
+    });
+}
+int bfd_challenger_destroy(bfd_challenger *c) {
+    delete c;
+    return BFD_OK;
+}
+int bfd_challenger_observe(bfd_challenger *c, const uint8_t *bytes, size_t len) {
+    return guarded([&] {
+        if (!c || (!bytes && len)) fail(BFD_ERR_ARG, "null argument");
+        c->c.observe(bytes, len);
+    });
+}
+int bfd_challenger_sample(bfd_challenger *c, uint8_t out[16]) {
+    return guarded([&] {
+        if (!c || !out) fail(BFD_ERR_ARG, "null argument");
+        store_elem(out, c->c.sample());
+    });
+}
+int bfd_challenger_sample_bits(bfd_challenger *c, uint32_t bits, uint64_t *out) {
+    return guarded([&] {
+        if (!c || !out || bits > 64) fail(BFD_ERR_ARG, "bad argument");
+        *out = c->c.sample_bits((int)bits);
+    });
+}
+
+int bfd_commit(bfd_ctx *ctx, const bfd_code *code, const uint8_t *evals, size_t height, size_t width,
+               bfd_commitment **out, uint8_t root[32]) {
+    return guarded([&] { do_commit(ctx, code, evals, false, height, width, out, root); });
+}
+int bfd_commit_device(bfd_ctx *ctx, const bfd_code *code, const void *d_evals, size_t height, size_t width,
+                      bfd_commitment **out, uint8_t root[32]) {
+    return guarded([&] { do_commit(ctx, code, d_evals, true, height, width, out, root); });
+}
+int bfd_commitment_root(const bfd_commitment *comm, uint8_t root[32]) {
+    return guarded([&] {
+        if (!comm || !root) fail(BFD_ERR_ARG, "null argument");
+        memcpy(root, comm->root, 32);
+    });
+}
+int bfd_commitment_destroy(bfd_commitment *comm) {
+    return guarded([&] {
+        if (!comm) return;
+        bfd_ctx *ctx = comm->ctx;
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        ctx->resident -= comm->bytes;
+        delete comm;
+    });
+}
+
+int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uint8_t *values) {
+    return guarded([&] {
+        if (!ctx || !comm || !point || !values) fail(BFD_ERR_ARG, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        const int n = (int)comm->n;
+        std::vector<gf> x(n);
+        for (int i = 0; i < n; i++) x[i] = load_elem(point + 16 * i);
+        if (n == 0) {
+            hip_check(hipMemcpyAsync(values, comm->data.p, 16, hipMemcpyDeviceToHost, st), "copy");
+            hip_check(hipStreamSynchronize(st), "eval");
+            return;
+        }
+        // evaluate (multilinear.rs:200-214): fix the top variable repeatedly
+        DevBuf tmp(16 * (comm->N / 2), st);
+        const gf *cur = comm->data.as<gf>();
+        for (int i = n - 1; i >= 0; i--) {
+            hip_check(launch::fix(st, cur, tmp.as<gf>(), (size_t)1 << i, x[i]), "fix");
+            cur = tmp.as<gf>();
+        }
+        hip_check(hipMemcpyAsync(values, cur, 16, hipMemcpyDeviceToHost, st), "copy");
+        hip_check(hipStreamSynchronize(st), "eval");
+    });
+}
+
+int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
+             const uint8_t *values, bfd_challenger *challenger, uint8_t **proof, size_t *proof_len) {
+    return guarded([&] {
+        if (!ctx || !code || !comm || !values || !challenger || !proof || !proof_len || (!point && comm->n))
+            fail(BFD_ERR_ARG, "null argument");
+        if (comm->code != code) fail(BFD_ERR_ARG, "commitment was made with another code");
+        std::vector<uint8_t> out;
+        do_open(ctx, code, comm, point, values, challenger->c, out);
+        uint8_t *p = (uint8_t *)malloc(out.size() ? out.size() : 1);
+        if (!p) fail(BFD_ERR_ARG, "host allocation failed");
+        memcpy(p, out.data(), out.size());
+        *proof = p;
+        *proof_len = out.size();
+    });
+}
+
+void bfd_proof_free(uint8_t *proof) { free(proof); }
+
+int bfd_sumcheck_product(bfd_ctx *ctx, uint32_t n, const uint8_t *f_b, const uint8_t *g_b, const uint8_t *r_b,
+                         bfd_challenger *challenger, uint8_t claim_out[16], uint8_t *round_polys,
+                         uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim_out[16]) {
+    return guarded([&] {
+        if (!ctx || !f_b || !g_b || !challenger || !claim_out || !final_fg || !subclaim_out ||
+            (n && (!r_b || !round_polys || !challenges)))
+            fail(BFD_ERR_ARG, "null argument");
+        if (n > 32) fail(BFD_ERR_ARG, "too many variables");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        Challenger &ch = challenger->c;
+        const size_t N = (size_t)1 << n;
+        if (n == 0) {  // prove_sumcheck returns before observing (sumcheck/src/lib.rs:63-65)
+            gf f0 = load_elem(f_b), g0 = load_elem(g_b), cl = hmul(f0, g0);
+            store_elem(claim_out, cl);
+            store_elem(final_fg, f0);
+            store_elem(final_fg + 16, g0);
+            store_elem(subclaim_out, cl);
+            return;
+        }
+        std::vector<gf> r(n);
+        for (uint32_t i = 0; i < n; i++) r[i] = load_elem(r_b + 16 * i);
+        const int hn = (int)n - 1;
+        std::vector<gf> cf(n + 1, gf_one());
+        for (int j = hn - 1; j >= 0; j--) {
+            gf t = gf_add(gf_one(), r[j]);
+            if (gf_eq(t, gf_zero())) fail(BFD_ERR_ARG, "r_j = 1 makes the halved eq table singular");
+            cf[j] = hmul(cf[j + 1], hinv(t));
+        }
+        DevBuf df(16 * N, st), dg(16 * N, st), T(16 * ((size_t)1 << hn), st);
+        hip_check(hipMemcpyAsync(df.p, f_b, 16 * N, hipMemcpyHostToDevice, st), "copy f");
+        hip_check(hipMemcpyAsync(dg.p, g_b, 16 * N, hipMemcpyHostToDevice, st), "copy g");
+        device_eq(ctx, r.data(), hn, T.as<gf>());
+        const unsigned kMaxPartials = 512;
+        DevBuf partials(16 * 3 * kMaxPartials, st);
+        ctx->pinned.ensure(16 * 3 * kMaxPartials + 64);
+        uint8_t *pin = (uint8_t *)ctx->pinned.p;
+        gf s = gf_one(), sub = gf_zero(), x = gf_zero();
+        for (int i = (int)n - 1; i >= 0; i--) {
+            const int k = (int)n - 1 - i;
+            unsigned nb = 0;
+            if (i == (int)n - 1)
+                ctx->timer.run(st, "k_prod_round", [&] {
+                    return launch::prod_round(st, df.as<gf>(), dg.as<gf>(), T.as<gf>(), (size_t)1 << i,
+                                              partials.as<gf>(), &nb);
+                });
+            else
+                ctx->timer.run(st, "k_prod_fold_round", [&] {
+                    return launch::prod_fold_round(st, df.as<gf>(), dg.as<gf>(), T.as<gf>(), (size_t)1 << i, x,
+                                                   partials.as<gf>(), &nb);
+                });
+            hip_check(hipMemcpyAsync(pin, partials.p, 48 * nb, hipMemcpyDeviceToHost, st), "copy partials");
+            hip_check(hipStreamSynchronize(st), "round sync");
+            gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
+            for (unsigned t = 0; t < nb; t++) {
+                A0 = gf_add(A0, load_elem(pin + 48 * t));
+                A1 = gf_add(A1, load_elem(pin + 48 * t + 16));
+                A2 = gf_add(A2, load_elem(pin + 48 * t + 32));
+            }
+            // q(X) = q0 + q1 X + q2 X^2 with q1 = A0 + A1 + A2 (characteristic 2)
+            const gf q0 = hmul(cf[i], A0), q2 = hmul(cf[i], A2), q1 = hmul(cf[i], gf_add(A0, gf_add(A1, A2)));
+            const gf e0 = gf_add(gf_one(), r[i]);  // eq(r_i, X) = (1 + r_i) + X
+            const gf c0 = hmul(s, hmul(e0, q0));
+            const gf c1 = hmul(s, gf_add(hmul(e0, q1), q0));
+            const gf c2 = hmul(s, gf_add(hmul(e0, q2), q1));
+            const gf c3 = hmul(s, q2);
+            if (i == (int)n - 1) {
+                sub = gf_add(c1, gf_add(c2, c3));  // claim = p(0) + p(1)
+                store_elem(claim_out, sub);
+                ch.observe_elem(sub);                // sumcheck/src/lib.rs:67
+            }
+            store_elem(round_polys + 48 * k, c0);
+            store_elem(round_polys + 48 * k + 16, c2);
+            store_elem(round_polys + 48 * k + 32, c3);
+            ch.observe_elem(c0);
+            ch.observe_elem(c2);
+            ch.observe_elem(c3);
+            x = ch.sample();
+            store_elem(challenges + 16 * k, x);
+            sub = gf_add(c0, hmul(x, gf_add(c1, hmul(x, gf_add(c2, hmul(x, c3))))));
+            s = hmul(s, gf_add(e0, x));
+        }
+        // last fold of the two remaining entries happens on the host
+        hip_check(hipMemcpyAsync(pin, df.p, 32, hipMemcpyDeviceToHost, st), "copy f");
+        hip_check(hipMemcpyAsync(pin + 32, dg.p, 32, hipMemcpyDeviceToHost, st), "copy g");
+        hip_check(hipStreamSynchronize(st), "final sync");
+        ctx->timer.collect();
+        gf f0 = load_elem(pin), f1 = load_elem(pin + 16), g0 = load_elem(pin + 32), g1 = load_elem(pin + 48);
+        store_elem(final_fg, gf_add(f0, hmul(x, gf_add(f1, f0))));
+        store_elem(final_fg + 16, gf_add(g0, hmul(x, gf_add(g1, g0))));
+        store_elem(subclaim_out, sub);
+    });
+}
+
+int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out) {
+    return guarded([&] {
+        if (!out && n) fail(BFD_ERR_ARG, "null out");
+        uint32_t key[8];
+        seed_key(seed, key);
+        const size_t words = 4 * n;
+        uint32_t blk[16];
+        for (size_t bidx = 0; bidx * 16 < words; bidx++) {
+            chacha_block(12, key, bidx, blk);
+            const size_t take = words - bidx * 16 < 16 ? words - bidx * 16 : 16;
+            memcpy(out + 64 * bidx, blk, 4 * take);
+        }
+    });
+}
+
+int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms) {
+    return guarded([&] {
+        if (!ctx || !ms) fail(BFD_ERR_ARG, "null argument");
+        if (what < 0 || what > 3 || iters < 1) fail(BFD_ERR_ARG, "unknown micro-benchmark");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        size_t bytes = what == 2 ? 200 * n : 32 * n;
+        DevBuf io(bytes, st);
+        std::vector<uint8_t> h(bytes);
+        for (size_t i = 0; i < bytes; i++) h[i] = (uint8_t)(i * 2654435761u >> 13);
+        hip_check(hipMemcpyAsync(io.p, h.data(), bytes, hipMemcpyHostToDevice, st), "copy");
+        hipEvent_t a, e;
+        hip_check(hipEventCreate(&a), "event");
+        hip_check(hipEventCreate(&e), "event");
+        auto go = [&](int it) {
+            if (what == 2) return launch::bench_keccak(st, io.as<uint64_t>(), n, it);
+            if (what == 3) return launch::copy16(st, io.as<gf>(), io.as<gf>() + n, n);
+            return launch::bench_mul(st, what, io.as<gf>(), n, it);
+        };
+        hip_check(go(1), "bench warm-up");
+        hip_check(hipEventRecord(a, st), "record");
+        hip_check(go(iters), "bench");
+        hip_check(hipEventRecord(e, st), "record");
+        hip_check(hipEventSynchronize(e), "sync");
+        float t = 0;
+        hip_check(hipEventElapsedTime(&t, a, e), "elapsed");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(e);
+        *ms = t;
+    });
+}
+
+int bfd_ctx_kernel_times(bfd_ctx *ctx, int enable, char *names, size_t name_len, double *ms,
+                         uint32_t *launches, size_t cap, size_t *count) {
+    return guarded([&] {
+        if (!ctx) fail(BFD_ERR_ARG, "null ctx");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        size_t i = 0;
+        for (auto &kv : ctx->timer.totals) {
+            if (i >= cap) break;
+            if (names && name_len) {
+                strncpy(names + i * name_len, kv.first.c_str(), name_len - 1);
+                names[i * name_len + name_len - 1] = 0;
+            }
+            if (ms) ms[i] = kv.second.first;
+            if (launches) launches[i] = kv.second.second;
+            i++;
+        }
+        if (count) *count = i;
+        ctx->timer.totals.clear();
+        ctx->timer.enabled = enable != 0;
+    });
+}
+
+}  // extern "C"

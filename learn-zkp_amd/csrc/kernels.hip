@@ -1,0 +1,473 @@
+// CDNA4 kernels of the Basefold commit/open path over POLYVAL GF(2^128).
+//
+// Data layout in HBM (all arrays in the reference's codeword order, 16 B per
+// element, 32 B per digest):
+//   codeword   cw[x*c + u]      u < c is the coset slot; slot u holds coset
+//                               t = bitrev_b(u), which makes the final
+//                               bit-reversal of encode_batch
+//                               (binary_reed_solomon.rs:81) the identity.
+//   fold table ft[s][j]         = twiddles[b+s][bitrev_{b+s}(j)], j < c*2^s:
+//                               the fold twiddle of level s
+//                               (binary_reed_solomon.rs:29-35, 88-96) and,
+//                               at index (p << b) | u, the NTT twiddle of
+//                               data-order layer s (binary.rs:98-137).
+//   Merkle     layers stored leaf layer first, each layer contiguous.
+//
+// In data order (index x of the input, not of its bit reversal) the
+// reference's coset NTT layer i = n-1-s pairs x and x + 2^s, so the n layers
+// run with strides 1, 2, ..., N/2 over contiguous memory.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gf128.hpp"
+#include "keccak.hpp"
+#include "kernels.hpp"
+
+namespace bfd {
+namespace {
+
+constexpr int CB = 128;  // block size of kernels that multiply (LDS comb tables)
+constexpr int KB = 256;  // block size of hashing kernels
+
+__device__ __forceinline__ uint32_t *comb_slot(uint32_t *lds) { return lds + threadIdx.x; }
+
+inline unsigned grid_for(size_t work, int block, unsigned cap) {
+    size_t g = (work + block - 1) / block;
+    if (g > cap) g = cap;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// twiddle tables
+// ---------------------------------------------------------------------------
+// ft[j] = XOR over set bits p of j of bt[p], j < count
+__global__ void k_xor_span(gf *ft, const gf *bt, int bits, size_t count) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < count;
+         j += (size_t)gridDim.x * blockDim.x) {
+        gf acc = gf_zero();
+        for (int p = 0; p < bits; p++)
+            if ((j >> p) & 1) acc = gf_add(acc, bt[p]);
+        gf_st(ft + j, acc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reed-Solomon encode: NTT layers in data order
+// ---------------------------------------------------------------------------
+// layer s = 0 (stride 1) reading the input and replicating it into all slots
+__global__ __launch_bounds__(CB) void k_ntt_first(const gf *__restrict__ data, gf *__restrict__ cw,
+                                                  const gf *__restrict__ ft0, int c, size_t half) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < half;
+         q += (size_t)gridDim.x * blockDim.x) {
+        gf lo0 = gf_ld(data + 2 * q), hi0 = gf_ld(data + 2 * q + 1);
+        for (int u = 0; u < c; u++) {
+            gf t = gf_ld(ft0 + u);
+            gf lo = gf_add(lo0, gf_mul(hi0, t, tab, CB));
+            gf hi = gf_add(hi0, lo);
+            gf_st(cw + (2 * q) * c + u, lo);
+            gf_st(cw + (2 * q + 1) * c + u, hi);
+        }
+    }
+}
+
+// general layer s >= 1, in place on the codeword buffer
+__global__ __launch_bounds__(CB) void k_ntt_layer(gf *__restrict__ cw, const gf *__restrict__ fts,
+                                                  int s, int b, size_t half) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    const int c = 1 << b;
+    const size_t pmask = ((size_t)1 << s) - 1;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < half;
+         q += (size_t)gridDim.x * blockDim.x) {
+        size_t p = q & pmask;
+        size_t x = ((q >> s) << (s + 1)) | p;
+        size_t y = x + ((size_t)1 << s);
+        for (int u = 0; u < c; u++) {
+            gf t = gf_ld(fts + ((p << b) | u));
+            gf lo = gf_ld(cw + x * c + u), hi = gf_ld(cw + y * c + u);
+            lo = gf_add(lo, gf_mul(hi, t, tab, CB));
+            hi = gf_add(hi, lo);
+            gf_st(cw + x * c + u, lo);
+            gf_st(cw + y * c + u, hi);
+        }
+    }
+}
+
+__global__ void k_replicate(const gf *__restrict__ data, gf *__restrict__ cw, int c, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n * c;
+         i += (size_t)gridDim.x * blockDim.x)
+        gf_st(cw + i, gf_ld(data + i / c));
+}
+
+// ---------------------------------------------------------------------------
+// Merkle hashing (Keccak-256)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void st_digest(uint64_t *p, const uint64_t d[4]) {
+    reinterpret_cast<ulonglong2 *>(p)[0] = ulonglong2{d[0], d[1]};
+    reinterpret_cast<ulonglong2 *>(p)[1] = ulonglong2{d[2], d[3]};
+}
+
+// leaf i = Keccak256(elem[i]) for width-1 rows (16-byte leaves)
+__global__ __launch_bounds__(KB) void k_leaves_elem(const gf *__restrict__ rows, uint64_t *__restrict__ out,
+                                                    size_t count) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x) {
+        gf v = gf_ld(rows + i);
+        uint64_t m[2] = {v.lo, v.hi}, d[4];
+        kk::keccak256_words<2>(m, d);
+        st_digest(out + 4 * i, d);
+    }
+}
+
+// leaf j = Keccak256(pi[j] || pi[j + half]) (pi-tree rows, basefold.rs:350-354)
+__global__ __launch_bounds__(KB) void k_leaves_pair(const gf *__restrict__ pi, uint64_t *__restrict__ out,
+                                                    size_t half) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
+         j += (size_t)gridDim.x * blockDim.x) {
+        gf a = gf_ld(pi + j), b = gf_ld(pi + j + half);
+        uint64_t m[4] = {a.lo, a.hi, b.lo, b.hi}, d[4];
+        kk::keccak256_words<4>(m, d);
+        st_digest(out + 4 * j, d);
+    }
+}
+
+// node i = Keccak256(prev[2i] || prev[2i+1])
+__global__ __launch_bounds__(KB) void k_nodes(const uint64_t *__restrict__ prev, uint64_t *__restrict__ out,
+                                              size_t count) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(prev + 8 * i);
+        ulonglong2 a = p[0], b = p[1], c = p[2], e = p[3];
+        uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
+        kk::keccak256_words<8>(m, d);
+        st_digest(out + 4 * i, d);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// open: sumcheck inner product, fold, fix_last_var, eq table
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ gf wave_xor(gf v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        v.lo ^= __shfl_xor(v.lo, off);
+        v.hi ^= __shfl_xor(v.hi, off);
+    }
+    return v;
+}
+
+// partial[block] = XOR_k a[k] * b[k]
+__global__ __launch_bounds__(CB) void k_dot(const gf *__restrict__ a, const gf *__restrict__ b, size_t n,
+                                            gf *__restrict__ partial) {
+    __shared__ uint32_t lds[80 * CB];
+    __shared__ gf red[CB / 64];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    gf acc = gf_zero();
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x)
+        acc = gf_add(acc, gf_mul(gf_ld(a + k), gf_ld(b + k), tab, CB));
+    acc = wave_xor(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        gf s = gf_zero();
+        for (int w = 0; w < CB / 64; w++) s = gf_add(s, red[w]);
+        gf_st(partial + blockIdx.x, s);
+    }
+}
+
+// out[j] = interpolate(ft[j], w[j], w[j+mid], r)  (binary_reed_solomon.rs:103-109)
+__global__ __launch_bounds__(CB) void k_fold(const gf *__restrict__ w, gf *__restrict__ out,
+                                             const gf *__restrict__ ft, size_t mid, gf r) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < mid;
+         j += (size_t)gridDim.x * blockDim.x) {
+        gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
+        hi = gf_add(hi, lo);
+        lo = gf_add(lo, gf_mul(hi, gf_ld(ft + j), tab, CB));
+        gf_st(out + j, gf_add(lo, gf_mul(gf_add(hi, lo), r, tab, CB)));
+    }
+}
+
+// out[j] = in[j] + x * (in[j+half] - in[j])  (multilinear.rs:183-198)
+__global__ __launch_bounds__(CB) void k_fix(const gf *__restrict__ in, gf *__restrict__ out, size_t half, gf x) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    gf_comb_table(tab, CB, x);
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
+         j += (size_t)gridDim.x * blockDim.x) {
+        gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
+        gf_st(out + j, gf_add(lo, gf_comb_mul(gf_add(hi, lo), tab, CB)));
+    }
+}
+
+// eq table as the tensor product of two host-built factors:
+// out[lo + (hi << m)] = A[lo] * B[hi]
+__global__ __launch_bounds__(CB) void k_outer(const gf *__restrict__ A, int m, const gf *__restrict__ B,
+                                              gf *__restrict__ out, size_t n) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    const size_t mask = ((size_t)1 << m) - 1;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        gf_st(out + i, gf_mul(gf_ld(A + (i & mask)), gf_ld(B + (i >> m)), tab, CB));
+}
+
+// ---------------------------------------------------------------------------
+// degree-3 product sumcheck  sum_b f(b) g(b) eq(r, b)
+//
+// With the top variable X split off, eq(r, (b, X)) = T[b] * cf * eq(r_i, X)
+// (T the halved eq table, cf its correcting factor), so the round polynomial
+// is s * ((1 + r_i) + X) * q(X) with q(X) = cf * sum_b T[b] f_b(X) g_b(X).
+// Per pair the kernel accumulates A0 = sum T f0 g0, A1 = sum T f1 g1 and
+// A2 = sum T (f0+f1)(g0+g1); the host turns them into q's coefficients.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void block_xor3(gf a0, gf a1, gf a2, gf *partial) {
+    __shared__ gf red[3][CB / 64];
+    a0 = wave_xor(a0);
+    a1 = wave_xor(a1);
+    a2 = wave_xor(a2);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a0;
+        red[1][threadIdx.x >> 6] = a1;
+        red[2][threadIdx.x >> 6] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        gf s = gf_zero();
+        for (int w = 0; w < CB / 64; w++) s = gf_add(s, red[threadIdx.x][w]);
+        gf_st(partial + 3 * blockIdx.x + threadIdx.x, s);
+    }
+}
+
+__device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf &a0, gf &a1, gf &a2,
+                                           uint32_t *tab) {
+    gf p0 = gf_mul(f0, g0, tab, CB);
+    gf p1 = gf_mul(f1, g1, tab, CB);
+    gf p2 = gf_mul(gf_add(f0, f1), gf_add(g0, g1), tab, CB);
+    gf_comb_table(tab, CB, t);
+    a0 = gf_add(a0, gf_comb_mul(p0, tab, CB));
+    a1 = gf_add(a1, gf_comb_mul(p1, tab, CB));
+    a2 = gf_add(a2, gf_comb_mul(p2, tab, CB));
+}
+
+// sums of the first round over tables of 2*half entries
+__global__ __launch_bounds__(CB) void k_prod_round(const gf *__restrict__ f, const gf *__restrict__ g,
+                                                   const gf *__restrict__ T, size_t half, gf *__restrict__ partial) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
+         j += (size_t)gridDim.x * blockDim.x)
+        prod_terms(gf_ld(f + j), gf_ld(f + j + half), gf_ld(g + j), gf_ld(g + j + half), gf_ld(T + j), a0, a1,
+                   a2, tab);
+    block_xor3(a0, a1, a2, partial);
+}
+
+// fixes the top variable of f and g (tables of 4*q entries -> 2*q) with x,
+// in place, and accumulates the next round's sums over the new tables
+__global__ __launch_bounds__(CB) void k_prod_fold_round(gf *__restrict__ f, gf *__restrict__ g,
+                                                        const gf *__restrict__ T, size_t q, gf x,
+                                                        gf *__restrict__ partial) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
+    const size_t h = 2 * q;
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < q;
+         j += (size_t)gridDim.x * blockDim.x) {
+        gf fa = gf_ld(f + j), fb = gf_ld(f + j + q), fc = gf_ld(f + j + h), fd = gf_ld(f + j + h + q);
+        gf ga = gf_ld(g + j), gb = gf_ld(g + j + q), gc = gf_ld(g + j + h), gd = gf_ld(g + j + h + q);
+        gf_comb_table(tab, CB, x);
+        gf f0 = gf_add(fa, gf_comb_mul(gf_add(fc, fa), tab, CB));
+        gf f1 = gf_add(fb, gf_comb_mul(gf_add(fd, fb), tab, CB));
+        gf g0 = gf_add(ga, gf_comb_mul(gf_add(gc, ga), tab, CB));
+        gf g1 = gf_add(gb, gf_comb_mul(gf_add(gd, gb), tab, CB));
+        gf_st(f + j, f0);
+        gf_st(f + j + q, f1);
+        gf_st(g + j, g0);
+        gf_st(g + j + q, g1);
+        prod_terms(f0, f1, g0, g1, gf_ld(T + j), a0, a1, a2, tab);
+    }
+    block_xor3(a0, a1, a2, partial);
+}
+
+// ---------------------------------------------------------------------------
+// query gathers
+// ---------------------------------------------------------------------------
+__global__ void k_gather_digest(const uint64_t *__restrict__ base, const uint64_t *__restrict__ idx,
+                                uint64_t *__restrict__ out, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(base + 4 * idx[i]);
+        ulonglong2 *o = reinterpret_cast<ulonglong2 *>(out + 4 * i);
+        o[0] = p[0];
+        o[1] = p[1];
+    }
+}
+__global__ void k_gather_elem(const gf *__restrict__ base, const uint64_t *__restrict__ idx, gf *__restrict__ out,
+                              size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        gf_st(out + i, gf_ld(base + idx[i]));
+}
+
+// ---------------------------------------------------------------------------
+// micro-benchmarks of the two multiply forms and of Keccak-f
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CB) void k_bench_mul_comb(gf *io, size_t n, int iters) {
+    __shared__ uint32_t lds[80 * CB];
+    uint32_t *tab = comb_slot(lds);
+    gf_comb_init(tab, CB);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        gf a = gf_ld(io + 2 * i), b = gf_ld(io + 2 * i + 1);
+        for (int k = 0; k < iters; k++) {
+            gf c = gf_mul(a, b, tab, CB);
+            b = a;
+            a = c;
+        }
+        gf_st(io + 2 * i, a);
+    }
+}
+__global__ __launch_bounds__(CB) void k_bench_mul_reg(gf *io, size_t n, int iters) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        gf a = gf_ld(io + 2 * i), b = gf_ld(io + 2 * i + 1);
+        for (int k = 0; k < iters; k++) {
+            gf c = gf_mul_reg(a, b);
+            b = a;
+            a = c;
+        }
+        gf_st(io + 2 * i, a);
+    }
+}
+__global__ __launch_bounds__(KB) void k_bench_keccak(uint64_t *io, size_t n, int iters) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t s[25];
+#pragma unroll
+        for (int k = 0; k < 25; k++) s[k] = io[25 * i + k];
+        for (int k = 0; k < iters; k++) kk::keccak_f(s);
+#pragma unroll
+        for (int k = 0; k < 25; k++) io[25 * i + k] = s[k];
+    }
+}
+
+// plain 16-byte-per-thread copy: the reference point for HBM byte counters
+__global__ void k_copy16(const gf *__restrict__ src, gf *__restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        gf_st(dst + i, gf_ld(src + i));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace launch {
+
+static const unsigned kCapMul = 256 * 8;
+static const unsigned kCapHash = 256 * 16;
+
+hipError_t xor_span(hipStream_t st, gf *ft, const gf *bt, int bits, size_t count) {
+    k_xor_span<<<grid_for(count, 256, 65535), 256, 0, st>>>(ft, bt, bits, count);
+    return hipGetLastError();
+}
+
+hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n) {
+    const int c = 1 << b;
+    if (n == 0) {
+        k_replicate<<<1, 64, 0, st>>>(data, cw, c, 1);
+        return hipGetLastError();
+    }
+    const size_t half = ((size_t)1 << n) / 2;
+    k_ntt_first<<<grid_for(half, CB, kCapMul), CB, 0, st>>>(data, cw, ft0, c, half);
+    return hipGetLastError();
+}
+hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n) {
+    const size_t half = ((size_t)1 << n) / 2;
+    k_ntt_layer<<<grid_for(half, CB, kCapMul), CB, 0, st>>>(cw, fts, s, b, half);
+    return hipGetLastError();
+}
+
+hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count) {
+    k_leaves_elem<<<grid_for(count, KB, kCapHash), KB, 0, st>>>(rows, out, count);
+    return hipGetLastError();
+}
+hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half) {
+    k_leaves_pair<<<grid_for(half, KB, kCapHash), KB, 0, st>>>(pi, out, half);
+    return hipGetLastError();
+}
+hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count) {
+    k_nodes<<<grid_for(count, KB, kCapHash), KB, 0, st>>>(prev, next, count);
+    return hipGetLastError();
+}
+
+hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks) {
+    unsigned g = grid_for(n, CB, 512);
+    k_dot<<<g, CB, 0, st>>>(a, b, n, partial);
+    *nblocks = g;
+    return hipGetLastError();
+}
+hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r) {
+    k_fold<<<grid_for(mid, CB, kCapMul), CB, 0, st>>>(w, out, ft, mid, r);
+    return hipGetLastError();
+}
+hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x) {
+    k_fix<<<grid_for(half, CB, kCapMul), CB, 0, st>>>(in, out, half, x);
+    return hipGetLastError();
+}
+hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_t n) {
+    k_outer<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(A, m, B, out, n);
+    return hipGetLastError();
+}
+hipError_t prod_round(hipStream_t st, const gf *f, const gf *g, const gf *T, size_t half, gf *partial,
+                      unsigned *nblocks) {
+    unsigned gsz = grid_for(half, CB, 512);
+    k_prod_round<<<gsz, CB, 0, st>>>(f, g, T, half, partial);
+    *nblocks = gsz;
+    return hipGetLastError();
+}
+hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, gf x, gf *partial,
+                           unsigned *nblocks) {
+    unsigned gsz = grid_for(q, CB, 512);
+    k_prod_fold_round<<<gsz, CB, 0, st>>>(f, g, T, q, x, partial);
+    *nblocks = gsz;
+    return hipGetLastError();
+}
+hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n) {
+    k_gather_digest<<<grid_for(n, 256, 4096), 256, 0, st>>>(base, idx, out, n);
+    return hipGetLastError();
+}
+hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *out, size_t n) {
+    k_gather_elem<<<grid_for(n, 256, 4096), 256, 0, st>>>(base, idx, out, n);
+    return hipGetLastError();
+}
+
+hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters) {
+    if (variant == 0)
+        k_bench_mul_comb<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
+    else
+        k_bench_mul_reg<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
+    return hipGetLastError();
+}
+hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n) {
+    k_copy16<<<grid_for(n, 256, 1 << 16), 256, 0, st>>>(src, dst, n);
+    return hipGetLastError();
+}
+hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters) {
+    k_bench_keccak<<<grid_for(n, KB, 1 << 20), KB, 0, st>>>(io, n, iters);
+    return hipGetLastError();
+}
+
+}  // namespace launch
+}  // namespace bfd

@@ -1,0 +1,38 @@
+// Launchers of the CDNA4 kernels (kernels.hip).  All take the stream first
+// and return the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "gf128.hpp"
+
+namespace bfd {
+namespace launch {
+
+// ft[j] = XOR over set bits p of j of bt[p]
+hipError_t xor_span(hipStream_t st, gf *ft, const gf *bt, int bits, size_t count);
+// RS encode: layer 0 (reads the 2^n evaluations, replicates into the c slots)
+// then layers s = 1..n-1 in place on the slot-interleaved codeword
+hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n);
+hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n);
+hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count);
+hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half);
+// one Merkle level: next[i] = H(prev[2i] || prev[2i+1])
+hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count);
+hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks);
+hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r);
+hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x);
+hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_t n);
+hipError_t prod_round(hipStream_t st, const gf *f, const gf *g, const gf *T, size_t half, gf *partial,
+                      unsigned *nblocks);
+hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, gf x, gf *partial,
+                           unsigned *nblocks);
+hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n);
+hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *out, size_t n);
+hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters);
+hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n);
+hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters);
+
+}  // namespace launch
+}  // namespace bfd

@@ -13,6 +13,13 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
+def bfd():
+    """The HIP library's Python mirror (package dir learn-zkp_amd/)."""
+    from __graft_entry__ import load_package
+    return load_package()
+
+
+@pytest.fixture(scope="session")
 def oracle():
     import oracle as O
     O.lib()
