@@ -74,6 +74,8 @@ def main() -> None:
     ap.add_argument("--cpu-log-n", type=int, default=20)
     ap.add_argument("--cpu-samples", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-ref", action="store_true",
+                    help="run one k_copy16 over 2^24 elements first (HBM byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -87,6 +89,8 @@ def main() -> None:
     n = args.log_n
     N = 1 << n
     ctx = bfd.Context(0)
+    if args.copy_ref:
+        ctx.microbench(3, 1 << 24, 1)
     pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(LAMBDA, LOG2_C, n)), ctx)
     evals = bfd.random_vec(0, N)
     point = bfd.random_vec(1, n)

@@ -17,169 +17,74 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     set_device(ctx);
     hipStream_t st = ctx->stream;
-    const int n = (int)cm->n, d = n, b = (int)cm->b, L = b + d;
-    const size_t M = cm->M, c = (size_t)1 << b;
-    std::vector<gf> r(n), rinv(n);
-    for (int i = 0; i < n; i++) {
-        r[i] = load_elem(point_b + 16 * i);
-        if (gf_eq(r[i], gf_zero()) || gf_eq(r[i], gf_one()))
-            fail(BFD_ERR_ARG, "point coordinates 0 and 1 are not invertible in the Eval prover");
-        rinv[i] = hinv(r[i]);                            // EvalImpr::new, eval_impr.rs:20-27
-    }
-    const gf claim = load_elem(values_b);
-    const int hn = n > 0 ? n - 1 : 0;                    // HalfEq::new, eval_impr.rs:69-79
-    // correcting_factor(i) = prod_{j=i}^{n-2} (1 - r_j)^-1  (eval_impr.rs:81-88)
-    std::vector<gf> cf(n + 1, gf_one());
-    for (int j = hn - 1; j >= 0; j--) cf[j] = hmul(cf[j + 1], hinv(gf_add(gf_one(), r[j])));
+    OpenRounds o;
+    o.init((int)cm->n, (int)cm->b, code->q, point_b, values_b);
+    const int d = o.d;
+    const size_t M = cm->M, c = (size_t)1 << o.b;
 
     // device scratch for the whole opening
-    const size_t half_eq_n = (size_t)1 << hn;
-    DevBuf half_eq(16 * half_eq_n, st);
-    if (n > 0) device_eq(ctx, r.data(), hn, half_eq.as<gf>());
+    const int hn = o.half_eq_vars();
+    DevBuf half_eq(16 * ((size_t)1 << hn), st);
+    if (o.n > 0) device_eq(ctx, o.r.data(), hn, half_eq.as<gf>());
     DevBuf poly(16 * (cm->N > 1 ? cm->N / 2 : 1), st);
-    // pi_k for k >= 1 (pi_0 is the codeword): sum_{k=1..d} M >> k = M - c
-    std::vector<size_t> pi_off(d + 1, 0), tree_off(d + 1, 0);
-    for (int k = 1; k <= d; k++) pi_off[k] = (k == 1 ? 0 : pi_off[k - 1] + (M >> (k - 1)));
-    DevBuf pis(16 * (M - c + 1), st);
-    size_t tree_total = 0;
-    for (int k = 0; k < d; k++) {
-        tree_off[k] = tree_total;
-        tree_total += (M >> k) - 1;
-    }
-    DevBuf trees(32 * (tree_total ? tree_total : 1), st);
+    OpenLayout lay;
+    lay.plan(M, d);
+    DevBuf pis(16 * lay.pis_elems(M, c), st), trees(32 * lay.pitree_digests(), st);
+    lay.cw = cm->cw.as<gf>();
+    lay.tree = cm->tree.as<uint64_t>();
+    lay.pis = pis.as<gf>();
+    lay.pitrees = trees.as<uint64_t>();
     const unsigned kMaxPartials = 512;
     DevBuf partials(16 * kMaxPartials, st);
     ctx->pinned.ensure(32 + 16 * kMaxPartials + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
 
-    auto pi_ptr = [&](int k) -> gf * { return k == 0 ? cm->cw.as<gf>() : pis.as<gf>() + pi_off[k]; };
-    auto tree_ptr = [&](int k) -> uint64_t * { return trees.as<uint64_t>() + 4 * tree_off[k]; };
-
-    std::vector<std::array<gf, 2>> comp(d);
-    std::vector<std::array<uint8_t, 32>> pi_roots(d);
-    gf subclaim = claim, inner_sub = gf_zero(), s_delta = gf_one();
     const gf *poly_cur = cm->data.as<gf>();
-
     for (int k = 0; k < d; k++) {
         const int i = d - 1 - k;
-        const size_t len = M >> k, half = len / 2;
-        gf *pik = pi_ptr(k);
-        uint64_t *tk = tree_ptr(k);
+        const size_t half = (M >> k) / 2, ph = (size_t)1 << i;
+        const gf *pik = lay.pi(k);
+        uint64_t *tk = trees.as<uint64_t>() + 4 * lay.tree_off[k];
         // pi tree over rows (pi[j], pi[j+half])  (basefold.rs:350-355)
         ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half); });
         tree_nodes(ctx, tk, half);
         hip_check(hipMemcpyAsync(pin, tk + 4 * (2 * half - 2), 32, hipMemcpyDeviceToHost, st), "copy pi root");
         // inner product of the halved eq table with the low half of poly (eval_impr.rs:187-197)
         unsigned nb = 0;
-        const size_t ph = (size_t)1 << i;
         ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, partials.as<gf>(), &nb); });
         hip_check(hipMemcpyAsync(pin + 32, partials.p, 16 * nb, hipMemcpyDeviceToHost, st), "copy partials");
         hip_check(hipStreamSynchronize(st), "round sync");
-        memcpy(pi_roots[k].data(), pin, 32);
         gf S = gf_zero();
         for (unsigned t = 0; t < nb; t++) S = gf_add(S, load_elem(pin + 32 + 16 * t));
-
-        ch.observe(pi_roots[k].data(), 32);                          // basefold.rs:356
-        const gf sub = (i == n - 1) ? subclaim : inner_sub;          // eval.rs:90-94
-        const gf coeff0 = hmul(cf[i], S);
-        const gf eval1 = hmul(gf_add(sub, hmul(gf_add(gf_one(), r[i]), coeff0)), rinv[i]);  // eval_impr.rs:41-43
-        const gf coeff1 = gf_add(eval1, coeff0);
-        // delta (eval_impr.rs:45-58): scalar = eq(r[i+1..], x[i+1..]); in
-        // characteristic 2 the factors are [s (1 + r_i), s]
-        const gf d0 = hmul(s_delta, gf_add(gf_one(), r[i]));
-        const gf d1 = s_delta;
-        const gf rp0 = hmul(coeff0, d0), rp1 = gf_add(hmul(coeff0, d1), hmul(coeff1, d0)), rp2 = hmul(coeff1, d1);
-        comp[k] = {rp0, rp2};                                        // eval.rs:64-66
-        ch.observe_elem(rp0);                                        // sumcheck/src/lib.rs:94
-        ch.observe_elem(rp2);
-        const gf x = ch.sample();                                    // lib.rs:96
-        inner_sub = gf_add(coeff0, hmul(coeff1, x));                 // eval.rs:110
-        subclaim = gf_add(rp0, hmul(x, gf_add(rp1, hmul(x, rp2))));  // lib.rs:100 (horner)
-        s_delta = hmul(s_delta, gf_add(gf_one(), gf_add(r[i], x)));
+        const gf x = o.round(k, pin, S, ch);
         // fold pi (binary_reed_solomon.rs:88-96) and fix the top variable of poly
-        gf *pnext = pi_ptr(k + 1);
+        gf *pnext = pis.as<gf>() + lay.pi_off[k + 1];
         const gf *ftl = code->ft.as<gf>() + code->ft_off[i];
         ctx->timer.run(st, "k_fold", [&] { return launch::fold(st, pik, pnext, ftl, half, x); });
         gf *pout = poly.as<gf>();
         ctx->timer.run(st, "k_fix", [&] { return launch::fix(st, poly_cur, pout, ph, x); });
         poly_cur = pout;
     }
-    // final_poly (basefold.rs:194-196)
     hip_check(hipMemcpyAsync(pin, poly_cur, 16, hipMemcpyDeviceToHost, st), "copy final");
     hip_check(hipStreamSynchronize(st), "final sync");
-    ctx->timer.collect();
-    const gf final_poly = load_elem(pin);
-    ch.observe_elem(final_poly);
+    o.finish(load_elem(pin), ch);
 
-    // queries (basefold.rs:200-222)
-    const size_t q = code->q;
-    std::vector<uint64_t> idx(q);
-    for (size_t t = 0; t < q; t++) idx[t] = ch.sample_bits(L);
-    // gather plan
-    std::vector<uint64_t> g_main_dig, g_pi_dig, g_cw, g_pi;
-    auto layer_off = [](size_t leaves, int t) { return 2 * leaves - 2 * (leaves >> t); };
-    for (size_t t = 0; t < q; t++) {
-        g_cw.push_back(idx[t]);
-        for (int u = 0; u < L; u++) g_main_dig.push_back(layer_off(M, u) + ((idx[t] >> u) ^ 1));
-        for (int k = 0; k < d; k++) {
-            const int i = d - 1 - k;
-            const size_t ni = (size_t)1 << (b + i);
-            const size_t sib = ((idx[t] / ni) & 1) ^ 1, row = idx[t] % ni;
-            if (k == 0) g_cw.push_back(row + sib * ni);
-            else g_pi.push_back(pi_off[k] + row + sib * ni);
-            for (int u = 0; u < b + i; u++) g_pi_dig.push_back(tree_off[k] + layer_off(ni, u) + ((row >> u) ^ 1));
-        }
-    }
-    auto gather = [&](const std::vector<uint64_t> &ix, bool digest, const void *base, std::vector<uint8_t> &host) {
-        const size_t nrec = ix.size(), rb = digest ? 32 : 16;
-        host.resize(nrec * rb);
-        if (!nrec) return;
-        DevBuf dix(8 * nrec, st), dout(rb * nrec, st);
-        hip_check(hipMemcpyAsync(dix.p, ix.data(), 8 * nrec, hipMemcpyHostToDevice, st), "copy gather idx");
-        if (digest)
-            ctx->timer.run(st, "k_gather", [&] { return launch::gather_digest(st, (const uint64_t *)base, dix.as<uint64_t>(), dout.as<uint64_t>(), nrec); });
-        else
-            ctx->timer.run(st, "k_gather", [&] { return launch::gather_elem(st, (const gf *)base, dix.as<uint64_t>(), dout.as<gf>(), nrec); });
-        hip_check(hipMemcpyAsync(host.data(), dout.p, rb * nrec, hipMemcpyDeviceToHost, st), "copy gather");
-        hip_check(hipStreamSynchronize(st), "gather sync");
-    };
-    std::vector<uint8_t> h_main_dig, h_pi_dig, h_cw, h_pi;
-    gather(g_main_dig, true, cm->tree.p, h_main_dig);
-    gather(g_pi_dig, true, trees.p, h_pi_dig);
-    gather(g_cw, false, cm->cw.p, h_cw);
-    gather(g_pi, false, pis.p, h_pi);
+    // openings: one gather of elements, one of digests
+    std::vector<uint64_t> ea, da;
+    plan_gathers(o, lay, ea, da);
+    std::vector<uint8_t> he(16 * ea.size()), hd(32 * da.size());
+    DevBuf dix(8 * (ea.size() + da.size()) + 8, st), dout(16 * ea.size() + 32 * da.size() + 16, st);
+    hip_check(hipMemcpyAsync(dix.p, ea.data(), 8 * ea.size(), hipMemcpyHostToDevice, st), "copy gather plan");
+    hip_check(hipMemcpyAsync(dix.as<uint64_t>() + ea.size(), da.data(), 8 * da.size(), hipMemcpyHostToDevice, st),
+              "copy gather plan");
+    uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * ea.size();
+    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, ea.size(), 16); });
+    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>() + ea.size(), od, da.size(), 32); });
+    if (!he.empty()) hip_check(hipMemcpyAsync(he.data(), oe, he.size(), hipMemcpyDeviceToHost, st), "copy gather");
+    if (!hd.empty()) hip_check(hipMemcpyAsync(hd.data(), od, hd.size(), hipMemcpyDeviceToHost, st), "copy gather");
+    hip_check(hipStreamSynchronize(st), "gather sync");
     ctx->timer.collect();
-
-    // bincode BasefoldProof (basefold.rs:49-66)
-    proof.clear();
-    put_u64(proof, (uint64_t)d);
-    for (int k = 0; k < d; k++) {
-        put_u64(proof, 2);
-        put_elem(proof, comp[k][0]);
-        put_elem(proof, comp[k][1]);
-    }
-    put_u64(proof, 1);
-    put_elem(proof, final_poly);
-    put_u64(proof, (uint64_t)d);
-    for (int k = 0; k < d; k++) put_bytes(proof, pi_roots[k].data(), 32);
-    put_u64(proof, (uint64_t)q);
-    size_t pm = 0, pp = 0, pc = 0, pv = 0;
-    for (size_t t = 0; t < q; t++) {
-        put_u64(proof, 1);
-        put_bytes(proof, &h_cw[16 * pc++], 16);
-        put_u64(proof, (uint64_t)L);
-        put_bytes(proof, &h_main_dig[32 * pm], 32 * (size_t)L);
-        pm += L;
-        put_u64(proof, (uint64_t)d);
-        for (int k = 0; k < d; k++) {
-            const int i = d - 1 - k;
-            if (k == 0) put_bytes(proof, &h_cw[16 * pc++], 16);
-            else put_bytes(proof, &h_pi[16 * pv++], 16);
-            put_u64(proof, (uint64_t)(b + i));
-            put_bytes(proof, &h_pi_dig[32 * pp], 32 * (size_t)(b + i));
-            pp += b + i;
-        }
-    }
+    assemble_proof(o, he.data(), hd.data(), proof);
 }
 
 }  // namespace

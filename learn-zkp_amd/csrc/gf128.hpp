@@ -120,38 +120,112 @@ BFD_D gf gf_mul(gf a, gf b, uint32_t *tab, int stride) {
     return gf_comb_mul(a, tab, stride);
 }
 
-// Register-only 32x32 carry-less product: integer products of operands
-// masked to every fourth bit; carries stay inside the three-bit holes.
-BFD_D uint64_t clmul32_holes(uint32_t x, uint32_t y) {
-    const uint32_t m0 = 0x11111111u, m1 = m0 << 1, m2 = m0 << 2, m3 = m0 << 3;
-    uint32_t x0 = x & m0, x1 = x & m1, x2 = x & m2, x3 = x & m3;
-    uint32_t y0 = y & m0, y1 = y & m1, y2 = y & m2, y3 = y & m3;
-    uint64_t z0 = ((uint64_t)x0 * y0) ^ ((uint64_t)x1 * y3) ^ ((uint64_t)x2 * y2) ^ ((uint64_t)x3 * y1);
-    uint64_t z1 = ((uint64_t)x0 * y1) ^ ((uint64_t)x1 * y0) ^ ((uint64_t)x2 * y3) ^ ((uint64_t)x3 * y2);
-    uint64_t z2 = ((uint64_t)x0 * y2) ^ ((uint64_t)x1 * y1) ^ ((uint64_t)x2 * y0) ^ ((uint64_t)x3 * y3);
-    uint64_t z3 = ((uint64_t)x0 * y3) ^ ((uint64_t)x1 * y2) ^ ((uint64_t)x2 * y1) ^ ((uint64_t)x3 * y0);
-    const uint64_t M0 = 0x1111111111111111ull;
-    return (z0 & M0) | (z1 & (M0 << 1)) | (z2 & (M0 << 2)) | (z3 & (M0 << 3));
+// ---------------------------------------------------------------------------
+// Register multiply (no LDS).  A 32x32 carry-less product is four masked
+// 32x32->64 integer multiply-adds per class: operands are split by the masks
+// 0x1111.. << k so every data bit is followed by a three-bit hole; an
+// integer product of two such pieces sums at most 8 terms per position, so
+// carries stay inside the hole and bit 4i+c of the sum over the class-c
+// pieces is the GF(2) coefficient.  128x128 uses two Karatsuba levels
+// (9 such products); sums of three terms use v_bitop3_b32.
+// ---------------------------------------------------------------------------
+BFD_D uint32_t x3(uint32_t a, uint32_t b, uint32_t c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
+
+// masked pieces of a 32-bit operand
+struct pieces {
+    uint32_t p[4];
+};
+BFD_D pieces split4(uint32_t x) {
+    const uint32_t m0 = 0x11111111u;
+    return pieces{{x & m0, x & (m0 << 1), x & (m0 << 2), x & (m0 << 3)}};
+}
+BFD_D pieces pxor(pieces a, pieces b) {
+    return pieces{{a.p[0] ^ b.p[0], a.p[1] ^ b.p[1], a.p[2] ^ b.p[2], a.p[3] ^ b.p[3]}};
 }
 
-BFD_D void clmul64_holes(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
-    uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-    uint64_t l = clmul32_holes(a0, b0), h = clmul32_holes(a1, b1);
-    uint64_t m = clmul32_holes(a0 ^ a1, b0 ^ b1) ^ l ^ h;
-    lo = l ^ (m << 32);
-    hi = h ^ (m >> 32);
+struct u64p {
+    uint32_t lo, hi;
+};
+
+BFD_D u64p clmul32p(const pieces &x, const pieces &y) {
+    uint64_t pr[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) pr[i][j] = (uint64_t)x.p[i] * y.p[j];
+    // class c collects products with i + j = c (mod 4)
+    uint32_t zl[4], zh[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        uint64_t a = pr[0][c], b = pr[1][(c + 3) & 3], d = pr[2][(c + 2) & 3], e = pr[3][(c + 1) & 3];
+        zl[c] = x3(x3((uint32_t)a, (uint32_t)b, (uint32_t)d), (uint32_t)e, 0u);
+        zh[c] = x3(x3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(d >> 32)), (uint32_t)(e >> 32), 0u);
+    }
+    const uint32_t m0 = 0x11111111u, m1 = m0 << 1, m2 = m0 << 2;
+    // disjoint masks: bit-field inserts pick each class's positions
+    uint32_t lo = (zl[0] & m0) | (~m0 & ((zl[1] & m1) | (~m1 & ((zl[2] & m2) | (~m2 & zl[3])))));
+    uint32_t hi = (zh[0] & m0) | (~m0 & ((zh[1] & m1) | (~m1 & ((zh[2] & m2) | (~m2 & zh[3])))));
+    return u64p{lo, hi};
 }
 
-// Register-only multiply (two-level Karatsuba over clmul32_holes).
-BFD_D gf gf_mul_reg(gf a, gf b) {
-    uint64_t l0, l1, h0, h1, m0, m1;
-    clmul64_holes(a.lo, b.lo, l0, l1);
-    clmul64_holes(a.hi, b.hi, h0, h1);
-    clmul64_holes(a.lo ^ a.hi, b.lo ^ b.hi, m0, m1);
-    m0 ^= l0 ^ h0;
-    m1 ^= l1 ^ h1;
-    return gf_mont_reduce(l0, l1 ^ m0, h0 ^ m1, h1);
+// b's masked pieces for the nine Karatsuba products, reusable across
+// multiplies by the same b
+struct gf_pre {
+    pieces q[9];
+};
+BFD_D gf_pre gf_prepare(gf b) {
+    pieces b0 = split4((uint32_t)b.lo), b1 = split4((uint32_t)(b.lo >> 32)), b2 = split4((uint32_t)b.hi),
+           b3 = split4((uint32_t)(b.hi >> 32));
+    pieces b01 = pxor(b0, b1), b23 = pxor(b2, b3), b02 = pxor(b0, b2), b13 = pxor(b1, b3), bx = pxor(b02, b13);
+    return gf_pre{{b0, b1, b01, b2, b3, b23, b02, b13, bx}};
 }
+
+// 64x64 Karatsuba from three 32x32 products: returns 4 dwords
+BFD_D void kara64(const pieces &a0, const pieces &a1, const pieces &a01, const pieces &b0, const pieces &b1,
+                  const pieces &b01, uint32_t r[4]) {
+    u64p l = clmul32p(a0, b0), h = clmul32p(a1, b1), m = clmul32p(a01, b01);
+    uint32_t ml = x3(m.lo, l.lo, h.lo), mh = x3(m.hi, l.hi, h.hi);
+    r[0] = l.lo;
+    r[1] = l.hi ^ ml;
+    r[2] = h.lo ^ mh;
+    r[3] = h.hi;
+}
+
+BFD_D gf gf_mul_pre(gf a, const gf_pre &pb) {
+    pieces a0 = split4((uint32_t)a.lo), a1 = split4((uint32_t)(a.lo >> 32)), a2 = split4((uint32_t)a.hi),
+           a3 = split4((uint32_t)(a.hi >> 32));
+    pieces a01 = pxor(a0, a1), a23 = pxor(a2, a3), a02 = pxor(a0, a2), a13 = pxor(a1, a3), ax = pxor(a02, a13);
+    uint32_t L[4], H[4], Mm[4];
+    kara64(a0, a1, a01, pb.q[0], pb.q[1], pb.q[2], L);
+    kara64(a2, a3, a23, pb.q[3], pb.q[4], pb.q[5], H);
+    kara64(a02, a13, ax, pb.q[6], pb.q[7], pb.q[8], Mm);
+    // 256-bit product w0..w7
+    uint32_t w[8];
+    w[0] = L[0];
+    w[1] = L[1];
+    w[2] = x3(L[2], Mm[0], x3(L[0], H[0], 0u));
+    w[3] = x3(L[3], Mm[1], x3(L[1], H[1], 0u));
+    w[4] = x3(H[0], Mm[2], x3(L[2], H[2], 0u));
+    w[5] = x3(H[1], Mm[3], x3(L[3], H[3], 0u));
+    w[6] = H[2];
+    w[7] = H[3];
+    // Montgomery fold (x86_64_avx2.rs:41-68) on 64-bit words v_k = w[2k] | w[2k+1] << 32
+    // v2 ^= v0 ^ v0>>1 ^ v0>>2 ^ v0>>7 ; v1 ^= v0<<63 ^ v0<<62 ^ v0<<57
+    uint32_t s1l = __builtin_amdgcn_alignbit(w[1], w[0], 1), s2l = __builtin_amdgcn_alignbit(w[1], w[0], 2),
+             s7l = __builtin_amdgcn_alignbit(w[1], w[0], 7);
+    w[4] = x3(x3(w[4], w[0], s1l), s2l, s7l);
+    w[5] = x3(x3(w[5], w[1], w[1] >> 1), w[1] >> 2, w[1] >> 7);
+    w[3] = x3(x3(w[3], w[0] << 31, w[0] << 30), w[0] << 25, 0u);
+    // v3 ^= v1 ^ v1>>1 ^ v1>>2 ^ v1>>7 ; v2 ^= v1<<63 ^ v1<<62 ^ v1<<57
+    uint32_t t1l = __builtin_amdgcn_alignbit(w[3], w[2], 1), t2l = __builtin_amdgcn_alignbit(w[3], w[2], 2),
+             t7l = __builtin_amdgcn_alignbit(w[3], w[2], 7);
+    w[6] = x3(x3(w[6], w[2], t1l), t2l, t7l);
+    w[7] = x3(x3(w[7], w[3], w[3] >> 1), w[3] >> 2, w[3] >> 7);
+    w[5] = x3(x3(w[5], w[2] << 31, w[2] << 30), w[2] << 25, 0u);
+    return gf{(uint64_t)w[4] | ((uint64_t)w[5] << 32), (uint64_t)w[6] | ((uint64_t)w[7] << 32)};
+}
+
+BFD_D gf gf_mul_reg(gf a, gf b) { return gf_mul_pre(a, gf_prepare(b)); }
 
 BFD_D gf gf_ld(const gf *p) {
     const uint4 v = *reinterpret_cast<const uint4 *>(p);

@@ -320,3 +320,171 @@ inline void put_bytes(std::vector<uint8_t> &v, const void *p, size_t n) {
 
 }  // namespace rt
 }  // namespace bfd
+
+namespace bfd {
+namespace rt {
+
+// ---------------------------------------------------------------------------
+// Host side of one width-1 Basefold opening: the Eval sumcheck
+// (sumcheck/src/function/{eval,eval_impr}.rs) interleaved with the folding
+// rounds (poly-commit/src/basefold.rs:335-369), the query sampling
+// (:200-203) and the bincode proof (:49-66, :205-231).  The device side only
+// supplies per-round pi-tree roots and inner products; plain and batched
+// openings share this so their transcripts are the same code.
+// ---------------------------------------------------------------------------
+struct OpenRounds {
+    int n = 0, b = 0, d = 0, L = 0;
+    size_t M = 0, q = 0;
+    std::vector<gf> r, rinv, cf;
+    gf claim{}, subclaim{}, inner_sub{}, s_delta{}, final_poly{};
+    std::vector<std::array<gf, 2>> comp;
+    std::vector<std::array<uint8_t, 32>> pi_roots;
+    std::vector<uint64_t> idx;
+
+    void init(int n_, int log2_c, size_t q_, const uint8_t *point_b, const uint8_t *claim_b) {
+        n = d = n_;
+        b = log2_c;
+        L = b + d;
+        M = (size_t)1 << L;
+        q = q_;
+        r.assign(n, gf_zero());
+        rinv.assign(n, gf_zero());
+        for (int i = 0; i < n; i++) {
+            r[i] = load_elem(point_b + 16 * i);
+            if (gf_eq(r[i], gf_zero()) || gf_eq(r[i], gf_one()))
+                fail(BFD_ERR_ARG, "point coordinates 0 and 1 are not invertible in the Eval prover");
+            rinv[i] = hinv(r[i]);                                  // EvalImpr::new, eval_impr.rs:20-27
+        }
+        claim = load_elem(claim_b);
+        // correcting_factor(i) = prod_{j=i}^{n-2} (1 - r_j)^-1  (eval_impr.rs:81-88)
+        const int hn = n > 0 ? n - 1 : 0;
+        cf.assign(n + 1, gf_one());
+        for (int j = hn - 1; j >= 0; j--) cf[j] = hmul(cf[j + 1], hinv(gf_add(gf_one(), r[j])));
+        subclaim = claim;
+        inner_sub = gf_zero();
+        s_delta = gf_one();
+        comp.assign(d, {gf_zero(), gf_zero()});
+        pi_roots.assign(d, {});
+    }
+    int half_eq_vars() const { return n > 0 ? n - 1 : 0; }
+
+    // Round k (level i = d-1-k): observe the pi-tree root, turn the inner
+    // product S = sum half_eq[b] * poly_lo[b] into the round polynomial,
+    // observe it, sample the challenge and advance both subclaims.
+    gf round(int k, const uint8_t root[32], gf S, Challenger &ch) {
+        const int i = d - 1 - k;
+        memcpy(pi_roots[k].data(), root, 32);
+        ch.observe(root, 32);                                           // basefold.rs:356
+        const gf sub = (i == n - 1) ? subclaim : inner_sub;             // eval.rs:90-94
+        const gf coeff0 = hmul(cf[i], S);
+        const gf eval1 = hmul(gf_add(sub, hmul(gf_add(gf_one(), r[i]), coeff0)), rinv[i]);  // eval_impr.rs:41-43
+        const gf coeff1 = gf_add(eval1, coeff0);
+        // delta (eval_impr.rs:45-58): scalar = eq(r[i+1..], x[i+1..]); in
+        // characteristic 2 the two factors are s (1 + r_i) and s
+        const gf d0 = hmul(s_delta, gf_add(gf_one(), r[i]));
+        const gf d1 = s_delta;
+        const gf rp0 = hmul(coeff0, d0), rp1 = gf_add(hmul(coeff0, d1), hmul(coeff1, d0)), rp2 = hmul(coeff1, d1);
+        comp[k] = {rp0, rp2};                                           // eval.rs:64-66
+        ch.observe_elem(rp0);                                           // sumcheck/src/lib.rs:94
+        ch.observe_elem(rp2);
+        const gf x = ch.sample();                                       // lib.rs:96
+        inner_sub = gf_add(coeff0, hmul(coeff1, x));                    // eval.rs:110
+        subclaim = gf_add(rp0, hmul(x, gf_add(rp1, hmul(x, rp2))));     // lib.rs:100 (horner)
+        s_delta = hmul(s_delta, gf_add(gf_one(), gf_add(r[i], x)));
+        return x;
+    }
+
+    // basefold.rs:194-203: observe the final polynomial, sample the queries
+    void finish(gf fin, Challenger &ch) {
+        final_poly = fin;
+        ch.observe_elem(fin);
+        idx.resize(q);
+        for (size_t t = 0; t < q; t++) idx[t] = ch.sample_bits(L);
+    }
+};
+
+// Device addresses of one polynomial's opening data.
+struct OpenLayout {
+    const gf *cw = nullptr;          // codeword (= pi of round 0)
+    const uint64_t *tree = nullptr;  // main tree, 2M-1 digests, leaf layer first
+    const gf *pis = nullptr;         // pi of rounds 1..d, at pi_off[k]
+    const uint64_t *pitrees = nullptr;  // pi trees of rounds 0..d-1, at tree_off[k]
+    std::vector<size_t> pi_off, tree_off;
+
+    void plan(size_t M, int d) {
+        pi_off.assign(d + 1, 0);
+        tree_off.assign(d + 1, 0);
+        for (int k = 2; k <= d; k++) pi_off[k] = pi_off[k - 1] + (M >> (k - 1));
+        size_t t = 0;
+        for (int k = 0; k < d; k++) {
+            tree_off[k] = t;
+            t += (M >> k) - 1;
+        }
+        tree_off[d] = t;
+    }
+    size_t pis_elems(size_t M, size_t c) const { return M - c + 1; }
+    size_t pitree_digests() const { return tree_off.empty() ? 1 : tree_off.back() + 1; }
+    const gf *pi(int k) const { return k == 0 ? cw : pis + pi_off[k]; }
+    const uint64_t *pitree(int k) const { return pitrees + 4 * tree_off[k]; }
+};
+
+inline size_t layer_off(size_t leaves, int t) { return 2 * leaves - 2 * (leaves >> t); }
+
+// Absolute addresses of every opened value, in proof order (basefold.rs:205-222):
+// per query the codeword row and its L siblings, then per round the pi
+// sibling value and its b+i siblings.
+inline void plan_gathers(const OpenRounds &o, const OpenLayout &lay, std::vector<uint64_t> &elems,
+                         std::vector<uint64_t> &digs) {
+    for (size_t t = 0; t < o.q; t++) {
+        const uint64_t id = o.idx[t];
+        elems.push_back((uint64_t)(lay.cw + id));
+        for (int u = 0; u < o.L; u++)
+            digs.push_back((uint64_t)(lay.tree + 4 * (layer_off(o.M, u) + ((id >> u) ^ 1))));
+        for (int k = 0; k < o.d; k++) {
+            const int i = o.d - 1 - k;
+            const size_t ni = (size_t)1 << (o.b + i);
+            const size_t sib = ((id / ni) & 1) ^ 1, row = id % ni;
+            elems.push_back((uint64_t)(lay.pi(k) + row + sib * ni));
+            for (int u = 0; u < o.b + i; u++)
+                digs.push_back((uint64_t)(lay.pitree(k) + 4 * (layer_off(ni, u) + ((row >> u) ^ 1))));
+        }
+    }
+}
+
+// bincode BasefoldProof from the gathered records (same order as plan_gathers)
+inline void assemble_proof(const OpenRounds &o, const uint8_t *elems, const uint8_t *digs,
+                           std::vector<uint8_t> &proof) {
+    proof.clear();
+    proof.reserve(64 + (size_t)o.d * 72 + o.q * (40 + 32 * (size_t)o.L + (size_t)o.d * (24 + 32 * (size_t)o.L)));
+    put_u64(proof, (uint64_t)o.d);
+    for (int k = 0; k < o.d; k++) {
+        put_u64(proof, 2);
+        put_elem(proof, o.comp[k][0]);
+        put_elem(proof, o.comp[k][1]);
+    }
+    put_u64(proof, 1);
+    put_elem(proof, o.final_poly);
+    put_u64(proof, (uint64_t)o.d);
+    for (int k = 0; k < o.d; k++) put_bytes(proof, o.pi_roots[k].data(), 32);
+    put_u64(proof, (uint64_t)o.q);
+    for (size_t t = 0; t < o.q; t++) {
+        put_u64(proof, 1);
+        put_bytes(proof, elems, 16);
+        elems += 16;
+        put_u64(proof, (uint64_t)o.L);
+        put_bytes(proof, digs, 32 * (size_t)o.L);
+        digs += 32 * (size_t)o.L;
+        put_u64(proof, (uint64_t)o.d);
+        for (int k = 0; k < o.d; k++) {
+            const int i = o.d - 1 - k;
+            put_bytes(proof, elems, 16);
+            elems += 16;
+            put_u64(proof, (uint64_t)(o.b + i));
+            put_bytes(proof, digs, 32 * (size_t)(o.b + i));
+            digs += 32 * (size_t)(o.b + i);
+        }
+    }
+}
+
+}  // namespace rt
+}  // namespace bfd

@@ -3,94 +3,112 @@
 // transcript: p3/src/merkle_tree.rs:8-21 (leaf = Keccak256(row bytes), node =
 // Keccak256(left || right)) over tiny-keccak's Keccak::v256.
 //
-// Every 64-bit lane lives in two VGPRs; rotations become v_alignbit_b32
-// pairs, chi becomes v_bfi_b32 (a ^ (~b & c) == bfi(b, a, a ^ c)).  The 25
-// lanes are plain scalars so the compiler keeps the whole state in registers
-// and renames away the pi permutation.
+// gfx950 specifics: every 64-bit lane is a pair of 32-bit halves; rotations
+// are two v_alignbit_b32; the five-way column parity of theta is two
+// three-input XORs and chi (a ^ (~b & c)) is one three-input LUT op, both via
+// v_bitop3_b32.  A round is ~190 VALU instructions; the 24 rounds are fully
+// unrolled so round constants become immediates and the zero padding lanes
+// of one-block messages fold away in the first round.
 #pragma once
 #include <stdint.h>
 #include <hip/hip_runtime.h>
 
 namespace kk {
 
-__device__ __constant__ static const uint64_t RC[24] = {
-    0x0000000000000001ull, 0x0000000000008082ull, 0x800000000000808Aull, 0x8000000080008000ull,
-    0x000000000000808Bull, 0x0000000080000001ull, 0x8000000080008081ull, 0x8000000000008009ull,
-    0x000000000000008Aull, 0x0000000000000088ull, 0x0000000080008009ull, 0x000000008000000Aull,
-    0x000000008000808Bull, 0x800000000000008Bull, 0x8000000000008089ull, 0x8000000000008003ull,
-    0x8000000000008002ull, 0x8000000000000080ull, 0x000000000000800Aull, 0x800000008000000Aull,
-    0x8000000080008081ull, 0x8000000000008080ull, 0x0000000080000001ull, 0x8000000080008008ull};
+__device__ __constant__ static const uint32_t RC_LO[24] = {
+    0x00000001u, 0x00008082u, 0x0000808Au, 0x80008000u, 0x0000808Bu, 0x80000001u,
+    0x80008081u, 0x00008009u, 0x0000008Au, 0x00000088u, 0x80008009u, 0x8000000Au,
+    0x8000808Bu, 0x0000008Bu, 0x00008089u, 0x00008003u, 0x00008002u, 0x00000080u,
+    0x0000800Au, 0x8000000Au, 0x80008081u, 0x00008080u, 0x80000001u, 0x80008008u};
+__device__ __constant__ static const uint32_t RC_HI[24] = {
+    0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x00000000u,
+    0x80000000u, 0x80000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u,
+    0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u,
+    0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x80000000u};
 
-__device__ __forceinline__ uint64_t rotl(uint64_t v, int r) {
-    return (v << r) | (v >> (64 - r));
+// v_bitop3_b32 truth tables: bit (a<<2 | b<<1 | c) of the immediate is
+// f(a, b, c) for source operands (a, b, c); 0xF0/0xCC/0xAA select a/b/c.
+constexpr uint32_t LUT_XOR3 = (0xF0 ^ 0xCC ^ 0xAA) & 0xFF;          // a ^ b ^ c
+constexpr uint32_t LUT_CHI = (0xF0 ^ (~0xCC & 0xAA)) & 0xFF;        // a ^ (~b & c)
+
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, LUT_XOR3);
 }
-// State lanes s[x + 5y].  One full permutation, 24 rounds.
-__device__ __forceinline__ void keccak_f(uint64_t s[25]) {
-#pragma unroll 1
+__device__ __forceinline__ uint32_t chi(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, LUT_CHI);
+}
+
+struct lane {
+    uint32_t lo, hi;
+};
+
+// rotate left by a compile-time constant
+template <int R>
+__device__ __forceinline__ lane rotl(lane v) {
+    if constexpr (R == 0) {
+        return v;
+    } else if constexpr (R == 32) {
+        return lane{v.hi, v.lo};
+    } else if constexpr (R < 32) {
+        return lane{__builtin_amdgcn_alignbit(v.lo, v.hi, 32 - R), __builtin_amdgcn_alignbit(v.hi, v.lo, 32 - R)};
+    } else {
+        return lane{__builtin_amdgcn_alignbit(v.hi, v.lo, 64 - R), __builtin_amdgcn_alignbit(v.lo, v.hi, 64 - R)};
+    }
+}
+
+__device__ __forceinline__ lane lx(lane a, lane b) { return lane{a.lo ^ b.lo, a.hi ^ b.hi}; }
+
+// one permutation on lanes s[x + 5y]
+__device__ __forceinline__ void keccak_f(lane s[25]) {
+#pragma unroll
     for (int round = 0; round < 24; round++) {
-        uint64_t c0 = s[0] ^ s[5] ^ s[10] ^ s[15] ^ s[20];
-        uint64_t c1 = s[1] ^ s[6] ^ s[11] ^ s[16] ^ s[21];
-        uint64_t c2 = s[2] ^ s[7] ^ s[12] ^ s[17] ^ s[22];
-        uint64_t c3 = s[3] ^ s[8] ^ s[13] ^ s[18] ^ s[23];
-        uint64_t c4 = s[4] ^ s[9] ^ s[14] ^ s[19] ^ s[24];
-        uint64_t d0 = c4 ^ rotl(c1, 1);
-        uint64_t d1 = c0 ^ rotl(c2, 1);
-        uint64_t d2 = c1 ^ rotl(c3, 1);
-        uint64_t d3 = c2 ^ rotl(c4, 1);
-        uint64_t d4 = c3 ^ rotl(c0, 1);
+        lane c[5];
+#pragma unroll
+        for (int x = 0; x < 5; x++) {
+            c[x].lo = xor3(xor3(s[x].lo, s[x + 5].lo, s[x + 10].lo), s[x + 15].lo, s[x + 20].lo);
+            c[x].hi = xor3(xor3(s[x].hi, s[x + 5].hi, s[x + 10].hi), s[x + 15].hi, s[x + 20].hi);
+        }
+        lane d[5];
+#pragma unroll
+        for (int x = 0; x < 5; x++) d[x] = lx(c[(x + 4) % 5], rotl<1>(c[(x + 1) % 5]));
         // theta + rho + pi: B[y + 5*((2x+3y)%5)] = rotl(A[x+5y] ^ D[x], r[x][y])
-        uint64_t b0 = s[0] ^ d0;
-        uint64_t b10 = rotl(s[1] ^ d1, 1);
-        uint64_t b20 = rotl(s[2] ^ d2, 62);
-        uint64_t b5 = rotl(s[3] ^ d3, 28);
-        uint64_t b15 = rotl(s[4] ^ d4, 27);
-        uint64_t b16 = rotl(s[5] ^ d0, 36);
-        uint64_t b1 = rotl(s[6] ^ d1, 44);
-        uint64_t b11 = rotl(s[7] ^ d2, 6);
-        uint64_t b21 = rotl(s[8] ^ d3, 55);
-        uint64_t b6 = rotl(s[9] ^ d4, 20);
-        uint64_t b7 = rotl(s[10] ^ d0, 3);
-        uint64_t b17 = rotl(s[11] ^ d1, 10);
-        uint64_t b2 = rotl(s[12] ^ d2, 43);
-        uint64_t b12 = rotl(s[13] ^ d3, 25);
-        uint64_t b22 = rotl(s[14] ^ d4, 39);
-        uint64_t b23 = rotl(s[15] ^ d0, 41);
-        uint64_t b8 = rotl(s[16] ^ d1, 45);
-        uint64_t b18 = rotl(s[17] ^ d2, 15);
-        uint64_t b3 = rotl(s[18] ^ d3, 21);
-        uint64_t b13 = rotl(s[19] ^ d4, 8);
-        uint64_t b14 = rotl(s[20] ^ d0, 18);
-        uint64_t b24 = rotl(s[21] ^ d1, 2);
-        uint64_t b9 = rotl(s[22] ^ d2, 61);
-        uint64_t b19 = rotl(s[23] ^ d3, 56);
-        uint64_t b4 = rotl(s[24] ^ d4, 14);
-        // chi
-        s[0] = b0 ^ (~b1 & b2);
-        s[1] = b1 ^ (~b2 & b3);
-        s[2] = b2 ^ (~b3 & b4);
-        s[3] = b3 ^ (~b4 & b0);
-        s[4] = b4 ^ (~b0 & b1);
-        s[5] = b5 ^ (~b6 & b7);
-        s[6] = b6 ^ (~b7 & b8);
-        s[7] = b7 ^ (~b8 & b9);
-        s[8] = b8 ^ (~b9 & b5);
-        s[9] = b9 ^ (~b5 & b6);
-        s[10] = b10 ^ (~b11 & b12);
-        s[11] = b11 ^ (~b12 & b13);
-        s[12] = b12 ^ (~b13 & b14);
-        s[13] = b13 ^ (~b14 & b10);
-        s[14] = b14 ^ (~b10 & b11);
-        s[15] = b15 ^ (~b16 & b17);
-        s[16] = b16 ^ (~b17 & b18);
-        s[17] = b17 ^ (~b18 & b19);
-        s[18] = b18 ^ (~b19 & b15);
-        s[19] = b19 ^ (~b15 & b16);
-        s[20] = b20 ^ (~b21 & b22);
-        s[21] = b21 ^ (~b22 & b23);
-        s[22] = b22 ^ (~b23 & b24);
-        s[23] = b23 ^ (~b24 & b20);
-        s[24] = b24 ^ (~b20 & b21);
-        s[0] ^= RC[round];
+        lane b[25];
+        b[0] = lx(s[0], d[0]);
+        b[10] = rotl<1>(lx(s[1], d[1]));
+        b[20] = rotl<62>(lx(s[2], d[2]));
+        b[5] = rotl<28>(lx(s[3], d[3]));
+        b[15] = rotl<27>(lx(s[4], d[4]));
+        b[16] = rotl<36>(lx(s[5], d[0]));
+        b[1] = rotl<44>(lx(s[6], d[1]));
+        b[11] = rotl<6>(lx(s[7], d[2]));
+        b[21] = rotl<55>(lx(s[8], d[3]));
+        b[6] = rotl<20>(lx(s[9], d[4]));
+        b[7] = rotl<3>(lx(s[10], d[0]));
+        b[17] = rotl<10>(lx(s[11], d[1]));
+        b[2] = rotl<43>(lx(s[12], d[2]));
+        b[12] = rotl<25>(lx(s[13], d[3]));
+        b[22] = rotl<39>(lx(s[14], d[4]));
+        b[23] = rotl<41>(lx(s[15], d[0]));
+        b[8] = rotl<45>(lx(s[16], d[1]));
+        b[18] = rotl<15>(lx(s[17], d[2]));
+        b[3] = rotl<21>(lx(s[18], d[3]));
+        b[13] = rotl<8>(lx(s[19], d[4]));
+        b[14] = rotl<18>(lx(s[20], d[0]));
+        b[24] = rotl<2>(lx(s[21], d[1]));
+        b[9] = rotl<61>(lx(s[22], d[2]));
+        b[19] = rotl<56>(lx(s[23], d[3]));
+        b[4] = rotl<14>(lx(s[24], d[4]));
+        // chi: A[x+5y] = B[x+5y] ^ (~B[x+1+5y] & B[x+2+5y])
+#pragma unroll
+        for (int y = 0; y < 25; y += 5)
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                s[y + x].lo = chi(b[y + x].lo, b[y + (x + 1) % 5].lo, b[y + (x + 2) % 5].lo);
+                s[y + x].hi = chi(b[y + x].hi, b[y + (x + 1) % 5].hi, b[y + (x + 2) % 5].hi);
+            }
+        // iota
+        s[0].lo ^= RC_LO[round];
+        s[0].hi ^= RC_HI[round];
     }
 }
 
@@ -98,13 +116,27 @@ __device__ __forceinline__ void keccak_f(uint64_t s[25]) {
 // Writes the 4-word digest.
 template <int NW>
 __device__ __forceinline__ void keccak256_words(const uint64_t *m, uint64_t out[4]) {
-    uint64_t s[25];
+    lane s[25];
 #pragma unroll
-    for (int i = 0; i < 25; i++) s[i] = i < NW ? m[i] : 0ull;
-    s[NW] ^= 0x01ull;                  // Keccak padding start (domain bit)
-    s[16] ^= 0x8000000000000000ull;    // final bit of the 136-byte rate
+    for (int i = 0; i < 25; i++) {
+        uint64_t v = i < NW ? m[i] : 0ull;
+        if (i == NW) v ^= 0x01ull;                    // Keccak padding start
+        if (i == 16) v ^= 0x8000000000000000ull;      // last bit of the 136-byte rate
+        s[i] = lane{(uint32_t)v, (uint32_t)(v >> 32)};
+    }
     keccak_f(s);
-    out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = (uint64_t)s[i].lo | ((uint64_t)s[i].hi << 32);
+}
+
+// permutation on 64-bit words (used by the micro-benchmark)
+__device__ __forceinline__ void keccak_f(uint64_t w[25]) {
+    lane s[25];
+#pragma unroll
+    for (int i = 0; i < 25; i++) s[i] = lane{(uint32_t)w[i], (uint32_t)(w[i] >> 32)};
+    keccak_f(s);
+#pragma unroll
+    for (int i = 0; i < 25; i++) w[i] = (uint64_t)s[i].lo | ((uint64_t)s[i].hi << 32);
 }
 
 }  // namespace kk

@@ -55,46 +55,37 @@ __global__ void k_xor_span(gf *ft, const gf *bt, int bits, size_t count) {
 // ---------------------------------------------------------------------------
 // Reed-Solomon encode: NTT layers in data order
 // ---------------------------------------------------------------------------
-// layer s = 0 (stride 1) reading the input and replicating it into all slots
-__global__ __launch_bounds__(CB) void k_ntt_first(const gf *__restrict__ data, gf *__restrict__ cw,
-                                                  const gf *__restrict__ ft0, int c, size_t half) {
-    __shared__ uint32_t lds[80 * CB];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < half;
+// One thread per (butterfly, slot): q = (hi * 2^s + p) * c + u, so lanes
+// touch consecutive 16-byte elements and consecutive twiddles; the twiddle
+// index (p << b) | u is simply q mod 2^(s+b).
+// Layer s = 0 (stride 1) reads the input and replicates it into all slots.
+__global__ __launch_bounds__(256) void k_ntt_first(const gf *__restrict__ data, gf *__restrict__ cw,
+                                                   const gf *__restrict__ ft0, int b, size_t work) {
+    const size_t cm = ((size_t)1 << b) - 1;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < work;
          q += (size_t)gridDim.x * blockDim.x) {
-        gf lo0 = gf_ld(data + 2 * q), hi0 = gf_ld(data + 2 * q + 1);
-        for (int u = 0; u < c; u++) {
-            gf t = gf_ld(ft0 + u);
-            gf lo = gf_add(lo0, gf_mul(hi0, t, tab, CB));
-            gf hi = gf_add(hi0, lo);
-            gf_st(cw + (2 * q) * c + u, lo);
-            gf_st(cw + (2 * q + 1) * c + u, hi);
-        }
+        const size_t u = q & cm, pr = q >> b;
+        gf lo = gf_ld(data + 2 * pr), hi = gf_ld(data + 2 * pr + 1);
+        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft0 + u)));
+        hi = gf_add(hi, lo);
+        gf_st(cw + ((2 * pr) << b) + u, lo);
+        gf_st(cw + ((2 * pr + 1) << b) + u, hi);
     }
 }
 
-// general layer s >= 1, in place on the codeword buffer
-__global__ __launch_bounds__(CB) void k_ntt_layer(gf *__restrict__ cw, const gf *__restrict__ fts,
-                                                  int s, int b, size_t half) {
-    __shared__ uint32_t lds[80 * CB];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
-    const int c = 1 << b;
-    const size_t pmask = ((size_t)1 << s) - 1;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < half;
+// general layer s >= 1, in place on the slot-interleaved codeword
+__global__ __launch_bounds__(256) void k_ntt_layer(gf *__restrict__ cw, const gf *__restrict__ fts, int s, int b,
+                                                   size_t work) {
+    const size_t tmask = ((size_t)1 << (s + b)) - 1;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < work;
          q += (size_t)gridDim.x * blockDim.x) {
-        size_t p = q & pmask;
-        size_t x = ((q >> s) << (s + 1)) | p;
-        size_t y = x + ((size_t)1 << s);
-        for (int u = 0; u < c; u++) {
-            gf t = gf_ld(fts + ((p << b) | u));
-            gf lo = gf_ld(cw + x * c + u), hi = gf_ld(cw + y * c + u);
-            lo = gf_add(lo, gf_mul(hi, t, tab, CB));
-            hi = gf_add(hi, lo);
-            gf_st(cw + x * c + u, lo);
-            gf_st(cw + y * c + u, hi);
-        }
+        const size_t lo_i = ((q >> (s + b)) << (s + b + 1)) | (q & tmask);
+        const size_t hi_i = lo_i + ((size_t)1 << (s + b));
+        gf lo = gf_ld(cw + lo_i), hi = gf_ld(cw + hi_i);
+        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(fts + (q & tmask))));
+        hi = gf_add(hi, lo);
+        gf_st(cw + lo_i, lo);
+        gf_st(cw + hi_i, hi);
     }
 }
 
@@ -164,14 +155,11 @@ __device__ __forceinline__ gf wave_xor(gf v) {
 // partial[block] = XOR_k a[k] * b[k]
 __global__ __launch_bounds__(CB) void k_dot(const gf *__restrict__ a, const gf *__restrict__ b, size_t n,
                                             gf *__restrict__ partial) {
-    __shared__ uint32_t lds[80 * CB];
     __shared__ gf red[CB / 64];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
     gf acc = gf_zero();
     for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
          k += (size_t)gridDim.x * blockDim.x)
-        acc = gf_add(acc, gf_mul(gf_ld(a + k), gf_ld(b + k), tab, CB));
+        acc = gf_add(acc, gf_mul_reg(gf_ld(a + k), gf_ld(b + k)));
     acc = wave_xor(acc);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -185,28 +173,23 @@ __global__ __launch_bounds__(CB) void k_dot(const gf *__restrict__ a, const gf *
 // out[j] = interpolate(ft[j], w[j], w[j+mid], r)  (binary_reed_solomon.rs:103-109)
 __global__ __launch_bounds__(CB) void k_fold(const gf *__restrict__ w, gf *__restrict__ out,
                                              const gf *__restrict__ ft, size_t mid, gf r) {
-    __shared__ uint32_t lds[80 * CB];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
+    const gf_pre pr = gf_prepare(r);
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < mid;
          j += (size_t)gridDim.x * blockDim.x) {
         gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
         hi = gf_add(hi, lo);
-        lo = gf_add(lo, gf_mul(hi, gf_ld(ft + j), tab, CB));
-        gf_st(out + j, gf_add(lo, gf_mul(gf_add(hi, lo), r, tab, CB)));
+        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
+        gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), pr)));
     }
 }
 
 // out[j] = in[j] + x * (in[j+half] - in[j])  (multilinear.rs:183-198)
 __global__ __launch_bounds__(CB) void k_fix(const gf *__restrict__ in, gf *__restrict__ out, size_t half, gf x) {
-    __shared__ uint32_t lds[80 * CB];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
-    gf_comb_table(tab, CB, x);
+    const gf_pre px = gf_prepare(x);
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
          j += (size_t)gridDim.x * blockDim.x) {
         gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
-        gf_st(out + j, gf_add(lo, gf_comb_mul(gf_add(hi, lo), tab, CB)));
+        gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
     }
 }
 
@@ -214,13 +197,10 @@ __global__ __launch_bounds__(CB) void k_fix(const gf *__restrict__ in, gf *__res
 // out[lo + (hi << m)] = A[lo] * B[hi]
 __global__ __launch_bounds__(CB) void k_outer(const gf *__restrict__ A, int m, const gf *__restrict__ B,
                                               gf *__restrict__ out, size_t n) {
-    __shared__ uint32_t lds[80 * CB];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
     const size_t mask = ((size_t)1 << m) - 1;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
-        gf_st(out + i, gf_mul(gf_ld(A + (i & mask)), gf_ld(B + (i >> m)), tab, CB));
+        gf_st(out + i, gf_mul_reg(gf_ld(A + (i & mask)), gf_ld(B + (i >> m))));
 }
 
 // ---------------------------------------------------------------------------
@@ -250,28 +230,24 @@ __device__ __forceinline__ void block_xor3(gf a0, gf a1, gf a2, gf *partial) {
     }
 }
 
-__device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf &a0, gf &a1, gf &a2,
-                                           uint32_t *tab) {
-    gf p0 = gf_mul(f0, g0, tab, CB);
-    gf p1 = gf_mul(f1, g1, tab, CB);
-    gf p2 = gf_mul(gf_add(f0, f1), gf_add(g0, g1), tab, CB);
-    gf_comb_table(tab, CB, t);
-    a0 = gf_add(a0, gf_comb_mul(p0, tab, CB));
-    a1 = gf_add(a1, gf_comb_mul(p1, tab, CB));
-    a2 = gf_add(a2, gf_comb_mul(p2, tab, CB));
+__device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf &a0, gf &a1, gf &a2) {
+    gf p0 = gf_mul_reg(f0, g0);
+    gf p1 = gf_mul_reg(f1, g1);
+    gf p2 = gf_mul_reg(gf_add(f0, f1), gf_add(g0, g1));
+    const gf_pre pt = gf_prepare(t);
+    a0 = gf_add(a0, gf_mul_pre(p0, pt));
+    a1 = gf_add(a1, gf_mul_pre(p1, pt));
+    a2 = gf_add(a2, gf_mul_pre(p2, pt));
 }
 
 // sums of the first round over tables of 2*half entries
 __global__ __launch_bounds__(CB) void k_prod_round(const gf *__restrict__ f, const gf *__restrict__ g,
                                                    const gf *__restrict__ T, size_t half, gf *__restrict__ partial) {
-    __shared__ uint32_t lds[80 * CB];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
     gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
          j += (size_t)gridDim.x * blockDim.x)
         prod_terms(gf_ld(f + j), gf_ld(f + j + half), gf_ld(g + j), gf_ld(g + j + half), gf_ld(T + j), a0, a1,
-                   a2, tab);
+                   a2);
     block_xor3(a0, a1, a2, partial);
 }
 
@@ -280,27 +256,182 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *__restrict__ f, con
 __global__ __launch_bounds__(CB) void k_prod_fold_round(gf *__restrict__ f, gf *__restrict__ g,
                                                         const gf *__restrict__ T, size_t q, gf x,
                                                         gf *__restrict__ partial) {
-    __shared__ uint32_t lds[80 * CB];
-    uint32_t *tab = comb_slot(lds);
-    gf_comb_init(tab, CB);
     gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
     const size_t h = 2 * q;
+    const gf_pre px = gf_prepare(x);
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < q;
          j += (size_t)gridDim.x * blockDim.x) {
         gf fa = gf_ld(f + j), fb = gf_ld(f + j + q), fc = gf_ld(f + j + h), fd = gf_ld(f + j + h + q);
         gf ga = gf_ld(g + j), gb = gf_ld(g + j + q), gc = gf_ld(g + j + h), gd = gf_ld(g + j + h + q);
-        gf_comb_table(tab, CB, x);
-        gf f0 = gf_add(fa, gf_comb_mul(gf_add(fc, fa), tab, CB));
-        gf f1 = gf_add(fb, gf_comb_mul(gf_add(fd, fb), tab, CB));
-        gf g0 = gf_add(ga, gf_comb_mul(gf_add(gc, ga), tab, CB));
-        gf g1 = gf_add(gb, gf_comb_mul(gf_add(gd, gb), tab, CB));
+        gf f0 = gf_add(fa, gf_mul_pre(gf_add(fc, fa), px));
+        gf f1 = gf_add(fb, gf_mul_pre(gf_add(fd, fb), px));
+        gf g0 = gf_add(ga, gf_mul_pre(gf_add(gc, ga), px));
+        gf g1 = gf_add(gb, gf_mul_pre(gf_add(gd, gb), px));
         gf_st(f + j, f0);
         gf_st(f + j + q, f1);
         gf_st(g + j, g0);
         gf_st(g + j + q, g1);
-        prod_terms(f0, f1, g0, g1, gf_ld(T + j), a0, a1, a2, tab);
+        prod_terms(f0, f1, g0, g1, gf_ld(T + j), a0, a1, a2);
     }
     block_xor3(a0, a1, a2, partial);
+}
+
+// ---------------------------------------------------------------------------
+// Segmented forms: one launch covers the same stage of many polynomials.
+// Segment g owns blocks [blk0, blk0 + nblk) and grid-strides over its own
+// `count` items; a block finds its segment by binary search on blk0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const Seg &my_seg(const Seg *segs, int nseg) {
+    int lo = 0, hi = nseg - 1;
+    const uint32_t bx = blockIdx.x;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].blk0 <= bx) lo = mid;
+        else hi = mid - 1;
+    }
+    return segs[lo];
+}
+#define SEG_LOOP(g, i)                                                                                  \
+    for (size_t i = (size_t)(blockIdx.x - g.blk0) * blockDim.x + threadIdx.x; i < g.count;               \
+         i += (size_t)g.nblk * blockDim.x)
+
+// in = evaluations, out = codeword, aux = level-0 twiddles, arg = b
+__global__ __launch_bounds__(256) void ks_ntt_first(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const int b = (int)g.arg;
+    const size_t cm = ((size_t)1 << b) - 1;
+    const gf *data = (const gf *)g.in, *ft0 = (const gf *)g.aux;
+    gf *cw = (gf *)g.out;
+    SEG_LOOP(g, q) {
+        const size_t u = q & cm, pr = q >> b;
+        gf lo = gf_ld(data + 2 * pr), hi = gf_ld(data + 2 * pr + 1);
+        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft0 + u)));
+        hi = gf_add(hi, lo);
+        gf_st(cw + ((2 * pr) << b) + u, lo);
+        gf_st(cw + ((2 * pr + 1) << b) + u, hi);
+    }
+}
+
+// out = codeword (in place), aux = level-s twiddles, arg = s + b
+__global__ __launch_bounds__(256) void ks_ntt_layer(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const int sb = (int)g.arg;
+    const size_t tmask = ((size_t)1 << sb) - 1;
+    gf *cw = (gf *)g.out;
+    const gf *fts = (const gf *)g.aux;
+    SEG_LOOP(g, q) {
+        const size_t lo_i = ((q >> sb) << (sb + 1)) | (q & tmask);
+        const size_t hi_i = lo_i + ((size_t)1 << sb);
+        gf lo = gf_ld(cw + lo_i), hi = gf_ld(cw + hi_i);
+        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(fts + (q & tmask))));
+        hi = gf_add(hi, lo);
+        gf_st(cw + lo_i, lo);
+        gf_st(cw + hi_i, hi);
+    }
+}
+
+__global__ __launch_bounds__(KB) void ks_leaves_elem(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const gf *rows = (const gf *)g.in;
+    uint64_t *out = (uint64_t *)g.out;
+    SEG_LOOP(g, i) {
+        gf v = gf_ld(rows + i);
+        uint64_t m[2] = {v.lo, v.hi}, d[4];
+        kk::keccak256_words<2>(m, d);
+        st_digest(out + 4 * i, d);
+    }
+}
+
+// arg = half
+__global__ __launch_bounds__(KB) void ks_leaves_pair(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const gf *pi = (const gf *)g.in;
+    uint64_t *out = (uint64_t *)g.out;
+    const size_t half = g.arg;
+    SEG_LOOP(g, j) {
+        gf a = gf_ld(pi + j), b = gf_ld(pi + j + half);
+        uint64_t m[4] = {a.lo, a.hi, b.lo, b.hi}, d[4];
+        kk::keccak256_words<4>(m, d);
+        st_digest(out + 4 * j, d);
+    }
+}
+
+__global__ __launch_bounds__(KB) void ks_nodes(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const uint64_t *prev = (const uint64_t *)g.in;
+    uint64_t *out = (uint64_t *)g.out;
+    SEG_LOOP(g, i) {
+        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(prev + 8 * i);
+        ulonglong2 a = p[0], b = p[1], c = p[2], e = p[3];
+        uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
+        kk::keccak256_words<8>(m, d);
+        st_digest(out + 4 * i, d);
+    }
+}
+
+// in = halved eq table, aux = poly; partial[blockIdx.x] = block sum
+__global__ __launch_bounds__(CB) void ks_dot(const Seg *segs, int nseg, gf *__restrict__ partial) {
+    const Seg &g = my_seg(segs, nseg);
+    __shared__ gf red[CB / 64];
+    const gf *a = (const gf *)g.in, *b = (const gf *)g.aux;
+    gf acc = gf_zero();
+    SEG_LOOP(g, k) acc = gf_add(acc, gf_mul_reg(gf_ld(a + k), gf_ld(b + k)));
+    acc = wave_xor(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        gf s = gf_zero();
+        for (int w = 0; w < CB / 64; w++) s = gf_add(s, red[w]);
+        gf_st(partial + blockIdx.x, s);
+    }
+}
+
+// in = pi (2*count), out = next pi (count), aux = level twiddles, x = r
+__global__ __launch_bounds__(CB) void ks_fold(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const gf *w = (const gf *)g.in, *ft = (const gf *)g.aux;
+    gf *out = (gf *)g.out;
+    const size_t mid = g.count;
+    const gf_pre pr = gf_prepare(g.x);
+    SEG_LOOP(g, j) {
+        gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
+        hi = gf_add(hi, lo);
+        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
+        gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), pr)));
+    }
+}
+
+// in = table (2*count), out = fixed table (count), x = challenge
+__global__ __launch_bounds__(CB) void ks_fix(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const gf *in = (const gf *)g.in;
+    gf *out = (gf *)g.out;
+    const size_t half = g.count;
+    const gf_pre px = gf_prepare(g.x);
+    SEG_LOOP(g, j) {
+        gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
+        gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+    }
+}
+
+// in = A (2^m), aux = B, out = table, arg = m
+__global__ __launch_bounds__(CB) void ks_outer(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const gf *A = (const gf *)g.in, *B = (const gf *)g.aux;
+    gf *out = (gf *)g.out;
+    const int m = (int)g.arg;
+    const size_t mask = ((size_t)1 << m) - 1;
+    SEG_LOOP(g, i) gf_st(out + i, gf_mul_reg(gf_ld(A + (i & mask)), gf_ld(B + (i >> m))));
+}
+
+// out[i] = the 16- or 32-byte record at absolute device address src[i]
+__global__ void k_gather_abs(const uint64_t *__restrict__ src, uint8_t *__restrict__ out, size_t n, int rb) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(src[i]);
+        ulonglong2 *o = reinterpret_cast<ulonglong2 *>(out + (size_t)rb * i);
+        o[0] = p[0];
+        if (rb == 32) o[1] = p[1];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -389,13 +520,13 @@ hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int 
         k_replicate<<<1, 64, 0, st>>>(data, cw, c, 1);
         return hipGetLastError();
     }
-    const size_t half = ((size_t)1 << n) / 2;
-    k_ntt_first<<<grid_for(half, CB, kCapMul), CB, 0, st>>>(data, cw, ft0, c, half);
+    const size_t work = ((size_t)1 << n) / 2 * c;
+    k_ntt_first<<<grid_for(work, 256, kCapMul), 256, 0, st>>>(data, cw, ft0, b, work);
     return hipGetLastError();
 }
 hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n) {
-    const size_t half = ((size_t)1 << n) / 2;
-    k_ntt_layer<<<grid_for(half, CB, kCapMul), CB, 0, st>>>(cw, fts, s, b, half);
+    const size_t work = ((size_t)1 << n) / 2 << b;
+    k_ntt_layer<<<grid_for(work, 256, kCapMul), 256, 0, st>>>(cw, fts, s, b, work);
     return hipGetLastError();
 }
 
@@ -453,6 +584,36 @@ hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *
     return hipGetLastError();
 }
 
+hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int nseg, uint32_t nblocks, gf *partial) {
+    if (nseg <= 0 || nblocks == 0) return hipSuccess;
+    switch (which) {
+    case SegKernel::NttFirst: ks_ntt_first<<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::NttLayer: ks_ntt_layer<<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::LeavesElem: ks_leaves_elem<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::LeavesPair: ks_leaves_pair<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::Nodes: ks_nodes<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::Dot: ks_dot<<<nblocks, CB, 0, st>>>(d_segs, nseg, partial); break;
+    case SegKernel::Fold: ks_fold<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::Fix: ks_fix<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::Outer: ks_outer<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
+    }
+    return hipGetLastError();
+}
+int seg_block_size(SegKernel which) {
+    switch (which) {
+    case SegKernel::NttFirst:
+    case SegKernel::NttLayer: return 256;
+    case SegKernel::LeavesElem:
+    case SegKernel::LeavesPair:
+    case SegKernel::Nodes: return KB;
+    default: return CB;
+    }
+}
+hipError_t gather_abs(hipStream_t st, const uint64_t *src, void *out, size_t n, int rb) {
+    if (!n) return hipSuccess;
+    k_gather_abs<<<grid_for(n, 256, 4096), 256, 0, st>>>(src, (uint8_t *)out, n, rb);
+    return hipGetLastError();
+}
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters) {
     if (variant == 0)
         k_bench_mul_comb<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);

@@ -8,7 +8,31 @@
 #include "gf128.hpp"
 
 namespace bfd {
+
+// One polynomial's share of a segmented launch (meaning of the pointers and
+// `arg` per kernel is documented at each segmented kernel).
+struct Seg {
+    const void *in;
+    void *out;
+    const void *aux;
+    uint64_t count;   // work items of this segment
+    uint64_t arg;
+    gf x;             // per-segment challenge (fold / fix)
+    uint32_t blk0;    // first block of the segment in the launch
+    uint32_t nblk;    // blocks of the segment
+};
+
+enum class SegKernel { NttFirst, NttLayer, LeavesElem, LeavesPair, Nodes, Dot, Fold, Fix, Outer };
+
 namespace launch {
+
+// segmented launch over nseg segments occupying nblocks blocks in total;
+// `partial` receives one element per block for SegKernel::Dot
+hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int nseg, uint32_t nblocks, gf *partial);
+int seg_block_size(SegKernel which);
+// out[i] = rb (16 or 32) bytes from absolute device address src[i]
+hipError_t gather_abs(hipStream_t st, const uint64_t *src, void *out, size_t n, int rb);
+
 
 // ft[j] = XOR over set bits p of j of bt[p]
 hipError_t xor_span(hipStream_t st, gf *ft, const gf *bt, int bits, size_t count);
