@@ -90,6 +90,22 @@ int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, con
              const uint8_t *values, bfd_challenger *challenger, uint8_t **proof, size_t *proof_len);
 void bfd_proof_free(uint8_t *proof);
 
+/* ---- batched commit+open of independent polynomials (uneven sizes) ----
+ * The reference commits one matrix height per Basefold instance
+ * (basefold.rs:94 TODO), so a batch is `count` independent Basefold
+ * commit+open runs, each with its own challenger; every proof is byte-equal
+ * to the plain bfd_commit + bfd_open proof of the same polynomial.
+ * evals[p]: 2^log_sizes[p] evaluations (device pointers if evals_on_device);
+ * points[p]: log_sizes[p] coordinates; claims[p]: the evaluation at the point.
+ * With observe_prefix set, each challenger first observes the root and the
+ * claim (the harness order of poly-commit/src/lib.rs:77-89).
+ * Outputs count x 32 roots and count library-allocated proofs. */
+int bfd_commit_open_batch(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t count,
+                          const uint32_t *log_sizes, const void *const *evals, int evals_on_device,
+                          const uint8_t *const *points, const uint8_t *const *claims,
+                          bfd_challenger *const *challengers, int observe_prefix, uint8_t *roots,
+                          uint8_t **proofs, size_t *proof_lens);
+
 /* ---- degree-3 product sumcheck  sum_b f(b) g(b) eq(r, b)  (no reference
  * counterpart; driver semantics of prove_sumcheck, sumcheck/src/lib.rs:56-103):
  * observes the claim, then per round observes [c0, c2, c3] and samples the

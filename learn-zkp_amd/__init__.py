@@ -82,6 +82,11 @@ def lib():
         L.bfd_sumcheck_product.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u8p, u8p, u8p, ctypes.c_void_p,
                                            u8p, u8p, u8p, u8p, u8p]
         L.bfd_random_vec.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u8p]
+        L.bfd_commit_open_batch.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_void_p),
+                                            ctypes.c_int, ctypes.POINTER(u8p), ctypes.POINTER(u8p),
+                                            ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, u8p,
+                                            ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
         L.bfd_microbench.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]
         L.bfd_ctx_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
@@ -303,6 +308,47 @@ class Basefold:
             return ctypes.string_at(proof, plen.value)
         finally:
             lib().bfd_proof_free(proof)
+
+
+def commit_open_batch(lambda_: int, log2_c: int, evals: Sequence, points: Sequence, claims: Sequence,
+                      challengers: Sequence[GenericChallenger], observe_prefix: bool = True,
+                      device_ptrs: Optional[Sequence[int]] = None, ctx: Optional[Context] = None):
+    """Commit and open many independent polynomials in one batched call.
+
+    evals[p]: numpy (2^n_p, 2) host evaluations (or pass device_ptrs and the
+    sizes through evals as ints = n_p).  Each proof equals the plain
+    commit+open proof of the same polynomial with the same challenger.
+    Returns (roots: list[bytes], proofs: list[bytes])."""
+    ctx = ctx or default_context()
+    count = len(points)
+    keep = []
+    if device_ptrs is None:
+        host = [_elems(e) for e in evals]
+        keep.extend(host)
+        sizes = [int(len(e)).bit_length() - 1 for e in host]
+        ptrs = (ctypes.c_void_p * count)(*[h.ctypes.data for h in host])
+        on_dev = 0
+    else:
+        sizes = [int(n) for n in evals]
+        ptrs = (ctypes.c_void_p * count)(*device_ptrs)
+        on_dev = 1
+    pts = [_elems(p) if len(p) else np.zeros((1, 2), dtype=np.uint64) for p in points]
+    cls = [_elems(c) for c in claims]
+    keep.extend(pts + cls)
+    sz = (ctypes.c_uint32 * count)(*sizes)
+    pp = (u8p * count)(*[_p(x) for x in pts])
+    cp = (u8p * count)(*[_p(x) for x in cls])
+    ch = (ctypes.c_void_p * count)(*[c.h.value if isinstance(c.h, ctypes.c_void_p) else c.h for c in challengers])
+    roots = np.zeros(32 * count, dtype=np.uint8)
+    proofs = (u8p * count)()
+    lens = (ctypes.c_size_t * count)()
+    _check(lib().bfd_commit_open_batch(ctx.h, lambda_, log2_c, count, sz, ptrs, on_dev, pp, cp, ch,
+                                       int(observe_prefix), _p(roots), proofs, lens))
+    out = []
+    for i in range(count):
+        out.append(ctypes.string_at(proofs[i], lens[i]))
+        lib().bfd_proof_free(proofs[i])
+    return [roots[32 * i:32 * (i + 1)].tobytes() for i in range(count)], out
 
 
 def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional[Context] = None):

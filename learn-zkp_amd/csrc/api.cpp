@@ -72,7 +72,8 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     // openings: one gather of elements, one of digests
     std::vector<uint64_t> ea, da;
     plan_gathers(o, lay, ea, da);
-    std::vector<uint8_t> he(16 * ea.size()), hd(32 * da.size());
+    ctx->gather_pin.ensure(16 * ea.size() + 32 * da.size() + 64);
+    uint8_t *he = (uint8_t *)ctx->gather_pin.p, *hd = he + 16 * ea.size();
     DevBuf dix(8 * (ea.size() + da.size()) + 8, st), dout(16 * ea.size() + 32 * da.size() + 16, st);
     hip_check(hipMemcpyAsync(dix.p, ea.data(), 8 * ea.size(), hipMemcpyHostToDevice, st), "copy gather plan");
     hip_check(hipMemcpyAsync(dix.as<uint64_t>() + ea.size(), da.data(), 8 * da.size(), hipMemcpyHostToDevice, st),
@@ -80,14 +81,72 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * ea.size();
     ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, ea.size(), 16); });
     ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>() + ea.size(), od, da.size(), 32); });
-    if (!he.empty()) hip_check(hipMemcpyAsync(he.data(), oe, he.size(), hipMemcpyDeviceToHost, st), "copy gather");
-    if (!hd.empty()) hip_check(hipMemcpyAsync(hd.data(), od, hd.size(), hipMemcpyDeviceToHost, st), "copy gather");
+    hip_check(hipMemcpyAsync(he, oe, 16 * ea.size() + 32 * da.size(), hipMemcpyDeviceToHost, st), "copy gather");
     hip_check(hipStreamSynchronize(st), "gather sync");
     ctx->timer.collect();
-    assemble_proof(o, he.data(), hd.data(), proof);
+    assemble_proof(o, he, hd, proof);
 }
 
 }  // namespace
+
+namespace bfd {
+namespace rt {
+
+bfd_code *create_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d) {
+    if (log2_c > 8 || d > 32 || log2_c + d > 34) fail(BFD_ERR_ARG, "code too large");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    set_device(ctx);
+    hipStream_t st = ctx->stream;
+    auto code = std::make_unique<bfd_code>();
+    code->ctx = ctx;
+    code->lambda = lambda;
+    code->log2_c = log2_c;
+    code->d = d;
+    code->q = num_queries(lambda, log2_c);
+    const int L = (int)(log2_c + d);
+    // level s table (fold order) = twiddles[b+s][bitrev_{b+s}(j)], j < 2^(b+s)
+    // = XOR over bits p of j of basis[b+s][b+s-1-p]
+    auto basis = ntt_basis(L);
+    code->ft_off.assign(d + 1, 0);
+    size_t total = 0;
+    for (uint32_t s = 0; s < d; s++) {
+        code->ft_off[s] = total;
+        total += (size_t)1 << (log2_c + s);
+    }
+    code->ft_off[d] = total;
+    code->ft = DevBuf(16 * (total ? total : 1), st);
+    std::vector<gf> bt;
+    std::vector<size_t> bt_off(d + 1, 0);
+    for (uint32_t s = 0; s < d; s++) {
+        const int K = (int)(log2_c + s);
+        bt_off[s] = bt.size();
+        for (int p = 0; p < K; p++) bt.push_back(basis[K][K - 1 - p]);
+    }
+    DevBuf dbt(16 * (bt.size() ? bt.size() : 1), st);
+    if (!bt.empty())
+        hip_check(hipMemcpyAsync(dbt.p, bt.data(), 16 * bt.size(), hipMemcpyHostToDevice, st), "copy basis");
+    for (uint32_t s = 0; s < d; s++) {
+        const int K = (int)(log2_c + s);
+        hip_check(launch::xor_span(st, code->ft.as<gf>() + code->ft_off[s], dbt.as<gf>() + bt_off[s], K,
+                                   (size_t)1 << K),
+                  "xor_span");
+    }
+    hip_check(hipStreamSynchronize(st), "code setup");
+    return code.release();
+}
+
+bfd_code *cached_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const uint64_t key = ((uint64_t)lambda << 32) | ((uint64_t)log2_c << 16) | d;
+    auto it = ctx->code_cache.find(key);
+    if (it != ctx->code_cache.end()) return it->second;
+    bfd_code *c = create_code(ctx, lambda, log2_c, d);
+    ctx->code_cache[key] = c;
+    return c;
+}
+
+}  // namespace rt
+}  // namespace bfd
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -120,6 +179,8 @@ int bfd_ctx_destroy(bfd_ctx *ctx) {
     return guarded([&] {
         if (!ctx) return;
         (void)hipSetDevice(ctx->device);
+        for (auto &kv : ctx->code_cache) delete kv.second;
+        ctx->code_cache.clear();
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -139,46 +200,7 @@ uint64_t bfd_ctx_resident_bytes(const bfd_ctx *ctx) { return ctx ? ctx->resident
 int bfd_code_create(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d, bfd_code **out) {
     return guarded([&] {
         if (!ctx || !out) fail(BFD_ERR_ARG, "null argument");
-        if (log2_c > 8 || d > 32 || log2_c + d > 34) fail(BFD_ERR_ARG, "code too large");
-        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-        set_device(ctx);
-        hipStream_t st = ctx->stream;
-        auto code = std::make_unique<bfd_code>();
-        code->ctx = ctx;
-        code->lambda = lambda;
-        code->log2_c = log2_c;
-        code->d = d;
-        code->q = num_queries(lambda, log2_c);
-        const int L = (int)(log2_c + d);
-        // level s table (fold order) = twiddles[b+s][bitrev_{b+s}(j)], j < 2^(b+s)
-        // = XOR over bits p of j of basis[b+s][b+s-1-p]
-        auto basis = ntt_basis(L);
-        code->ft_off.assign(d + 1, 0);
-        size_t total = 0;
-        for (uint32_t s = 0; s < d; s++) {
-            code->ft_off[s] = total;
-            total += (size_t)1 << (log2_c + s);
-        }
-        code->ft_off[d] = total;
-        code->ft = DevBuf(16 * (total ? total : 1), st);
-        std::vector<gf> bt;
-        std::vector<size_t> bt_off(d + 1, 0);
-        for (uint32_t s = 0; s < d; s++) {
-            const int K = (int)(log2_c + s);
-            bt_off[s] = bt.size();
-            for (int p = 0; p < K; p++) bt.push_back(basis[K][K - 1 - p]);
-        }
-        DevBuf dbt(16 * (bt.size() ? bt.size() : 1), st);
-        if (!bt.empty())
-            hip_check(hipMemcpyAsync(dbt.p, bt.data(), 16 * bt.size(), hipMemcpyHostToDevice, st), "copy basis");
-        for (uint32_t s = 0; s < d; s++) {
-            const int K = (int)(log2_c + s);
-            hip_check(launch::xor_span(st, code->ft.as<gf>() + code->ft_off[s], dbt.as<gf>() + bt_off[s], K,
-                                       (size_t)1 << K),
-                      "xor_span");
-        }
-        hip_check(hipStreamSynchronize(st), "code setup");
-        *out = code.release();
+        *out = create_code(ctx, lambda, log2_c, d);
     });
 }
 

@@ -46,6 +46,24 @@ inline gf hinv(gf a) {
     return r;
 }
 
+// In-place inversion of every element with one field inversion
+// (prefix products; all inputs must be nonzero).
+inline void batch_inverse(std::vector<gf> &v) {
+    if (v.empty()) return;
+    std::vector<gf> pre(v.size());
+    gf acc = gf_one();
+    for (size_t i = 0; i < v.size(); i++) {
+        pre[i] = acc;
+        acc = hmul(acc, v[i]);
+    }
+    gf inv = hinv(acc);
+    for (size_t i = v.size(); i-- > 0;) {
+        gf vi = v[i];
+        v[i] = hmul(inv, pre[i]);
+        inv = hmul(inv, vi);
+    }
+}
+
 inline gf hfrom_canonical(uint64_t lo, uint64_t hi) {
     // R^2 = X^256 mod P in canonical form (p3/src/polyval/portable.rs:10-13)
     return hmul(gf{lo, hi}, gf{0x4563df92ea7081b5ull, 0x1e563df92ea7081bull});

@@ -161,13 +161,20 @@ struct KernelTimer {
 // ---------------------------------------------------------------------------
 // handles
 // ---------------------------------------------------------------------------
+struct bfd_code;
+
 struct bfd_ctx {
     int device = 0;
+    // codes made on demand by the batched and serving paths, keyed by
+    // (lambda, log2_c, d); owned by the context
+    std::map<uint64_t, bfd_code *> code_cache;
     hipStream_t stream = nullptr;
     uint64_t budget = 0;
     uint64_t resident = 0;
     std::recursive_mutex mu;
-    bfd::rt::PinnedBuf pinned;
+    bfd::rt::PinnedBuf pinned;      // per-round read-backs (roots, partial sums)
+    bfd::rt::PinnedBuf gather_pin;  // opened values of a proof, device -> host
+    bfd::rt::PinnedBuf seg_pin;     // segment tables of batched launches, host -> device
     bfd::rt::KernelTimer timer;
 };
 
@@ -197,6 +204,11 @@ struct bfd_challenger {
 
 namespace bfd {
 namespace rt {
+
+// BinaryReedSolomonCode::new with its device twiddle table (api.cpp)
+bfd_code *create_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d);
+// cached code for the batched and serving paths
+bfd_code *cached_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d);
 
 inline void set_device(bfd_ctx *ctx) { hip_check(hipSetDevice(ctx->device), "hipSetDevice"); }
 
@@ -237,7 +249,7 @@ inline std::vector<gf> host_eq(const gf *r, int m) {
 // two host-built halves.
 inline void device_eq(bfd_ctx *ctx, const gf *r, int m, gf *out) {
     hipStream_t st = ctx->stream;
-    int lo = m < 12 ? m : 12, hi = m - lo;
+    int lo = (m + 1) / 2, hi = m - lo;
     std::vector<gf> A = host_eq(r, lo), B = host_eq(r + lo, hi);
     DevBuf dA(A.size() * 16, st), dB(B.size() * 16, st);
     hip_check(hipMemcpyAsync(dA.p, A.data(), A.size() * 16, hipMemcpyHostToDevice, st), "copy eq A");
@@ -348,18 +360,22 @@ struct OpenRounds {
         M = (size_t)1 << L;
         q = q_;
         r.assign(n, gf_zero());
-        rinv.assign(n, gf_zero());
         for (int i = 0; i < n; i++) {
             r[i] = load_elem(point_b + 16 * i);
             if (gf_eq(r[i], gf_zero()) || gf_eq(r[i], gf_one()))
                 fail(BFD_ERR_ARG, "point coordinates 0 and 1 are not invertible in the Eval prover");
-            rinv[i] = hinv(r[i]);                                  // EvalImpr::new, eval_impr.rs:20-27
         }
         claim = load_elem(claim_b);
-        // correcting_factor(i) = prod_{j=i}^{n-2} (1 - r_j)^-1  (eval_impr.rs:81-88)
+        // r^-1 (EvalImpr::new, eval_impr.rs:20-27) and (1 - r_j)^-1 (HalfEq::new,
+        // :69-79) with one batched inversion
         const int hn = n > 0 ? n - 1 : 0;
+        std::vector<gf> inv(r);
+        for (int j = 0; j < hn; j++) inv.push_back(gf_add(gf_one(), r[j]));
+        batch_inverse(inv);
+        rinv.assign(inv.begin(), inv.begin() + n);
+        // correcting_factor(i) = prod_{j=i}^{n-2} (1 - r_j)^-1  (eval_impr.rs:81-88)
         cf.assign(n + 1, gf_one());
-        for (int j = hn - 1; j >= 0; j--) cf[j] = hmul(cf[j + 1], hinv(gf_add(gf_one(), r[j])));
+        for (int j = hn - 1; j >= 0; j--) cf[j] = hmul(cf[j + 1], inv[n + j]);
         subclaim = claim;
         inner_sub = gf_zero();
         s_delta = gf_one();

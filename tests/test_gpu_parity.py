@@ -149,3 +149,39 @@ def test_basefold_full_size_verifies(bfd, oracle, ctx):
     bad = bytearray(proof)
     bad[-40] ^= 1
     assert oracle.basefold_verify(128, log2_c, root, point, claim, bytes(bad)) != 0
+
+
+def test_batch_equals_plain_and_oracle(bfd, oracle, ctx):
+    """Uneven batch: every proof equals the plain path's and the oracle's."""
+    rng = np.random.default_rng(7)
+    sizes = [int(v) for v in rng.integers(1, 11, size=12)] + [12, 3, 1]
+    for log2_c in (1, 2):
+        _say(f"batch log2_c={log2_c} sizes={sizes}")
+        evals = [bfd.random_vec(1000 + i, 1 << n) for i, n in enumerate(sizes)]
+        points = [bfd.random_vec(2000 + i, n) for i, n in enumerate(sizes)]
+        claims = [oracle.mle_evaluate(e, p) for e, p in zip(evals, points)]
+        chs = [bfd.GenericChallenger.keccak256() for _ in sizes]
+        roots, proofs = bfd.commit_open_batch(128, log2_c, evals, points, claims, chs, True, ctx=ctx)
+        for i, n in enumerate(sizes):
+            root_o, claim_o, proof_o = oracle.basefold_prove(128, log2_c, evals[i], points[i])
+            assert roots[i] == root_o, (log2_c, i, n)
+            assert proofs[i] == proof_o, (log2_c, i, n)
+            root_p, _, proof_p = gpu_prove(bfd, ctx, 128, log2_c, evals[i], points[i])
+            assert proof_p == proofs[i]
+
+
+def test_batch_uneven_mix_verifies(bfd, oracle, ctx):
+    """The benchmark mix shape (sizes 12..22) at reduced count: roots equal the
+    plain commitments and every proof verifies on the CPU."""
+    rng = np.random.default_rng(0)
+    sizes = [int(v) for v in rng.integers(12, 23, size=6)]
+    _say(f"batch mix sizes={sizes}")
+    evals = [bfd.random_vec(3000 + i, 1 << n) for i, n in enumerate(sizes)]
+    points = [bfd.random_vec(4000 + i, n) for i, n in enumerate(sizes)]
+    claims = [oracle.mle_evaluate(e, p) for e, p in zip(evals, points)]
+    chs = [bfd.GenericChallenger.keccak256() for _ in sizes]
+    roots, proofs = bfd.commit_open_batch(128, 1, evals, points, claims, chs, True, ctx=ctx)
+    for i, n in enumerate(sizes):
+        assert oracle.basefold_verify(128, 1, roots[i], points[i], claims[i], proofs[i]) == 0, (i, n)
+        root_p, _, proof_p = gpu_prove(bfd, ctx, 128, 1, evals[i], points[i])
+        assert root_p == roots[i] and proof_p == proofs[i], (i, n)
