@@ -88,7 +88,14 @@ int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uin
  * pass a clone to keep the original).  *proof is allocated by the library. */
 int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
              const uint8_t *values, bfd_challenger *challenger, uint8_t **proof, size_t *proof_len);
+/* Same, writing into a caller buffer of `capacity` bytes (at least
+ * bfd_proof_size); *proof_len receives the size written. */
+int bfd_open_into(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
+                  const uint8_t *values, bfd_challenger *challenger, uint8_t *proof, size_t capacity,
+                  size_t *proof_len);
 void bfd_proof_free(uint8_t *proof);
+/* Exact bincode size of the proof for (lambda, log2_c, d). */
+uint64_t bfd_proof_size(uint32_t lambda, uint32_t log2_c, uint32_t d);
 
 /* ---- batched commit+open of independent polynomials (uneven sizes) ----
  * The reference commits one matrix height per Basefold instance
@@ -99,7 +106,9 @@ void bfd_proof_free(uint8_t *proof);
  * points[p]: log_sizes[p] coordinates; claims[p]: the evaluation at the point.
  * With observe_prefix set, each challenger first observes the root and the
  * claim (the harness order of poly-commit/src/lib.rs:77-89).
- * Outputs count x 32 roots and count library-allocated proofs. */
+ * Outputs count x 32 roots and count proofs: a non-NULL proofs[p] on input is
+ * a caller buffer of proof_lens[p] bytes (at least bfd_proof_size), otherwise
+ * the library allocates it (free with bfd_proof_free). */
 int bfd_commit_open_batch(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t count,
                           const uint32_t *log_sizes, const void *const *evals, int evals_on_device,
                           const uint8_t *const *points, const uint8_t *const *claims,

@@ -79,6 +79,10 @@ def lib():
                                ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
         L.bfd_proof_free.argtypes = [u8p]
         L.bfd_proof_free.restype = None
+        L.bfd_proof_size.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.bfd_proof_size.restype = ctypes.c_uint64
+        L.bfd_open_into.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u8p, u8p, ctypes.c_void_p,
+                                    u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.bfd_sumcheck_product.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u8p, u8p, u8p, ctypes.c_void_p,
                                            u8p, u8p, u8p, u8p, u8p]
         L.bfd_random_vec.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u8p]
@@ -127,9 +131,9 @@ class Context:
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
-            lib().bfd_ctx_destroy(self.h)
-            self.h = None
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bfd_ctx_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:
@@ -194,9 +198,9 @@ class GenericChallenger:
         return GenericChallenger(h)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().bfd_challenger_destroy(self.h)
-            self.h = None
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bfd_challenger_destroy(self.h)
+        self.h = None
 
     def observe(self, data: bytes) -> None:
         buf = (ctypes.c_uint8 * max(1, len(data))).from_buffer_copy(bytes(data) or b"\0")
@@ -242,9 +246,9 @@ class BasefoldCommitmentData:
     _owner: "Basefold" = field(repr=False, default=None)
 
     def __del__(self):
-        if getattr(self, "handle", None):
-            lib().bfd_commitment_destroy(self.handle)
-            self.handle = None
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.bfd_commitment_destroy(self.handle)
+        self.handle = None
 
 
 class Basefold:
@@ -263,9 +267,9 @@ class Basefold:
         return cls(config, ctx)
 
     def __del__(self):
-        if getattr(self, "code_h", None):
-            lib().bfd_code_destroy(self.code_h)
-            self.code_h = None
+        if getattr(self, "code_h", None) and _lib is not None:
+            _lib.bfd_code_destroy(self.code_h)
+        self.code_h = None
 
     @property
     def num_queries(self) -> int:
@@ -318,7 +322,7 @@ def commit_open_batch(lambda_: int, log2_c: int, evals: Sequence, points: Sequen
     evals[p]: numpy (2^n_p, 2) host evaluations (or pass device_ptrs and the
     sizes through evals as ints = n_p).  Each proof equals the plain
     commit+open proof of the same polynomial with the same challenger.
-    Returns (roots: list[bytes], proofs: list[bytes])."""
+    Returns (roots: list[bytes], proofs: list[numpy uint8 arrays])."""
     ctx = ctx or default_context()
     count = len(points)
     keep = []
@@ -340,15 +344,17 @@ def commit_open_batch(lambda_: int, log2_c: int, evals: Sequence, points: Sequen
     cp = (u8p * count)(*[_p(x) for x in cls])
     ch = (ctypes.c_void_p * count)(*[c.h.value if isinstance(c.h, ctypes.c_void_p) else c.h for c in challengers])
     roots = np.zeros(32 * count, dtype=np.uint8)
-    proofs = (u8p * count)()
-    lens = (ctypes.c_size_t * count)()
+    bufs = [np.empty(int(lib().bfd_proof_size(lambda_, log2_c, n)), dtype=np.uint8) for n in sizes]
+    proofs = (u8p * count)(*[_p(b) for b in bufs])
+    lens = (ctypes.c_size_t * count)(*[len(b) for b in bufs])
     _check(lib().bfd_commit_open_batch(ctx.h, lambda_, log2_c, count, sz, ptrs, on_dev, pp, cp, ch,
                                        int(observe_prefix), _p(roots), proofs, lens))
-    out = []
-    for i in range(count):
-        out.append(ctypes.string_at(proofs[i], lens[i]))
-        lib().bfd_proof_free(proofs[i])
-    return [roots[32 * i:32 * (i + 1)].tobytes() for i in range(count)], out
+    return [roots[32 * i:32 * (i + 1)].tobytes() for i in range(count)], [b[:lens[i]] for i, b in enumerate(bufs)]
+
+
+def proof_size(lambda_: int, log2_c: int, d: int) -> int:
+    """Exact bincode size of a Basefold proof."""
+    return int(lib().bfd_proof_size(lambda_, log2_c, d))
 
 
 def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional[Context] = None):

@@ -13,7 +13,7 @@ namespace {
 // Basefold::open for width 1 (basefold.rs:133-232) with the interleaved
 // folding sumcheck (:335-369) and the Eval prover (eval.rs, eval_impr.rs).
 void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const uint8_t *point_b,
-             const uint8_t *values_b, Challenger &ch, std::vector<uint8_t> &proof) {
+             const uint8_t *values_b, Challenger &ch, uint8_t **proof, size_t *proof_len) {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     set_device(ctx);
     hipStream_t st = ctx->stream;
@@ -70,21 +70,29 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     o.finish(load_elem(pin), ch);
 
     // openings: one gather of elements, one of digests
-    std::vector<uint64_t> ea, da;
-    plan_gathers(o, lay, ea, da);
-    ctx->gather_pin.ensure(16 * ea.size() + 32 * da.size() + 64);
-    uint8_t *he = (uint8_t *)ctx->gather_pin.p, *hd = he + 16 * ea.size();
-    DevBuf dix(8 * (ea.size() + da.size()) + 8, st), dout(16 * ea.size() + 32 * da.size() + 16, st);
-    hip_check(hipMemcpyAsync(dix.p, ea.data(), 8 * ea.size(), hipMemcpyHostToDevice, st), "copy gather plan");
-    hip_check(hipMemcpyAsync(dix.as<uint64_t>() + ea.size(), da.data(), 8 * da.size(), hipMemcpyHostToDevice, st),
-              "copy gather plan");
-    uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * ea.size();
-    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, ea.size(), 16); });
-    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>() + ea.size(), od, da.size(), 32); });
-    hip_check(hipMemcpyAsync(he, oe, 16 * ea.size() + 32 * da.size(), hipMemcpyDeviceToHost, st), "copy gather");
+    double t0 = now_ms();
+    size_t ne = 0, nd = 0;
+    gather_counts(o, ne, nd);
+    std::vector<uint64_t> plan(ne + nd);
+    plan_gathers(o, lay, plan.data(), plan.data() + ne);
+    ctx->gather_pin.ensure(16 * ne + 32 * nd + 64);
+    uint8_t *he = (uint8_t *)ctx->gather_pin.p, *hd = he + 16 * ne;
+    DevBuf dix(8 * (ne + nd) + 8, st), dout(16 * ne + 32 * nd + 16, st);
+    hip_check(hipMemcpyAsync(dix.p, plan.data(), 8 * (ne + nd), hipMemcpyHostToDevice, st), "copy gather plan");
+    uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * ne;
+    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, ne, 16); });
+    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>() + ne, od, nd, 32); });
+    hip_check(hipMemcpyAsync(he, oe, 16 * ne + 32 * nd, hipMemcpyDeviceToHost, st), "copy gather");
     hip_check(hipStreamSynchronize(st), "gather sync");
     ctx->timer.collect();
-    assemble_proof(o, he, hd, proof);
+    const size_t sz = proof_size(o.b, o.d, o.q);
+    if (*proof && *proof_len < sz) fail(BFD_ERR_ARG, "proof buffer too small");
+    uint8_t *dst = *proof ? *proof : (uint8_t *)malloc(sz ? sz : 1);
+    if (!dst) fail(BFD_ERR_ARG, "host allocation failed");
+    assemble_proof(o, he, hd, dst);
+    *proof = dst;
+    *proof_len = sz;
+    ctx->timer.host("proof", now_ms() - t0);
 }
 
 }  // namespace
@@ -309,14 +317,30 @@ int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, con
         if (!ctx || !code || !comm || !values || !challenger || !proof || !proof_len || (!point && comm->n))
             fail(BFD_ERR_ARG, "null argument");
         if (comm->code != code) fail(BFD_ERR_ARG, "commitment was made with another code");
-        std::vector<uint8_t> out;
-        do_open(ctx, code, comm, point, values, challenger->c, out);
-        uint8_t *p = (uint8_t *)malloc(out.size() ? out.size() : 1);
-        if (!p) fail(BFD_ERR_ARG, "host allocation failed");
-        memcpy(p, out.data(), out.size());
+        uint8_t *p = nullptr;
+        size_t n = 0;
+        do_open(ctx, code, comm, point, values, challenger->c, &p, &n);
         *proof = p;
-        *proof_len = out.size();
+        *proof_len = n;
     });
+}
+
+int bfd_open_into(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
+                  const uint8_t *values, bfd_challenger *challenger, uint8_t *proof, size_t capacity,
+                  size_t *proof_len) {
+    return guarded([&] {
+        if (!ctx || !code || !comm || !values || !challenger || !proof || !proof_len || (!point && comm->n))
+            fail(BFD_ERR_ARG, "null argument");
+        if (comm->code != code) fail(BFD_ERR_ARG, "commitment was made with another code");
+        uint8_t *p = proof;
+        size_t n = capacity;
+        do_open(ctx, code, comm, point, values, challenger->c, &p, &n);
+        *proof_len = n;
+    });
+}
+
+uint64_t bfd_proof_size(uint32_t lambda, uint32_t log2_c, uint32_t d) {
+    return proof_size((int)log2_c, (int)d, num_queries(lambda, log2_c));
 }
 
 void bfd_proof_free(uint8_t *proof) { free(proof); }

@@ -103,7 +103,7 @@ namespace rt {
 void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t count, const uint32_t *log_sizes,
                        const void *const *evals, bool on_device, const uint8_t *const *points,
                        const uint8_t *const *claims, Challenger *const *chs, bool observe_prefix, uint8_t *roots,
-                       std::vector<std::vector<uint8_t>> &proofs) {
+                       uint8_t **proofs, size_t *proof_lens) {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     set_device(ctx);
     hipStream_t st = ctx->stream;
@@ -326,32 +326,43 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
     hip_check(hipMemcpyAsync(pin, dgat.p, 16 * count, hipMemcpyDeviceToHost, st), "copy finals");
     hip_check(hipStreamSynchronize(st), "final sync");
     segl.synced();
-    std::vector<uint64_t> ea, da;
+    double t0 = now_ms();
     std::vector<size_t> e0(count + 1, 0), d0(count + 1, 0);
+    // query sampling is independent per transcript
+    parallel_for(count, [&](size_t p) { P[p].o.finish(load_elem(pin + 16 * p), *P[p].ch); });
     for (size_t p = 0; p < count; p++) {
-        P[p].o.finish(load_elem(pin + 16 * p), *P[p].ch);
-        e0[p] = ea.size();
-        d0[p] = da.size();
-        plan_gathers(P[p].o, P[p].lay, ea, da);
+        size_t ne = 0, nd = 0;
+        gather_counts(P[p].o, ne, nd);
+        e0[p + 1] = e0[p] + ne;
+        d0[p + 1] = d0[p] + nd;
     }
-    e0[count] = ea.size();
-    d0[count] = da.size();
+    const size_t NE = e0[count], ND = d0[count];
+    std::vector<uint64_t> plan(NE + ND);
+    parallel_for(count, [&](size_t p) { plan_gathers(P[p].o, P[p].lay, plan.data() + e0[p], plan.data() + NE + d0[p]); });
+    ctx->timer.host("plan", now_ms() - t0);
     PinnedBuf &hout = ctx->gather_pin;
-    hout.ensure(16 * ea.size() + 32 * da.size() + 64);
-    DevBuf dix(8 * (ea.size() + da.size()) + 8, st), dout(16 * ea.size() + 32 * da.size() + 64, st);
-    hip_check(hipMemcpyAsync(dix.p, ea.data(), 8 * ea.size(), hipMemcpyHostToDevice, st), "copy gather plan");
-    hip_check(hipMemcpyAsync(dix.as<uint64_t>() + ea.size(), da.data(), 8 * da.size(), hipMemcpyHostToDevice, st),
-              "copy gather plan");
-    uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * ea.size();
-    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, ea.size(), 16); });
-    ctx->timer.run(st, "k_gather_abs",
-                   [&] { return launch::gather_abs(st, dix.as<uint64_t>() + ea.size(), od, da.size(), 32); });
-    hip_check(hipMemcpyAsync(hout.p, oe, 16 * ea.size() + 32 * da.size(), hipMemcpyDeviceToHost, st), "copy gather");
+    hout.ensure(16 * NE + 32 * ND + 64);
+    DevBuf dix(8 * (NE + ND) + 8, st), dout(16 * NE + 32 * ND + 64, st);
+    hip_check(hipMemcpyAsync(dix.p, plan.data(), 8 * (NE + ND), hipMemcpyHostToDevice, st), "copy gather plan");
+    uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * NE;
+    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, NE, 16); });
+    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>() + NE, od, ND, 32); });
+    hip_check(hipMemcpyAsync(hout.p, oe, 16 * NE + 32 * ND, hipMemcpyDeviceToHost, st), "copy gather");
     hip_check(hipStreamSynchronize(st), "gather sync");
     ctx->timer.collect();
-    const uint8_t *he = (const uint8_t *)hout.p, *hd = he + 16 * ea.size();
-    proofs.resize(count);
-    for (size_t p = 0; p < count; p++) assemble_proof(P[p].o, he + 16 * e0[p], hd + 32 * d0[p], proofs[p]);
+    t0 = now_ms();
+    const uint8_t *he = (const uint8_t *)hout.p, *hd = he + 16 * NE;
+    for (size_t p = 0; p < count; p++) {
+        const size_t sz = proof_size(P[p].o.b, P[p].o.d, P[p].o.q);
+        if (proofs[p] && proof_lens[p] < sz) fail(BFD_ERR_ARG, "proof buffer too small");
+        if (!proofs[p]) {
+            proofs[p] = (uint8_t *)malloc(sz ? sz : 1);
+            if (!proofs[p]) fail(BFD_ERR_ARG, "host allocation failed");
+        }
+        proof_lens[p] = sz;
+    }
+    parallel_for(count, [&](size_t p) { assemble_proof(P[p].o, he + 16 * e0[p], hd + 32 * d0[p], proofs[p]); });
+    ctx->timer.host("assemble", now_ms() - t0);
 }
 
 }  // namespace rt
@@ -371,14 +382,7 @@ extern "C" int bfd_commit_open_batch(bfd_ctx *ctx, uint32_t lambda, uint32_t log
             if (!challengers[p] || !evals[p] || !points[p] || !claims[p]) fail(BFD_ERR_ARG, "null argument");
             chs[p] = &challengers[p]->c;
         }
-        std::vector<std::vector<uint8_t>> out;
         batch_commit_open(ctx, lambda, log2_c, count, log_sizes, evals, evals_on_device != 0, points, claims,
-                          chs.data(), observe_prefix != 0, roots, out);
-        for (size_t p = 0; p < count; p++) {
-            proofs[p] = (uint8_t *)malloc(out[p].size() ? out[p].size() : 1);
-            if (!proofs[p]) fail(BFD_ERR_ARG, "host allocation failed");
-            memcpy(proofs[p], out[p].data(), out[p].size());
-            proof_lens[p] = out[p].size();
-        }
+                          chs.data(), observe_prefix != 0, roots, proofs, proof_lens);
     });
 }

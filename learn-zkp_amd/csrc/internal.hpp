@@ -9,6 +9,9 @@
 #include <string.h>
 
 #include <array>
+#include <atomic>
+#include <chrono>
+#include <thread>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -135,6 +138,13 @@ struct KernelTimer {
         hip_check(launch(), name);
         hip_check(hipEventRecord(r.b, st), "hipEventRecord");
         pending.push_back(r);
+    }
+    // host-side phase time, reported next to the kernels as "host:<name>"
+    void host(const char *name, double ms) {
+        if (!enabled) return;
+        auto &t = totals[std::string("host:") + name];
+        t.first += ms;
+        t.second += 1;
     }
     // call after a stream synchronisation
     void collect() {
@@ -446,60 +456,105 @@ struct OpenLayout {
 
 inline size_t layer_off(size_t leaves, int t) { return 2 * leaves - 2 * (leaves >> t); }
 
-// Absolute addresses of every opened value, in proof order (basefold.rs:205-222):
-// per query the codeword row and its L siblings, then per round the pi
-// sibling value and its b+i siblings.
-inline void plan_gathers(const OpenRounds &o, const OpenLayout &lay, std::vector<uint64_t> &elems,
-                         std::vector<uint64_t> &digs) {
+// Records one proof opens: per query the codeword row and L main-tree
+// siblings, then per round a pi sibling value and b+i pi-tree siblings.
+inline void gather_counts(const OpenRounds &o, size_t &n_elems, size_t &n_digs) {
+    size_t per_d = (size_t)o.L;
+    for (int k = 0; k < o.d; k++) per_d += (size_t)(o.b + (o.d - 1 - k));
+    n_elems = o.q * (1 + (size_t)o.d);
+    n_digs = o.q * per_d;
+}
+
+// Absolute addresses of every opened value, in proof order (basefold.rs:205-222).
+inline void plan_gathers(const OpenRounds &o, const OpenLayout &lay, uint64_t *elems, uint64_t *digs) {
     for (size_t t = 0; t < o.q; t++) {
         const uint64_t id = o.idx[t];
-        elems.push_back((uint64_t)(lay.cw + id));
-        for (int u = 0; u < o.L; u++)
-            digs.push_back((uint64_t)(lay.tree + 4 * (layer_off(o.M, u) + ((id >> u) ^ 1))));
+        *elems++ = (uint64_t)(lay.cw + id);
+        for (int u = 0; u < o.L; u++) *digs++ = (uint64_t)(lay.tree + 4 * (layer_off(o.M, u) + ((id >> u) ^ 1)));
         for (int k = 0; k < o.d; k++) {
             const int i = o.d - 1 - k;
             const size_t ni = (size_t)1 << (o.b + i);
             const size_t sib = ((id / ni) & 1) ^ 1, row = id % ni;
-            elems.push_back((uint64_t)(lay.pi(k) + row + sib * ni));
-            for (int u = 0; u < o.b + i; u++)
-                digs.push_back((uint64_t)(lay.pitree(k) + 4 * (layer_off(ni, u) + ((row >> u) ^ 1))));
+            *elems++ = (uint64_t)(lay.pi(k) + row + sib * ni);
+            const uint64_t *pt = lay.pitree(k);
+            for (int u = 0; u < o.b + i; u++) *digs++ = (uint64_t)(pt + 4 * (layer_off(ni, u) + ((row >> u) ^ 1)));
         }
     }
 }
 
-// bincode BasefoldProof from the gathered records (same order as plan_gathers)
-inline void assemble_proof(const OpenRounds &o, const uint8_t *elems, const uint8_t *digs,
-                           std::vector<uint8_t> &proof) {
-    proof.clear();
-    proof.reserve(64 + (size_t)o.d * 72 + o.q * (40 + 32 * (size_t)o.L + (size_t)o.d * (24 + 32 * (size_t)o.L)));
-    put_u64(proof, (uint64_t)o.d);
+// Exact bincode size of BasefoldProof for d rounds, L = b + d, q queries.
+inline size_t proof_size(int b, int d, size_t q) {
+    const size_t L = (size_t)(b + d);
+    size_t per_q = 8 + 16 + 8 + 32 * L + 8;
+    for (int k = 0; k < d; k++) per_q += 16 + 8 + 32 * (size_t)(b + (d - 1 - k));
+    return 8 + 40 * (size_t)d + 8 + 16 + 8 + 32 * (size_t)d + 8 + q * per_q;
+}
+
+struct Writer {
+    uint8_t *p;
+    void u64(uint64_t v) { memcpy(p, &v, 8); p += 8; }
+    void elem(gf v) { memcpy(p, &v.lo, 8); memcpy(p + 8, &v.hi, 8); p += 16; }
+    void bytes(const void *src, size_t n) { memcpy(p, src, n); p += n; }
+};
+
+// bincode BasefoldProof (basefold.rs:49-66) from the gathered records (same
+// order as plan_gathers) into `out`, which holds proof_size(b, d, q) bytes.
+inline void assemble_proof(const OpenRounds &o, const uint8_t *elems, const uint8_t *digs, uint8_t *out) {
+    Writer w{out};
+    w.u64((uint64_t)o.d);
     for (int k = 0; k < o.d; k++) {
-        put_u64(proof, 2);
-        put_elem(proof, o.comp[k][0]);
-        put_elem(proof, o.comp[k][1]);
+        w.u64(2);
+        w.elem(o.comp[k][0]);
+        w.elem(o.comp[k][1]);
     }
-    put_u64(proof, 1);
-    put_elem(proof, o.final_poly);
-    put_u64(proof, (uint64_t)o.d);
-    for (int k = 0; k < o.d; k++) put_bytes(proof, o.pi_roots[k].data(), 32);
-    put_u64(proof, (uint64_t)o.q);
+    w.u64(1);
+    w.elem(o.final_poly);
+    w.u64((uint64_t)o.d);
+    for (int k = 0; k < o.d; k++) w.bytes(o.pi_roots[k].data(), 32);
+    w.u64((uint64_t)o.q);
     for (size_t t = 0; t < o.q; t++) {
-        put_u64(proof, 1);
-        put_bytes(proof, elems, 16);
+        w.u64(1);
+        w.bytes(elems, 16);
         elems += 16;
-        put_u64(proof, (uint64_t)o.L);
-        put_bytes(proof, digs, 32 * (size_t)o.L);
+        w.u64((uint64_t)o.L);
+        w.bytes(digs, 32 * (size_t)o.L);
         digs += 32 * (size_t)o.L;
-        put_u64(proof, (uint64_t)o.d);
+        w.u64((uint64_t)o.d);
         for (int k = 0; k < o.d; k++) {
             const int i = o.d - 1 - k;
-            put_bytes(proof, elems, 16);
+            w.bytes(elems, 16);
             elems += 16;
-            put_u64(proof, (uint64_t)(o.b + i));
-            put_bytes(proof, digs, 32 * (size_t)(o.b + i));
+            w.u64((uint64_t)(o.b + i));
+            w.bytes(digs, 32 * (size_t)(o.b + i));
             digs += 32 * (size_t)(o.b + i);
         }
     }
+}
+
+inline double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Runs f(i) for i in [0, n) on up to `threads` host threads.
+template <class F>
+void parallel_for(size_t n, F &&f, unsigned threads = 0) {
+    if (!threads) {
+        threads = std::thread::hardware_concurrency();
+        if (threads > 16) threads = 16;
+        if (threads < 1) threads = 1;
+    }
+    if (n <= 1 || threads == 1) {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    const unsigned t = (unsigned)std::min<size_t>(threads, n);
+    for (unsigned k = 0; k < t; k++)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto &th : pool) th.join();
 }
 
 }  // namespace rt

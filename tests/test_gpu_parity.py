@@ -165,9 +165,9 @@ def test_batch_equals_plain_and_oracle(bfd, oracle, ctx):
         for i, n in enumerate(sizes):
             root_o, claim_o, proof_o = oracle.basefold_prove(128, log2_c, evals[i], points[i])
             assert roots[i] == root_o, (log2_c, i, n)
-            assert proofs[i] == proof_o, (log2_c, i, n)
+            assert proofs[i].tobytes() == proof_o, (log2_c, i, n)
             root_p, _, proof_p = gpu_prove(bfd, ctx, 128, log2_c, evals[i], points[i])
-            assert proof_p == proofs[i]
+            assert proof_p == proofs[i].tobytes()
 
 
 def test_batch_uneven_mix_verifies(bfd, oracle, ctx):
@@ -182,6 +182,6 @@ def test_batch_uneven_mix_verifies(bfd, oracle, ctx):
     chs = [bfd.GenericChallenger.keccak256() for _ in sizes]
     roots, proofs = bfd.commit_open_batch(128, 1, evals, points, claims, chs, True, ctx=ctx)
     for i, n in enumerate(sizes):
-        assert oracle.basefold_verify(128, 1, roots[i], points[i], claims[i], proofs[i]) == 0, (i, n)
+        assert oracle.basefold_verify(128, 1, roots[i], points[i], claims[i], proofs[i].tobytes()) == 0, (i, n)
         root_p, _, proof_p = gpu_prove(bfd, ctx, 128, 1, evals[i], points[i])
-        assert root_p == roots[i] and proof_p == proofs[i], (i, n)
+        assert root_p == roots[i] and proof_p == proofs[i].tobytes(), (i, n)

@@ -44,3 +44,13 @@ def test_errors_without_handles(bfd):
     h = ctypes.c_void_p()
     rc = L.bfd_code_create(None, 128, 1, 4, ctypes.byref(h))
     assert rc == 1 and b"null" in L.bfd_last_error()
+
+
+def test_proof_size_matches_oracle(bfd, oracle):
+    for log2_c in (0, 1, 2):
+        for n in (0, 1, 5, 9):
+            e, p = oracle.random_vec(1, 1 << n), oracle.random_vec(2, n)
+            _, _, proof = oracle.basefold_prove(128, log2_c, e, p)
+            assert bfd.proof_size(128, log2_c, n) == len(proof)
+    assert bfd.proof_size(128, 1, 16) == 1645080
+    assert bfd.proof_size(128, 2, 24) == 2235000
