@@ -45,6 +45,18 @@ def algorithmic_bytes(n: int, log2_c: int) -> dict:
     return {"commit": commit, "open": opn, "total": commit + opn}
 
 
+def traffic_ratio(kernel: str):
+    path = os.path.join(ROOT, "profiles", "basefold_2e24_rate4_kernels.md")
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_ratio" and parts[1] == kernel:
+                return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(args) -> dict:
     """The CPU restatement (oracle/) with OpenMP on this host's cores, on a
     bounded sample: commit+open of 2^SAMPLE evaluations at the same rate."""
@@ -203,8 +215,24 @@ def main() -> None:
     else:
         alg_bytes = None
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if alg_bytes else None
+    # HBM traffic per launch: algorithmic bytes per launch times the measured
+    # (FETCH_SIZE + WRITE_SIZE) / algorithmic ratio of the committed rocprof
+    # summary for this kernel, when one exists
     traffic = None
+    ratio = traffic_ratio(name)
+    if alg_bytes and ratio:
+        traffic = ratio * alg_bytes / launches
     ab = algorithmic_bytes(n, LOG2_C)
+    # integer-VALU position of the dominant Keccak kernel: permutations per
+    # second inside it against an isolated, memory-free Keccak-f kernel
+    perms_per_step = {"k_nodes": per_step_units["k_nodes"], "k_leaves_elem": M,
+                      "k_leaves_pair": per_step_units["k_leaves_pair"]}.get(name)
+    valu = None
+    if perms_per_step:
+        ctx.microbench(2, 1 << 20, 2)
+        peak = (1 << 20) * 16 / (ctx.microbench(2, 1 << 20, 16) * 1e-3)
+        rate = perms_per_step * args.steps / (kms * 1e-3)
+        valu = {"keccak_perm_per_s": rate, "isolated_keccak_perm_per_s": peak, "frac": rate / peak}
 
     result = {
         "metric": "Basefold commit+open field-elems/s (2^24 evals, rate 1/4)",
@@ -225,6 +253,7 @@ def main() -> None:
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "avg_launch_ms": kms / launches, "launches_per_step": launches / args.steps},
+        "valu": valu,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(ktimes.items())},
         "proof_bytes": len(proof),
     }
