@@ -133,6 +133,64 @@ def batched_vs_plain(bfd, ctx, torch, reps: int) -> dict:
             "batched_elems_per_s": total / t_b, "speedup": t_p / t_b, "proofs_identical": True}
 
 
+def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int) -> dict:
+    """Config 5: Poisson arrivals (mean `rate` req/s, numpy seed 0) of
+    commit-then-open requests, log2 size uniform over {14..24}, rate 1/2 or 1/4
+    chosen 50/50, evaluations from a seeded pool resident in HBM (2 per size),
+    random points; the server admits by HBM budget and batches continuously.
+    Latency = submit to proof ready."""
+    import threading
+    rng = np.random.default_rng(0)
+    gaps = rng.exponential(1.0 / rate, size=n_req)
+    sizes = rng.integers(14, 25, size=n_req)
+    rates = rng.choice([1, 2], size=n_req)
+    pool = {}
+    for n in range(14, 25):
+        for j in range(2):
+            e = bfd.random_vec(100 * n + j, 1 << n)
+            pool[(n, j)] = torch.from_numpy(e.view(np.int64).reshape(-1)).to("cuda:0")
+    points = [bfd.random_vec(1_000_000 + i, int(n)) for i, n in enumerate(sizes)]
+    torch.cuda.synchronize()
+    srv = bfd.Server(hbm_budget=budget, ctx=ctx)
+    # warm-up: one request per size builds every code table
+    warm = [srv.submit(pool[(n, 0)].data_ptr(), n, bfd.random_vec(7, n), c) for n in range(14, 25) for c in (1, 2)]
+    for t in warm:
+        srv.wait(t)
+    tickets = [None] * n_req
+    t0 = time.perf_counter()
+
+    def client():
+        at = 0.0
+        for i in range(n_req):
+            at += gaps[i]
+            d = at - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            n = int(sizes[i])
+            tickets[i] = srv.submit(pool[(n, i % 2)].data_ptr(), n, points[i], int(rates[i]))
+
+    th = threading.Thread(target=client)
+    th.start()
+    lat = []
+    peak_resident = 0
+    for i in range(n_req):
+        while tickets[i] is None:
+            time.sleep(0.0005)
+        lat.append(srv.wait(tickets[i])[3])
+        peak_resident = max(peak_resident, srv.stats()["resident_bytes"])
+    th.join()
+    makespan = time.perf_counter() - t0
+    timing = srv.timing()
+    srv.close()
+    total = int(sum(1 << int(n) for n in sizes))
+    lat = np.array(lat)
+    return {"workload": f"{n_req} requests, Poisson {rate:g}/s, log2 n U{{14..24}}, rate 1/2|1/4 50/50",
+            "budget_bytes": budget, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "elems_per_s": total / makespan, "makespan_s": makespan,
+            "arrival_span_s": float(gaps.sum()), "requests_per_s_served": n_req / makespan,
+            "peak_resident_sampled_bytes": peak_resident, "server_timing": timing}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +201,9 @@ def main() -> None:
     ap.add_argument("--cpu-samples", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain figure")
+    ap.add_argument("--serve-requests", type=int, default=400, help="serving trace length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--batch-reps", type=int, default=2)
     ap.add_argument("--copy-ref", action="store_true",
                     help="run one k_copy16 over 2^24 elements first (HBM byte-counter calibration)")
@@ -259,6 +320,9 @@ def main() -> None:
     }
     if not args.no_batch:
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps)
+    if args.serve_requests > 0:
+        result["serving"] = serving_trace(bfd, ctx, torch, args.serve_requests, args.serve_rate,
+                                          int(args.serve_budget_gb * 1e9))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result))

@@ -115,6 +115,32 @@ int bfd_commit_open_batch(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t
                           bfd_challenger *const *challengers, int observe_prefix, uint8_t *roots,
                           uint8_t **proofs, size_t *proof_lens);
 
+/* ---- serving: continuous batching of commit-then-open requests ----
+ * A worker thread runs commit ticks and open-round ticks on two streams; a
+ * request is admitted when its resident footprint fits hbm_budget_bytes
+ * (0 = unlimited) and at most max_active (0 = unlimited) are in flight.
+ * Each request: evaluations already in device memory (kept valid until its
+ * proof is collected), n point coordinates; the claim is evaluated on the
+ * device and a fresh challenger observes root and claim before opening, so
+ * the proof equals a plain commit + eval + open of the same polynomial. */
+typedef struct bfd_server bfd_server;
+int bfd_server_create(bfd_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t max_active, bfd_server **out);
+/* BFD_ERR_BUDGET if the request alone exceeds the budget */
+int bfd_server_submit(bfd_server *srv, uint32_t lambda, uint32_t log2_c, uint32_t n, const void *d_evals,
+                      const uint8_t *point, uint64_t *ticket);
+/* Waits up to timeout_ms (< 0: forever; BFD_ERR_STATE on timeout) and hands
+ * over the result: root, claim, proof (free with bfd_proof_free) and the
+ * submit-to-done latency. */
+int bfd_server_wait(bfd_server *srv, uint64_t ticket, int64_t timeout_ms, uint8_t root[32], uint8_t claim[16],
+                    uint8_t **proof, size_t *proof_len, double *latency_ms);
+int bfd_server_stats(bfd_server *srv, uint64_t *pending, uint64_t *active, uint64_t *resident_bytes,
+                     uint64_t *completed);
+/* out: commit ticks, open ticks, commit-lane busy ms, open-lane busy ms,
+ * host ms in tick finishes, host ms enqueueing ticks, job-rounds in open ticks */
+int bfd_server_timing(bfd_server *srv, double out[8]);
+/* finishes every admitted and pending request, then stops */
+int bfd_server_destroy(bfd_server *srv);
+
 /* ---- degree-3 product sumcheck  sum_b f(b) g(b) eq(r, b)  (no reference
  * counterpart; driver semantics of prove_sumcheck, sumcheck/src/lib.rs:56-103):
  * observes the claim, then per round observes [c0, c2, c3] and samples the

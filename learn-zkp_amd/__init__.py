@@ -51,6 +51,13 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise BfdError(-1, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # PyTorch ships its own libamdhip64.so.7.  Load it first when it is
+        # installed so this library binds to the same HIP runtime: two
+        # runtimes in one process cannot both own the GPU.
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         L = ctypes.CDLL(str(LIB_PATH))
         L.bfd_last_error.restype = ctypes.c_char_p
         L.bfd_ctx_resident_bytes.restype = ctypes.c_uint64
@@ -91,6 +98,16 @@ def lib():
                                             ctypes.c_int, ctypes.POINTER(u8p), ctypes.POINTER(u8p),
                                             ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, u8p,
                                             ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+        L.bfd_server_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        L.bfd_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_void_p, u8p, ctypes.POINTER(ctypes.c_uint64)]
+        L.bfd_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, u8p, u8p,
+                                      ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t),
+                                      ctypes.POINTER(ctypes.c_double)]
+        L.bfd_server_stats.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_uint64)] * 4
+        L.bfd_server_destroy.argtypes = [ctypes.c_void_p]
+        L.bfd_server_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.bfd_microbench.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]
         L.bfd_ctx_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
@@ -355,6 +372,64 @@ def commit_open_batch(lambda_: int, log2_c: int, evals: Sequence, points: Sequen
 def proof_size(lambda_: int, log2_c: int, d: int) -> int:
     """Exact bincode size of a Basefold proof."""
     return int(lib().bfd_proof_size(lambda_, log2_c, d))
+
+
+class Server:
+    """Continuous-batching commit+open service (bfd_server_*): submit requests
+    whose evaluations live in device memory; collect root, claim, proof and
+    the submit-to-done latency with wait()."""
+
+    def __init__(self, hbm_budget: int = 0, max_active: int = 0, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        _check(lib().bfd_server_create(self.ctx.h, hbm_budget, max_active, ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, device_ptr: int, n: int, point, log2_c: int, lambda_: int = 128, keep=None) -> int:
+        point = _elems(point) if n else np.zeros((1, 2), dtype=np.uint64)
+        t = ctypes.c_uint64()
+        _check(lib().bfd_server_submit(self.h, lambda_, log2_c, n, ctypes.c_void_p(device_ptr), _p(point),
+                                       ctypes.byref(t)))
+        self._keep[t.value] = keep
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        root = (ctypes.c_uint8 * 32)()
+        claim = np.zeros((1, 2), dtype=np.uint64)
+        proof = u8p()
+        plen = ctypes.c_size_t()
+        lat = ctypes.c_double()
+        _check(lib().bfd_server_wait(self.h, ticket, timeout_ms, root, _p(claim), ctypes.byref(proof),
+                                     ctypes.byref(plen), ctypes.byref(lat)))
+        self._keep.pop(ticket, None)
+        try:
+            data = ctypes.string_at(proof, plen.value)
+        finally:
+            lib().bfd_proof_free(proof)
+        return bytes(root), claim, data, lat.value
+
+    def stats(self) -> dict:
+        v = [ctypes.c_uint64() for _ in range(4)]
+        _check(lib().bfd_server_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["pending", "active", "resident_bytes", "completed"], [x.value for x in v]))
+
+    def timing(self) -> dict:
+        v = (ctypes.c_double * 8)()
+        _check(lib().bfd_server_timing(self.h, v))
+        return dict(zip(["commit_ticks", "open_ticks", "commit_busy_ms", "open_busy_ms", "finish_ms",
+                         "enqueue_ms", "open_job_rounds"], list(v)[:7]))
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bfd_server_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional[Context] = None):

@@ -187,6 +187,7 @@ int bfd_ctx_destroy(bfd_ctx *ctx) {
     return guarded([&] {
         if (!ctx) return;
         (void)hipSetDevice(ctx->device);
+        ctx->lane.reset();
         for (auto &kv : ctx->code_cache) delete kv.second;
         ctx->code_cache.clear();
         (void)hipStreamSynchronize(ctx->stream);

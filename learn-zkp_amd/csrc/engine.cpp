@@ -1,0 +1,480 @@
+// Continuous-batching engine (see engine.hpp).
+#include "engine.hpp"
+
+namespace bfd {
+namespace rt {
+
+namespace {
+
+size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+Seg make_seg(const void *in, void *out, const void *aux, uint64_t count, uint64_t arg, gf x = gf_zero()) {
+    Seg g;
+    g.in = in;
+    g.out = out;
+    g.aux = aux;
+    g.count = count;
+    g.arg = arg;
+    g.x = x;
+    g.blk0 = g.nblk = 0;
+    return g;
+}
+
+// grows a stream-ordered device buffer (contents are not preserved)
+void ensure(DevBuf &b, size_t bytes, hipStream_t st) {
+    if (b.bytes >= bytes) return;
+    b = DevBuf(bytes + bytes / 4 + 4096, st);
+}
+
+}  // namespace
+
+size_t job_bytes(const Job &j) {
+    const size_t c = (size_t)1 << j.b;
+    return align_up(16 * j.M) + align_up(32 * (2 * j.M - 1)) + align_up(16 * ((size_t)1 << (j.n - 1))) +
+           align_up(16 * (j.N / 2)) + align_up(16 * j.lay.pis_elems(j.M, c)) + align_up(32 * j.lay.pitree_digests());
+}
+
+void job_prepare(bfd_ctx *ctx, Job &j, uint32_t lambda, uint32_t log2_c, int n, const void *d_evals,
+                 const uint8_t *point, const uint8_t *claim) {
+    if (n < 1 || n > 30) fail(BFD_ERR_ARG, "batched and served polynomials need 1..30 variables");
+    if (log2_c > 8) fail(BFD_ERR_ARG, "log2_c too large");
+    j.n = n;
+    j.b = (int)log2_c;
+    j.L = n + j.b;
+    j.N = (size_t)1 << n;
+    j.M = j.N << j.b;
+    j.code = cached_code(ctx, lambda, log2_c, (uint32_t)n);
+    j.data = (const gf *)d_evals;
+    j.point.assign(point, point + 16 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        gf r = load_elem(point + 16 * i);
+        if (gf_eq(r, gf_zero()) || gf_eq(r, gf_one()))
+            fail(BFD_ERR_ARG, "point coordinates 0 and 1 are not invertible in the Eval prover");
+    }
+    j.have_claim = claim != nullptr;
+    if (claim) j.claim = load_elem(claim);
+    j.lay.plan(j.M, n);
+    j.bytes = job_bytes(j);
+}
+
+Arena::Arena(size_t bytes) : size_(bytes) {
+    hip_check(hipMalloc((void **)&base_, bytes), "hipMalloc arena");
+    free_[0] = bytes;
+}
+Arena::~Arena() {
+    if (base_) (void)hipFree(base_);
+}
+bool Arena::alloc(size_t bytes, size_t &off) {
+    bytes = align_up(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        off = it->first;
+        const size_t rest = it->second - bytes;
+        free_.erase(it);
+        if (rest) free_[off + bytes] = rest;
+        return true;
+    }
+    return false;
+}
+void Arena::free(size_t off, size_t bytes) {
+    bytes = align_up(bytes);
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+// Carves a job's buffers out of `j.base` when set, else out of one
+// allocation of its own on `st`.
+static void job_alloc(Job &j, hipStream_t st) {
+    const size_t c = (size_t)1 << j.b;
+    uint8_t *cur = j.base;
+    if (!cur) {
+        j.slab = DevBuf(j.bytes, st);
+        cur = j.slab.as<uint8_t>();
+    }
+    auto take = [&](size_t n) {
+        uint8_t *r = cur;
+        cur += align_up(n);
+        return r;
+    };
+    j.cw = (gf *)take(16 * j.M);
+    j.tree = (uint64_t *)take(32 * (2 * j.M - 1));
+    j.half_eq = (gf *)take(16 * ((size_t)1 << (j.n - 1)));
+    j.poly = (gf *)take(16 * (j.N / 2));
+    j.pis = (gf *)take(16 * j.lay.pis_elems(j.M, c));
+    j.pitrees = (uint64_t *)take(32 * j.lay.pitree_digests());
+    j.lay.cw = j.cw;
+    j.lay.tree = j.tree;
+    j.lay.pis = j.pis;
+    j.lay.pitrees = j.pitrees;
+}
+
+void LaneDeleter::operator()(Lane *l) const { delete l; }
+
+void Lane::reset_ring() { seg_used_ = 0; }
+
+uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial, uint32_t max_blocks_per_seg) {
+    if (segs.empty()) return 0;
+    if (seg_used_ + segs.size() > seg_cap_) {
+        // the tick outgrew its staging ring: drain the stream, then start over
+        hip_check(hipStreamSynchronize(st_), "segment ring sync");
+        seg_used_ = 0;
+        if (segs.size() > seg_cap_) {
+            seg_cap_ = 2 * segs.size() + 256;
+            seg_pin_.ensure(sizeof(Seg) * seg_cap_);
+            seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
+        }
+    }
+    const int bs = launch::seg_block_size(k);
+    uint32_t blocks = 0;
+    for (auto &g : segs) {
+        uint64_t want = (g.count + bs - 1) / bs;
+        if (want < 1) want = 1;
+        if (want > max_blocks_per_seg) want = max_blocks_per_seg;
+        g.blk0 = blocks;
+        g.nblk = (uint32_t)want;
+        blocks += g.nblk;
+    }
+    Seg *h = (Seg *)seg_pin_.p + seg_used_;
+    memcpy(h, segs.data(), sizeof(Seg) * segs.size());
+    Seg *d = seg_dev_.as<Seg>() + seg_used_;
+    hip_check(hipMemcpyAsync(d, h, sizeof(Seg) * segs.size(), hipMemcpyHostToDevice, st_), "copy segs");
+    seg_used_ += segs.size();
+    auto go = [&] { return launch::seg_launch(st_, k, d, (int)segs.size(), blocks, partial); };
+    if (timer_) timer_->run(st_, name, go);
+    else hip_check(go(), name);
+    return blocks;
+}
+
+// ---------------------------------------------------------------------------
+// commit tick: encode, Merkle tree and (when no claim was given) the claim
+// ---------------------------------------------------------------------------
+void Lane::commit_tick(std::vector<Job *> &jobs) {
+    if (jobs.empty()) return;
+    reset_ring();
+    const size_t need = 160 * jobs.size() + 256;
+    if (seg_cap_ < need) {
+        seg_cap_ = need;
+        seg_pin_.ensure(sizeof(Seg) * seg_cap_);
+        seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
+    }
+    int max_n = 0, max_L = 0;
+    for (size_t a = 0; a < jobs.size(); a++) {
+        Job &j = *jobs[a];
+        if (!j.cw) job_alloc(j, st_);
+        j.slot = a;
+        max_n = std::max(max_n, j.n);
+        max_L = std::max(max_L, j.L);
+    }
+    auto &segs = segs_;
+    for (int s = 0; s < max_n; s++) {
+        segs.clear();
+        for (Job *jp : jobs) {
+            Job &j = *jp;
+            if (j.n <= s) continue;
+            const gf *ft = j.code->ft.as<gf>() + j.code->ft_off[s];
+            const size_t work = (j.N / 2) << j.b;
+            if (s == 0) segs.push_back(make_seg(j.data, j.cw, ft, work, (uint64_t)j.b));
+            else segs.push_back(make_seg(nullptr, j.cw, ft, work, (uint64_t)(s + j.b)));
+        }
+        seg(s == 0 ? SegKernel::NttFirst : SegKernel::NttLayer, segs, s == 0 ? "ks_ntt_first" : "ks_ntt_layer");
+    }
+    segs.clear();
+    for (Job *jp : jobs) segs.push_back(make_seg(jp->cw, jp->tree, nullptr, jp->M, 0));
+    seg(SegKernel::LeavesElem, segs, "ks_leaves_elem");
+    for (int t = 0; t < max_L; t++) {
+        segs.clear();
+        for (Job *jp : jobs) {
+            Job &j = *jp;
+            if (t < j.L)
+                segs.push_back(make_seg(j.tree + 4 * layer_off(j.M, t), j.tree + 4 * layer_off(j.M, t + 1), nullptr,
+                                        j.M >> (t + 1), 0));
+        }
+        seg(SegKernel::Nodes, segs, "ks_nodes");
+    }
+    // claim = MLE(evals)(point) by fixing the top variable repeatedly
+    // (multilinear.rs:200-214), in the job's poly buffer
+    for (int lvl = 0; lvl < max_n; lvl++) {
+        segs.clear();
+        for (Job *jp : jobs) {
+            Job &j = *jp;
+            if (j.have_claim || lvl >= j.n) continue;
+            const gf x = load_elem(j.point.data() + 16 * (j.n - 1 - lvl));
+            segs.push_back(make_seg(lvl == 0 ? j.data : j.poly, j.poly, nullptr, j.N >> (lvl + 1), 0, x));
+        }
+        seg(SegKernel::Fix, segs, "ks_fix");
+    }
+    // read back roots (and claims)
+    const size_t m = jobs.size();
+    std::vector<uint64_t> addr(2 * m);
+    for (size_t a = 0; a < m; a++) {
+        const Job &j = *jobs[a];
+        addr[a] = (uint64_t)(j.tree + 4 * (2 * j.M - 2));
+        addr[m + a] = (uint64_t)(j.have_claim ? j.data : j.poly);
+    }
+    ensure(addr_dev_, 16 * m, st_);
+    ensure(rb_dev_, 64 * m, st_);
+    rb_pin_.ensure(64 * m + 64);
+    // the address list lives in pinned memory until the copy has run
+    uint64_t *pa = (uint64_t *)((uint8_t *)rb_pin_.p + 48 * m);
+    memcpy(pa, addr.data(), 16 * m);
+    hip_check(hipMemcpyAsync(addr_dev_.p, pa, 16 * m, hipMemcpyHostToDevice, st_), "copy addr");
+    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>(), rb_dev_.p, m, 32), "gather roots");
+    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>() + m, rb_dev_.as<uint8_t>() + 32 * m, m, 16),
+              "gather claims");
+    hip_check(hipMemcpyAsync(rb_pin_.p, rb_dev_.p, 48 * m, hipMemcpyDeviceToHost, st_), "copy roots");
+}
+
+void Lane::commit_finish(std::vector<Job *> &jobs) {
+    const size_t m = jobs.size();
+    const uint8_t *rb = (const uint8_t *)rb_pin_.p;
+    parallel_for(m, [&](size_t a) {
+        Job &j = *jobs[a];
+        memcpy(j.root, rb + 32 * a, 32);
+        if (!j.have_claim) j.claim = load_elem(rb + 32 * m + 16 * a);
+        uint8_t cb[16];
+        store_elem(cb, j.claim);
+        j.o.init(j.n, j.b, j.code->q, j.point.data(), cb);
+        const int hn = j.o.half_eq_vars();
+        j.eq_lo = (hn + 1) / 2;
+        j.eqA = host_eq(j.o.r.data(), j.eq_lo);
+        j.eqB = host_eq(j.o.r.data() + j.eq_lo, hn - j.eq_lo);
+        if (j.observe_prefix) {                // poly-commit/src/lib.rs:77, 87-89
+            j.ch->observe(j.root, 32);
+            j.ch->observe_elem(j.claim);
+        }
+        j.stage = Job::Open;
+        j.k = 0;
+        j.poly_cur = j.data;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// open tick: every job in the Open stage advances one round; jobs in the
+// Gather stage gather their openings
+// ---------------------------------------------------------------------------
+void Lane::open_tick(std::vector<Job *> &jobs) {
+    if (jobs.empty()) return;
+    reset_ring();
+    const size_t need = 64 * jobs.size() + 256;
+    if (seg_cap_ < need) {
+        seg_cap_ = need;
+        seg_pin_.ensure(sizeof(Seg) * seg_cap_);
+        seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
+    }
+    auto &segs = segs_;
+    const size_t m = jobs.size();
+    for (size_t a = 0; a < m; a++) jobs[a]->slot = a;
+
+    // (A) halved eq tables of jobs entering their first round
+    {
+        size_t elems = 0;
+        for (Job *jp : jobs)
+            if (jp->stage == Job::Open && jp->k == 0) elems += jp->eqA.size() + jp->eqB.size();
+        if (elems) {
+            ensure(eq_dev_, 16 * elems, st_);
+            std::vector<gf> h;
+            h.reserve(elems);
+            segs.clear();
+            for (Job *jp : jobs) {
+                Job &j = *jp;
+                if (j.stage != Job::Open || j.k != 0) continue;
+                const gf *A = eq_dev_.as<gf>() + h.size();
+                h.insert(h.end(), j.eqA.begin(), j.eqA.end());
+                const gf *B = eq_dev_.as<gf>() + h.size();
+                h.insert(h.end(), j.eqB.begin(), j.eqB.end());
+                segs.push_back(make_seg(A, j.half_eq, B, (size_t)1 << j.o.half_eq_vars(), (uint64_t)j.eq_lo));
+            }
+            stage_pin_.ensure(16 * elems);
+            memcpy(stage_pin_.p, h.data(), 16 * elems);
+            hip_check(hipMemcpyAsync(eq_dev_.p, stage_pin_.p, 16 * elems, hipMemcpyHostToDevice, st_), "copy eq halves");
+            seg(SegKernel::Outer, segs, "ks_outer");
+        }
+    }
+    // (B) fold pi and fix poly with the previous round's challenge
+    segs.clear();
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        if (j.stage != Job::Open || j.k == 0) continue;
+        const int kp = j.k - 1, i = j.n - 1 - kp;
+        const size_t half = (j.M >> kp) / 2;
+        segs.push_back(make_seg(j.lay.pi(kp), (void *)j.lay.pi(kp + 1), j.code->ft.as<gf>() + j.code->ft_off[i], half,
+                                0, j.x_prev));
+    }
+    seg(SegKernel::Fold, segs, "ks_fold");
+    segs.clear();
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        if (j.stage != Job::Open || j.k == 0) continue;
+        const int i = j.n - j.k;
+        segs.push_back(make_seg(j.poly_cur, j.poly, nullptr, (size_t)1 << i, 0, j.x_prev));
+        j.poly_cur = j.poly;
+    }
+    seg(SegKernel::Fix, segs, "ks_fix");
+    // (C) this round's pi tree and inner product
+    segs.clear();
+    int max_levels = 0;
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        if (j.stage != Job::Open || j.k >= j.n) continue;
+        const size_t half = (j.M >> j.k) / 2;
+        segs.push_back(make_seg(j.lay.pi(j.k), (void *)j.lay.pitree(j.k), nullptr, half, half));
+        max_levels = std::max(max_levels, j.L - j.k - 1);
+    }
+    seg(SegKernel::LeavesPair, segs, "ks_leaves_pair");
+    for (int t = 0; t < max_levels; t++) {
+        segs.clear();
+        for (Job *jp : jobs) {
+            Job &j = *jp;
+            if (j.stage != Job::Open || j.k >= j.n) continue;
+            const size_t half = (j.M >> j.k) / 2;
+            if ((half >> t) > 1) {
+                uint64_t *tk = (uint64_t *)j.lay.pitree(j.k);
+                segs.push_back(make_seg(tk + 4 * layer_off(half, t), tk + 4 * layer_off(half, t + 1), nullptr,
+                                        half >> (t + 1), 0));
+            }
+        }
+        seg(SegKernel::Nodes, segs, "ks_nodes");
+    }
+    segs.clear();
+    std::vector<Job *> dotted;
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        if (j.stage != Job::Open || j.k >= j.n) continue;
+        const int i = j.n - 1 - j.k;
+        segs.push_back(make_seg(j.half_eq, nullptr, j.poly_cur, (size_t)1 << i, 0));
+        dotted.push_back(jp);
+    }
+    ensure(partials_, 16 * 64 * m + 16, st_);
+    const uint32_t nblk = seg(SegKernel::Dot, segs, "ks_dot", partials_.as<gf>(), 64);
+    for (size_t a = 0; a < dotted.size(); a++) {
+        dotted[a]->dot_blk0 = segs[a].blk0;
+        dotted[a]->dot_nblk = segs[a].nblk;
+    }
+    // (D) small read-backs: this round's pi root, or the final polynomial
+    std::vector<uint64_t> addr32(m, 0), addr16(m, 0);
+    for (size_t a = 0; a < m; a++) {
+        const Job &j = *jobs[a];
+        if (j.stage != Job::Open) continue;
+        if (j.k < j.n) {
+            const size_t half = (j.M >> j.k) / 2;
+            addr32[a] = (uint64_t)(j.lay.pitree(j.k) + 4 * (2 * half - 2));
+        } else {
+            addr16[a] = (uint64_t)j.poly_cur;
+        }
+    }
+    // unused slots read a harmless address (the job's own root)
+    for (size_t a = 0; a < m; a++) {
+        if (!addr32[a]) addr32[a] = (uint64_t)(jobs[a]->tree + 4 * (2 * jobs[a]->M - 2));
+        if (!addr16[a]) addr16[a] = (uint64_t)jobs[a]->cw;
+    }
+    const size_t rb_bytes = 48 * m + 16 * (size_t)nblk;
+    rb_pin_.ensure(rb_bytes + 16 * m + 64);
+    ensure(addr_dev_, 16 * m, st_);
+    ensure(rb_dev_, 48 * m + 64, st_);
+    uint64_t *pa = (uint64_t *)((uint8_t *)rb_pin_.p + rb_bytes);
+    memcpy(pa, addr32.data(), 8 * m);
+    memcpy(pa + m, addr16.data(), 8 * m);
+    hip_check(hipMemcpyAsync(addr_dev_.p, pa, 16 * m, hipMemcpyHostToDevice, st_), "copy addr");
+    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>(), rb_dev_.p, m, 32), "gather roots");
+    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>() + m, rb_dev_.as<uint8_t>() + 32 * m, m, 16),
+              "gather finals");
+    hip_check(hipMemcpyAsync(rb_pin_.p, rb_dev_.p, 48 * m, hipMemcpyDeviceToHost, st_), "copy roots");
+    if (nblk)
+        hip_check(hipMemcpyAsync((uint8_t *)rb_pin_.p + 48 * m, partials_.p, 16 * (size_t)nblk,
+                                 hipMemcpyDeviceToHost, st_),
+                  "copy partials");
+    // (E) openings of jobs whose queries are known
+    size_t NE = 0, ND = 0;
+    gather_at_.assign(m, {0, 0});
+    for (size_t a = 0; a < m; a++) {
+        Job &j = *jobs[a];
+        if (j.stage != Job::Gather) continue;
+        gather_at_[a] = {NE, ND};
+        NE += j.ne;
+        ND += j.nd;
+    }
+    gather_NE_ = NE;
+    if (NE + ND) {
+        plan_pin_.ensure(8 * (NE + ND));
+        uint64_t *plan = (uint64_t *)plan_pin_.p;
+        parallel_for(m, [&](size_t a) {
+            Job &j = *jobs[a];
+            if (j.stage != Job::Gather) return;
+            plan_gathers(j.o, j.lay, plan + gather_at_[a].first, plan + NE + gather_at_[a].second);
+        });
+        ensure(plan_dev_, 8 * (NE + ND), st_);
+        ensure(gather_dev_, 16 * NE + 32 * ND + 64, st_);
+        gather_pin_.ensure(16 * NE + 32 * ND + 64);
+        hip_check(hipMemcpyAsync(plan_dev_.p, plan, 8 * (NE + ND), hipMemcpyHostToDevice, st_), "copy plan");
+        uint8_t *oe = gather_dev_.as<uint8_t>(), *od = oe + 16 * NE;
+        auto g1 = [&] { return launch::gather_abs(st_, plan_dev_.as<uint64_t>(), oe, NE, 16); };
+        auto g2 = [&] { return launch::gather_abs(st_, plan_dev_.as<uint64_t>() + NE, od, ND, 32); };
+        if (timer_) {
+            timer_->run(st_, "k_gather_abs", g1);
+            timer_->run(st_, "k_gather_abs", g2);
+        } else {
+            hip_check(g1(), "gather");
+            hip_check(g2(), "gather");
+        }
+        hip_check(hipMemcpyAsync(gather_pin_.p, oe, 16 * NE + 32 * ND, hipMemcpyDeviceToHost, st_), "copy gather");
+    }
+}
+
+void Lane::open_finish(std::vector<Job *> &jobs) {
+    const size_t m = jobs.size();
+    const uint8_t *rb = (const uint8_t *)rb_pin_.p;
+    // proofs of jobs that gathered this tick
+    const uint8_t *he = (const uint8_t *)gather_pin_.p, *hd = he + 16 * gather_NE_;
+    std::vector<size_t> gidx;
+    for (size_t a = 0; a < m; a++)
+        if (jobs[a]->stage == Job::Gather) gidx.push_back(a);
+    for (size_t a : gidx) {
+        Job &j = *jobs[a];
+        const size_t sz = proof_size(j.o.b, j.o.d, j.o.q);
+        if (j.proof && j.proof_cap < sz) fail(BFD_ERR_ARG, "proof buffer too small");
+        if (!j.proof) {
+            j.proof = (uint8_t *)malloc(sz ? sz : 1);
+            if (!j.proof) fail(BFD_ERR_ARG, "host allocation failed");
+            j.proof_owned = true;
+        }
+        j.proof_len = sz;
+    }
+    parallel_for(gidx.size(), [&](size_t g) {
+        Job &j = *jobs[gidx[g]];
+        assemble_proof(j.o, he + 16 * gather_at_[gidx[g]].first, hd + 32 * gather_at_[gidx[g]].second, j.proof);
+    });
+    for (size_t a : gidx) {
+        jobs[a]->stage = Job::Done;
+        jobs[a]->t_done = now_ms();
+    }
+    // rounds: transcripts advance (a few field operations each: sequential);
+    // finished jobs sample their queries
+    for (size_t a = 0; a < m; a++) {
+        Job &j = *jobs[a];
+        if (j.stage != Job::Open) continue;
+        if (j.k < j.n) {
+            gf S = gf_zero();
+            for (uint32_t t = 0; t < j.dot_nblk; t++) S = gf_add(S, load_elem(rb + 48 * m + 16 * (j.dot_blk0 + t)));
+            j.x_prev = j.o.round(j.k, rb + 32 * a, S, *j.ch);
+            j.k++;
+        } else {
+            j.o.finish(load_elem(rb + 32 * m + 16 * a), *j.ch);
+            gather_counts(j.o, j.ne, j.nd);
+            j.stage = Job::Gather;
+        }
+    }
+}
+
+}  // namespace rt
+}  // namespace bfd

@@ -10,6 +10,8 @@
 
 #include <array>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <chrono>
 #include <thread>
 #include <map>
@@ -172,6 +174,14 @@ struct KernelTimer {
 // handles
 // ---------------------------------------------------------------------------
 struct bfd_code;
+namespace bfd {
+namespace rt {
+class Lane;
+struct LaneDeleter {
+    void operator()(Lane *l) const;  // engine.cpp
+};
+}  // namespace rt
+}  // namespace bfd
 
 struct bfd_ctx {
     int device = 0;
@@ -186,6 +196,8 @@ struct bfd_ctx {
     bfd::rt::PinnedBuf gather_pin;  // opened values of a proof, device -> host
     bfd::rt::PinnedBuf seg_pin;     // segment tables of batched launches, host -> device
     bfd::rt::KernelTimer timer;
+    // engine lane on `stream` for batched calls (made on first use)
+    std::unique_ptr<bfd::rt::Lane, bfd::rt::LaneDeleter> lane;
 };
 
 struct bfd_code {
@@ -535,26 +547,86 @@ inline double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Runs f(i) for i in [0, n) on up to `threads` host threads.
+// Persistent host worker pool for per-polynomial host work (proof planning
+// and assembly, transcript set-up).  run() hands out indices through an
+// atomic counter; the calling thread works too.  One run at a time.
+class HostPool {
+  public:
+    static HostPool &get() {
+        static HostPool pool;
+        return pool;
+    }
+    template <class F>
+    void run(size_t n, F &&f) {
+        if (n == 0) return;
+        if (n == 1 || workers_.empty()) {
+            for (size_t i = 0; i < n; i++) f(i);
+            return;
+        }
+        std::lock_guard<std::mutex> one(run_mu_);
+        std::function<void(size_t)> fn = f;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_.store(0);
+            busy_ = workers_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        for (size_t i; (i = next_.fetch_add(1)) < n;) fn(i);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return busy_ == 0; });
+        fn_ = nullptr;
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            gen_++;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+
+  private:
+    HostPool() {
+        unsigned t = std::thread::hardware_concurrency();
+        if (t > 16) t = 16;
+        for (unsigned k = 1; k < t; k++) workers_.emplace_back([this] { loop(); });
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            std::function<void(size_t)> *fn;
+            size_t n;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                fn = fn_;
+                n = n_;
+            }
+            for (size_t i; (i = next_.fetch_add(1)) < n;) (*fn)(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--busy_ == 0) done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_, run_mu_;
+    std::condition_variable cv_, done_cv_;
+    std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0, busy_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    std::atomic<size_t> next_{0};
+};
+
+// Runs f(i) for i in [0, n) on the host pool.
 template <class F>
-void parallel_for(size_t n, F &&f, unsigned threads = 0) {
-    if (!threads) {
-        threads = std::thread::hardware_concurrency();
-        if (threads > 16) threads = 16;
-        if (threads < 1) threads = 1;
-    }
-    if (n <= 1 || threads == 1) {
-        for (size_t i = 0; i < n; i++) f(i);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    std::vector<std::thread> pool;
-    const unsigned t = (unsigned)std::min<size_t>(threads, n);
-    for (unsigned k = 0; k < t; k++)
-        pool.emplace_back([&] {
-            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-        });
-    for (auto &th : pool) th.join();
+void parallel_for(size_t n, F &&f) {
+    HostPool::get().run(n, std::forward<F>(f));
 }
 
 }  // namespace rt

@@ -185,3 +185,50 @@ def test_batch_uneven_mix_verifies(bfd, oracle, ctx):
         assert oracle.basefold_verify(128, 1, roots[i], points[i], claims[i], proofs[i].tobytes()) == 0, (i, n)
         root_p, _, proof_p = gpu_prove(bfd, ctx, 128, 1, evals[i], points[i])
         assert root_p == roots[i] and proof_p == proofs[i].tobytes(), (i, n)
+
+
+def test_server_mix_equals_alone(bfd, oracle, ctx):
+    """Continuous batching: requests of mixed sizes and rates submitted
+    together (so their rounds share ticks) give the same root, claim and proof
+    as each request alone on the plain path, and the oracle's."""
+    import torch
+    rng = np.random.default_rng(11)
+    reqs = [(int(n), int(c)) for n, c in zip(rng.integers(1, 13, size=16), rng.choice([1, 2], size=16))]
+    _say(f"server requests {reqs}")
+    evals = [bfd.random_vec(5000 + i, 1 << n) for i, (n, c) in enumerate(reqs)]
+    points = [bfd.random_vec(6000 + i, n) for i, (n, c) in enumerate(reqs)]
+    dev = [torch.from_numpy(e.view(np.int64).reshape(-1)).to("cuda:0") for e in evals]
+    torch.cuda.synchronize()
+    srv = bfd.Server(hbm_budget=0, ctx=ctx)
+    tickets = [srv.submit(d.data_ptr(), n, p, c, keep=d) for d, p, (n, c) in zip(dev, points, reqs)]
+    for i, t in enumerate(tickets):
+        n, c = reqs[i]
+        root, claim, proof, lat = srv.wait(t, 120000)
+        root_o, claim_o, proof_o = oracle.basefold_prove(128, c, evals[i], points[i])
+        assert root == root_o and (claim == claim_o).all() and proof == proof_o, (i, n, c)
+        root_p, claim_p, proof_p = gpu_prove(bfd, ctx, 128, c, evals[i], points[i])
+        assert proof_p == proof and lat > 0
+    srv.close()
+
+
+def test_server_budget_admission(bfd, oracle, ctx):
+    """A tight HBM budget serialises admission but every request completes;
+    a request larger than the budget is refused at submit."""
+    import torch
+    n, c = 10, 1
+    evals = [bfd.random_vec(7000 + i, 1 << n) for i in range(6)]
+    points = [bfd.random_vec(7100 + i, n) for i in range(6)]
+    dev = [torch.from_numpy(e.view(np.int64).reshape(-1)).to("cuda:0") for e in evals]
+    torch.cuda.synchronize()
+    one = 16 * (2 << n) + 32 * (2 * (2 << n)) + 16 * (1 << n) * 2 + 16 * (2 << n) + 32 * (2 << n) + 8192
+    srv = bfd.Server(hbm_budget=int(1.5 * one), ctx=ctx)
+    tickets = [srv.submit(d.data_ptr(), n, p, c, keep=d) for d, p in zip(dev, points)]
+    for i, t in enumerate(tickets):
+        root, claim, proof, lat = srv.wait(t, 120000)
+        assert oracle.basefold_verify(128, c, root, points[i], claim, proof) == 0
+    big = bfd.random_vec(1, 1 << 14)
+    dbig = torch.from_numpy(big.view(np.int64).reshape(-1)).to("cuda:0")
+    with pytest.raises(bfd.BfdError) as e:
+        srv.submit(dbig.data_ptr(), 14, bfd.random_vec(2, 14), 1)
+    assert e.value.code == 3
+    srv.close()
