@@ -23,6 +23,7 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     const size_t M = cm->M, c = (size_t)1 << o.b;
 
     // device scratch for the whole opening
+    double t_setup = now_ms();
     const int hn = o.half_eq_vars();
     DevBuf half_eq(16 * ((size_t)1 << hn), st);
     if (o.n > 0) device_eq(ctx, o.r.data(), hn, half_eq.as<gf>());
@@ -40,6 +41,8 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
 
     const gf *poly_cur = cm->data.as<gf>();
+    ctx->timer.host("open_setup", now_ms() - t_setup);
+    double t_host = 0;
     for (int k = 0; k < d; k++) {
         const int i = d - 1 - k;
         const size_t half = (M >> k) / 2, ph = (size_t)1 << i;
@@ -54,9 +57,11 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
         ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, partials.as<gf>(), &nb); });
         hip_check(hipMemcpyAsync(pin + 32, partials.p, 16 * nb, hipMemcpyDeviceToHost, st), "copy partials");
         hip_check(hipStreamSynchronize(st), "round sync");
+        const double th = now_ms();
         gf S = gf_zero();
         for (unsigned t = 0; t < nb; t++) S = gf_add(S, load_elem(pin + 32 + 16 * t));
         const gf x = o.round(k, pin, S, ch);
+        t_host += now_ms() - th;
         // fold pi (binary_reed_solomon.rs:88-96) and fix the top variable of poly
         gf *pnext = pis.as<gf>() + lay.pi_off[k + 1];
         const gf *ftl = code->ft.as<gf>() + code->ft_off[i];
@@ -65,6 +70,7 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
         ctx->timer.run(st, "k_fix", [&] { return launch::fix(st, poly_cur, pout, ph, x); });
         poly_cur = pout;
     }
+    ctx->timer.host("round_transcript", t_host);
     hip_check(hipMemcpyAsync(pin, poly_cur, 16, hipMemcpyDeviceToHost, st), "copy final");
     hip_check(hipStreamSynchronize(st), "final sync");
     o.finish(load_elem(pin), ch);

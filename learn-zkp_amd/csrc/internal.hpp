@@ -109,6 +109,10 @@ struct PinnedBuf {
     }
 };
 
+inline double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 // Optional per-kernel timing with HIP events on the context's stream.
 struct KernelTimer {
     bool enabled = false;
@@ -318,9 +322,11 @@ inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, boo
     cm->bytes = 16 * cm->N + 16 * cm->M + 32 * (2 * cm->M - 1);
     if (ctx->budget && ctx->resident + cm->bytes > ctx->budget)
         fail(BFD_ERR_BUDGET, "commitment exceeds the context's HBM budget");
+    const double t0 = now_ms();
     cm->data = DevBuf(16 * cm->N, st);
     cm->cw = DevBuf(16 * cm->M, st);
     cm->tree = DevBuf(32 * (2 * cm->M - 1), st);
+    ctx->timer.host("commit_alloc", now_ms() - t0);
     hip_check(hipMemcpyAsync(cm->data.p, evals, 16 * cm->N,
                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
               "copy evaluations");
@@ -541,10 +547,6 @@ inline void assemble_proof(const OpenRounds &o, const uint8_t *elems, const uint
             digs += 32 * (size_t)(o.b + i);
         }
     }
-}
-
-inline double now_ms() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 // Persistent host worker pool for per-polynomial host work (proof planning
