@@ -25,12 +25,12 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     // device scratch for the whole opening
     double t_setup = now_ms();
     const int hn = o.half_eq_vars();
-    DevBuf half_eq(16 * ((size_t)1 << hn), st);
+    DevBuf half_eq = ctx_buf(ctx, 16 * ((size_t)1 << hn));
     if (o.n > 0) device_eq(ctx, o.r.data(), hn, half_eq.as<gf>());
-    DevBuf poly(16 * (cm->N > 1 ? cm->N / 2 : 1), st);
+    DevBuf poly = ctx_buf(ctx, 16 * (cm->N > 1 ? cm->N / 2 : 1));
     OpenLayout lay;
     lay.plan(M, d);
-    DevBuf pis(16 * lay.pis_elems(M, c), st), trees(32 * lay.pitree_digests(), st);
+    DevBuf pis = ctx_buf(ctx, 16 * lay.pis_elems(M, c)), trees = ctx_buf(ctx, 32 * lay.pitree_digests());
     lay.cw = cm->cw.as<gf>();
     lay.tree = cm->tree.as<uint64_t>();
     lay.pis = pis.as<gf>();
@@ -83,7 +83,7 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     plan_gathers(o, lay, plan.data(), plan.data() + ne);
     ctx->gather_pin.ensure(16 * ne + 32 * nd + 64);
     uint8_t *he = (uint8_t *)ctx->gather_pin.p, *hd = he + 16 * ne;
-    DevBuf dix(8 * (ne + nd) + 8, st), dout(16 * ne + 32 * nd + 16, st);
+    DevBuf dix = ctx_buf(ctx, 8 * (ne + nd) + 8), dout = ctx_buf(ctx, 16 * ne + 32 * nd + 16);
     hip_check(hipMemcpyAsync(dix.p, plan.data(), 8 * (ne + nd), hipMemcpyHostToDevice, st), "copy gather plan");
     uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * ne;
     ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, ne, 16); });
@@ -180,6 +180,11 @@ int bfd_ctx_create(int device, uint64_t hbm_budget_bytes, bfd_ctx **out) {
         ctx->budget = hbm_budget_bytes;
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        ctx->cache.st = ctx->stream;
+        ctx->cache.cap = total_b / 4;
+        if (hbm_budget_bytes && hbm_budget_bytes < ctx->cache.cap) ctx->cache.cap = hbm_budget_bytes;
         hipMemPool_t pool;
         if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
             uint64_t thr = UINT64_MAX;
@@ -196,6 +201,7 @@ int bfd_ctx_destroy(bfd_ctx *ctx) {
         ctx->lane.reset();
         for (auto &kv : ctx->code_cache) delete kv.second;
         ctx->code_cache.clear();
+        ctx->cache.flush();
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -307,7 +313,7 @@ int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uin
             return;
         }
         // evaluate (multilinear.rs:200-214): fix the top variable repeatedly
-        DevBuf tmp(16 * (comm->N / 2), st);
+        DevBuf tmp = ctx_buf(ctx, 16 * (comm->N / 2));
         const gf *cur = comm->data.as<gf>();
         for (int i = n - 1; i >= 0; i--) {
             hip_check(launch::fix(st, cur, tmp.as<gf>(), (size_t)1 << i, x[i]), "fix");
@@ -382,7 +388,7 @@ int bfd_sumcheck_product(bfd_ctx *ctx, uint32_t n, const uint8_t *f_b, const uin
             if (gf_eq(t, gf_zero())) fail(BFD_ERR_ARG, "r_j = 1 makes the halved eq table singular");
             cf[j] = hmul(cf[j + 1], hinv(t));
         }
-        DevBuf df(16 * N, st), dg(16 * N, st), T(16 * ((size_t)1 << hn), st);
+        DevBuf df = ctx_buf(ctx, 16 * N), dg = ctx_buf(ctx, 16 * N), T = ctx_buf(ctx, 16 * ((size_t)1 << hn));
         hip_check(hipMemcpyAsync(df.p, f_b, 16 * N, hipMemcpyHostToDevice, st), "copy f");
         hip_check(hipMemcpyAsync(dg.p, g_b, 16 * N, hipMemcpyHostToDevice, st), "copy g");
         device_eq(ctx, r.data(), hn, T.as<gf>());

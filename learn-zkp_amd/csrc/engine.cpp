@@ -135,7 +135,8 @@ uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *pa
             seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
         }
     }
-    const int bs = launch::seg_block_size(k);
+    // items per block: one subtree per block for Subtree, else one per thread
+    const int bs = k == SegKernel::Subtree ? 1 : launch::seg_block_size(k);
     uint32_t blocks = 0;
     for (auto &g : segs) {
         uint64_t want = (g.count + bs - 1) / bs;
@@ -156,6 +157,33 @@ uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *pa
     return blocks;
 }
 
+// Node levels of every tree in trees_ (leaf layer, leaf count): step t of
+// each tree's launch plan (tree_steps) goes into one k_nodes-style and one
+// subtree-style segmented launch shared by all trees.
+void Lane::tree_levels() {
+    plans_.resize(trees_.size());
+    size_t max_steps = 0;
+    for (size_t a = 0; a < trees_.size(); a++) {
+        tree_steps(trees_[a].second, plans_[a]);
+        max_steps = std::max(max_steps, plans_[a].size());
+    }
+    std::vector<Seg> &segs = segs_;
+    std::vector<Seg> subs;
+    for (size_t t = 0; t < max_steps; t++) {
+        segs.clear();
+        subs.clear();
+        for (size_t a = 0; a < trees_.size(); a++) {
+            if (t >= plans_[a].size()) continue;
+            const TreeStep &s = plans_[a][t];
+            uint64_t *prev = trees_[a].first + 4 * s.prev;
+            if (s.lv == 0) segs.push_back(make_seg(prev, prev + 4 * s.h, nullptr, s.h / 2, 0));
+            else subs.push_back(make_seg(prev, nullptr, nullptr, s.h >> s.lv, s.h));
+        }
+        seg(SegKernel::Nodes, segs, "ks_nodes");
+        seg(SegKernel::Subtree, subs, "ks_subtree");
+    }
+}
+
 // ---------------------------------------------------------------------------
 // commit tick: encode, Merkle tree and (when no claim was given) the claim
 // ---------------------------------------------------------------------------
@@ -168,13 +196,12 @@ void Lane::commit_tick(std::vector<Job *> &jobs) {
         seg_pin_.ensure(sizeof(Seg) * seg_cap_);
         seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
     }
-    int max_n = 0, max_L = 0;
+    int max_n = 0;
     for (size_t a = 0; a < jobs.size(); a++) {
         Job &j = *jobs[a];
         if (!j.cw) job_alloc(j, st_);
         j.slot = a;
         max_n = std::max(max_n, j.n);
-        max_L = std::max(max_L, j.L);
     }
     auto &segs = segs_;
     for (int s = 0; s < max_n; s++) {
@@ -192,16 +219,9 @@ void Lane::commit_tick(std::vector<Job *> &jobs) {
     segs.clear();
     for (Job *jp : jobs) segs.push_back(make_seg(jp->cw, jp->tree, nullptr, jp->M, 0));
     seg(SegKernel::LeavesElem, segs, "ks_leaves_elem");
-    for (int t = 0; t < max_L; t++) {
-        segs.clear();
-        for (Job *jp : jobs) {
-            Job &j = *jp;
-            if (t < j.L)
-                segs.push_back(make_seg(j.tree + 4 * layer_off(j.M, t), j.tree + 4 * layer_off(j.M, t + 1), nullptr,
-                                        j.M >> (t + 1), 0));
-        }
-        seg(SegKernel::Nodes, segs, "ks_nodes");
-    }
+    trees_.clear();
+    for (Job *jp : jobs) trees_.push_back({jp->tree, jp->M});
+    tree_levels();
     // claim = MLE(evals)(point) by fixing the top variable repeatedly
     // (multilinear.rs:200-214), in the job's poly buffer
     for (int lvl = 0; lvl < max_n; lvl++) {
@@ -323,29 +343,20 @@ void Lane::open_tick(std::vector<Job *> &jobs) {
     seg(SegKernel::Fix, segs, "ks_fix");
     // (C) this round's pi tree and inner product
     segs.clear();
-    int max_levels = 0;
     for (Job *jp : jobs) {
         Job &j = *jp;
         if (j.stage != Job::Open || j.k >= j.n) continue;
         const size_t half = (j.M >> j.k) / 2;
         segs.push_back(make_seg(j.lay.pi(j.k), (void *)j.lay.pitree(j.k), nullptr, half, half));
-        max_levels = std::max(max_levels, j.L - j.k - 1);
     }
     seg(SegKernel::LeavesPair, segs, "ks_leaves_pair");
-    for (int t = 0; t < max_levels; t++) {
-        segs.clear();
-        for (Job *jp : jobs) {
-            Job &j = *jp;
-            if (j.stage != Job::Open || j.k >= j.n) continue;
-            const size_t half = (j.M >> j.k) / 2;
-            if ((half >> t) > 1) {
-                uint64_t *tk = (uint64_t *)j.lay.pitree(j.k);
-                segs.push_back(make_seg(tk + 4 * layer_off(half, t), tk + 4 * layer_off(half, t + 1), nullptr,
-                                        half >> (t + 1), 0));
-            }
-        }
-        seg(SegKernel::Nodes, segs, "ks_nodes");
+    trees_.clear();
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        if (j.stage != Job::Open || j.k >= j.n) continue;
+        trees_.push_back({(uint64_t *)j.lay.pitree(j.k), (j.M >> j.k) / 2});
     }
+    tree_levels();
     segs.clear();
     std::vector<Job *> dotted;
     for (Job *jp : jobs) {

@@ -96,6 +96,7 @@ class Lane {
     uint32_t seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial = nullptr,
                  uint32_t max_blocks_per_seg = 2048);
     void reset_ring();
+    void tree_levels();
     hipStream_t st_;
     KernelTimer *timer_;
     // pinned staging; reused from tick to tick (a tick starts only after the
@@ -104,6 +105,8 @@ class Lane {
     DevBuf seg_dev_, rb_dev_, addr_dev_, partials_, gather_dev_, plan_dev_, eq_dev_;
     size_t seg_cap_ = 0, seg_used_ = 0;
     std::vector<Seg> segs_;
+    std::vector<std::pair<uint64_t *, size_t>> trees_;  // leaf layer, leaf count
+    std::vector<std::vector<TreeStep>> plans_;
     std::vector<std::pair<size_t, size_t>> gather_at_;  // per job: first element / digest record
     size_t gather_NE_ = 0;
 };

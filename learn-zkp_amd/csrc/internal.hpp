@@ -65,28 +65,84 @@ int guarded(F &&f) {
 
 // ---------------------------------------------------------------------------
 // device memory: stream-ordered allocations from the device's default pool
-// (kept cached across calls so repeated commits/opens do not hit the driver)
 // ---------------------------------------------------------------------------
+// Freed blocks of one stream kept for reuse: a block of at least kMin bytes
+// waits here for the next request of exactly its size on the same stream, so
+// repeated commits and opens of one shape make no allocator calls.  Reuse is
+// safe by stream order (the block's last use and its next use are on the
+// same stream).  Holds at most `cap` bytes; a block that does not fit empties
+// the cache first.
+struct BlockCache {
+    static constexpr size_t kMin = (size_t)1 << 20;
+    std::mutex m;
+    std::multimap<size_t, void *> blocks;
+    size_t held = 0, cap = 0;
+    hipStream_t st = nullptr;
+    void *take(size_t n) {
+        if (n < kMin) return nullptr;
+        std::lock_guard<std::mutex> lk(m);
+        auto it = blocks.find(n);
+        if (it == blocks.end()) return nullptr;
+        void *p = it->second;
+        blocks.erase(it);
+        held -= n;
+        return p;
+    }
+    // false when the caller must free the block itself
+    bool put(void *p, size_t n) {
+        if (n < kMin || n > cap) return false;
+        std::lock_guard<std::mutex> lk(m);
+        if (held + n > cap) drop_locked();
+        blocks.emplace(n, p);
+        held += n;
+        return true;
+    }
+    void flush() {
+        std::lock_guard<std::mutex> lk(m);
+        drop_locked();
+    }
+    // the owner flushes while the stream still exists; this is a fallback
+    ~BlockCache() {
+        for (auto &kv : blocks) (void)hipFree(kv.second);
+    }
+
+  private:
+    void drop_locked() {
+        for (auto &kv : blocks) (void)hipFreeAsync(kv.second, st);
+        blocks.clear();
+        held = 0;
+    }
+};
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     hipStream_t st = nullptr;
+    BlockCache *cache = nullptr;
     DevBuf() = default;
-    DevBuf(size_t n, hipStream_t s) : bytes(n), st(s) {
-        if (n) hip_check(hipMallocAsync(&p, n, s), "hipMallocAsync");
+    DevBuf(size_t n, hipStream_t s, BlockCache *c = nullptr) : bytes(n), st(s), cache(c) {
+        if (!n) return;
+        if (c && c->st == s && (p = c->take(n))) return;
+        hipError_t e = hipMallocAsync(&p, n, s);
+        if (e != hipSuccess && c) {  // cached blocks may be what is missing
+            (void)hipGetLastError();
+            c->flush();
+            e = hipMallocAsync(&p, n, s);
+        }
+        hip_check(e, "hipMallocAsync");
     }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     DevBuf(DevBuf &&o) noexcept { *this = std::move(o); }
     DevBuf &operator=(DevBuf &&o) noexcept {
         release();
-        p = o.p; bytes = o.bytes; st = o.st;
+        p = o.p; bytes = o.bytes; st = o.st; cache = o.cache;
         o.p = nullptr; o.bytes = 0;
         return *this;
     }
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFreeAsync(p, st);
+        if (p && !(cache && cache->st == st && cache->put(p, bytes))) (void)hipFreeAsync(p, st);
         p = nullptr;
         bytes = 0;
     }
@@ -199,7 +255,10 @@ struct bfd_ctx {
     bfd::rt::PinnedBuf pinned;      // per-round read-backs (roots, partial sums)
     bfd::rt::PinnedBuf gather_pin;  // opened values of a proof, device -> host
     bfd::rt::PinnedBuf seg_pin;     // segment tables of batched launches, host -> device
+    bfd::rt::PinnedBuf eq_pin;      // eq-table halves, host -> device
     bfd::rt::KernelTimer timer;
+    // blocks of the plain path on `stream`, reused across calls
+    bfd::rt::BlockCache cache;
     // engine lane on `stream` for batched calls (made on first use)
     std::unique_ptr<bfd::rt::Lane, bfd::rt::LaneDeleter> lane;
 };
@@ -238,6 +297,9 @@ bfd_code *cached_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d
 
 inline void set_device(bfd_ctx *ctx) { hip_check(hipSetDevice(ctx->device), "hipSetDevice"); }
 
+// a device buffer on the context stream, drawn from and returned to its cache
+inline DevBuf ctx_buf(bfd_ctx *ctx, size_t bytes) { return DevBuf(bytes, ctx->stream, &ctx->cache); }
+
 inline size_t num_queries(uint32_t lambda, uint32_t log2_c) {
     double c = (double)(1ull << log2_c);
     double delta = (1.0 + 1.0 / c) / 2.0;
@@ -273,26 +335,68 @@ inline std::vector<gf> host_eq(const gf *r, int m) {
 
 // Builds eq(r[0..m], .) of 2^m entries on the device as the tensor product of
 // two host-built halves.
+// The halves go through ctx->eq_pin without a stream synchronisation: every
+// caller synchronises the stream before it returns, so the staging buffer is
+// idle again whenever device_eq is entered.
 inline void device_eq(bfd_ctx *ctx, const gf *r, int m, gf *out) {
     hipStream_t st = ctx->stream;
     int lo = (m + 1) / 2, hi = m - lo;
     std::vector<gf> A = host_eq(r, lo), B = host_eq(r + lo, hi);
-    DevBuf dA(A.size() * 16, st), dB(B.size() * 16, st);
-    hip_check(hipMemcpyAsync(dA.p, A.data(), A.size() * 16, hipMemcpyHostToDevice, st), "copy eq A");
-    hip_check(hipMemcpyAsync(dB.p, B.data(), B.size() * 16, hipMemcpyHostToDevice, st), "copy eq B");
-    ctx->timer.run(st, "k_outer", [&] { return launch::outer(st, dA.as<gf>(), lo, dB.as<gf>(), out, (size_t)1 << m); });
-    // the pageable host halves must outlive the copies
-    hip_check(hipStreamSynchronize(st), "eq table");
+    const size_t na = A.size(), nb = B.size();
+    ctx->eq_pin.ensure(16 * (na + nb));
+    gf *h = (gf *)ctx->eq_pin.p;
+    memcpy(h, A.data(), 16 * na);
+    memcpy(h + na, B.data(), 16 * nb);
+    DevBuf d(16 * (na + nb), st);
+    hip_check(hipMemcpyAsync(d.p, h, 16 * (na + nb), hipMemcpyHostToDevice, st), "copy eq halves");
+    ctx->timer.run(st, "k_outer", [&] { return launch::outer(st, d.as<gf>(), lo, d.as<gf>() + na, out, (size_t)1 << m); });
+}
+
+// The node levels of a Merkle tree over a power-of-two number of leaf digests,
+// as launches.  A level of more than kSubtreeFrom / 2 nodes fills the device
+// by itself and gets a k_nodes launch of its own; above that, each launch
+// hashes up to 9 levels per block through LDS (k_subtree), so the narrow top
+// of the tree costs two launches instead of one per level.  kSubtreeFrom =
+// 2^17 makes the first subtree launch 256 blocks, one per CU: a narrow level
+// then runs as at most one wave per SIMD, i.e. at one Keccak-f latency, where
+// two blocks per CU would stack two part-empty waves on a SIMD.
+struct TreeStep {
+    size_t prev;  // offset in digests of the input layer from the leaf layer
+    size_t h;     // digests in the input layer
+    int lv;       // levels hashed; 0 = one k_nodes level
+};
+constexpr size_t kSubtreeFrom = (size_t)1 << 17;
+inline void tree_steps(size_t leaves, std::vector<TreeStep> &out) {
+    if (leaves & (leaves - 1)) fail(BFD_ERR_ARG, "Merkle leaf count must be a power of two");
+    out.clear();
+    size_t off = 0, h = leaves;
+    while (h > 1) {
+        if (h > kSubtreeFrom) {
+            out.push_back({off, h, 0});
+            off += h;
+            h >>= 1;
+            continue;
+        }
+        const int lv = std::min(9, __builtin_ctzll(h));
+        out.push_back({off, h, lv});
+        for (int i = 0; i < lv; i++) {
+            off += h;
+            h >>= 1;
+        }
+    }
 }
 
 // all Merkle levels above `leaves` digests at layer0 (layers contiguous)
 inline void tree_nodes(bfd_ctx *ctx, uint64_t *layer0, size_t leaves) {
     hipStream_t st = ctx->stream;
-    uint64_t *prev = layer0;
-    for (size_t h = leaves; h > 1; h >>= 1) {
-        uint64_t *next = prev + 4 * h;
-        ctx->timer.run(st, "k_nodes", [&] { return launch::nodes(st, prev, next, h / 2); });
-        prev = next;
+    std::vector<TreeStep> steps;
+    tree_steps(leaves, steps);
+    for (const TreeStep &s : steps) {
+        uint64_t *prev = layer0 + 4 * s.prev;
+        if (s.lv == 0)
+            ctx->timer.run(st, "k_nodes", [&] { return launch::nodes(st, prev, prev + 4 * s.h, s.h / 2); });
+        else
+            ctx->timer.run(st, "k_subtree", [&] { return launch::subtree(st, prev, s.h, s.lv); });
     }
 }
 
@@ -300,8 +404,18 @@ inline void encode(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, i
     hipStream_t st = ctx->stream;
     const gf *ft = code->ft.as<gf>();
     ctx->timer.run(st, "k_ntt_first", [&] { return launch::ntt_first(st, data, cw, ft + code->ft_off[0], b, n); });
-    for (int s = 1; s < n; s++)
+    // remaining layers three at a time (one pass over the codeword per three)
+    int s = 1;
+    for (; s + 3 <= n; s += 3) {
+        const gf *t0 = ft + code->ft_off[s], *t1 = ft + code->ft_off[s + 1], *t2 = ft + code->ft_off[s + 2];
+        ctx->timer.run(st, "k_ntt_multi", [&] { return launch::ntt_multi(st, cw, t0, t1, t2, 3, s, b, n); });
+    }
+    if (s + 2 == n) {
+        const gf *t0 = ft + code->ft_off[s], *t1 = ft + code->ft_off[s + 1];
+        ctx->timer.run(st, "k_ntt_multi", [&] { return launch::ntt_multi(st, cw, t0, t1, nullptr, 2, s, b, n); });
+    } else if (s + 1 == n) {
         ctx->timer.run(st, "k_ntt_layer", [&] { return launch::ntt_layer(st, cw, ft + code->ft_off[s], s, b, n); });
+    }
 }
 
 inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, bool on_device, size_t height,
@@ -323,9 +437,9 @@ inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, boo
     if (ctx->budget && ctx->resident + cm->bytes > ctx->budget)
         fail(BFD_ERR_BUDGET, "commitment exceeds the context's HBM budget");
     const double t0 = now_ms();
-    cm->data = DevBuf(16 * cm->N, st);
-    cm->cw = DevBuf(16 * cm->M, st);
-    cm->tree = DevBuf(32 * (2 * cm->M - 1), st);
+    cm->data = ctx_buf(ctx, 16 * cm->N);
+    cm->cw = ctx_buf(ctx, 16 * cm->M);
+    cm->tree = ctx_buf(ctx, 32 * (2 * cm->M - 1));
     ctx->timer.host("commit_alloc", now_ms() - t0);
     hip_check(hipMemcpyAsync(cm->data.p, evals, 16 * cm->N,
                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),

@@ -22,7 +22,7 @@ struct Seg {
     uint32_t nblk;    // blocks of the segment
 };
 
-enum class SegKernel { NttFirst, NttLayer, LeavesElem, LeavesPair, Nodes, Dot, Fold, Fix, Outer };
+enum class SegKernel { NttFirst, NttLayer, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer };
 
 namespace launch {
 
@@ -40,10 +40,15 @@ hipError_t xor_span(hipStream_t st, gf *ft, const gf *bt, int bits, size_t count
 // then layers s = 1..n-1 in place on the slot-interleaved codeword
 hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n);
 hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n);
+// R = 2 or 3 layers s .. s+R-1 in one pass; t_r is the table of layer s + r
+hipError_t ntt_multi(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n);
 hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count);
 hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half);
 // one Merkle level: next[i] = H(prev[2i] || prev[2i+1])
 hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count);
+// lv Merkle levels above the layer `prev` of h digests (h a power of two,
+// 1 <= lv <= 9, lv <= log2 h), written to the layers that follow it
+hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv);
 hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks);
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r);
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x);

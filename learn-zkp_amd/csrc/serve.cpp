@@ -170,6 +170,14 @@ int bfd_server_create(bfd_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t max_acti
         hip_check(hipStreamCreateWithFlags(&S->s_open, hipStreamNonBlocking), "hipStreamCreate");
         S->commit_lane.reset(new Lane(S->s_commit, nullptr));
         S->open_lane.reset(new Lane(S->s_open, nullptr));
+        // hand memory held for the plain path back to the device first
+        {
+            std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+            ctx->cache.flush();
+            hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+        }
         // reserve the budget once (unlimited budget: 3/4 of the free memory)
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
