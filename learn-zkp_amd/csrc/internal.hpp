@@ -377,7 +377,7 @@ inline void tree_steps(size_t leaves, std::vector<TreeStep> &out) {
             h >>= 1;
             continue;
         }
-        const int lv = std::min(9, __builtin_ctzll(h));
+        const int lv = std::min(launch::subtree_max_levels(), __builtin_ctzll(h));
         out.push_back({off, h, lv});
         for (int i = 0; i < lv; i++) {
             off += h;

@@ -195,50 +195,56 @@ __global__ __launch_bounds__(KB) void k_nodes(const uint64_t *__restrict__ prev,
 
 // Upper Merkle levels, several per launch.  `prev` is a layer of h digests
 // (h a power of two) and the next levels follow it contiguously.  Subtree
-// `sub` covers children [sub << lv, (sub + 1) << lv): its 2^(lv-1) level-1
-// nodes are hashed from global memory, every further level from LDS, and each
-// level's digests are written to its layer.  lv <= 9 (256 level-1 nodes).
-// From level 2 on, node t runs on thread ((t & 3) << 6) | (t >> 2), which
-// spreads a narrow level over all four waves instead of stacking it on one.
+// `sub` covers the C = 2^lv children [sub * C, (sub + 1) * C), lv <=
+// kSubMaxLevels: the block copies them into LDS, hashes level after level
+// inside LDS (one Keccak call site, so the straight-line permutation is
+// fetched into the instruction cache once), and finally writes every level's
+// digests out with coalesced 16-byte stores.  buf holds the children at
+// [0, C) and level l at [2C - 2^(lv-l+1), 2C - 2^(lv-l)).  Node t of a level
+// runs on thread t, so a narrow level occupies as few waves as possible: a
+// block's waves share SIMDs, and a level spread over all of them runs slower
+// than the same level packed into one wave.
+constexpr int kSubThreads = 256;
+constexpr int kSubMaxLevels = 9;               // 2^(levels-1) level-1 nodes = kSubThreads
+constexpr int kSubLds = 2 << kSubMaxLevels;    // digests of LDS per block (2C - 1 used)
 __device__ __forceinline__ void subtree(uint64_t *__restrict__ prev, size_t h, int lv, size_t sub,
-                                        ulonglong2 (*buf)[KB][2]) {
+                                        ulonglong2 (*buf)[2]) {
     const int tid = threadIdx.x;
-    size_t width = (size_t)1 << (lv - 1);
-    uint64_t *lvl = prev + 4 * h;
-    size_t cnt = h >> 1;
-    if ((size_t)tid < width) {
-        const size_t i = sub * width + tid;
-        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(prev + 8 * i);
-        ulonglong2 a = p[0], b = p[1], c = p[2], e = p[3];
-        uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
-        kk::keccak256_words<8>(m, d);
-        st_digest(lvl + 4 * i, d);
-        buf[0][tid][0] = ulonglong2{d[0], d[1]};
-        buf[0][tid][1] = ulonglong2{d[2], d[3]};
-    }
+    const int C = 1 << lv;
+    const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(prev + 4 * (sub << lv));
+    for (int k = tid; k < 2 * C; k += kSubThreads) buf[k >> 1][k & 1] = src[k];
     __syncthreads();
-    const size_t node = ((size_t)(tid & 63) << 2) | (size_t)(tid >> 6);
-    int cur = 0;
-    for (int l = 2; l <= lv; l++) {
-        lvl += 4 * cnt;
-        cnt >>= 1;
-        width >>= 1;
-        if (node < width) {
-            const ulonglong2 a = buf[cur][2 * node][0], b = buf[cur][2 * node][1];
-            const ulonglong2 c = buf[cur][2 * node + 1][0], e = buf[cur][2 * node + 1][1];
+    int in = 0, out = C;
+    for (int l = 1; l <= lv; l++) {
+        const int w = C >> l;
+        if (tid < w) {
+            const ulonglong2 a = buf[in + 2 * tid][0], b = buf[in + 2 * tid][1];
+            const ulonglong2 c = buf[in + 2 * tid + 1][0], e = buf[in + 2 * tid + 1][1];
             uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
             kk::keccak256_words<8>(m, d);
-            st_digest(lvl + 4 * (sub * width + node), d);
-            buf[cur ^ 1][node][0] = ulonglong2{d[0], d[1]};
-            buf[cur ^ 1][node][1] = ulonglong2{d[2], d[3]};
+            buf[out + tid][0] = ulonglong2{d[0], d[1]};
+            buf[out + tid][1] = ulonglong2{d[2], d[3]};
         }
         __syncthreads();
-        cur ^= 1;
+        in = out;
+        out += w;
     }
+    uint64_t *lvl = prev + 4 * h;
+    size_t cnt = h >> 1;
+    int off = C;
+    for (int l = 1; l <= lv; l++) {
+        const int w = C >> l;
+        ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(lvl + 4 * (sub * (size_t)w));
+        for (int k = tid; k < 2 * w; k += kSubThreads) dst[k] = buf[off + (k >> 1)][k & 1];
+        lvl += 4 * cnt;
+        cnt >>= 1;
+        off += w;
+    }
+    __syncthreads();  // buf is refilled by the block's next subtree
 }
 
-__global__ __launch_bounds__(KB) void k_subtree(uint64_t *__restrict__ prev, size_t h, int lv) {
-    __shared__ ulonglong2 buf[2][KB][2];
+__global__ __launch_bounds__(kSubThreads) void k_subtree(uint64_t *__restrict__ prev, size_t h, int lv) {
+    __shared__ ulonglong2 buf[kSubLds][2];
     const size_t subs = h >> lv;
     for (size_t sub = blockIdx.x; sub < subs; sub += gridDim.x) subtree(prev, h, lv, sub, buf);
 }
@@ -474,8 +480,8 @@ __global__ __launch_bounds__(KB) void ks_nodes(const Seg *segs, int nseg) {
 
 // in = layer of arg = h digests, count = h >> lv subtrees (both powers of two,
 // so lv = log2(h / count)); see subtree()
-__global__ __launch_bounds__(KB) void ks_subtree(const Seg *segs, int nseg) {
-    __shared__ ulonglong2 buf[2][KB][2];
+__global__ __launch_bounds__(kSubThreads) void ks_subtree(const Seg *segs, int nseg) {
+    __shared__ ulonglong2 buf[kSubLds][2];
     const Seg &g = my_seg(segs, nseg);
     uint64_t *prev = (uint64_t *)g.in;
     const int lv = __builtin_ctzll(g.arg) - __builtin_ctzll(g.count);
@@ -664,8 +670,8 @@ hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t co
     return hipGetLastError();
 }
 hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv) {
-    if (lv < 1 || lv > 9 || (h & (h - 1)) || (h >> lv) == 0) return hipErrorInvalidValue;
-    k_subtree<<<grid_for(h >> lv, 1, kCapHash), KB, 0, st>>>(prev, h, lv);
+    if (lv < 1 || lv > kSubMaxLevels || (h & (h - 1)) || (h >> lv) == 0) return hipErrorInvalidValue;
+    k_subtree<<<grid_for(h >> lv, 1, kCapHash), kSubThreads, 0, st>>>(prev, h, lv);
     return hipGetLastError();
 }
 
@@ -718,7 +724,7 @@ hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int ns
     case SegKernel::LeavesElem: ks_leaves_elem<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::LeavesPair: ks_leaves_pair<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Nodes: ks_nodes<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
-    case SegKernel::Subtree: ks_subtree<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::Subtree: ks_subtree<<<nblocks, kSubThreads, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Dot: ks_dot<<<nblocks, CB, 0, st>>>(d_segs, nseg, partial); break;
     case SegKernel::Fold: ks_fold<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Fix: ks_fix<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
@@ -732,11 +738,12 @@ int seg_block_size(SegKernel which) {
     case SegKernel::NttLayer: return 256;
     case SegKernel::LeavesElem:
     case SegKernel::LeavesPair:
-    case SegKernel::Nodes:
-    case SegKernel::Subtree: return KB;
+    case SegKernel::Nodes: return KB;
+    case SegKernel::Subtree: return kSubThreads;
     default: return CB;
     }
 }
+int subtree_max_levels() { return kSubMaxLevels; }
 hipError_t gather_abs(hipStream_t st, const uint64_t *src, void *out, size_t n, int rb) {
     if (!n) return hipSuccess;
     k_gather_abs<<<grid_for(n, 256, 4096), 256, 0, st>>>(src, (uint8_t *)out, n, rb);

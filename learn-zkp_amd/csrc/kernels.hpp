@@ -47,8 +47,10 @@ hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half)
 // one Merkle level: next[i] = H(prev[2i] || prev[2i+1])
 hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count);
 // lv Merkle levels above the layer `prev` of h digests (h a power of two,
-// 1 <= lv <= 9, lv <= log2 h), written to the layers that follow it
+// 1 <= lv <= subtree_max_levels(), lv <= log2 h), written to the layers that follow it
 hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv);
+// largest lv subtree() and SegKernel::Subtree accept
+int subtree_max_levels();
 hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks);
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r);
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x);
