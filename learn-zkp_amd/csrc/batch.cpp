@@ -34,7 +34,7 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
         all[p] = jobs[p].get();
         if (!on_device) staged += 16 * ((size_t)1 << log_sizes[p]);
     }
-    DevBuf stage(staged ? staged : 1, st);
+    DevBuf stage = ctx_buf(ctx, staged ? staged : 1);
     size_t off = 0;
     for (size_t p = 0; p < count; p++) {
         Job &j = *jobs[p];
@@ -54,9 +54,11 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
     }
     if (ctx->budget && ctx->resident + bytes > ctx->budget)
         fail(BFD_ERR_BUDGET, "batch exceeds the context's HBM budget");
-    // one slab for every job: the stream-ordered pool hands the same block
-    // back to a repeated batch of the same shape
-    DevBuf slab(bytes, st);
+    // one slab for every job, from the context's block cache: a repeated
+    // batch of the same shape gets the same block back without touching the
+    // allocator (taking it from the pool after plain calls have split the
+    // pool's free memory would map new memory instead)
+    DevBuf slab = ctx_buf(ctx, bytes);
     {
         size_t o = 0;
         for (Job *j : all) {

@@ -157,30 +157,43 @@ uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *pa
     return blocks;
 }
 
-// Node levels of every tree in trees_ (leaf layer, leaf count): step t of
-// each tree's launch plan (tree_steps) goes into one k_nodes-style and one
-// subtree-style segmented launch shared by all trees.
+// Node levels of every tree in trees_ (leaf layer, leaf count).  The single
+// levels (k_nodes form) run bottom-aligned, one segmented launch per level
+// index shared by all trees; then the subtree steps of all trees run
+// together, so the latency-bound tops of every tree share two or three
+// launches instead of being spread over the tick.
 void Lane::tree_levels() {
+    const size_t from = subtree_from(trees_.size());
     plans_.resize(trees_.size());
-    size_t max_steps = 0;
+    singles_.assign(trees_.size(), 0);
+    size_t max_single = 0, max_sub = 0;
     for (size_t a = 0; a < trees_.size(); a++) {
-        tree_steps(trees_[a].second, plans_[a]);
-        max_steps = std::max(max_steps, plans_[a].size());
+        tree_steps(trees_[a].second, plans_[a], from);
+        size_t k = 0;
+        while (k < plans_[a].size() && plans_[a][k].lv == 0) k++;
+        singles_[a] = k;
+        max_single = std::max(max_single, k);
+        max_sub = std::max(max_sub, plans_[a].size() - k);
     }
     std::vector<Seg> &segs = segs_;
-    std::vector<Seg> subs;
-    for (size_t t = 0; t < max_steps; t++) {
+    for (size_t t = 0; t < max_single; t++) {
         segs.clear();
-        subs.clear();
         for (size_t a = 0; a < trees_.size(); a++) {
-            if (t >= plans_[a].size()) continue;
+            if (t >= singles_[a]) continue;
             const TreeStep &s = plans_[a][t];
             uint64_t *prev = trees_[a].first + 4 * s.prev;
-            if (s.lv == 0) segs.push_back(make_seg(prev, prev + 4 * s.h, nullptr, s.h / 2, 0));
-            else subs.push_back(make_seg(prev, nullptr, nullptr, s.h >> s.lv, s.h));
+            segs.push_back(make_seg(prev, prev + 4 * s.h, nullptr, s.h / 2, 0));
         }
         seg(SegKernel::Nodes, segs, "ks_nodes");
-        seg(SegKernel::Subtree, subs, "ks_subtree");
+    }
+    for (size_t v = 0; v < max_sub; v++) {
+        segs.clear();
+        for (size_t a = 0; a < trees_.size(); a++) {
+            if (singles_[a] + v >= plans_[a].size()) continue;
+            const TreeStep &s = plans_[a][singles_[a] + v];
+            segs.push_back(make_seg(trees_[a].first + 4 * s.prev, nullptr, nullptr, s.h >> s.lv, s.h));
+        }
+        seg(SegKernel::Subtree, segs, "ks_subtree");
     }
 }
 
@@ -204,17 +217,27 @@ void Lane::commit_tick(std::vector<Job *> &jobs) {
         max_n = std::max(max_n, j.n);
     }
     auto &segs = segs_;
-    for (int s = 0; s < max_n; s++) {
-        segs.clear();
+    segs.clear();
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        segs.push_back(make_seg(j.data, j.cw, j.code->ft.as<gf>() + j.code->ft_off[0], (j.N / 2) << j.b, (uint64_t)j.b));
+    }
+    seg(SegKernel::NttFirst, segs, "ks_ntt_first");
+    // layers 1.. in passes of up to three (k_ntt_multi's grouping); a job
+    // with fewer layers left takes the two- or one-layer form
+    std::vector<Seg> by_r[4];
+    for (int s = 1; s < max_n; s += 3) {
+        for (auto &v : by_r) v.clear();
         for (Job *jp : jobs) {
             Job &j = *jp;
             if (j.n <= s) continue;
+            const int R = std::min(3, j.n - s);
             const gf *ft = j.code->ft.as<gf>() + j.code->ft_off[s];
-            const size_t work = (j.N / 2) << j.b;
-            if (s == 0) segs.push_back(make_seg(j.data, j.cw, ft, work, (uint64_t)j.b));
-            else segs.push_back(make_seg(nullptr, j.cw, ft, work, (uint64_t)(s + j.b)));
+            by_r[R].push_back(make_seg(nullptr, j.cw, ft, j.M >> (R == 1 ? 1 : R), (uint64_t)(s + j.b)));
         }
-        seg(s == 0 ? SegKernel::NttFirst : SegKernel::NttLayer, segs, s == 0 ? "ks_ntt_first" : "ks_ntt_layer");
+        seg(SegKernel::NttMulti3, by_r[3], "ks_ntt_multi");
+        seg(SegKernel::NttMulti2, by_r[2], "ks_ntt_multi");
+        seg(SegKernel::NttLayer, by_r[1], "ks_ntt_layer");
     }
     segs.clear();
     for (Job *jp : jobs) segs.push_back(make_seg(jp->cw, jp->tree, nullptr, jp->M, 0));

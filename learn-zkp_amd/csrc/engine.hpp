@@ -107,6 +107,7 @@ class Lane {
     std::vector<Seg> segs_;
     std::vector<std::pair<uint64_t *, size_t>> trees_;  // leaf layer, leaf count
     std::vector<std::vector<TreeStep>> plans_;
+    std::vector<size_t> singles_;  // leading k_nodes-form steps of each plan
     std::vector<std::pair<size_t, size_t>> gather_at_;  // per job: first element / digest record
     size_t gather_NE_ = 0;
 };

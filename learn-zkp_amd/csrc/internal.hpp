@@ -353,25 +353,35 @@ inline void device_eq(bfd_ctx *ctx, const gf *r, int m, gf *out) {
 }
 
 // The node levels of a Merkle tree over a power-of-two number of leaf digests,
-// as launches.  A level of more than kSubtreeFrom / 2 nodes fills the device
-// by itself and gets a k_nodes launch of its own; above that, each launch
-// hashes up to 9 levels per block through LDS (k_subtree), so the narrow top
-// of the tree costs two launches instead of one per level.  kSubtreeFrom =
-// 2^17 makes the first subtree launch 256 blocks, one per CU: a narrow level
-// then runs as at most one wave per SIMD, i.e. at one Keccak-f latency, where
-// two blocks per CU would stack two part-empty waves on a SIMD.
+// as launches.  A level whose input layer holds more than `from` digests gets
+// a k_nodes launch of its own; from there up, each launch hashes up to
+// subtree_max_levels() levels per block through LDS (k_subtree), so the narrow
+// top of the tree costs two launches instead of one per level.  For one tree,
+// from = kSubtreeFrom = 2^17 makes the first subtree launch 256 blocks, one
+// per CU, so every narrow level runs as one wave per SIMD at one Keccak-f
+// latency; with T trees hashed together (segmented launches) subtree_from()
+// divides it by T so that the first subtree launch still has about 256 blocks.
 struct TreeStep {
     size_t prev;  // offset in digests of the input layer from the leaf layer
     size_t h;     // digests in the input layer
     int lv;       // levels hashed; 0 = one k_nodes level
 };
 constexpr size_t kSubtreeFrom = (size_t)1 << 17;
-inline void tree_steps(size_t leaves, std::vector<TreeStep> &out) {
+inline size_t subtree_from(size_t trees) {
+    size_t from = kSubtreeFrom;
+    while (trees > 1 && from > 2) {
+        from >>= 1;
+        trees = (trees + 1) >> 1;
+    }
+    return from;
+}
+// steps with lv == 0 come first, then the subtree steps
+inline void tree_steps(size_t leaves, std::vector<TreeStep> &out, size_t from = kSubtreeFrom) {
     if (leaves & (leaves - 1)) fail(BFD_ERR_ARG, "Merkle leaf count must be a power of two");
     out.clear();
     size_t off = 0, h = leaves;
     while (h > 1) {
-        if (h > kSubtreeFrom) {
+        if (h > from) {
             out.push_back({off, h, 0});
             off += h;
             h >>= 1;

@@ -98,48 +98,49 @@ __global__ __launch_bounds__(256) void k_ntt_layer(gf *__restrict__ cw, const gf
 // waves per SIMD (at most 256 VGPRs) hide one wave's loads behind the other's
 // multiplies: 10.9 ms for the 23 layers of a 2^26-element codeword against
 // 12.2 ms at the compiler's one-wave allocation.
+// The group body, expanded in place in both kernels below: written out as a
+// function and inlined, it makes the register allocator spill 650 bytes per
+// lane at two waves per SIMD, against 20 bytes when it is in the kernel body.
+#define NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g)                                   \
+    do {                                                                                  \
+        constexpr int E_ = 1 << (R);                                                      \
+        const size_t low = (g) & (mK);                                                    \
+        gf *p = (cw) + ((((g) >> (K)) << ((K) + (R))) | low);                             \
+        gf v[E_];                                                                         \
+        _Pragma("unroll") for (int i = 0; i < E_; i++) v[i] = gf_ld(p + i * (stride));    \
+        {                                                                                 \
+            const gf_pre w = gf_prepare(gf_ld((t0) + low));                               \
+            _Pragma("unroll") for (int i = 0; i < E_; i += 2) {                           \
+                v[i] = gf_add(v[i], gf_mul_pre(v[i + 1], w));                             \
+                v[i + 1] = gf_add(v[i + 1], v[i]);                                        \
+            }                                                                             \
+        }                                                                                 \
+        if constexpr ((R) >= 2) {                                                         \
+            _Pragma("unroll") for (int a = 0; a < 2; a++) {                               \
+                const gf_pre w = gf_prepare(gf_ld((t1) + (low | ((size_t)a << (K)))));    \
+                _Pragma("unroll") for (int i = a; i < E_; i += 4) {                       \
+                    v[i] = gf_add(v[i], gf_mul_pre(v[i + 2], w));                         \
+                    v[i + 2] = gf_add(v[i + 2], v[i]);                                    \
+                }                                                                         \
+            }                                                                             \
+        }                                                                                 \
+        if constexpr ((R) >= 3) {                                                         \
+            _Pragma("unroll") for (int a = 0; a < 4; a++) {                               \
+                v[a] = gf_add(v[a], gf_mul_reg(v[a + 4], gf_ld((t2) + (low | ((size_t)a << (K)))))); \
+                v[a + 4] = gf_add(v[a + 4], v[a]);                                        \
+            }                                                                             \
+        }                                                                                 \
+        _Pragma("unroll") for (int i = 0; i < E_; i++) gf_st(p + i * (stride), v[i]);     \
+    } while (0)
+
 template <int R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ntt_multi(gf *__restrict__ cw, const gf *__restrict__ t0,
-                                                   const gf *__restrict__ t1, const gf *__restrict__ t2, int K,
-                                                   size_t groups) {
-    constexpr int E = 1 << R;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ntt_multi(
+    gf *__restrict__ cw, const gf *__restrict__ t0, const gf *__restrict__ t1, const gf *__restrict__ t2, int K,
+    size_t groups) {
     const size_t mK = ((size_t)1 << K) - 1, stride = (size_t)1 << K;
     for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < groups;
-         g += (size_t)gridDim.x * blockDim.x) {
-        const size_t low = g & mK;
-        gf *p = cw + (((g >> K) << (K + R)) | low);
-        gf v[E];
-#pragma unroll
-        for (int i = 0; i < E; i++) v[i] = gf_ld(p + i * stride);
-        {
-            const gf_pre w = gf_prepare(gf_ld(t0 + low));
-#pragma unroll
-            for (int i = 0; i < E; i += 2) {
-                v[i] = gf_add(v[i], gf_mul_pre(v[i + 1], w));
-                v[i + 1] = gf_add(v[i + 1], v[i]);
-            }
-        }
-        if constexpr (R >= 2) {
-#pragma unroll
-            for (int a = 0; a < 2; a++) {
-                const gf_pre w = gf_prepare(gf_ld(t1 + (low | ((size_t)a << K))));
-#pragma unroll
-                for (int i = a; i < E; i += 4) {
-                    v[i] = gf_add(v[i], gf_mul_pre(v[i + 2], w));
-                    v[i + 2] = gf_add(v[i + 2], v[i]);
-                }
-            }
-        }
-        if constexpr (R >= 3) {
-#pragma unroll
-            for (int a = 0; a < 4; a++) {
-                v[a] = gf_add(v[a], gf_mul_reg(v[a + 4], gf_ld(t2 + (low | ((size_t)a << K)))));
-                v[a + 4] = gf_add(v[a + 4], v[a]);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < E; i++) gf_st(p + i * stride, v[i]);
-    }
+         g += (size_t)gridDim.x * blockDim.x)
+        NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g);
 }
 
 __global__ void k_replicate(const gf *__restrict__ data, gf *__restrict__ cw, int c, size_t n) {
@@ -439,6 +440,22 @@ __global__ __launch_bounds__(256) void ks_ntt_layer(const Seg *segs, int nseg) {
     }
 }
 
+// out = codeword (in place), aux = table of layer s, arg = K = s + b,
+// count = groups of 2^R elements; the tables of layers s+1 and s+2 follow
+// that of layer s contiguously (2^K and 2^(K+1) entries after it)
+template <int R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ks_ntt_multi(const Seg *segs,
+                                                                                            int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    const int K = (int)g.arg;
+    const gf *__restrict__ t0 = (const gf *)g.aux;
+    const gf *__restrict__ t1 = t0 + ((size_t)1 << K);
+    const gf *__restrict__ t2 = t1 + ((size_t)2 << K);
+    gf *__restrict__ cw = (gf *)g.out;
+    const size_t mK = ((size_t)1 << K) - 1, stride = (size_t)1 << K;
+    SEG_LOOP(g, q) NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, q);
+}
+
 __global__ __launch_bounds__(KB) void ks_leaves_elem(const Seg *segs, int nseg) {
     const Seg &g = my_seg(segs, nseg);
     const gf *rows = (const gf *)g.in;
@@ -721,6 +738,8 @@ hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int ns
     switch (which) {
     case SegKernel::NttFirst: ks_ntt_first<<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
     case SegKernel::NttLayer: ks_ntt_layer<<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::NttMulti2: ks_ntt_multi<2><<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::NttMulti3: ks_ntt_multi<3><<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
     case SegKernel::LeavesElem: ks_leaves_elem<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::LeavesPair: ks_leaves_pair<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Nodes: ks_nodes<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
@@ -735,7 +754,9 @@ hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int ns
 int seg_block_size(SegKernel which) {
     switch (which) {
     case SegKernel::NttFirst:
-    case SegKernel::NttLayer: return 256;
+    case SegKernel::NttLayer:
+    case SegKernel::NttMulti2:
+    case SegKernel::NttMulti3: return 256;
     case SegKernel::LeavesElem:
     case SegKernel::LeavesPair:
     case SegKernel::Nodes: return KB;

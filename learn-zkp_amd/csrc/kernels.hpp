@@ -22,7 +22,7 @@ struct Seg {
     uint32_t nblk;    // blocks of the segment
 };
 
-enum class SegKernel { NttFirst, NttLayer, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer };
+enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer };
 
 namespace launch {
 
