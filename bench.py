@@ -45,6 +45,29 @@ def algorithmic_bytes(n: int, log2_c: int) -> dict:
     return {"commit": commit, "open": opn, "total": commit + opn}
 
 
+def valu_work(n: int, log2_c: int) -> tuple:
+    """Keccak-f permutations K and GF(2^128) multiplications G of one
+    commit+open: K = (2M-1) + (2M-2c-n) (leaves and nodes of the main tree and
+    of every pi tree), G = n*M/2 (NTT) + (2M-2c) (two per fold output) +
+    (2N-2) (fix_last_var and the inner products) + N/2 (eq table)."""
+    N, c = 1 << n, 1 << log2_c
+    M = N * c
+    return (2 * M - 1) + (2 * M - 2 * c - n), n * M // 2 + (2 * M - 2 * c) + (2 * N - 2) + N // 2
+
+
+SUBTREE_FROM = 1 << 17
+
+
+def single_level_nodes(leaves: int) -> int:
+    """Nodes hashed by k_nodes launches in a tree over `leaves` digests: every
+    level whose input layer holds more than SUBTREE_FROM digests."""
+    nodes, h = 0, leaves
+    while h > SUBTREE_FROM:
+        nodes += h // 2
+        h //= 2
+    return nodes
+
+
 def traffic_ratio(kernel: str):
     path = os.path.join(ROOT, "profiles", "basefold_2e24_rate4_kernels.md")
     try:
@@ -73,8 +96,20 @@ def cpu_baseline(args) -> dict:
         times.append(time.perf_counter() - t0)
     t = statistics.median(times)
     return {"value": (1 << n) / t, "unit": "elems/s", "cores": cores, "kind": "port",
-            "sample": f"commit+open of 2^{n} evaluations at rate 1/4 (median of {len(times)}, "
-                      f"C restatement of the reference CPU path, OpenMP {cores} threads, PCLMUL)"}
+            "sample": f"commit+open of 2^{n} evaluations at rate 1/4 (median of {len(times)} after a warm-up, "
+                      f"C restatement of the reference CPU path, OpenMP {cores} threads, PCLMUL; host "
+                      f"{cpu_model()}, nproc {os.cpu_count()})"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def batched_vs_plain(bfd, ctx, torch, reps: int) -> dict:
@@ -198,7 +233,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log-n", type=int, default=LOG_N)
     ap.add_argument("--cpu-log-n", type=int, default=20)
-    ap.add_argument("--cpu-samples", type=int, default=3)
+    ap.add_argument("--cpu-samples", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain figure")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving trace length (0: skip)")
@@ -262,12 +297,19 @@ def main() -> None:
     name, (kms, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
     M = N << LOG2_C
     c = 1 << LOG2_C
+    # Merkle levels whose input layer holds more than SUBTREE_FROM digests run
+    # as one k_nodes launch each; the levels above them run in k_subtree (the
+    # library's launch plan for one tree).  Trees per step: the main tree over
+    # M leaves and the pi tree of every round k over (M >> k) / 2 leaves.
+    trees = [M] + [(M >> k) // 2 for k in range(n)]
+    node_launches = sum(max(0, t.bit_length() - 1 - (SUBTREE_FROM.bit_length() - 1)) for t in trees)
     per_step_units = {
-        # Keccak node hashing: main tree M-1 nodes + pi trees sum(L_k/2 - 1)
-        "k_nodes": (M - 1) + sum((M >> k) // 2 - 1 for k in range(n)),
+        "k_nodes": sum(single_level_nodes(t) for t in trees),
         "k_leaves_elem": M,
         "k_leaves_pair": sum((M >> k) // 2 for k in range(n)),
     }
+    if "k_nodes" in ktimes and ktimes["k_nodes"][1] != node_launches * args.steps:
+        raise SystemExit(f"k_nodes launches {ktimes['k_nodes'][1]} differ from the plan's {node_launches * args.steps}")
     bytes_per_unit = {"k_nodes": 96, "k_leaves_elem": 48, "k_leaves_pair": 64}
     if name in per_step_units:
         alg_bytes = per_step_units[name] * bytes_per_unit[name] * args.steps
@@ -288,12 +330,21 @@ def main() -> None:
     # second inside it against an isolated, memory-free Keccak-f kernel
     perms_per_step = {"k_nodes": per_step_units["k_nodes"], "k_leaves_elem": M,
                       "k_leaves_pair": per_step_units["k_leaves_pair"]}.get(name)
+    ctx.microbench(2, 1 << 20, 2)
+    r_k = (1 << 20) * 64 / (ctx.microbench(2, 1 << 20, 64) * 1e-3)
+    ctx.microbench(1, 1 << 22, 2)
+    r_g = (1 << 22) * 64 / (ctx.microbench(1, 1 << 22, 64) * 1e-3)
     valu = None
     if perms_per_step:
-        ctx.microbench(2, 1 << 20, 2)
-        peak = (1 << 20) * 16 / (ctx.microbench(2, 1 << 20, 16) * 1e-3)
         rate = perms_per_step * args.steps / (kms * 1e-3)
-        valu = {"keccak_perm_per_s": rate, "isolated_keccak_perm_per_s": peak, "frac": rate / peak}
+        valu = {"keccak_perm_per_s": rate, "isolated_keccak_perm_per_s": r_k, "frac": rate / r_k}
+    # whole-step VALU position: time the step's Keccak-f permutations and
+    # GF(2^128) multiplications would take at the isolated kernels' rates,
+    # over the measured step time
+    K, G = valu_work(n, LOG2_C)
+    t_step = t / args.steps
+    valu_step = {"keccak_perms": K, "gf_muls": G, "isolated_keccak_perm_per_s": r_k,
+                 "isolated_gf_mul_per_s": r_g, "frac": (K / r_k + G / r_g) / t_step}
 
     result = {
         "metric": "Basefold commit+open field-elems/s (2^24 evals, rate 1/4)",
@@ -315,6 +366,7 @@ def main() -> None:
                      "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic, "avg_launch_ms": kms / launches, "launches_per_step": launches / args.steps},
         "valu": valu,
+        "valu_step": valu_step,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(ktimes.items())},
         "proof_bytes": len(proof),
     }

@@ -618,14 +618,20 @@ __global__ __launch_bounds__(CB) void k_bench_mul_reg(gf *io, size_t n, int iter
         gf_st(io + 2 * i, a);
     }
 }
+// `iters` Keccak-f permutations per state, states stored lane-major
+// (io[k * n + i]) so loads and stores are coalesced: the VALU ceiling of the
+// hashing kernels
 __global__ __launch_bounds__(KB) void k_bench_keccak(uint64_t *io, size_t n, int iters) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        uint64_t s[25];
+        kk::lane s[25];
 #pragma unroll
-        for (int k = 0; k < 25; k++) s[k] = io[25 * i + k];
+        for (int k = 0; k < 25; k++) {
+            const uint64_t v = io[k * n + i];
+            s[k] = kk::lane{(uint32_t)v, (uint32_t)(v >> 32)};
+        }
         for (int k = 0; k < iters; k++) kk::keccak_f(s);
 #pragma unroll
-        for (int k = 0; k < 25; k++) io[25 * i + k] = s[k];
+        for (int k = 0; k < 25; k++) io[k * n + i] = (uint64_t)s[k].lo | ((uint64_t)s[k].hi << 32);
     }
 }
 
