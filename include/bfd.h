@@ -157,8 +157,9 @@ int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out);
 
 /* ---- measurement helpers ---- */
 /* Times `iters` dependent multiplies (variant 0: LDS comb, 1: register
- * Karatsuba), Keccak-f permutations (2) on n lanes, or one copy of n
- * 16-byte elements (3); returns milliseconds. */
+ * Karatsuba), `iters` chained Keccak-256 hashes of 64-byte messages (2, the
+ * Merkle node hash) on n lanes, or one copy of n 16-byte elements (3);
+ * returns milliseconds.  No reference counterpart (measurement only). */
 int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms);
 /* Per-kernel timing of the last commit/open on this context: fills up to
  * `cap` (name, total ms, launches) records; returns the count in *count. */

@@ -5,12 +5,17 @@
 // p3/src/polyval/portable.rs:20-28): 16 bytes, little-endian, low 64 bits first.
 // Addition is XOR; multiplication is the POLYVAL "dot" a*b*X^-128.
 //
-// CDNA4 has no carry-less multiply.  The device multiply is a 4-bit-window
-// comb: per lane, a 16-entry table of v*b (v a 4-bit polynomial, entries of
-// 131 bits = 5 dwords) lives in LDS; the 32 nibbles of a index it, eight
-// nibble positions per 32-bit word are processed together with one 256-bit
-// shift by 4 between positions.  The 256-bit product is then reduced with
-// the two-step Montgomery fold of the reference backends (x86_64_avx2.rs:41-68).
+// CDNA4 has no carry-less multiply.  The device multiply used by every kernel
+// is gf_mul_reg / gf_mul_pre (register Karatsuba over 32x32 carry-less
+// products built from masked integer multiply-adds, below); about 520 VALU
+// instructions per product.  The 256-bit product is reduced with the two-step
+// Montgomery fold of the reference backends (x86_64_avx2.rs:41-68).
+//
+// gf_comb_* is a second device multiply kept for the micro-benchmark only: a
+// 4-bit-window comb with a per-lane 16-entry table of v*b in LDS (entries of
+// 131 bits = 5 dwords), indexed by the 32 nibbles of a, eight nibble
+// positions per 32-bit word together with one 256-bit shift by 4 between
+// positions.  It measures about 30% slower than the register form.
 #pragma once
 #include <stdint.h>
 

@@ -618,20 +618,25 @@ __global__ __launch_bounds__(CB) void k_bench_mul_reg(gf *io, size_t n, int iter
         gf_st(io + 2 * i, a);
     }
 }
-// `iters` Keccak-f permutations per state, states stored lane-major
-// (io[k * n + i]) so loads and stores are coalesced: the VALU ceiling of the
-// hashing kernels
+// `iters` chained Keccak-256 hashes of 64-byte messages per thread (the node
+// hash of k_nodes: next message = digest || digest ^ previous digest), the
+// messages stored word-major (io[k * n + i]) so loads and stores are
+// coalesced: the memory-free ceiling of the Merkle hashing kernels
 __global__ __launch_bounds__(KB) void k_bench_keccak(uint64_t *io, size_t n, int iters) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        kk::lane s[25];
+        uint64_t m[8], d[4];
 #pragma unroll
-        for (int k = 0; k < 25; k++) {
-            const uint64_t v = io[k * n + i];
-            s[k] = kk::lane{(uint32_t)v, (uint32_t)(v >> 32)};
+        for (int k = 0; k < 8; k++) m[k] = io[k * n + i];
+        for (int k = 0; k < iters; k++) {
+            kk::keccak256_words<8>(m, d);
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                m[4 + w] = m[w] ^ d[w];
+                m[w] = d[w];
+            }
         }
-        for (int k = 0; k < iters; k++) kk::keccak_f(s);
 #pragma unroll
-        for (int k = 0; k < 25; k++) io[k * n + i] = (uint64_t)s[k].lo | ((uint64_t)s[k].hi << 32);
+        for (int k = 0; k < 8; k++) io[k * n + i] = m[k];
     }
 }
 
