@@ -168,6 +168,41 @@ def batched_vs_plain(bfd, ctx, torch, reps: int) -> dict:
             "batched_elems_per_s": total / t_b, "speedup": t_p / t_b, "proofs_identical": True}
 
 
+def reference_shapes(bfd, ctx, torch, reps: int) -> list:
+    """The reference's own criterion benchmarks (poly-commit/benches/poly-commit.rs:95-96,
+    147-158): commit, then open with a fresh GenericChallenger::keccak256(), of a
+    2^log_n x 256 row-major matrix (one polynomial per column) at rate 1/2, log_n 16..19.
+    Random device-resident data (the benchmark draws DenseMatrix::rand); medians of `reps`."""
+    out = []
+    w, log2_c = 256, 1
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(0)
+    for log_n in range(16, 20):
+        N = 1 << log_n
+        data = torch.randint(-(2 ** 63), 2 ** 63 - 1, (N * w * 2,), dtype=torch.int64, device="cuda:0",
+                             generator=gen)
+        point = bfd.random_vec(log_n, log_n)
+        pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(LAMBDA, log2_c, log_n)), ctx)
+        torch.cuda.synchronize()
+        tc, to = [], []
+        for _ in range(reps + 1):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            cd = pcs.commit_device(data.data_ptr(), N, w)
+            tc.append(time.perf_counter() - t0)
+            values = pcs.eval(cd, point)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            pcs.open(cd, [bfd.PolyEvals(values, point)], bfd.GenericChallenger.keccak256())
+            to.append(time.perf_counter() - t0)
+            del cd
+        c, o = statistics.median(tc[1:]), statistics.median(to[1:])
+        out.append({"log_n": log_n, "log_polys": 8, "rate": "1/2", "commit_ms": 1e3 * c, "open_ms": 1e3 * o,
+                    "elems_per_s": N * w / (c + o)})
+        del data
+    return out
+
+
 def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int) -> dict:
     """Config 5: Poisson arrivals (mean `rate` req/s, numpy seed 0) of
     commit-then-open requests, log2 size uniform over {14..24}, rate 1/2 or 1/4
@@ -236,6 +271,8 @@ def main() -> None:
     ap.add_argument("--cpu-samples", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain figure")
+    ap.add_argument("--no-ref-shapes", action="store_true",
+                    help="skip the reference benchmark's own shapes (2^16..2^19 rows x 256 columns)")
     ap.add_argument("--serve-requests", type=int, default=400, help="serving trace length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -370,6 +407,8 @@ def main() -> None:
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(ktimes.items())},
         "proof_bytes": len(proof),
     }
+    if not args.no_ref_shapes:
+        result["reference_shapes"] = reference_shapes(bfd, ctx, torch, 3)
     if not args.no_batch:
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps)
     if args.serve_requests > 0:

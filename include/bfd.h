@@ -70,7 +70,8 @@ int bfd_challenger_sample_bits(bfd_challenger *c, uint32_t bits, uint64_t *out);
 
 /* ---- PolyCommitScheme ---- */
 /* Basefold::commit (basefold.rs:112-120): evals is the height x width
- * row-major matrix of evaluations (width 1 supported), height = 2^d. */
+ * row-major matrix of evaluations, one polynomial per column (width 1 ..
+ * 2^24), height = 2^d; each Merkle leaf hashes one 16*width-byte row. */
 int bfd_commit(bfd_ctx *ctx, const bfd_code *code, const uint8_t *evals, size_t height, size_t width,
                bfd_commitment **out, uint8_t root[32]);
 /* Same, with the evaluations already resident in device memory (not copied
@@ -80,22 +81,27 @@ int bfd_commit_device(bfd_ctx *ctx, const bfd_code *code, const void *d_evals, s
 int bfd_commitment_root(const bfd_commitment *comm, uint8_t root[32]);
 int bfd_commitment_destroy(bfd_commitment *comm);
 
-/* Basefold::eval (basefold.rs:126-131): values[col] = MLE(col)(point), point has d elements */
+/* Basefold::eval (basefold.rs:126-131): values[col] = MLE(col)(point) for
+ * every column (width x 16 bytes), point has d elements */
 int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uint8_t *values);
 
-/* Basefold::open (basefold.rs:133-232) for one PolyEvals {point, values}.
- * The challenger is advanced in place (the reference takes it by value:
- * pass a clone to keep the original).  *proof is allocated by the library. */
+/* Basefold::open (basefold.rs:133-232) for one PolyEvals {point, values}
+ * (values: width x 16 bytes).  Width > 1 first samples the random scalars by
+ * eq from the challenger (basefold.rs:166-183).  The challenger is advanced
+ * in place (the reference takes it by value: pass a clone to keep the
+ * original).  *proof is allocated by the library. */
 int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
              const uint8_t *values, bfd_challenger *challenger, uint8_t **proof, size_t *proof_len);
 /* Same, writing into a caller buffer of `capacity` bytes (at least
- * bfd_proof_size); *proof_len receives the size written. */
+ * bfd_proof_size_width); *proof_len receives the size written. */
 int bfd_open_into(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
                   const uint8_t *values, bfd_challenger *challenger, uint8_t *proof, size_t capacity,
                   size_t *proof_len);
 void bfd_proof_free(uint8_t *proof);
-/* Exact bincode size of the proof for (lambda, log2_c, d). */
+/* Exact bincode size of the proof for (lambda, log2_c, d), width 1, and for
+ * a commitment of `width` columns (each opening carries a whole row). */
 uint64_t bfd_proof_size(uint32_t lambda, uint32_t log2_c, uint32_t d);
+uint64_t bfd_proof_size_width(uint32_t lambda, uint32_t log2_c, uint32_t d, uint64_t width);
 
 /* ---- batched commit+open of independent polynomials (uneven sizes) ----
  * The reference commits one matrix height per Basefold instance

@@ -35,6 +35,10 @@ _lib = None
 u8p = ctypes.POINTER(ctypes.c_uint8)
 
 
+# status codes of include/bfd.h
+BFD_OK, BFD_ERR_ARG, BFD_ERR_HIP, BFD_ERR_BUDGET, BFD_ERR_STATE = 0, 1, 2, 3, 4
+
+
 class BfdError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"bfd error {code}: {msg}")
@@ -88,6 +92,8 @@ def lib():
         L.bfd_proof_free.restype = None
         L.bfd_proof_size.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.bfd_proof_size.restype = ctypes.c_uint64
+        L.bfd_proof_size_width.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+        L.bfd_proof_size_width.restype = ctypes.c_uint64
         L.bfd_open_into.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u8p, u8p, ctypes.c_void_p,
                                     u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.bfd_sumcheck_product.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u8p, u8p, u8p, ctypes.c_void_p,
@@ -261,6 +267,7 @@ class BasefoldCommitmentData:
     handle: ctypes.c_void_p
     data: np.ndarray
     _owner: "Basefold" = field(repr=False, default=None)
+    width: int = 1
 
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
@@ -292,27 +299,36 @@ class Basefold:
     def num_queries(self) -> int:
         return int(lib().bfd_code_num_queries(self.code_h))
 
-    def commit(self, data) -> BasefoldCommitmentData:
-        data = _elems(data)
+    def commit(self, data, width: Optional[int] = None) -> BasefoldCommitmentData:
+        """RowMajorMatrix commit (basefold.rs:112-120).  ``data`` is the
+        height x width matrix as an array of shape (height, width, 2) (one
+        polynomial per column), or flat with ``width`` given (default 1)."""
+        arr = np.ascontiguousarray(data, dtype=np.uint64)
+        if width is None:
+            width = arr.shape[1] if arr.ndim == 3 else 1
+        flat = arr.reshape(-1, 2)
+        if width < 1 or len(flat) % width:
+            raise BfdError(BFD_ERR_ARG, "data does not hold whole rows of the given width")
         root = (ctypes.c_uint8 * 32)()
         h = ctypes.c_void_p()
-        _check(lib().bfd_commit(self.ctx.h, self.code_h, _p(data), len(data), 1, ctypes.byref(h), root))
-        return BasefoldCommitmentData(bytes(root), h, data, self)
+        _check(lib().bfd_commit(self.ctx.h, self.code_h, _p(flat), len(flat) // width, width, ctypes.byref(h), root))
+        return BasefoldCommitmentData(bytes(root), h, flat, self, width)
 
-    def commit_device(self, ptr: int, height: int) -> BasefoldCommitmentData:
-        """Commit evaluations already resident in device memory at ``ptr``."""
+    def commit_device(self, ptr: int, height: int, width: int = 1) -> BasefoldCommitmentData:
+        """Commit a height x width row-major matrix already resident in device memory at ``ptr``."""
         root = (ctypes.c_uint8 * 32)()
         h = ctypes.c_void_p()
-        _check(lib().bfd_commit_device(self.ctx.h, self.code_h, ctypes.c_void_p(ptr), height, 1,
+        _check(lib().bfd_commit_device(self.ctx.h, self.code_h, ctypes.c_void_p(ptr), height, width,
                                        ctypes.byref(h), root))
-        return BasefoldCommitmentData(bytes(root), h, None, self)
+        return BasefoldCommitmentData(bytes(root), h, None, self, width)
 
     def data(self, comm_data: BasefoldCommitmentData) -> np.ndarray:
         return comm_data.data
 
     def eval(self, comm_data: BasefoldCommitmentData, point) -> np.ndarray:
+        """One value per column (basefold.rs:126-131): shape (width, 2)."""
         point = _elems(point)
-        out = np.zeros((1, 2), dtype=np.uint64)
+        out = np.zeros((comm_data.width, 2), dtype=np.uint64)
         _check(lib().bfd_eval(self.ctx.h, comm_data.handle, _p(point), _p(out)))
         return out
 
@@ -321,6 +337,8 @@ class Basefold:
         if not evals:
             return b""
         point, values = _elems(evals[0].point), _elems(evals[0].values)
+        if len(values) != comm_data.width:
+            raise BfdError(BFD_ERR_ARG, "open needs one value per committed column")
         proof = u8p()
         plen = ctypes.c_size_t()
         _check(lib().bfd_open(self.ctx.h, self.code_h, comm_data.handle, _p(point), _p(values),
@@ -369,9 +387,9 @@ def commit_open_batch(lambda_: int, log2_c: int, evals: Sequence, points: Sequen
     return [roots[32 * i:32 * (i + 1)].tobytes() for i in range(count)], [b[:lens[i]] for i, b in enumerate(bufs)]
 
 
-def proof_size(lambda_: int, log2_c: int, d: int) -> int:
-    """Exact bincode size of a Basefold proof."""
-    return int(lib().bfd_proof_size(lambda_, log2_c, d))
+def proof_size(lambda_: int, log2_c: int, d: int, width: int = 1) -> int:
+    """Exact bincode size of a Basefold proof (openings carry whole rows of ``width``)."""
+    return int(lib().bfd_proof_size_width(lambda_, log2_c, d, width))
 
 
 class Server:

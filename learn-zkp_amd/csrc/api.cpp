@@ -17,13 +17,42 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     set_device(ctx);
     hipStream_t st = ctx->stream;
+    double t_setup = now_ms();
+    const size_t w = cm->w;
+    // width > 1 (basefold.rs:166-183): random scalars by eq combine the
+    // columns of the claims, the evaluations and the codeword rows; the rest
+    // of the opening runs on the combination
+    uint8_t claim_b[16];
+    const uint8_t *claim_p = values_b;
+    DevBuf comb;
+    const gf *poly0 = cm->data.as<gf>(), *pi0 = cm->cw.as<gf>();
+    if (w > 1) {
+        int lw = 0;
+        while (((size_t)1 << lw) < w) lw++;
+        std::vector<gf> rs(lw);
+        for (int i = 0; i < lw; i++) rs[i] = ch.sample();  // sample_ext_vec (challenger.rs:17-19)
+        std::vector<gf> sc = host_eq(rs.data(), lw);          // MultiPoly::eq, truncated to w
+        gf claim = gf_zero();
+        for (size_t t = 0; t < w; t++) claim = gf_add(claim, hmul(sc[t], load_elem(values_b + 16 * t)));
+        store_elem(claim_b, claim);
+        claim_p = claim_b;
+        comb = ctx_buf(ctx, 16 * (cm->N + cm->M + w));
+        gf *p_poly = comb.as<gf>(), *p_pi = p_poly + cm->N, *p_sc = p_pi + cm->M;
+        ctx->scalar_pin.ensure(16 * w);
+        memcpy(ctx->scalar_pin.p, sc.data(), 16 * w);
+        hip_check(hipMemcpyAsync(p_sc, ctx->scalar_pin.p, 16 * w, hipMemcpyHostToDevice, st), "copy scalars");
+        ctx->timer.run(st, "k_rowdot", [&] { return launch::rowdot(st, cm->data.as<gf>(), w, p_sc, p_poly, cm->N); });
+        ctx->timer.run(st, "k_rowdot", [&] { return launch::rowdot(st, cm->cw.as<gf>(), w, p_sc, p_pi, cm->M); });
+        poly0 = p_poly;
+        pi0 = p_pi;
+    }
     OpenRounds o;
-    o.init((int)cm->n, (int)cm->b, code->q, point_b, values_b);
+    o.init((int)cm->n, (int)cm->b, code->q, point_b, claim_p);
+    o.w = w;
     const int d = o.d;
     const size_t M = cm->M, c = (size_t)1 << o.b;
 
     // device scratch for the whole opening
-    double t_setup = now_ms();
     const int hn = o.half_eq_vars();
     DevBuf half_eq = ctx_buf(ctx, 16 * ((size_t)1 << hn));
     if (o.n > 0) device_eq(ctx, o.r.data(), hn, half_eq.as<gf>());
@@ -31,7 +60,11 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     OpenLayout lay;
     lay.plan(M, d);
     DevBuf pis = ctx_buf(ctx, 16 * lay.pis_elems(M, c)), trees = ctx_buf(ctx, 32 * lay.pitree_digests());
-    lay.cw = cm->cw.as<gf>();
+    lay.cw = pi0;
+    if (w > 1) {
+        lay.rows = cm->cw.as<gf>();
+        lay.w = w;
+    }
     lay.tree = cm->tree.as<uint64_t>();
     lay.pis = pis.as<gf>();
     lay.pitrees = trees.as<uint64_t>();
@@ -40,7 +73,7 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     ctx->pinned.ensure(32 + 16 * kMaxPartials + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
 
-    const gf *poly_cur = cm->data.as<gf>();
+    const gf *poly_cur = poly0;
     ctx->timer.host("open_setup", now_ms() - t_setup);
     double t_host = 0;
     for (int k = 0; k < d; k++) {
@@ -91,7 +124,7 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     hip_check(hipMemcpyAsync(he, oe, 16 * ne + 32 * nd, hipMemcpyDeviceToHost, st), "copy gather");
     hip_check(hipStreamSynchronize(st), "gather sync");
     ctx->timer.collect();
-    const size_t sz = proof_size(o.b, o.d, o.q);
+    const size_t sz = proof_size(o.b, o.d, o.q, o.w);
     if (*proof && *proof_len < sz) fail(BFD_ERR_ARG, "proof buffer too small");
     uint8_t *dst = *proof ? *proof : (uint8_t *)malloc(sz ? sz : 1);
     if (!dst) fail(BFD_ERR_ARG, "host allocation failed");
@@ -305,21 +338,24 @@ int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uin
         set_device(ctx);
         hipStream_t st = ctx->stream;
         const int n = (int)comm->n;
+        const size_t w = comm->w;
         std::vector<gf> x(n);
         for (int i = 0; i < n; i++) x[i] = load_elem(point + 16 * i);
         if (n == 0) {
-            hip_check(hipMemcpyAsync(values, comm->data.p, 16, hipMemcpyDeviceToHost, st), "copy");
+            hip_check(hipMemcpyAsync(values, comm->data.p, 16 * w, hipMemcpyDeviceToHost, st), "copy");
             hip_check(hipStreamSynchronize(st), "eval");
             return;
         }
-        // evaluate (multilinear.rs:200-214): fix the top variable repeatedly
-        DevBuf tmp = ctx_buf(ctx, 16 * (comm->N / 2));
+        // evaluate every column (basefold.rs:126-131, multilinear.rs:200-214):
+        // fix the top variable repeatedly; on the row-major matrix that is one
+        // flat fix over 2^i rows of w elements
+        DevBuf tmp = ctx_buf(ctx, 16 * (comm->N / 2) * w);
         const gf *cur = comm->data.as<gf>();
         for (int i = n - 1; i >= 0; i--) {
-            hip_check(launch::fix(st, cur, tmp.as<gf>(), (size_t)1 << i, x[i]), "fix");
+            hip_check(launch::fix(st, cur, tmp.as<gf>(), ((size_t)1 << i) * w, x[i]), "fix");
             cur = tmp.as<gf>();
         }
-        hip_check(hipMemcpyAsync(values, cur, 16, hipMemcpyDeviceToHost, st), "copy");
+        hip_check(hipMemcpyAsync(values, cur, 16 * w, hipMemcpyDeviceToHost, st), "copy");
         hip_check(hipStreamSynchronize(st), "eval");
     });
 }
@@ -354,6 +390,10 @@ int bfd_open_into(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm
 
 uint64_t bfd_proof_size(uint32_t lambda, uint32_t log2_c, uint32_t d) {
     return proof_size((int)log2_c, (int)d, num_queries(lambda, log2_c));
+}
+
+uint64_t bfd_proof_size_width(uint32_t lambda, uint32_t log2_c, uint32_t d, uint64_t width) {
+    return proof_size((int)log2_c, (int)d, num_queries(lambda, log2_c), width ? width : 1);
 }
 
 void bfd_proof_free(uint8_t *proof) { free(proof); }

@@ -256,6 +256,7 @@ struct bfd_ctx {
     bfd::rt::PinnedBuf gather_pin;  // opened values of a proof, device -> host
     bfd::rt::PinnedBuf seg_pin;     // segment tables of batched launches, host -> device
     bfd::rt::PinnedBuf eq_pin;      // eq-table halves, host -> device
+    bfd::rt::PinnedBuf scalar_pin;  // random scalars of a width > 1 opening, host -> device
     bfd::rt::KernelTimer timer;
     // blocks of the plain path on `stream`, reused across calls
     bfd::rt::BlockCache cache;
@@ -276,8 +277,9 @@ struct bfd_commitment {
     const bfd_code *code = nullptr;
     uint32_t n = 0, b = 0;
     size_t N = 0, M = 0;
-    bfd::rt::DevBuf data;                   // N evaluations
-    bfd::rt::DevBuf cw;                     // M codeword elements
+    size_t w = 1;                           // polynomials (columns of the row-major matrices)
+    bfd::rt::DevBuf data;                   // N x w evaluations
+    bfd::rt::DevBuf cw;                     // M x w codeword elements
     bfd::rt::DevBuf tree;                   // 2M-1 digests, leaf layer first
     uint8_t root[32];
     uint64_t bytes = 0;
@@ -410,6 +412,25 @@ inline void tree_nodes(bfd_ctx *ctx, uint64_t *layer0, size_t leaves) {
     }
 }
 
+// RS encode of a width-w row-major matrix (binary_reed_solomon.rs:67-82):
+// layer 0, then the remaining layers three per pass
+inline void encode_w(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, int b, int n, size_t w) {
+    hipStream_t st = ctx->stream;
+    const gf *ft = code->ft.as<gf>();
+    if (n == 0) {  // c copies of the single row
+        for (size_t u = 0; u < ((size_t)1 << b); u++)
+            hip_check(hipMemcpyAsync(cw + u * w, data, 16 * w, hipMemcpyDeviceToDevice, st), "replicate");
+        return;
+    }
+    ctx->timer.run(st, "k_ntt_first_w", [&] { return launch::ntt_first_w(st, data, cw, ft + code->ft_off[0], b, n, w); });
+    for (int s = 1; s < n; s += 3) {
+        const int R = std::min(3, n - s);
+        const gf *t0 = ft + code->ft_off[s];
+        const gf *t1 = R >= 2 ? ft + code->ft_off[s + 1] : nullptr, *t2 = R >= 3 ? ft + code->ft_off[s + 2] : nullptr;
+        ctx->timer.run(st, "k_ntt_multi_w", [&] { return launch::ntt_multi_w(st, cw, t0, t1, t2, R, s, b, n, w); });
+    }
+}
+
 inline void encode(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, int b, int n) {
     hipStream_t st = ctx->stream;
     const gf *ft = code->ft.as<gf>();
@@ -431,7 +452,7 @@ inline void encode(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, i
 inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, bool on_device, size_t height,
                size_t width, bfd_commitment **out, uint8_t root[32]) {
     if (!ctx || !code || !out || !evals) fail(BFD_ERR_ARG, "null argument");
-    if (width != 1) fail(BFD_ERR_ARG, "only width-1 commitments are supported");
+    if (width < 1 || width > ((size_t)1 << 24)) fail(BFD_ERR_ARG, "width must be 1..2^24");
     if (height != ((size_t)1 << code->d)) fail(BFD_ERR_ARG, "height must equal 2^d of the code");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     set_device(ctx);
@@ -443,21 +464,28 @@ inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, boo
     cm->b = code->log2_c;
     cm->N = height;
     cm->M = height << code->log2_c;
-    cm->bytes = 16 * cm->N + 16 * cm->M + 32 * (2 * cm->M - 1);
+    cm->w = width;
+    const size_t w = width;
+    cm->bytes = 16 * cm->N * w + 16 * cm->M * w + 32 * (2 * cm->M - 1);
     if (ctx->budget && ctx->resident + cm->bytes > ctx->budget)
         fail(BFD_ERR_BUDGET, "commitment exceeds the context's HBM budget");
     const double t0 = now_ms();
-    cm->data = ctx_buf(ctx, 16 * cm->N);
-    cm->cw = ctx_buf(ctx, 16 * cm->M);
+    cm->data = ctx_buf(ctx, 16 * cm->N * w);
+    cm->cw = ctx_buf(ctx, 16 * cm->M * w);
     cm->tree = ctx_buf(ctx, 32 * (2 * cm->M - 1));
     ctx->timer.host("commit_alloc", now_ms() - t0);
-    hip_check(hipMemcpyAsync(cm->data.p, evals, 16 * cm->N,
+    hip_check(hipMemcpyAsync(cm->data.p, evals, 16 * cm->N * w,
                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
               "copy evaluations");
     gf *data = cm->data.as<gf>(), *cw = cm->cw.as<gf>();
-    encode(ctx, code, data, cw, (int)cm->b, (int)cm->n);
     uint64_t *tree = cm->tree.as<uint64_t>();
-    ctx->timer.run(st, "k_leaves_elem", [&] { return launch::leaves_elem(st, cw, tree, cm->M); });
+    if (w == 1) {
+        encode(ctx, code, data, cw, (int)cm->b, (int)cm->n);
+        ctx->timer.run(st, "k_leaves_elem", [&] { return launch::leaves_elem(st, cw, tree, cm->M); });
+    } else {
+        encode_w(ctx, code, data, cw, (int)cm->b, (int)cm->n, w);
+        ctx->timer.run(st, "k_leaves_rows", [&] { return launch::leaves_rows(st, cw, w, tree, cm->M); });
+    }
     tree_nodes(ctx, tree, cm->M);
     hip_check(hipMemcpyAsync(cm->root, tree + 4 * (2 * cm->M - 2), 32, hipMemcpyDeviceToHost, st), "copy root");
     hip_check(hipStreamSynchronize(st), "commit");
@@ -499,6 +527,7 @@ namespace rt {
 struct OpenRounds {
     int n = 0, b = 0, d = 0, L = 0;
     size_t M = 0, q = 0;
+    size_t w = 1;  // elements per opened codeword row
     std::vector<gf> r, rinv, cf;
     gf claim{}, subclaim{}, inner_sub{}, s_delta{}, final_poly{};
     std::vector<std::array<gf, 2>> comp;
@@ -573,7 +602,9 @@ struct OpenRounds {
 
 // Device addresses of one polynomial's opening data.
 struct OpenLayout {
-    const gf *cw = nullptr;          // codeword (= pi of round 0)
+    const gf *cw = nullptr;          // pi of round 0: the codeword (width 1) or its scalar combination
+    const gf *rows = nullptr;        // codeword matrix, w elements per row (nullptr: cw, w = 1)
+    size_t w = 1;
     const uint64_t *tree = nullptr;  // main tree, 2M-1 digests, leaf layer first
     const gf *pis = nullptr;         // pi of rounds 1..d, at pi_off[k]
     const uint64_t *pitrees = nullptr;  // pi trees of rounds 0..d-1, at tree_off[k]
@@ -598,12 +629,12 @@ struct OpenLayout {
 
 inline size_t layer_off(size_t leaves, int t) { return 2 * leaves - 2 * (leaves >> t); }
 
-// Records one proof opens: per query the codeword row and L main-tree
-// siblings, then per round a pi sibling value and b+i pi-tree siblings.
+// Records one proof opens: per query the codeword row (w elements) and L
+// main-tree siblings, then per round a pi sibling value and b+i pi-tree siblings.
 inline void gather_counts(const OpenRounds &o, size_t &n_elems, size_t &n_digs) {
     size_t per_d = (size_t)o.L;
     for (int k = 0; k < o.d; k++) per_d += (size_t)(o.b + (o.d - 1 - k));
-    n_elems = o.q * (1 + (size_t)o.d);
+    n_elems = o.q * (o.w + (size_t)o.d);
     n_digs = o.q * per_d;
 }
 
@@ -611,7 +642,10 @@ inline void gather_counts(const OpenRounds &o, size_t &n_elems, size_t &n_digs) 
 inline void plan_gathers(const OpenRounds &o, const OpenLayout &lay, uint64_t *elems, uint64_t *digs) {
     for (size_t t = 0; t < o.q; t++) {
         const uint64_t id = o.idx[t];
-        *elems++ = (uint64_t)(lay.cw + id);
+        if (lay.rows)
+            for (size_t c = 0; c < lay.w; c++) *elems++ = (uint64_t)(lay.rows + id * lay.w + c);
+        else
+            *elems++ = (uint64_t)(lay.cw + id);
         for (int u = 0; u < o.L; u++) *digs++ = (uint64_t)(lay.tree + 4 * (layer_off(o.M, u) + ((id >> u) ^ 1)));
         for (int k = 0; k < o.d; k++) {
             const int i = o.d - 1 - k;
@@ -624,10 +658,11 @@ inline void plan_gathers(const OpenRounds &o, const OpenLayout &lay, uint64_t *e
     }
 }
 
-// Exact bincode size of BasefoldProof for d rounds, L = b + d, q queries.
-inline size_t proof_size(int b, int d, size_t q) {
+// Exact bincode size of BasefoldProof for d rounds, L = b + d, q queries and
+// w elements per opened codeword row.
+inline size_t proof_size(int b, int d, size_t q, size_t w = 1) {
     const size_t L = (size_t)(b + d);
-    size_t per_q = 8 + 16 + 8 + 32 * L + 8;
+    size_t per_q = 8 + 16 * w + 8 + 32 * L + 8;
     for (int k = 0; k < d; k++) per_q += 16 + 8 + 32 * (size_t)(b + (d - 1 - k));
     return 8 + 40 * (size_t)d + 8 + 16 + 8 + 32 * (size_t)d + 8 + q * per_q;
 }
@@ -655,9 +690,9 @@ inline void assemble_proof(const OpenRounds &o, const uint8_t *elems, const uint
     for (int k = 0; k < o.d; k++) w.bytes(o.pi_roots[k].data(), 32);
     w.u64((uint64_t)o.q);
     for (size_t t = 0; t < o.q; t++) {
-        w.u64(1);
-        w.bytes(elems, 16);
-        elems += 16;
+        w.u64((uint64_t)o.w);
+        w.bytes(elems, 16 * o.w);
+        elems += 16 * o.w;
         w.u64((uint64_t)o.L);
         w.bytes(digs, 32 * (size_t)o.L);
         digs += 32 * (size_t)o.L;

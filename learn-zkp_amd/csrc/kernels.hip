@@ -101,13 +101,13 @@ __global__ __launch_bounds__(256) void k_ntt_layer(gf *__restrict__ cw, const gf
 // The group body, expanded in place in both kernels below: written out as a
 // function and inlined, it makes the register allocator spill 650 bytes per
 // lane at two waves per SIMD, against 20 bytes when it is in the kernel body.
-#define NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g)                                   \
+#define NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g, pitch, col)                        \
     do {                                                                                  \
         constexpr int E_ = 1 << (R);                                                      \
         const size_t low = (g) & (mK);                                                    \
-        gf *p = (cw) + ((((g) >> (K)) << ((K) + (R))) | low);                             \
+        gf *p = (cw) + ((((g) >> (K)) << ((K) + (R))) | low) * (pitch) + (col);           \
         gf v[E_];                                                                         \
-        _Pragma("unroll") for (int i = 0; i < E_; i++) v[i] = gf_ld(p + i * (stride));    \
+        _Pragma("unroll") for (int i = 0; i < E_; i++) v[i] = gf_ld(p + i * (stride) * (pitch)); \
         {                                                                                 \
             const gf_pre w = gf_prepare(gf_ld((t0) + low));                               \
             _Pragma("unroll") for (int i = 0; i < E_; i += 2) {                           \
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_ntt_layer(gf *__restrict__ cw, const gf
                 v[a + 4] = gf_add(v[a + 4], v[a]);                                        \
             }                                                                             \
         }                                                                                 \
-        _Pragma("unroll") for (int i = 0; i < E_; i++) gf_st(p + i * (stride), v[i]);     \
+        _Pragma("unroll") for (int i = 0; i < E_; i++) gf_st(p + i * (stride) * (pitch), v[i]); \
     } while (0)
 
 template <int R>
@@ -140,7 +140,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const size_t mK = ((size_t)1 << K) - 1, stride = (size_t)1 << K;
     for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < groups;
          g += (size_t)gridDim.x * blockDim.x)
-        NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g);
+        NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g, 1, 0);
+}
+
+// Width-w codeword (row-major, w elements per row): the same passes with one
+// thread per (group, column); every column of a row uses the row's twiddles.
+template <int R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ntt_multi_w(
+    gf *__restrict__ cw, const gf *__restrict__ t0, const gf *__restrict__ t1, const gf *__restrict__ t2, int K,
+    size_t groups, size_t w) {
+    const size_t mK = ((size_t)1 << K) - 1, stride = (size_t)1 << K;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < groups * w;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = q / w, col = q - g * w;
+        NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g, w, col);
+    }
+}
+
+// layer 0 of a width-w matrix: data rows 2pr, 2pr+1 into codeword rows
+// (2pr << b) + u and ((2pr + 1) << b) + u, one thread per (pair, slot, column)
+__global__ __launch_bounds__(256) void k_ntt_first_w(const gf *__restrict__ data, gf *__restrict__ cw,
+                                                     const gf *__restrict__ ft0, int b, size_t work, size_t w) {
+    const size_t cm = ((size_t)1 << b) - 1;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < work * w;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const size_t it = q / w, col = q - it * w;
+        const size_t u = it & cm, pr = it >> b;
+        gf lo = gf_ld(data + (2 * pr) * w + col), hi = gf_ld(data + (2 * pr + 1) * w + col);
+        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft0 + u)));
+        hi = gf_add(hi, lo);
+        gf_st(cw + (((2 * pr) << b) + u) * w + col, lo);
+        gf_st(cw + (((2 * pr + 1) << b) + u) * w + col, hi);
+    }
 }
 
 __global__ void k_replicate(const gf *__restrict__ data, gf *__restrict__ cw, int c, size_t n) {
@@ -178,6 +209,56 @@ __global__ __launch_bounds__(KB) void k_leaves_pair(const gf *__restrict__ pi, u
         uint64_t m[4] = {a.lo, a.hi, b.lo, b.hi}, d[4];
         kk::keccak256_words<4>(m, d);
         st_digest(out + 4 * j, d);
+    }
+}
+
+// leaf i = Keccak256 of row i of a width-w matrix (16w bytes, 2w 64-bit
+// words): full 17-word blocks, then the last block with the remaining words
+// and the Keccak padding (0x01 after the message, 0x80 in the last rate byte)
+__global__ __launch_bounds__(KB) void k_leaves_rows(const gf *__restrict__ rows, size_t w, uint64_t *__restrict__ out,
+                                                    size_t count) {
+    const size_t nw = 2 * w, full = nw / 17, rem = nw % 17;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t *m = reinterpret_cast<const uint64_t *>(rows + i * w);
+        kk::lane s[25];
+#pragma unroll
+        for (int k = 0; k < 25; k++) s[k] = kk::lane{0u, 0u};
+        for (size_t blk = 0; blk < full; blk++, m += 17) {
+#pragma unroll
+            for (int k = 0; k < 17; k++) {
+                const uint64_t v = m[k];
+                s[k].lo ^= (uint32_t)v;
+                s[k].hi ^= (uint32_t)(v >> 32);
+            }
+            kk::keccak_f(s);
+        }
+#pragma unroll
+        for (int k = 0; k < 17; k++) {
+            uint64_t v = (size_t)k < rem ? m[k] : 0ull;
+            if ((size_t)k == rem) v ^= 0x01ull;
+            if (k == 16) v ^= 0x8000000000000000ull;
+            s[k].lo ^= (uint32_t)v;
+            s[k].hi ^= (uint32_t)(v >> 32);
+        }
+        kk::keccak_f(s);
+        uint64_t d[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) d[k] = (uint64_t)s[k].lo | ((uint64_t)s[k].hi << 32);
+        st_digest(out + 4 * i, d);
+    }
+}
+
+// out[j] = sum_t scalars[t] * m[j * w + t]  (row-wise dot_product,
+// basefold.rs:168-183)
+__global__ __launch_bounds__(CB) void k_rowdot(const gf *__restrict__ m, size_t w, const gf *__restrict__ scalars,
+                                               gf *__restrict__ out, size_t rows) {
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < rows;
+         j += (size_t)gridDim.x * blockDim.x) {
+        const gf *row = m + j * w;
+        gf acc = gf_zero();
+        for (size_t t = 0; t < w; t++) acc = gf_add(acc, gf_mul_reg(gf_ld(row + t), gf_ld(scalars + t)));
+        gf_st(out + j, acc);
     }
 }
 
@@ -453,7 +534,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const gf *__restrict__ t2 = t1 + ((size_t)2 << K);
     gf *__restrict__ cw = (gf *)g.out;
     const size_t mK = ((size_t)1 << K) - 1, stride = (size_t)1 << K;
-    SEG_LOOP(g, q) NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, q);
+    SEG_LOOP(g, q) NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, q, 1, 0);
 }
 
 __global__ __launch_bounds__(KB) void ks_leaves_elem(const Seg *segs, int nseg) {
@@ -685,6 +766,30 @@ hipError_t ntt_multi(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const g
     return hipGetLastError();
 }
 
+hipError_t ntt_first_w(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, size_t w) {
+    if (n < 1 || w < 1) return hipErrorInvalidValue;
+    const size_t work = ((size_t)1 << n) / 2 * ((size_t)1 << b);
+    k_ntt_first_w<<<grid_for(work * w, 256, kCapMul), 256, 0, st>>>(data, cw, ft0, b, work, w);
+    return hipGetLastError();
+}
+hipError_t ntt_multi_w(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n,
+                       size_t w) {
+    if (R < 1 || R > 3 || s + R > n || w < 1) return hipErrorInvalidValue;
+    const size_t groups = ((size_t)1 << (n + b)) >> R;
+    const unsigned g = grid_for(groups * w, 256, kCapMul);
+    if (R == 1) k_ntt_multi_w<1><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
+    else if (R == 2) k_ntt_multi_w<2><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
+    else k_ntt_multi_w<3><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
+    return hipGetLastError();
+}
+hipError_t leaves_rows(hipStream_t st, const gf *rows, size_t w, uint64_t *out, size_t count) {
+    k_leaves_rows<<<grid_for(count, KB, kCapHash), KB, 0, st>>>(rows, w, out, count);
+    return hipGetLastError();
+}
+hipError_t rowdot(hipStream_t st, const gf *m, size_t w, const gf *scalars, gf *out, size_t rows) {
+    k_rowdot<<<grid_for(rows, CB, kCapMul), CB, 0, st>>>(m, w, scalars, out, rows);
+    return hipGetLastError();
+}
 hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count) {
     k_leaves_elem<<<grid_for(count, KB, kCapHash), KB, 0, st>>>(rows, out, count);
     return hipGetLastError();

@@ -42,6 +42,15 @@ hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int 
 hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n);
 // R = 2 or 3 layers s .. s+R-1 in one pass; t_r is the table of layer s + r
 hipError_t ntt_multi(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n);
+// width-w forms (row-major matrix, w elements per row): layer 0, and R = 1..3
+// layers per pass on the codeword matrix
+hipError_t ntt_first_w(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, size_t w);
+hipError_t ntt_multi_w(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n,
+                       size_t w);
+// leaf i = Keccak256 of the 16w bytes of row i
+hipError_t leaves_rows(hipStream_t st, const gf *rows, size_t w, uint64_t *out, size_t count);
+// out[j] = sum_t scalars[t] * m[j*w + t]
+hipError_t rowdot(hipStream_t st, const gf *m, size_t w, const gf *scalars, gf *out, size_t rows);
 hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count);
 hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half);
 // one Merkle level: next[i] = H(prev[2i] || prev[2i+1])

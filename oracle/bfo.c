@@ -772,10 +772,40 @@ static void prover_free(prover *p) {
     rs_free(&p->rs);
 }
 
+/* sample_random_scalars_by_eq (basefold.rs:398-408): log2(next_pow2(w))
+ * challenges, their eq table truncated to w entries */
+static void scalars_by_eq(bfo_challenger *ch, size_t w, u128 *scalars) {
+    int lw = 0;
+    while (((size_t)1 << lw) < w) lw++;
+    u128 rr[64];
+    for (int i = 0; i < lw; i++) rr[i] = ch_sample(ch);     /* sample_ext_vec, challenger.rs:17-19 */
+    u128 *e = malloc(sizeof(u128) * ((size_t)1 << lw));
+    eq_table(rr, lw, ONE, e);
+    memcpy(scalars, e, sizeof(u128) * w);
+    free(e);
+}
+
+/* row-wise dot_product(&scalars, row) over a rows x w row-major matrix (basefold.rs:168-183) */
+static void row_dots(const u128 *m, size_t rows, size_t w, const u128 *scalars, u128 *out) {
+#pragma omp parallel for schedule(static) if (rows >= 1024)
+    for (size_t j = 0; j < rows; j++) {
+        u128 acc = 0;
+        for (size_t t = 0; t < w; t++) acc ^= fmul(scalars[t], m[j * w + t]);
+        out[j] = acc;
+    }
+}
+
 int bfo_basefold_prove(int lambda, int log2_c, int n, const uint8_t *evals_b,
                        const uint8_t *point_b, uint8_t root_out[32], uint8_t claim_out[16],
                        uint8_t **proof_out, size_t *proof_len) {
-    if (n < 0 || n > 40 || log2_c < 0 || log2_c > 8) return -1;
+    return bfo_basefold_prove_w(lambda, log2_c, n, 1, evals_b, point_b, root_out, claim_out, proof_out, proof_len);
+}
+
+int bfo_basefold_prove_w(int lambda, int log2_c, int n, size_t width, const uint8_t *evals_b,
+                         const uint8_t *point_b, uint8_t root_out[32], uint8_t *claims_out,
+                         uint8_t **proof_out, size_t *proof_len) {
+    if (n < 0 || n > 40 || log2_c < 0 || log2_c > 8 || width < 1) return -1;
+    const size_t w = width;
     prover P;
     memset(&P, 0, sizeof P);
     P.n = n; P.b = log2_c; P.d = n; P.L = log2_c + n;
@@ -784,29 +814,48 @@ int bfo_basefold_prove(int lambda, int log2_c, int n, const uint8_t *evals_b,
     rs_new(&P.rs, log2_c, n);
     int b = P.b, d = P.d;
     size_t N = P.N, M = P.M;
-    P.data = malloc(sizeof(u128) * N);
-    P.cw = malloc(sizeof(u128) * M);
-    memcpy(P.data, evals_b, 16 * N);
+    P.data = malloc(sizeof(u128) * N * w);
+    P.cw = malloc(sizeof(u128) * M * w);
+    memcpy(P.data, evals_b, 16 * N * w);
     u128 r[64], rinv[64];
     for (int i = 0; i < n; i++) r[i] = ld(point_b + 16 * i);
 
-    /* commit: encode(clone(data)) then Merkle over width-1 rows (basefold.rs:112-120) */
-    rs_encode(&P.rs, P.data, 1, P.cw);
-    merkle_build(&P.main, (const uint8_t *)P.cw, M, 16);
+    /* commit: encode_batch(clone(data)) then Merkle over the w-element rows (basefold.rs:112-120) */
+    rs_encode(&P.rs, P.data, w, P.cw);
+    merkle_build(&P.main, (const uint8_t *)P.cw, M, 16 * w);
     memcpy(root_out, merkle_root_ptr(&P.main), 32);
 
-    /* harness transcript prefix: observe(root), observe(claims) (poly-commit/src/lib.rs:72-90) */
+    /* harness transcript prefix: observe(root), observe_ext_slice(values) (poly-commit/src/lib.rs:82-92);
+     * values = one MLE evaluation per column (basefold.rs:126-131) */
     bfo_challenger *ch = bfo_challenger_new();
     bfo_challenger_observe(ch, merkle_root_ptr(&P.main), 32);
-    P.claim = mle_eval(P.data, n, r);
-    st(claim_out, P.claim);
-    ch_observe_elem(ch, P.claim);
+    u128 *claims = malloc(sizeof(u128) * w), *col = malloc(sizeof(u128) * N);
+    for (size_t t = 0; t < w; t++) {
+        for (size_t j = 0; j < N; j++) col[j] = P.data[j * w + t];
+        claims[t] = mle_eval(col, n, r);
+        st(claims_out + 16 * t, claims[t]);
+        ch_observe_elem(ch, claims[t]);
+    }
+    free(col);
 
-    /* open, width-1 branch (basefold.rs:159-165, 184-185) */
+    /* open (basefold.rs:157-185): width 1 takes the data and the codeword as they are; otherwise
+     * random scalars by eq combine the columns of data, claims and codeword rows */
     u128 *pi = malloc(sizeof(u128) * M);
-    memcpy(pi, P.cw, 16 * M);
     u128 *poly = malloc(sizeof(u128) * N);
-    memcpy(poly, P.data, 16 * N);
+    if (w == 1) {
+        memcpy(pi, P.cw, 16 * M);
+        memcpy(poly, P.data, 16 * N);
+        P.claim = claims[0];
+    } else {
+        u128 *sc = malloc(sizeof(u128) * w);
+        scalars_by_eq(ch, w, sc);
+        row_dots(P.data, N, w, sc, poly);
+        P.claim = 0;
+        for (size_t t = 0; t < w; t++) P.claim ^= fmul(sc[t], claims[t]);
+        row_dots(P.cw, M, w, sc, pi);
+        free(sc);
+    }
+    free(claims);
     for (int i = 0; i < n; i++) {
         if (r[i] == 0 || r[i] == ONE) { fprintf(stderr, "bfo: degenerate point coordinate\n"); abort(); }
         rinv[i] = finv(r[i]);                         /* EvalImpr::new, eval_impr.rs:20-27 */
@@ -874,8 +923,8 @@ int bfo_basefold_prove(int lambda, int log2_c, int n, const uint8_t *evals_b,
     bb_u64(&pb, (u64)P.q);
     for (size_t qi = 0; qi < P.q; qi++) {
         size_t idx = (size_t)bfo_challenger_sample_bits(ch, P.L);
-        bb_u64(&pb, 1);                               /* leaves: Vec<F> of the width-1 row */
-        bb_elem(&pb, P.cw[idx]);
+        bb_u64(&pb, (u64)w);                          /* leaves: Vec<F>, the codeword row */
+        for (size_t t = 0; t < w; t++) bb_elem(&pb, P.cw[idx * w + t]);
         bb_u64(&pb, (u64)P.main.depth);
         for (int t = 0; t < P.main.depth; t++) bb_put(&pb, P.main.layer[t] + 32 * ((idx >> t) ^ 1), 32);
         bb_u64(&pb, (u64)d);
@@ -904,12 +953,19 @@ void bfo_free(void *p) { free(p); }
 int bfo_basefold_verify(int lambda, int log2_c, int n, const uint8_t root[32],
                         const uint8_t *point_b, const uint8_t claim_b[16],
                         const uint8_t *proof, size_t proof_len) {
+    return bfo_basefold_verify_w(lambda, log2_c, n, 1, root, point_b, claim_b, proof, proof_len);
+}
+
+int bfo_basefold_verify_w(int lambda, int log2_c, int n, size_t width, const uint8_t root[32],
+                          const uint8_t *point_b, const uint8_t *claims_b,
+                          const uint8_t *proof, size_t proof_len) {
+    if (width < 1 || width > ((size_t)1 << 24)) return 4;
+    const size_t w = width;
     int d = n, b = log2_c, L = b + d;
     size_t q = bfo_num_queries(lambda, log2_c);
     brd rd = {proof, proof_len, 0, 0};
     u128 r[64];
     for (int i = 0; i < n; i++) r[i] = ld(point_b + 16 * i);
-    u128 claim = ld(claim_b);
     if (br_u64(&rd) != (u64)d) return 4;
     u128 comp[64][2];
     for (int k = 0; k < d; k++) {
@@ -924,7 +980,17 @@ int bfo_basefold_verify(int lambda, int log2_c, int n, const uint8_t root[32],
 
     bfo_challenger *ch = bfo_challenger_new();
     bfo_challenger_observe(ch, root, 32);
-    ch_observe_elem(ch, claim);
+    for (size_t t = 0; t < w; t++) ch_observe_elem(ch, ld(claims_b + 16 * t));
+    /* scalars and claim (basefold.rs:252-258) */
+    u128 *sc = malloc(sizeof(u128) * w), claim;
+    if (w == 1) {
+        sc[0] = ONE;
+        claim = ld(claims_b);
+    } else {
+        scalars_by_eq(ch, w, sc);
+        claim = 0;
+        for (size_t t = 0; t < w; t++) claim ^= fmul(sc[t], ld(claims_b + 16 * t));
+    }
     u128 subclaim = claim, chal[64];
     for (int k = 0; k < d; k++) {
         int i = d - 1 - k;
@@ -940,7 +1006,7 @@ int bfo_basefold_verify(int lambda, int log2_c, int n, const uint8_t root[32],
     }
     ch_observe_elem(ch, final_poly);
     /* final check (:273-276): evaluate(final, point_lo=[]) * eq_eval(point, r) */
-    if (fmul(final_poly, eq_eval(r, chal, (size_t)n)) != subclaim) { bfo_challenger_free(ch); return 1; }
+    if (fmul(final_poly, eq_eval(r, chal, (size_t)n)) != subclaim) { bfo_challenger_free(ch); free(sc); return 1; }
 
     rs_code rs;
     rs_new(&rs, log2_c, n);
@@ -948,15 +1014,15 @@ int bfo_basefold_verify(int lambda, int log2_c, int n, const uint8_t root[32],
     if (br_u64(&rd) != (u64)q) { rc = 4; goto out; }
     for (size_t qi = 0; qi < q && !rc; qi++) {
         size_t idx = (size_t)bfo_challenger_sample_bits(ch, L);
-        if (br_u64(&rd) != 1) { rc = 4; break; }
-        u128 leaf = br_elem(&rd);
+        if (br_u64(&rd) != (u64)w) { rc = 4; break; }
+        const uint8_t *lb = br_bytes(&rd, 16 * w);
         u64 np = br_u64(&rd);
         if (rd.err || np > 64) { rc = 4; break; }
         const uint8_t *sib = br_bytes(&rd, 32 * np);
         if (rd.err) { rc = 4; break; }
-        uint8_t lb[16];
-        st(lb, leaf);
-        if (!merkle_verify(root, (size_t)1 << L, idx, lb, 16, sib, np)) { rc = 3; break; }
+        if (!merkle_verify(root, (size_t)1 << L, idx, lb, 16 * w, sib, np)) { rc = 3; break; }
+        u128 leaf = 0;                                /* dot_product(scalars, opening row) (:297) */
+        for (size_t t = 0; t < w; t++) leaf ^= fmul(sc[t], ld(lb + 16 * t));
         if (br_u64(&rd) != (u64)d) { rc = 4; break; }
         u128 folded = leaf;
         for (int k = 0; k < d && !rc; k++) {
@@ -980,6 +1046,7 @@ int bfo_basefold_verify(int lambda, int log2_c, int n, const uint8_t root[32],
 out:
     rs_free(&rs);
     bfo_challenger_free(ch);
+    free(sc);
     if (!rc && rd.pos != rd.len) rc = 4;
     return rc;
 }

@@ -82,6 +82,18 @@ int bfo_basefold_prove(int lambda, int log2_c, int n, const uint8_t *evals,
 int bfo_basefold_verify(int lambda, int log2_c, int n, const uint8_t root[32],
                         const uint8_t *point, const uint8_t claim[16],
                         const uint8_t *proof, size_t proof_len);
+/*
+ * The same for a row-major 2^n x width matrix of evaluations (one polynomial
+ * per column): claims = width values (one per column), observed in column
+ * order by the harness; width > 1 takes the random-scalars-by-eq branch of
+ * open/verify (basefold.rs:166-183, 252-258, 398-408).
+ */
+int bfo_basefold_prove_w(int lambda, int log2_c, int n, size_t width, const uint8_t *evals,
+                         const uint8_t *point, uint8_t root[32], uint8_t *claims,
+                         uint8_t **proof, size_t *proof_len);
+int bfo_basefold_verify_w(int lambda, int log2_c, int n, size_t width, const uint8_t root[32],
+                          const uint8_t *point, const uint8_t *claims,
+                          const uint8_t *proof, size_t proof_len);
 void bfo_free(void *p);
 
 /*

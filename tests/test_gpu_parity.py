@@ -68,6 +68,64 @@ def test_basefold_medium_bit_exact(bfd, oracle, ctx, n, log2_c):
     assert root == root_o and (claim == claim_o).all() and proof == proof_o
 
 
+def gpu_prove_w(bfd, ctx, lam, log2_c, evals, width, point):
+    n = len(point)
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(lam, log2_c, n)), ctx)
+    cd = pcs.commit(evals, width)
+    claims = pcs.eval(cd, point)
+    ch = bfd.GenericChallenger.keccak256()
+    ch.observe(cd.comm)
+    ch.observe_ext_slice(claims)
+    proof = pcs.open(cd, [bfd.PolyEvals(claims, point)], ch)
+    return cd.comm, claims, proof
+
+
+def test_width_grid_bit_exact(bfd, oracle, ctx):
+    """The reference harness grid (poly-commit/src/lib.rs:73-77: num_vars 0..9 x num_polys 1..9),
+    rate 1/2: root, the width claims and the proof bytes equal the restatement's."""
+    for n in range(0, 10):
+        for w in range(1, 10):
+            _say(f"case width n={n} w={w}")
+            evals, point = bfd.random_vec(1000 + 10 * n + w, (1 << n) * w), bfd.random_vec(2000 + n, n)
+            root, claims, proof = gpu_prove_w(bfd, ctx, 128, 1, evals, w, point)
+            root_o, claims_o, proof_o = oracle.basefold_prove_w(128, 1, evals, w, point)
+            assert root == root_o, (n, w)
+            assert (claims == claims_o).all(), (n, w)
+            assert proof == proof_o, (n, w)
+            assert len(proof) == bfd.proof_size(128, 1, n, w)
+
+
+@pytest.mark.parametrize("n,log2_c,w", [(16, 1, 256), (12, 2, 33)])
+def test_width_reference_bench_shape_bit_exact(bfd, oracle, ctx, n, log2_c, w):
+    """poly-commit/benches/poly-commit.rs:95-96 commits 2^16..2^19 rows x 256 columns at rate 1/2;
+    the smallest of those shapes (and an odd width at rate 1/4) against the restatement."""
+    _say(f"case width n={n} log2_c={log2_c} w={w}")
+    evals, point = bfd.random_vec(31 + w, (1 << n) * w), bfd.random_vec(32, n)
+    root, claims, proof = gpu_prove_w(bfd, ctx, 128, log2_c, evals, w, point)
+    root_o, claims_o, proof_o = oracle.basefold_prove_w(128, log2_c, evals, w, point)
+    assert root == root_o and (claims == claims_o).all() and proof == proof_o
+    assert oracle.basefold_verify_w(128, log2_c, w, root, point, claims, proof) == 0
+
+
+def test_width_commit_device_and_reopen(bfd, oracle, ctx):
+    """Device-resident width-w matrix: the same root and proof as from host memory, twice."""
+    import torch as torch_mod
+    n, log2_c, w = 10, 1, 6
+    evals, point = bfd.random_vec(77, (1 << n) * w), bfd.random_vec(78, n)
+    dev = torch_mod.from_numpy(evals.view(np.int64).reshape(-1)).to("cuda:0")
+    torch_mod.cuda.synchronize()
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, n)), ctx)
+    cd = pcs.commit_device(dev.data_ptr(), 1 << n, w)
+    claims = pcs.eval(cd, point)
+    root_o, claims_o, proof_o = oracle.basefold_prove_w(128, log2_c, evals, w, point)
+    assert cd.comm == root_o and (claims == claims_o).all()
+    for _ in range(2):
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(cd.comm)
+        ch.observe_ext_slice(claims)
+        assert pcs.open(cd, [bfd.PolyEvals(claims, point)], ch) == proof_o
+
+
 def test_encode_matches_oracle_codeword_root(bfd, oracle, ctx):
     """The main-tree root pins the whole codeword (every leaf)."""
     for n, log2_c in [(5, 0), (6, 1), (7, 2), (12, 2)]:

@@ -54,3 +54,11 @@ def test_proof_size_matches_oracle(bfd, oracle):
             assert bfd.proof_size(128, log2_c, n) == len(proof)
     assert bfd.proof_size(128, 1, 16) == 1645080
     assert bfd.proof_size(128, 2, 24) == 2235000
+
+
+def test_proof_size_width_matches_oracle(bfd, oracle):
+    """bfd_proof_size_width (pure host arithmetic) against the restatement's bincode length."""
+    for n, log2_c, w in [(0, 1, 3), (5, 1, 1), (6, 2, 4), (7, 0, 9)]:
+        e, p = oracle.random_vec(5, (1 << n) * w), oracle.random_vec(6, n)
+        _, _, proof = oracle.basefold_prove_w(128, log2_c, e, w, p)
+        assert bfd.proof_size(128, log2_c, n, w) == len(proof)

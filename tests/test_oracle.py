@@ -197,3 +197,117 @@ def test_product_sumcheck_invariants(oracle):
         assert (fin[0:1] == fx).all() and (fin[1:2] == gx).all()
         eq = O.mle_evaluate(O.eq_table(r), x)
         assert (O.mul(O.mul(fx, gx), eq) == sub).all()
+
+
+# ---- width > 1 (one polynomial per column of a row-major matrix) ----
+def one(O):
+    return O.from_canonical(O.from_int(1))
+
+
+def eq_by_product(O, r, count):
+    """eq(r, j) = prod_i (r_i if bit i of j else 1 + r_i), straight from the definition."""
+    o = one(O)
+    out = []
+    for j in range(count):
+        acc = o
+        for i in range(len(r)):
+            f = r[i:i + 1] if (j >> i) & 1 else (r[i:i + 1] ^ o)
+            acc = O.mul(acc, f)
+        out.append(acc[0])
+    return np.array(out, dtype=np.uint64).reshape(-1, 2)
+
+
+def test_width_prove_verify_grid(oracle):
+    """The reference's own harness grid (poly-commit/src/lib.rs:73-77: num_vars 0..9 x num_polys
+    1..9) at rate 1/2: every proof verifies, claims are the column MLEs."""
+    O = oracle
+    for n in range(0, 10):
+        for w in range(1, 10):
+            e, p = rnd(O, (1 << n) * w, 300 + 10 * n + w), rnd(O, n, 400 + n)
+            root, claims, proof = O.basefold_prove_w(128, 1, e, w, p)
+            assert O.basefold_verify_w(128, 1, w, root, p, claims, proof) == 0, (n, w)
+            cols = e.reshape(1 << n, w, 2)
+            for t in range(w):
+                assert (claims[t:t + 1] == O.mle_evaluate(np.ascontiguousarray(cols[:, t]), p)).all()
+
+
+def test_width_encode_is_columnwise(oracle):
+    """encode_batch of a w-column matrix equals the width-1 encoding of each column."""
+    O = oracle
+    for log2_c, d, w in [(1, 4, 3), (2, 5, 5), (0, 3, 2)]:
+        m = rnd(O, (1 << d) * w, 500 + w)
+        cw = O.rs_encode(log2_c, d, m, w).reshape(-1, w, 2)
+        cols = m.reshape(1 << d, w, 2)
+        for t in range(w):
+            assert (cw[:, t] == O.rs_encode(log2_c, d, np.ascontiguousarray(cols[:, t]), 1)).all()
+
+
+def parse_head(proof, d):
+    """bincode head of BasefoldProof: round polys, final poly, pi roots."""
+    pos, u64 = 0, lambda b, o: int.from_bytes(b[o:o + 8], "little")
+    assert u64(proof, pos) == d
+    pos += 8
+    comps = []
+    for _ in range(d):
+        assert u64(proof, pos) == 2
+        comps.append((proof[pos + 8:pos + 24], proof[pos + 24:pos + 40]))
+        pos += 40
+    assert u64(proof, pos) == 1
+    final = proof[pos + 8:pos + 24]
+    pos += 24
+    assert u64(proof, pos) == d
+    roots = [proof[pos + 8 + 32 * k:pos + 40 + 32 * k] for k in range(d)]
+    return comps, final, roots
+
+
+def test_width_scalars_pinned_by_final_poly(oracle):
+    """basefold.rs:166-183, 398-408 restated independently: replay the verifier transcript
+    (root, claims, log2(next_pow2(w)) scalar samples, then per round pi root, c0, c2 and the
+    challenge), build the scalars by the eq product definition, combine the columns, and check
+    that the proof's final poly is that combination's MLE at the round challenges."""
+    O = oracle
+    for n, w, log2_c in [(5, 3, 1), (6, 5, 2), (4, 8, 1), (3, 9, 0)]:
+        e, p = rnd(O, (1 << n) * w, 600 + w), rnd(O, n, 700 + n)
+        root, claims, proof = O.basefold_prove_w(128, log2_c, e, w, p)
+        comps, final, roots = parse_head(proof, n)
+        ch = O.Challenger()
+        ch.observe(root)
+        for t in range(w):
+            ch.observe(claims[t].tobytes())
+        lw = (w - 1).bit_length()
+        r = np.concatenate([ch.sample() for _ in range(lw)]) if lw else np.zeros((0, 2), np.uint64)
+        s = eq_by_product(O, r, w)
+        chal = [None] * n
+        for k in range(n):
+            ch.observe(roots[k])
+            ch.observe(comps[k][0])
+            ch.observe(comps[k][1])
+            chal[n - 1 - k] = ch.sample()[0]
+        cols = e.reshape(1 << n, w, 2)
+        comb = np.zeros((1 << n, 2), np.uint64)
+        for t in range(w):
+            comb ^= O.mul(np.ascontiguousarray(cols[:, t]), np.repeat(s[t:t + 1], 1 << n, axis=0))
+        x = np.array(chal, dtype=np.uint64).reshape(-1, 2)
+        assert O.mle_evaluate(comb, x).tobytes() == final, (n, w)
+
+
+def test_width_tampering_and_size(oracle):
+    """A flipped byte of an opened row fails the Merkle check, a wrong claim fails, and the
+    proof grows by q * 16 * (w - 1) bytes over width 1 (the opened rows carry w elements)."""
+    O = oracle
+    n, log2_c = 6, 1
+    q = O.num_queries(128, log2_c)
+    p = rnd(O, n, 801)
+    _, _, p1 = O.basefold_prove_w(128, log2_c, rnd(O, 1 << n, 800), 1, p)
+    for w in [2, 4, 7]:
+        e = rnd(O, (1 << n) * w, 810 + w)
+        root, claims, proof = O.basefold_prove_w(128, log2_c, e, w, p)
+        assert len(proof) == len(p1) + q * 16 * (w - 1)
+        comps, final, roots = parse_head(proof, n)
+        head = 8 + 40 * n + 24 + 8 + 32 * n + 8
+        bad = bytearray(proof)
+        bad[head + 8 + 3] ^= 1  # first opened row, first element
+        assert O.basefold_verify_w(128, log2_c, w, root, p, claims, bytes(bad)) == 3
+        bc = claims.copy()
+        bc[w - 1, 1] ^= 4
+        assert O.basefold_verify_w(128, log2_c, w, root, p, bc, proof) != 0
