@@ -157,6 +157,27 @@ int bfd_sumcheck_product(bfd_ctx *ctx, uint32_t n, const uint8_t *f, const uint8
                          bfd_challenger *challenger, uint8_t claim[16], uint8_t *round_polys,
                          uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim[16]);
 
+/* ---- Quadratic sumcheck  sum_b sum_k s_k P_{a_k}(b) P_{b_k}(b)
+ * (sumcheck/src/function/quadratic.rs:13-134) driven by prove_sumcheck
+ * (sumcheck/src/lib.rs:56-103) over m tables of 2^n evaluations (polys:
+ * m x 2^n x 16 bytes, table-major), npairs pairs (pair_idx = a_0, b_0, a_1,
+ * b_1, ...) with one scalar each.  The claim is the function's sum
+ * (compute_sum); it is observed first, then [c0, c2] per round.  Outputs
+ * claim, n x 2 compressed round polys, n challenges (round order), the m
+ * final evaluations and the final subclaim.  A Batch (sumcheck/src/function/
+ * batch.rs) of Quadratics with equal num_vars is this call on the union of
+ * their tables with the pairs' scalars multiplied by alpha^k. ---- */
+int bfd_sumcheck_quadratic(bfd_ctx *ctx, uint32_t n, uint32_t m, const uint8_t *polys, uint32_t npairs,
+                           const uint8_t *scalars, const uint32_t *pair_idx, bfd_challenger *challenger,
+                           uint8_t claim[16], uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals,
+                           uint8_t subclaim[16]);
+
+/* ---- Polyval arithmetic on the host (p3/src/polyval.rs:30-252):
+ * Montgomery-form product a*b*X^-128 (Mul) and inverse (invert_or_zero,
+ * 0 for 0), little-endian 16-byte elements ---- */
+int bfd_field_mul(const uint8_t a[16], const uint8_t b[16], uint8_t out[16]);
+int bfd_field_inv(const uint8_t a[16], uint8_t out[16]);
+
 /* ---- input generation: Polyval::random_vec(n, StdRng::seed_from_u64(seed))
  * (p3/src/field/from_uniform_bytes.rs:5-27) ---- */
 int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out);

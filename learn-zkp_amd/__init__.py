@@ -98,6 +98,11 @@ def lib():
                                     u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.bfd_sumcheck_product.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u8p, u8p, u8p, ctypes.c_void_p,
                                            u8p, u8p, u8p, u8p, u8p]
+        L.bfd_sumcheck_quadratic.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, u8p, ctypes.c_uint32,
+                                             u8p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p, u8p, u8p, u8p,
+                                             u8p, u8p]
+        L.bfd_field_mul.argtypes = [u8p, u8p, u8p]
+        L.bfd_field_inv.argtypes = [u8p, u8p]
         L.bfd_random_vec.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u8p]
         L.bfd_commit_open_batch.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_void_p),
@@ -467,6 +472,70 @@ def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional
     _check(lib().bfd_sumcheck_product(ctx.h, n, _p(f), _p(g), _p(rr), challenger.h, _p(claim), _p(polys),
                                       _p(chal), _p(fin), _p(sub)))
     return claim, polys[:3 * n].reshape(n, 3, 2), chal[:n], fin, sub
+
+
+@dataclass
+class Quadratic:
+    """Quadratic<Polyval, Polyval> (sumcheck/src/function/quadratic.rs:13-28):
+    sum over pairs (scalar, a, b) of scalar * P_a * P_b, with its tables."""
+    polys: np.ndarray                       # (m, 2^n, 2)
+    pairs: Sequence                         # [(scalar (2,) uint64, a, b), ...]
+
+
+def prove_sumcheck_quadratic(f: Quadratic, challenger: GenericChallenger, ctx: Optional[Context] = None):
+    """prove_sumcheck (sumcheck/src/lib.rs:56-103) of a Quadratic function; the
+    claim is the function's sum (compute_sum).
+
+    Returns (claim, compressed round polys (n, 2, 2), challenges (n, 2),
+    final evaluations of every table (m, 2), subclaim)."""
+    ctx = ctx or default_context()
+    P = np.ascontiguousarray(f.polys, dtype=np.uint64)
+    m, size = P.shape[0], P.shape[1]
+    n = size.bit_length() - 1
+    if size != 1 << n:
+        raise BfdError(BFD_ERR_ARG, "tables must hold 2^n evaluations")
+    sc = np.ascontiguousarray(np.array([np.asarray(p[0], dtype=np.uint64).reshape(2) for p in f.pairs]))
+    idx = np.ascontiguousarray(np.array([[p[1], p[2]] for p in f.pairs], dtype=np.uint32).reshape(-1))
+    claim = np.zeros((1, 2), dtype=np.uint64)
+    polys = np.zeros((max(1, 2 * n), 2), dtype=np.uint64)
+    chal = np.zeros((max(1, n), 2), dtype=np.uint64)
+    fin = np.zeros((m, 2), dtype=np.uint64)
+    sub = np.zeros((1, 2), dtype=np.uint64)
+    _check(lib().bfd_sumcheck_quadratic(ctx.h, n, m, _p(P.reshape(-1, 2)), len(sc), _p(sc),
+                                        idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), challenger.h,
+                                        _p(claim), _p(polys), _p(chal), _p(fin), _p(sub)))
+    return claim, polys[:2 * n].reshape(n, 2, 2), chal[:n], fin, sub
+
+
+def batch_quadratics(fs: Sequence[Quadratic], alpha) -> Quadratic:
+    """Batch::new(fs, claims, alpha) (sumcheck/src/function/batch.rs:15-34) for
+    Quadratics of equal num_vars, the case that needs neither E::TWO powers nor
+    halve() (both degenerate in characteristic 2): the sum of alpha^k f_k is the
+    Quadratic over the union of their tables with every pair's scalar times
+    alpha^k.  Its round polys, challenges and final evaluations equal the
+    Batch's (evaluations() concatenates the functions' tables in order)."""
+    sizes = {np.asarray(f.polys).shape[1] for f in fs}
+    if len(sizes) != 1:
+        raise BfdError(BFD_ERR_ARG, "batched Quadratics must have equal num_vars")
+    alpha = np.asarray(alpha, dtype=np.uint64).reshape(1, 2)
+    power = np.array([[1, 0xC200000000000000]], dtype=np.uint64)  # alpha^0 = ONE (Montgomery form)
+    polys, pairs, base = [], [], 0
+    for f in fs:
+        P = np.asarray(f.polys, dtype=np.uint64)
+        for (s, a, b) in f.pairs:
+            pairs.append((_mul_host(np.asarray(s, dtype=np.uint64).reshape(1, 2), power)[0], base + a, base + b))
+        polys.append(P)
+        base += P.shape[0]
+        power = _mul_host(power, alpha)
+    return Quadratic(np.concatenate(polys, axis=0), pairs)
+
+
+def _mul_host(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """POLYVAL Montgomery product of single elements on the host (the library's
+    host multiply, through bfd_field_mul)."""
+    out = np.zeros((1, 2), dtype=np.uint64)
+    _check(lib().bfd_field_mul(_p(np.ascontiguousarray(a)), _p(np.ascontiguousarray(b)), _p(out)))
+    return out
 
 
 def header_symbols() -> List[str]:

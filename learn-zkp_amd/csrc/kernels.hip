@@ -468,6 +468,40 @@ __global__ __launch_bounds__(CB) void k_prod_fold_round(gf *__restrict__ f, gf *
 }
 
 // ---------------------------------------------------------------------------
+// Quadratic sumcheck  sum_b sum_k s_k P_{a_k}(b) P_{b_k}(b)
+// (sumcheck/src/function/quadratic.rs:105-127): the m tables are stored
+// table-major with a fixed stride and fixed in place; pair blockIdx.y sums
+// A0 = sum f0 g0, A1 = sum f1 g1, A2 = sum (f0+f1)(g0+g1) over the current
+// halves (A2 is the X^2 coefficient: (f1-f0)(g1-g0) in characteristic 2);
+// the host applies the scalars.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(CB) void k_quad_sums(const gf *__restrict__ P, size_t stride,
+                                                  const uint32_t *__restrict__ pairs, size_t h,
+                                                  gf *__restrict__ partial) {
+    const size_t a = pairs[2 * blockIdx.y], b = pairs[2 * blockIdx.y + 1];
+    const gf *f = P + a * stride, *g = P + b * stride;
+    gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < h; j += (size_t)gridDim.x * blockDim.x) {
+        const gf f0 = gf_ld(f + j), f1 = gf_ld(f + j + h), g0 = gf_ld(g + j), g1 = gf_ld(g + j + h);
+        a0 = gf_add(a0, gf_mul_reg(f0, g0));
+        a1 = gf_add(a1, gf_mul_reg(f1, g1));
+        a2 = gf_add(a2, gf_mul_reg(gf_add(f0, f1), gf_add(g0, g1)));
+    }
+    block_xor3(a0, a1, a2, partial + 3 * (size_t)gridDim.x * blockIdx.y);
+}
+
+// fix_last_var of table blockIdx.y (2*half entries -> half), in place
+__global__ __launch_bounds__(CB) void k_fix_multi(gf *__restrict__ P, size_t stride, size_t half, gf x) {
+    gf *f = P + blockIdx.y * stride;
+    const gf_pre px = gf_prepare(x);
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
+         j += (size_t)gridDim.x * blockDim.x) {
+        const gf lo = gf_ld(f + j), hi = gf_ld(f + j + half);
+        gf_st(f + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Segmented forms: one launch covers the same stage of many polynomials.
 // Segment g owns blocks [blk0, blk0 + nblk) and grid-strides over its own
 // `count` items; a block finds its segment by binary search on blk0.
@@ -838,6 +872,19 @@ hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, 
     unsigned gsz = grid_for(q, CB, 512);
     k_prod_fold_round<<<gsz, CB, 0, st>>>(f, g, T, q, x, partial);
     *nblocks = gsz;
+    return hipGetLastError();
+}
+hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
+                     gf *partial, unsigned max_blocks, unsigned *nblocks) {
+    if (npairs == 0 || npairs > 65535 || max_blocks == 0) return hipErrorInvalidValue;
+    const unsigned gx = grid_for(h, CB, max_blocks);
+    k_quad_sums<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, partial);
+    *nblocks = gx;
+    return hipGetLastError();
+}
+hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, gf x) {
+    if (m == 0 || m > 65535) return hipErrorInvalidValue;
+    k_fix_multi<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(P, stride, half, x);
     return hipGetLastError();
 }
 hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n) {

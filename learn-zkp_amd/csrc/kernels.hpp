@@ -68,6 +68,12 @@ hipError_t prod_round(hipStream_t st, const gf *f, const gf *g, const gf *T, siz
                       unsigned *nblocks);
 hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, gf x, gf *partial,
                            unsigned *nblocks);
+// Quadratic sumcheck: per pair (blockIdx.y) and block, the three sums
+// A0, A1, A2 over halves of h entries at partial[3 * (pair * nblocks + block)]
+hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
+                     gf *partial, unsigned max_blocks, unsigned *nblocks);
+// fix the top variable of m tables (stride apart) from 2*half to half entries
+hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, gf x);
 hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n);
 hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *out, size_t n);
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters);

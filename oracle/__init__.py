@@ -296,5 +296,23 @@ def product_sumcheck(f, g, r):
     return claim, polys.reshape(n, 3, 2), chal, fin, sub
 
 
+def quadratic_sumcheck(polys, scalars, pairs):
+    """polys: m tables of 2^n elements (m x 2^n x 2), scalars: one per pair,
+    pairs: [(a, b), ...]. Returns (claim, round polys (n x 2), challenges,
+    final evaluations (m), subclaim)."""
+    P = np.ascontiguousarray(polys, dtype=np.uint64)
+    m = P.shape[0]
+    n = (P.shape[1]).bit_length() - 1
+    s = as_elems(scalars)
+    idx = np.ascontiguousarray(np.array(pairs, dtype=np.uint32).reshape(-1))
+    claim, rp, ch, fin, sub = elems(1), elems(2 * n), elems(n), elems(m), elems(1)
+    rc = lib().bfo_sumcheck_quadratic(n, m, _p(P.reshape(-1, 2)), len(s), _p(s),
+                                      idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                      _p(claim), _p(rp), _p(ch), _p(fin), _p(sub))
+    if rc != 0:
+        raise RuntimeError(f"oracle quadratic sumcheck failed ({rc})")
+    return claim, rp.reshape(n, 2, 2), ch, fin, sub
+
+
 if os.environ.get("BFO_THREADS"):
     set_threads(int(os.environ["BFO_THREADS"]))

@@ -1108,3 +1108,70 @@ int bfo_product_sumcheck(int n, const uint8_t *f_b, const uint8_t *g_b, const ui
     free(f); free(g); free(e);
     return 0;
 }
+
+/* ===================================================================== */
+/* Quadratic sumcheck  sum_b sum_k s_k P_{a_k}(b) P_{b_k}(b)                */
+/* (sumcheck/src/function/quadratic.rs:13-134) driven by prove_sumcheck     */
+/* (sumcheck/src/lib.rs:56-103).  Per round (top variable first) the round  */
+/* poly is c0 = sum_k s_k sum_b f0 g0, c2 = sum_k s_k sum_b (f1-f0)(g1-g0), */
+/* c1 = subclaim - 2 c0 - c2 (quadratic.rs:70-80; 2 c0 = 0 here); the      */
+/* compressed poly [c0, c2] is observed, x sampled, every table fixed.      */
+/* ===================================================================== */
+int bfo_sumcheck_quadratic(int n, int m, const uint8_t *polys_b, int npairs, const uint8_t *scalars_b,
+                           const uint32_t *pair_idx, uint8_t claim_out[16], uint8_t *round_polys,
+                           uint8_t *challenges, uint8_t *final_evals, uint8_t subclaim_out[16]) {
+    if (n < 0 || n > 40 || m < 1 || npairs < 1) return -1;
+    for (int k = 0; k < 2 * npairs; k++)
+        if (pair_idx[k] >= (uint32_t)m) return -1;
+    size_t N = (size_t)1 << n;
+    u128 *P = malloc(sizeof(u128) * N * (size_t)m);
+    memcpy(P, polys_b, 16 * N * (size_t)m);
+    u128 *s = malloc(sizeof(u128) * (size_t)npairs);
+    for (int k = 0; k < npairs; k++) s[k] = ld(scalars_b + 16 * k);
+    /* claim = compute_sum (quadratic.rs:86-99) */
+    u128 claim = 0;
+    for (int k = 0; k < npairs; k++) {
+        const u128 *f = P + N * pair_idx[2 * k], *g = P + N * pair_idx[2 * k + 1];
+        u128 acc = 0;
+        for (size_t j = 0; j < N; j++) acc ^= fmul(f[j], g[j]);
+        claim ^= fmul(s[k], acc);
+    }
+    st(claim_out, claim);
+    u128 sub = claim;
+    bfo_challenger *ch = bfo_challenger_new();
+    if (n > 0) ch_observe_elem(ch, claim);               /* lib.rs:67 */
+    for (int i = n - 1; i >= 0; i--) {
+        int k = n - 1 - i;
+        size_t h = (size_t)1 << i;
+        u128 c0 = 0, c2 = 0;
+        for (int t = 0; t < npairs; t++) {
+            const u128 *f = P + N * pair_idx[2 * t], *g = P + N * pair_idx[2 * t + 1];
+            u128 a0 = 0, a2 = 0;
+            for (size_t j = 0; j < h; j++) {
+                a0 ^= fmul(f[j], g[j]);
+                a2 ^= fmul(f[j + h] ^ f[j], g[j + h] ^ g[j]);
+            }
+            c0 ^= fmul(s[t], a0);
+            c2 ^= fmul(s[t], a2);
+        }
+        u128 c1 = sub ^ c2;                              /* decompress_round_poly, char 2 */
+        st(round_polys + 32 * k, c0);
+        st(round_polys + 32 * k + 16, c2);
+        ch_observe_elem(ch, c0);
+        ch_observe_elem(ch, c2);
+        u128 x = ch_sample(ch);
+        st(challenges + 16 * k, x);
+        for (int p = 0; p < m; p++) {                    /* fix_last_var (multilinear.rs:183-188) */
+            u128 *f = P + N * (size_t)p;
+            for (size_t j = 0; j < h; j++) f[j] ^= fmul(x, f[j] ^ f[j + h]);
+        }
+        u128 rp[3] = {c0, c1, c2};
+        sub = horner(rp, 3, x);
+    }
+    for (int p = 0; p < m; p++) st(final_evals + 16 * p, P[N * (size_t)p]);
+    st(subclaim_out, sub);
+    bfo_challenger_free(ch);
+    free(P);
+    free(s);
+    return 0;
+}

@@ -107,6 +107,17 @@ int bfo_product_sumcheck(int n, const uint8_t *f, const uint8_t *g, const uint8_
                          uint8_t claim[16], uint8_t *round_polys, uint8_t *challenges,
                          uint8_t final_fg[32], uint8_t subclaim[16]);
 
+/*
+ * Quadratic sumcheck  sum_b sum_k s_k P_{a_k}(b) P_{b_k}(b)  over m tables of
+ * 2^n evaluations (polys: m x 2^n, table-major), pair_idx = [a_0, b_0, a_1,
+ * b_1, ...]; prove_sumcheck driver (sumcheck/src/lib.rs:56-103) with a fresh
+ * keccak challenger.  Outputs the claim, n x 2 compressed round polys
+ * [c0, c2], the n challenges, the m final evaluations and the subclaim.
+ */
+int bfo_sumcheck_quadratic(int n, int m, const uint8_t *polys, int npairs, const uint8_t *scalars,
+                           const uint32_t *pair_idx, uint8_t claim[16], uint8_t *round_polys,
+                           uint8_t *challenges, uint8_t *final_evals, uint8_t subclaim[16]);
+
 #ifdef __cplusplus
 }
 #endif

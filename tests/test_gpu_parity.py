@@ -126,6 +126,60 @@ def test_width_commit_device_and_reopen(bfd, oracle, ctx):
         assert pcs.open(cd, [bfd.PolyEvals(claims, point)], ch) == proof_o
 
 
+def quad_case(bfd, n, seed):
+    """quadratic.rs:150-171's shape: 5..9 tables, 5..9 pairs of random indices with random scalars."""
+    rng = np.random.default_rng(seed)
+    m, npairs = int(rng.integers(5, 10)), int(rng.integers(5, 10))
+    polys = bfd.random_vec(seed, m << n).reshape(m, 1 << n, 2)
+    scalars = bfd.random_vec(seed + 1, npairs)
+    pairs = [(scalars[k], int(rng.integers(0, m)), int(rng.integers(0, m))) for k in range(npairs)]
+    return bfd.Quadratic(polys, pairs)
+
+
+def test_quadratic_sumcheck_bit_exact(bfd, oracle, ctx):
+    """Quadratic sumcheck (quadratic.rs + prove_sumcheck): every output equals the restatement's."""
+    for n in list(range(0, 11)) + [14]:
+        _say(f"case quadratic n={n}")
+        f = quad_case(bfd, n, 900 + n)
+        got = bfd.prove_sumcheck_quadratic(f, bfd.GenericChallenger.keccak256(), ctx)
+        want = oracle.quadratic_sumcheck(f.polys, [p[0] for p in f.pairs], [(p[1], p[2]) for p in f.pairs])
+        for g, w in zip(got, want):
+            assert (np.asarray(g).reshape(-1, 2) == np.asarray(w).reshape(-1, 2)).all(), n
+
+
+def test_batch_quadratic_bit_exact(bfd, oracle, ctx):
+    """Batch of three equal-size Quadratics (batch.rs) as one merged call, against the restatement."""
+    n = 9
+    fs = [quad_case(bfd, n, 950 + k) for k in range(3)]
+    merged = bfd.batch_quadratics(fs, bfd.random_vec(960, 1))
+    got = bfd.prove_sumcheck_quadratic(merged, bfd.GenericChallenger.keccak256(), ctx)
+    want = oracle.quadratic_sumcheck(merged.polys, [p[0] for p in merged.pairs],
+                                     [(p[1], p[2]) for p in merged.pairs])
+    for g, w in zip(got, want):
+        assert (np.asarray(g).reshape(-1, 2) == np.asarray(w).reshape(-1, 2)).all()
+
+
+def test_quadratic_sumcheck_large_invariants(bfd, oracle, ctx):
+    """2^22 tables: the verifier's final identity with independently evaluated finals
+    (sum_k s_k P_a(r) P_b(r) = subclaim) and the transcript replay of every challenge."""
+    n = 22
+    _say(f"case quadratic n={n}")
+    f = quad_case(bfd, n, 970)
+    claim, rps, chal, fin, sub = bfd.prove_sumcheck_quadratic(f, bfd.GenericChallenger.keccak256(), ctx)
+    ch = oracle.Challenger()
+    ch.observe(claim.tobytes())
+    for k in range(n):
+        ch.observe(rps[k].tobytes())
+        assert (ch.sample() == chal[k]).all()
+    x = chal[::-1].copy()
+    for p in [0, len(f.polys) - 1]:
+        assert (oracle.mle_evaluate(f.polys[p], x) == fin[p:p + 1]).all()
+    expect = np.zeros((1, 2), np.uint64)
+    for (s, a, b) in f.pairs:
+        expect ^= oracle.mul(oracle.mul(fin[a:a + 1], fin[b:b + 1]), np.asarray(s).reshape(1, 2))
+    assert (sub == expect).all()
+
+
 def test_encode_matches_oracle_codeword_root(bfd, oracle, ctx):
     """The main-tree root pins the whole codeword (every leaf)."""
     for n, log2_c in [(5, 0), (6, 1), (7, 2), (12, 2)]:

@@ -62,3 +62,36 @@ def test_proof_size_width_matches_oracle(bfd, oracle):
         e, p = oracle.random_vec(5, (1 << n) * w), oracle.random_vec(6, n)
         _, _, proof = oracle.basefold_prove_w(128, log2_c, e, w, p)
         assert bfd.proof_size(128, log2_c, n, w) == len(proof)
+
+
+def test_field_mul_inv_match_oracle(bfd, oracle):
+    a, b = oracle.random_vec(11, 64), oracle.random_vec(12, 64)
+    prod = oracle.mul(a, b)
+    for i in range(64):
+        assert (bfd._mul_host(a[i:i + 1], b[i:i + 1]) == prod[i:i + 1]).all()
+    inv = oracle.inv(a)
+    out = np.zeros((1, 2), dtype=np.uint64)
+    for i in range(8):
+        bfd.lib().bfd_field_inv(bfd._p(np.ascontiguousarray(a[i:i + 1])), bfd._p(out))
+        assert (out == inv[i:i + 1]).all()
+
+
+def test_batch_quadratics_is_the_alpha_combination(bfd, oracle):
+    """Batch (sumcheck/src/function/batch.rs:15-44) of equal-size Quadratics: the merged
+    function's claim is sum_k alpha^k claim_k and its final identity holds at its challenges."""
+    n = 5
+    fs = []
+    for k in range(3):
+        polys = oracle.random_vec(50 + k, 2 << n).reshape(2, 1 << n, 2)
+        fs.append(bfd.Quadratic(polys, [(oracle.random_vec(60 + k, 1)[0], 0, 1), (oracle.random_vec(70 + k, 1)[0], 1, 1)]))
+    alpha = oracle.random_vec(80, 1)
+    merged = bfd.batch_quadratics(fs, alpha)
+    claims = [oracle.quadratic_sumcheck(f.polys, [p[0] for p in f.pairs], [(p[1], p[2]) for p in f.pairs])[0]
+              for f in fs]
+    mc, rps, chal, fin, sub = oracle.quadratic_sumcheck(merged.polys, [p[0] for p in merged.pairs],
+                                                        [(p[1], p[2]) for p in merged.pairs])
+    expect, power = np.zeros((1, 2), np.uint64), oracle.from_canonical(oracle.from_int(1))
+    for c in claims:
+        expect ^= oracle.mul(c, power)
+        power = oracle.mul(power, alpha)
+    assert (mc == expect).all()

@@ -311,3 +311,45 @@ def test_width_tampering_and_size(oracle):
         bc = claims.copy()
         bc[w - 1, 1] ^= 4
         assert O.basefold_verify_w(128, log2_c, w, root, p, bc, proof) != 0
+
+
+# ---- Quadratic sumcheck (sumcheck/src/function/quadratic.rs) ----
+def quad_case(O, n, seed):
+    """The reference test's shape (quadratic.rs:150-171): 5..9 polynomials, 5..9 pairs of random
+    indices with random scalars."""
+    rng = np.random.default_rng(seed)
+    m, npairs = int(rng.integers(5, 10)), int(rng.integers(5, 10))
+    polys = O.random_vec(seed, m << n).reshape(m, 1 << n, 2)
+    scalars = O.random_vec(seed + 1, npairs)
+    pairs = [(int(rng.integers(0, m)), int(rng.integers(0, m))) for _ in range(npairs)]
+    return polys, scalars, pairs
+
+
+def test_quadratic_sumcheck_pinned(oracle):
+    """Claim by brute force; the challenges replay from the transcript (observe claim, then
+    [c0, c2] per round, lib.rs:67,94-96); and the verifier's final identity: the last subclaim
+    equals sum_k s_k P_a(r) P_b(r) with every P(r) evaluated independently as an MLE."""
+    O = oracle
+    for n in range(1, 9):
+        polys, scalars, pairs = quad_case(O, n, 40 + n)
+        claim, rps, chal, fin, sub = O.quadratic_sumcheck(polys, scalars, pairs)
+        brute = np.zeros((1, 2), np.uint64)
+        for (a, b), s in zip(pairs, scalars):
+            prod = O.mul(polys[a], polys[b])
+            acc = np.bitwise_xor.reduce(prod, axis=0).reshape(1, 2)
+            brute ^= O.mul(acc, s.reshape(1, 2))
+        assert (claim == brute).all(), n
+        ch = O.Challenger()
+        ch.observe(claim.tobytes())
+        for k in range(n):
+            ch.observe(rps[k, 0].tobytes())
+            ch.observe(rps[k, 1].tobytes())
+            assert (ch.sample() == chal[k]).all(), (n, k)
+        x = chal[::-1].copy()
+        evals = [O.mle_evaluate(polys[p], x) for p in range(len(polys))]
+        for p in range(len(polys)):
+            assert (fin[p:p + 1] == evals[p]).all()
+        expect = np.zeros((1, 2), np.uint64)
+        for (a, b), s in zip(pairs, scalars):
+            expect ^= O.mul(O.mul(evals[a], evals[b]), s.reshape(1, 2))
+        assert (sub == expect).all(), n
