@@ -172,6 +172,23 @@ int bfd_sumcheck_quadratic(bfd_ctx *ctx, uint32_t n, uint32_t m, const uint8_t *
                            uint8_t claim[16], uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals,
                            uint8_t subclaim[16]);
 
+/* ---- Basefold::verify (basefold.rs:234-325) of `count` proofs made with
+ * `code`, each over a commitment of `width` columns: roots (count x 32
+ * bytes), points[p] (d elements), values[p] (width elements), proofs[p] of
+ * proof_lens[p] bytes, challengers[p] in the state open started from (the
+ * harness prefix observed; advanced in place).  The proof bytes are decoded
+ * and the transcripts replayed on the host; every query's fold chain and
+ * Merkle paths of all proofs are checked on the device in two launches.
+ * results[p]: 0 accepted, 1 InvalidFinalPoly, 2 InvalidQuery, 3 Mmcs (a
+ * Merkle path), 4 malformed proof bytes (the reference's first error in
+ * query order). ---- */
+int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t count, const uint8_t *roots,
+                     const uint8_t *const *points, const uint8_t *const *values, const uint8_t *const *proofs,
+                     const size_t *proof_lens, bfd_challenger *const *challengers, int *results);
+int bfd_verify(bfd_ctx *ctx, const bfd_code *code, size_t width, const uint8_t root[32], const uint8_t *point,
+               const uint8_t *values, const uint8_t *proof, size_t proof_len, bfd_challenger *challenger,
+               int *result);
+
 /* ---- Polyval arithmetic on the host (p3/src/polyval.rs:30-252):
  * Montgomery-form product a*b*X^-128 (Mul) and inverse (invert_or_zero,
  * 0 for 0), little-endian 16-byte elements ---- */

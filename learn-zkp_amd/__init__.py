@@ -101,6 +101,10 @@ def lib():
         L.bfd_sumcheck_quadratic.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, u8p, ctypes.c_uint32,
                                              u8p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p, u8p, u8p, u8p,
                                              u8p, u8p]
+        L.bfd_verify_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, u8p,
+                                       ctypes.POINTER(u8p), ctypes.POINTER(u8p), ctypes.POINTER(u8p),
+                                       ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_int)]
         L.bfd_field_mul.argtypes = [u8p, u8p, u8p]
         L.bfd_field_inv.argtypes = [u8p, u8p]
         L.bfd_random_vec.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u8p]
@@ -337,6 +341,21 @@ class Basefold:
         _check(lib().bfd_eval(self.ctx.h, comm_data.handle, _p(point), _p(out)))
         return out
 
+    def verify(self, comm: bytes, evals: Sequence[PolyEvals], proof: bytes, challenger: GenericChallenger,
+               width: Optional[int] = None) -> int:
+        """Basefold::verify (basefold.rs:234-325); returns VERIFY_OK (0) or the
+        error kind (INVALID_FINAL_POLY, INVALID_QUERY, MMCS_ERROR, MALFORMED_PROOF).
+        ``width`` defaults to the number of values."""
+        values = _elems(evals[0].values)
+        return _verify_batch(self, width or len(values), [comm], [_elems(evals[0].point)], [values], [proof],
+                             [challenger])[0]
+
+    def verify_batch(self, comms: Sequence[bytes], evals: Sequence[PolyEvals], proofs: Sequence[bytes],
+                     challengers: Sequence[GenericChallenger], width: int = 1) -> List[int]:
+        """Many proofs of this code (each over `width` columns) in one device pass."""
+        return _verify_batch(self, width, comms, [_elems(e.point) for e in evals], [_elems(e.values) for e in evals],
+                             proofs, challengers)
+
     def open(self, comm_data: BasefoldCommitmentData, evals: Sequence[PolyEvals],
              challenger: GenericChallenger) -> bytes:
         if not evals:
@@ -352,6 +371,26 @@ class Basefold:
             return ctypes.string_at(proof, plen.value)
         finally:
             lib().bfd_proof_free(proof)
+
+
+# results of Basefold::verify (basefold.rs:234-325) as bfd_verify_batch reports them
+VERIFY_OK, INVALID_FINAL_POLY, INVALID_QUERY, MMCS_ERROR, MALFORMED_PROOF = 0, 1, 2, 3, 4
+
+
+def _verify_batch(pcs: "Basefold", width: int, roots, points, values, proofs, challengers) -> List[int]:
+    count = len(proofs)
+    rb = np.frombuffer(b"".join(bytes(r) for r in roots) or b"\0", dtype=np.uint8).copy()
+    pts = [np.ascontiguousarray(_elems(p)) if len(p) else np.zeros((1, 2), np.uint64) for p in points]
+    vals = [np.ascontiguousarray(_elems(v)) for v in values]
+    bufs = [np.frombuffer(bytes(pr) or b"\0", dtype=np.uint8).copy() for pr in proofs]
+    P = (u8p * max(1, count))(*[_p(a) for a in pts])
+    V = (u8p * max(1, count))(*[_p(a) for a in vals])
+    B = (u8p * max(1, count))(*[_p(a) for a in bufs])
+    lens = (ctypes.c_size_t * max(1, count))(*[len(bytes(pr)) for pr in proofs])
+    chs = (ctypes.c_void_p * max(1, count))(*[c.h for c in challengers])
+    res = (ctypes.c_int * max(1, count))()
+    _check(lib().bfd_verify_batch(pcs.ctx.h, pcs.code_h, width, count, _p(rb), P, V, B, lens, chs, res))
+    return [int(res[i]) for i in range(count)]
 
 
 def commit_open_batch(lambda_: int, log2_c: int, evals: Sequence, points: Sequence, claims: Sequence,

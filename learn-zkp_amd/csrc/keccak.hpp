@@ -129,6 +129,36 @@ __device__ __forceinline__ void keccak256_words(const uint64_t *m, uint64_t out[
     for (int i = 0; i < 4; i++) out[i] = (uint64_t)s[i].lo | ((uint64_t)s[i].hi << 32);
 }
 
+// Keccak-256 of nw 64-bit words (8-byte aligned, runtime length): full
+// 17-word blocks, then the last block with the remaining words and the
+// padding (0x01 after the message, 0x80 in the last rate byte)
+__device__ __forceinline__ void keccak256_var(const uint64_t *m, size_t nw, uint64_t out[4]) {
+    const size_t full = nw / 17, rem = nw % 17;
+    lane s[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) s[k] = lane{0u, 0u};
+    for (size_t blk = 0; blk < full; blk++, m += 17) {
+#pragma unroll
+        for (int k = 0; k < 17; k++) {
+            const uint64_t v = m[k];
+            s[k].lo ^= (uint32_t)v;
+            s[k].hi ^= (uint32_t)(v >> 32);
+        }
+        keccak_f(s);
+    }
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+        uint64_t v = (size_t)k < rem ? m[k] : 0ull;
+        if ((size_t)k == rem) v ^= 0x01ull;
+        if (k == 16) v ^= 0x8000000000000000ull;
+        s[k].lo ^= (uint32_t)v;
+        s[k].hi ^= (uint32_t)(v >> 32);
+    }
+    keccak_f(s);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = (uint64_t)s[k].lo | ((uint64_t)s[k].hi << 32);
+}
+
 // permutation on 64-bit words (used by the micro-benchmark)
 __device__ __forceinline__ void keccak_f(uint64_t w[25]) {
     lane s[25];

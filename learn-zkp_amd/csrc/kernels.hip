@@ -212,39 +212,13 @@ __global__ __launch_bounds__(KB) void k_leaves_pair(const gf *__restrict__ pi, u
     }
 }
 
-// leaf i = Keccak256 of row i of a width-w matrix (16w bytes, 2w 64-bit
-// words): full 17-word blocks, then the last block with the remaining words
-// and the Keccak padding (0x01 after the message, 0x80 in the last rate byte)
+// leaf i = Keccak256 of row i of a width-w matrix (16w bytes, 2w 64-bit words)
 __global__ __launch_bounds__(KB) void k_leaves_rows(const gf *__restrict__ rows, size_t w, uint64_t *__restrict__ out,
                                                     size_t count) {
-    const size_t nw = 2 * w, full = nw / 17, rem = nw % 17;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
          i += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t *m = reinterpret_cast<const uint64_t *>(rows + i * w);
-        kk::lane s[25];
-#pragma unroll
-        for (int k = 0; k < 25; k++) s[k] = kk::lane{0u, 0u};
-        for (size_t blk = 0; blk < full; blk++, m += 17) {
-#pragma unroll
-            for (int k = 0; k < 17; k++) {
-                const uint64_t v = m[k];
-                s[k].lo ^= (uint32_t)v;
-                s[k].hi ^= (uint32_t)(v >> 32);
-            }
-            kk::keccak_f(s);
-        }
-#pragma unroll
-        for (int k = 0; k < 17; k++) {
-            uint64_t v = (size_t)k < rem ? m[k] : 0ull;
-            if ((size_t)k == rem) v ^= 0x01ull;
-            if (k == 16) v ^= 0x8000000000000000ull;
-            s[k].lo ^= (uint32_t)v;
-            s[k].hi ^= (uint32_t)(v >> 32);
-        }
-        kk::keccak_f(s);
         uint64_t d[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) d[k] = (uint64_t)s[k].lo | ((uint64_t)s[k].hi << 32);
+        kk::keccak256_var(reinterpret_cast<const uint64_t *>(rows + i * w), 2 * w, d);
         st_digest(out + 4 * i, d);
     }
 }
@@ -498,6 +472,107 @@ __global__ __launch_bounds__(CB) void k_fix_multi(gf *__restrict__ P, size_t str
          j += (size_t)gridDim.x * blockDim.x) {
         const gf lo = gf_ld(f + j), hi = gf_ld(f + j + half);
         gf_st(f + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Verifier (basefold.rs:234-325) over many proofs of one code.  A proof's
+// query records are read in place from its bincode bytes (8-byte aligned
+// copy in device memory; every field is a multiple of 8 bytes).  Record
+// layout: u64 w | w elements | u64 L | L digests | u64 d | per round k
+// (level i = d-1-k): element | u64 (b+i) | (b+i) digests.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ gf ld8(const uint8_t *p) {
+    const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+    return gf{q[0], q[1]};
+}
+__device__ __forceinline__ gf interpolate(gf t, gf lo, gf hi, gf r) {  // binary_reed_solomon.rs:103-109
+    hi = gf_add(hi, lo);
+    lo = gf_add(lo, gf_mul_reg(hi, t));
+    return gf_add(lo, gf_mul_reg(gf_add(hi, lo), r));
+}
+
+// thread per (proof, query): the fold chain of the opened row through every
+// round; vals[(p*q + t)*d + k] = the two pi values of round k, fold_ok = the
+// final value equals final_poly (pi_0 = encode0(final_poly) is constant)
+__global__ __launch_bounds__(CB) void k_verify_fold(const VerifyJob *__restrict__ jobs, uint32_t count,
+                                                    uint32_t q, int b, int d, size_t w, const gf *__restrict__ ft,
+                                                    gf *__restrict__ vals, uint8_t *__restrict__ fold_ok) {
+    const size_t L = (size_t)(b + d);
+    for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < (size_t)count * q;
+         it += (size_t)gridDim.x * blockDim.x) {
+        const VerifyJob &J = jobs[it / q];
+        const size_t t = it % q;
+        const uint8_t *rec = J.proof + J.open_off + t * J.rec_bytes;
+        const uint64_t idx = J.idx[t];
+        gf folded = gf_zero();
+        for (size_t c = 0; c < w; c++) folded = gf_add(folded, gf_mul_reg(gf_ld(J.scalars + c), ld8(rec + 8 + 16 * c)));
+        size_t off = 24 + 16 * w + 32 * L;
+        for (int k = 0; k < d; k++) {
+            const int i = d - 1 - k;
+            const uint64_t ni = (uint64_t)1 << (b + i);
+            const uint64_t sib = ((idx >> (b + i)) & 1) ^ 1, row = idx & (ni - 1);
+            const gf sv = ld8(rec + off);
+            const gf v0 = sib == 0 ? sv : folded, v1 = sib == 0 ? folded : sv;
+            gf_st(vals + 2 * (it * d + k), v0);
+            gf_st(vals + 2 * (it * d + k) + 1, v1);
+            const gf *lvl = ft + (((size_t)1 << b) * (((size_t)1 << i) - 1));  // fold table of level i
+            folded = interpolate(gf_ld(lvl + row), v0, v1, gf_ld(J.chal + i));
+            off += 24 + 32 * (size_t)(b + i);
+        }
+        fold_ok[it] = gf_eq(folded, J.final_poly) ? 1 : 0;
+    }
+}
+
+// thread per (proof, query, path): path 0 climbs the main tree from the
+// opened row, path k+1 the pi tree of round k from its two values
+__global__ __launch_bounds__(KB) void k_verify_paths(const VerifyJob *__restrict__ jobs, uint32_t count, uint32_t q,
+                                                     int b, int d, size_t w, const gf *__restrict__ vals,
+                                                     uint8_t *__restrict__ path_ok) {
+    const size_t L = (size_t)(b + d), P = (size_t)d + 1;
+    for (size_t it = blockIdx.x * (size_t)blockDim.x + threadIdx.x; it < (size_t)count * q * P;
+         it += (size_t)gridDim.x * blockDim.x) {
+        const size_t pq = it / P, j = it % P;
+        const VerifyJob &J = jobs[pq / q];
+        const size_t t = pq % q;
+        const uint8_t *rec = J.proof + J.open_off + t * J.rec_bytes;
+        const uint64_t idx = J.idx[t];
+        uint64_t cur[4];
+        const uint8_t *sib, *root;
+        uint64_t index;
+        size_t len;
+        if (j == 0) {
+            kk::keccak256_var(reinterpret_cast<const uint64_t *>(rec + 8), 2 * w, cur);
+            sib = rec + 16 + 16 * w;
+            len = L;
+            index = idx;
+            root = J.root;
+        } else {
+            const int k = (int)j - 1, i = d - 1 - k;
+            size_t off = 24 + 16 * w + 32 * L;
+            for (int kk2 = 0; kk2 < k; kk2++) off += 24 + 32 * (size_t)(b + d - 1 - kk2);
+            const gf v0 = gf_ld(vals + 2 * (pq * d + k)), v1 = gf_ld(vals + 2 * (pq * d + k) + 1);
+            uint64_t m[4] = {v0.lo, v0.hi, v1.lo, v1.hi};
+            kk::keccak256_words<4>(m, cur);
+            sib = rec + off + 24;
+            len = (size_t)(b + i);
+            index = idx & ((((uint64_t)1) << (b + i)) - 1);
+            root = J.proof + J.roots_off + 32 * (size_t)k;
+        }
+        for (size_t u = 0; u < len; u++, index >>= 1) {
+            const uint64_t *s8 = reinterpret_cast<const uint64_t *>(sib + 32 * u);
+            uint64_t m[8];
+            if ((index & 1) == 0) {
+                m[0] = cur[0]; m[1] = cur[1]; m[2] = cur[2]; m[3] = cur[3];
+                m[4] = s8[0]; m[5] = s8[1]; m[6] = s8[2]; m[7] = s8[3];
+            } else {
+                m[0] = s8[0]; m[1] = s8[1]; m[2] = s8[2]; m[3] = s8[3];
+                m[4] = cur[0]; m[5] = cur[1]; m[6] = cur[2]; m[7] = cur[3];
+            }
+            kk::keccak256_words<8>(m, cur);
+        }
+        const uint64_t *r8 = reinterpret_cast<const uint64_t *>(root);
+        path_ok[it] = (cur[0] == r8[0] && cur[1] == r8[1] && cur[2] == r8[2] && cur[3] == r8[3]) ? 1 : 0;
     }
 }
 
@@ -885,6 +960,20 @@ hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t 
 hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, gf x) {
     if (m == 0 || m > 65535) return hipErrorInvalidValue;
     k_fix_multi<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(P, stride, half, x);
+    return hipGetLastError();
+}
+hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
+                       const gf *ft, gf *vals, uint8_t *fold_ok) {
+    const size_t n = (size_t)count * q;
+    if (!n) return hipSuccess;
+    k_verify_fold<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(jobs, count, q, b, d, w, ft, vals, fold_ok);
+    return hipGetLastError();
+}
+hipError_t verify_paths(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
+                        const gf *vals, uint8_t *path_ok) {
+    const size_t n = (size_t)count * q * (size_t)(d + 1);
+    if (!n) return hipSuccess;
+    k_verify_paths<<<grid_for(n, KB, kCapHash), KB, 0, st>>>(jobs, count, q, b, d, w, vals, path_ok);
     return hipGetLastError();
 }
 hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n) {

@@ -22,6 +22,19 @@ struct Seg {
     uint32_t nblk;    // blocks of the segment
 };
 
+// One proof of a verification batch (device copy; pointers into device memory).
+struct VerifyJob {
+    const uint8_t *proof;   // bincode bytes, 8-byte aligned
+    const uint8_t *root;    // 32-byte main-tree root
+    const uint64_t *idx;    // q query indices
+    const gf *chal;         // chal[i] = challenge of level i
+    const gf *scalars;      // w column scalars (ONE for width 1)
+    gf final_poly;
+    uint64_t open_off;      // byte offset of the first query record
+    uint64_t rec_bytes;     // bytes per query record
+    uint64_t roots_off;     // byte offset of the pi-tree roots
+};
+
 enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer };
 
 namespace launch {
@@ -74,6 +87,12 @@ hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t 
                      gf *partial, unsigned max_blocks, unsigned *nblocks);
 // fix the top variable of m tables (stride apart) from 2*half to half entries
 hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, gf x);
+// verifier kernels: fold chains (thread per proof x query) and Merkle paths
+// (thread per proof x query x path, d + 1 paths)
+hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
+                       const gf *ft, gf *vals, uint8_t *fold_ok);
+hipError_t verify_paths(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
+                        const gf *vals, uint8_t *path_ok);
 hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n);
 hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *out, size_t n);
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters);

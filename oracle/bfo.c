@@ -1011,7 +1011,7 @@ int bfo_basefold_verify_w(int lambda, int log2_c, int n, size_t width, const uin
     rs_code rs;
     rs_new(&rs, log2_c, n);
     int rc = 0;
-    if (br_u64(&rd) != (u64)q) { rc = 4; goto out; }
+    if (br_u64(&rd) != (u64)q || rd.err) { rc = 4; goto out; }
     for (size_t qi = 0; qi < q && !rc; qi++) {
         size_t idx = (size_t)bfo_challenger_sample_bits(ch, L);
         if (br_u64(&rd) != (u64)w) { rc = 4; break; }
@@ -1047,7 +1047,7 @@ out:
     rs_free(&rs);
     bfo_challenger_free(ch);
     free(sc);
-    if (!rc && rd.pos != rd.len) rc = 4;
+    if (!rc && (rd.err || rd.pos != rd.len)) rc = 4;  /* bincode: truncated or trailing bytes */
     return rc;
 }
 

@@ -180,6 +180,91 @@ def test_quadratic_sumcheck_large_invariants(bfd, oracle, ctx):
     assert (sub == expect).all()
 
 
+def tamper_offsets(n, log2_c, w, q):
+    """Byte offsets inside a bincode BasefoldProof (basefold.rs:49-66) for d = n rounds."""
+    d, L = n, n + log2_c
+    open_off = 72 * d + 48
+    rec = 8 + 16 * w + 8 + 32 * L + 8 + sum(16 + 8 + 32 * (log2_c + d - 1 - k) for k in range(d))
+    r0 = open_off
+    return {
+        "round_poly": 8 + 40 * min(1, d - 1) + 8,
+        "final_poly": 8 + 40 * d + 8,
+        "pi_root": 40 * d + 40,
+        "row": r0 + 8,
+        "main_sibling": r0 + 16 + 16 * w + 5,
+        "pi_value": r0 + 24 + 16 * w + 32 * L,
+        "pi_sibling": r0 + 24 + 16 * w + 32 * L + 24 + 3,
+        "last_query_row": open_off + (q - 1) * rec + 8 + 16 * (w - 1),
+        "query_count": open_off - 8,
+    }
+
+
+def verify_both(bfd, oracle, ctx, pcs, log2_c, w, root, point, claims, proof):
+    ch = bfd.GenericChallenger.keccak256()
+    ch.observe(root)
+    ch.observe_ext_slice(claims)
+    got = pcs.verify(root, [bfd.PolyEvals(claims, point)], proof, ch, w)
+    want = oracle.basefold_verify_w(128, log2_c, w, root, point, claims, proof)
+    return got, want
+
+
+@pytest.mark.parametrize("n,log2_c,w", [(10, 1, 1), (9, 2, 3), (6, 0, 9), (1, 1, 2), (0, 1, 4)])
+def test_verifier_matches_restatement(bfd, oracle, ctx, n, log2_c, w):
+    """Basefold::verify on the device against the restatement's verifier: honest proofs accept,
+    and every tampered part of the wire format gets the same error kind."""
+    _say(f"case verify n={n} log2_c={log2_c} w={w}")
+    evals, point = bfd.random_vec(600 + n, (1 << n) * w), bfd.random_vec(601 + n, n)
+    root, claims, proof = gpu_prove_w(bfd, ctx, 128, log2_c, evals, w, point)
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, n)), ctx)
+    got, want = verify_both(bfd, oracle, ctx, pcs, log2_c, w, root, point, claims, proof)
+    assert got == want == 0
+    q = oracle.num_queries(128, log2_c)
+    offs = tamper_offsets(n, log2_c, w, q)
+    for name, off in offs.items():
+        if (n == 0 and name in ("round_poly", "pi_root", "pi_value", "pi_sibling")) or q == 0:
+            continue
+        if n + log2_c == 0 and name == "main_sibling":
+            continue
+        bad = bytearray(proof)
+        bad[off] ^= 0x04
+        got, want = verify_both(bfd, oracle, ctx, pcs, log2_c, w, root, point, claims, bytes(bad))
+        assert got == want != 0, (name, got, want)
+    for cut in [1, 8, len(proof) // 2]:
+        got, want = verify_both(bfd, oracle, ctx, pcs, log2_c, w, root, point, claims, proof[:-cut])
+        assert got == want == 4, cut
+    bad_claims = claims.copy()
+    bad_claims[0, 0] ^= 2
+    got, want = verify_both(bfd, oracle, ctx, pcs, log2_c, w, root, point, bad_claims, proof)
+    assert got == want != 0
+
+
+def test_verifier_batch(bfd, oracle, ctx):
+    """Many proofs in one call: every result equals the restatement's for its proof."""
+    n, log2_c, w = 8, 1, 3
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, n)), ctx)
+    offs = tamper_offsets(n, log2_c, w, oracle.num_queries(128, log2_c))
+    kinds = [None, "row", "final_poly", "pi_sibling", None, "last_query_row", None, "main_sibling"] * 3
+    roots, evals, proofs, chs, want = [], [], [], [], []
+    for k, kind in enumerate(kinds):
+        e, pt = bfd.random_vec(700 + k, (1 << n) * w), bfd.random_vec(800 + k, n)
+        root, claims, proof = gpu_prove_w(bfd, ctx, 128, log2_c, e, w, pt)
+        if kind:
+            b = bytearray(proof)
+            b[offs[kind]] ^= 0x10
+            proof = bytes(b)
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(root)
+        ch.observe_ext_slice(claims)
+        roots.append(root)
+        evals.append(bfd.PolyEvals(claims, pt))
+        proofs.append(proof)
+        chs.append(ch)
+        want.append(oracle.basefold_verify_w(128, log2_c, w, root, pt, claims, proof))
+    got = pcs.verify_batch(roots, evals, proofs, chs, w)
+    assert got == want
+    assert want.count(0) == 9
+
+
 def test_encode_matches_oracle_codeword_root(bfd, oracle, ctx):
     """The main-tree root pins the whole codeword (every leaf)."""
     for n, log2_c in [(5, 0), (6, 1), (7, 2), (12, 2)]:
@@ -260,7 +345,14 @@ def test_basefold_full_size_verifies(bfd, oracle, ctx):
     assert oracle.basefold_verify(128, log2_c, root, point, claim, proof) == 0
     bad = bytearray(proof)
     bad[-40] ^= 1
-    assert oracle.basefold_verify(128, log2_c, root, point, claim, bytes(bad)) != 0
+    want = oracle.basefold_verify(128, log2_c, root, point, claim, bytes(bad))
+    assert want != 0
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, n)), ctx)
+    for pr, expect in [(proof, 0), (bytes(bad), want)]:
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(root)
+        ch.observe_ext_slice(claim)
+        assert pcs.verify(root, [bfd.PolyEvals(claim, point)], pr, ch) == expect
 
 
 def test_batch_equals_plain_and_oracle(bfd, oracle, ctx):

@@ -353,3 +353,16 @@ def test_quadratic_sumcheck_pinned(oracle):
         for (a, b), s in zip(pairs, scalars):
             expect ^= O.mul(O.mul(evals[a], evals[b]), s.reshape(1, 2))
         assert (sub == expect).all(), n
+
+
+def test_verify_rejects_truncated_proofs(oracle):
+    """A proof cut short anywhere is malformed (bincode cannot decode it), including the
+    rate-1 case whose proof ends with an empty query list."""
+    O = oracle
+    for log2_c in [0, 1]:
+        e, p = rnd(O, 1 << 5, 90), rnd(O, 5, 91)
+        root, claim, proof = O.basefold_prove(128, log2_c, e, p)
+        assert O.basefold_verify(128, log2_c, root, p, claim, proof) == 0
+        for cut in [1, 8, 16, len(proof) // 2]:
+            assert O.basefold_verify(128, log2_c, root, p, claim, proof[:-cut]) == 4, (log2_c, cut)
+        assert O.basefold_verify(128, log2_c, root, p, claim, proof + b"\0") == 4
