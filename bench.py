@@ -95,7 +95,18 @@ def cpu_baseline(args) -> dict:
         O.basefold_prove(LAMBDA, LOG2_C, ev, pt)
         times.append(time.perf_counter() - t0)
     t = statistics.median(times)
+    # the restatement's verifier on one proof of the batched-verification workload (2^16, rate 1/2)
+    O.set_threads(1)
+    ev1, pt1 = O.random_vec(3000, 1 << 16), O.random_vec(4000, 16)
+    root1, claim1, proof1 = O.basefold_prove(LAMBDA, 1, ev1, pt1)
+    vt = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        assert O.basefold_verify(LAMBDA, 1, root1, pt1, claim1, proof1) == 0
+        vt.append(time.perf_counter() - t0)
+    O.set_threads(cores)
     return {"value": (1 << n) / t, "unit": "elems/s", "cores": cores, "kind": "port",
+            "verify_ms_per_proof": 1e3 * statistics.median(vt), "verify_cores": 1,
             "sample": f"commit+open of 2^{n} evaluations at rate 1/4 (median of {len(times)} after a warm-up, "
                       f"C restatement of the reference CPU path, OpenMP {cores} threads, PCLMUL; host "
                       f"{cpu_model()}, nproc {os.cpu_count()})"}
@@ -203,6 +214,49 @@ def reference_shapes(bfd, ctx, torch, reps: int) -> list:
     return out
 
 
+def verification(bfd, ctx, torch, count: int) -> dict:
+    """Basefold::verify of `count` proofs of 2^16 evaluations at rate 1/2 (the config-1 shape,
+    309 queries each) in one device call; cpu_baseline times the restatement's verifier on one."""
+    n, log2_c = 16, 1
+    evals = [bfd.random_vec(3000 + i, 1 << n) for i in range(count)]
+    points = [bfd.random_vec(4000 + i, n) for i in range(count)]
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(LAMBDA, log2_c, n)), ctx)
+    roots, claims, proofs = [], [], []
+    for e, pt in zip(evals, points):
+        cd = pcs.commit(e)
+        cl = pcs.eval(cd, pt)
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(cd.comm)
+        ch.observe_ext_slice(cl)
+        proofs.append(pcs.open(cd, [bfd.PolyEvals(cl, pt)], ch))
+        roots.append(cd.comm)
+        claims.append(cl)
+        del cd
+
+    def challengers():
+        out = []
+        for r, cl in zip(roots, claims):
+            ch = bfd.GenericChallenger.keccak256()
+            ch.observe(r)
+            ch.observe_ext_slice(cl)
+            out.append(ch)
+        return out
+
+    evs = [bfd.PolyEvals(cl, pt) for cl, pt in zip(claims, points)]
+    assert pcs.verify_batch(roots, evs, proofs, challengers()) == [0] * count
+    ts = []
+    for _ in range(3):
+        chs = challengers()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        res = pcs.verify_batch(roots, evs, proofs, chs)
+        ts.append(time.perf_counter() - t0)
+    assert res == [0] * count
+    t = statistics.median(ts)
+    return {"workload": f"{count} proofs of 2^16 evaluations, rate 1/2, 309 queries each", "batch_ms": 1e3 * t,
+            "proofs_per_s": count / t}
+
+
 def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int) -> dict:
     """Config 5: Poisson arrivals (mean `rate` req/s, numpy seed 0) of
     commit-then-open requests, log2 size uniform over {14..24}, rate 1/2 or 1/4
@@ -277,6 +331,8 @@ def main() -> None:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--batch-reps", type=int, default=2)
+    ap.add_argument("--verify-proofs", type=int, default=64,
+                    help="proofs in the batched-verification figure (0: skip)")
     ap.add_argument("--copy-ref", action="store_true",
                     help="run one k_copy16 over 2^24 elements first (HBM byte-counter calibration)")
     args = ap.parse_args()
@@ -411,6 +467,8 @@ def main() -> None:
         result["reference_shapes"] = reference_shapes(bfd, ctx, torch, 3)
     if not args.no_batch:
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps)
+    if args.verify_proofs > 0:
+        result["verify"] = verification(bfd, ctx, torch, args.verify_proofs)
     if args.serve_requests > 0:
         result["serving"] = serving_trace(bfd, ctx, torch, args.serve_requests, args.serve_rate,
                                           int(args.serve_budget_gb * 1e9))

@@ -382,10 +382,10 @@ def _verify_batch(pcs: "Basefold", width: int, roots, points, values, proofs, ch
     rb = np.frombuffer(b"".join(bytes(r) for r in roots) or b"\0", dtype=np.uint8).copy()
     pts = [np.ascontiguousarray(_elems(p)) if len(p) else np.zeros((1, 2), np.uint64) for p in points]
     vals = [np.ascontiguousarray(_elems(v)) for v in values]
-    bufs = [np.frombuffer(bytes(pr) or b"\0", dtype=np.uint8).copy() for pr in proofs]
+    bufs = [bytes(pr) or b"\0" for pr in proofs]  # no copy for bytes; the library reads them in place
     P = (u8p * max(1, count))(*[_p(a) for a in pts])
     V = (u8p * max(1, count))(*[_p(a) for a in vals])
-    B = (u8p * max(1, count))(*[_p(a) for a in bufs])
+    B = (u8p * max(1, count))(*[ctypes.cast(ctypes.c_char_p(b), u8p) for b in bufs])
     lens = (ctypes.c_size_t * max(1, count))(*[len(bytes(pr)) for pr in proofs])
     chs = (ctypes.c_void_p * max(1, count))(*[c.h for c in challengers])
     res = (ctypes.c_int * max(1, count))()

@@ -257,6 +257,7 @@ struct bfd_ctx {
     bfd::rt::PinnedBuf seg_pin;     // segment tables of batched launches, host -> device
     bfd::rt::PinnedBuf eq_pin;      // eq-table halves, host -> device
     bfd::rt::PinnedBuf scalar_pin;  // random scalars of a width > 1 opening, host -> device
+    bfd::rt::PinnedBuf verify_pin;  // proofs of a verification batch, host -> device
     bfd::rt::KernelTimer timer;
     // blocks of the plain path on `stream`, reused across calls
     bfd::rt::BlockCache cache;

@@ -91,20 +91,22 @@ int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t co
         const int b = (int)code->log2_c, d = (int)code->d, L = b + d;
         const size_t q = code->q, w = width;
         std::vector<Parsed> parsed(count);
-        std::vector<size_t> live;                      // proofs left for the device
-        std::vector<uint64_t> idx;
-        std::vector<gf> chal, scal;
-        for (size_t p = 0; p < count; p++) {
-            results[p] = 0;
+        for (size_t p = 0; p < count; p++)
             if (!proofs[p] || !challengers[p] || !values[p] || (d && !points[p])) fail(BFD_ERR_ARG, "null argument");
+        // decode and replay every transcript, one proof per pool task
+        std::vector<std::vector<uint64_t>> idx_p(count);
+        std::vector<std::vector<gf>> chal_p(count), scal_p(count);
+        parallel_for(count, [&](size_t p) {
+            results[p] = 0;
             Parsed &P = parsed[p];
             if (!decode(proofs[p], proof_lens[p], b, d, q, w, P)) {
                 results[p] = 4;
-                continue;
+                return;
             }
             Challenger &ch = challengers[p]->c;
             // scalars and claim (basefold.rs:252-258)
-            std::vector<gf> sc(w);
+            std::vector<gf> &sc = scal_p[p];
+            sc.assign(w, gf_zero());
             gf claim = gf_zero();
             if (w == 1) {
                 sc[0] = gf_one();
@@ -122,7 +124,8 @@ int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t co
             }
             // sumcheck rounds interleaved with the pi-tree roots (:371-396),
             // Eval::decompress_round_poly: c1 = subclaim - 2 c0 - c2
-            std::vector<gf> ch_i(d);
+            std::vector<gf> &ch_i = chal_p[p];
+            ch_i.assign(d, gf_zero());
             gf sub = claim;
             for (int k = 0; k < d; k++) {
                 ch.observe(proofs[p] + P.roots_off + 32 * (size_t)k, 32);
@@ -140,12 +143,20 @@ int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t co
                 eq = hmul(eq, gf_add(gf_one(), gf_add(load_elem(points[p] + 16 * i), ch_i[i])));
             if (!gf_eq(hmul(P.final_poly, eq), sub)) {
                 results[p] = 1;
-                continue;
+                return;
             }
-            for (size_t t = 0; t < q; t++) idx.push_back(ch.sample_bits(L));
-            chal.insert(chal.end(), ch_i.begin(), ch_i.end());
-            scal.insert(scal.end(), sc.begin(), sc.end());
+            idx_p[p].resize(q);
+            for (size_t t = 0; t < q; t++) idx_p[p][t] = ch.sample_bits(L);
+        });
+        std::vector<size_t> live;  // proofs left for the device
+        std::vector<uint64_t> idx;
+        std::vector<gf> chal, scal;
+        for (size_t p = 0; p < count; p++) {
+            if (results[p]) continue;
             live.push_back(p);
+            idx.insert(idx.end(), idx_p[p].begin(), idx_p[p].end());
+            chal.insert(chal.end(), chal_p[p].begin(), chal_p[p].end());
+            scal.insert(scal.end(), scal_p[p].begin(), scal_p[p].end());
         }
         const size_t nl = live.size();
         if (!nl || !q) return;
@@ -168,12 +179,17 @@ int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t co
         gf *d_ch = reinterpret_cast<gf *>(reinterpret_cast<uint8_t *>(d_idx) + sz_idx);
         gf *d_sc = reinterpret_cast<gf *>(reinterpret_cast<uint8_t *>(d_ch) + sz_ch);
         VerifyJob *d_jobs = reinterpret_cast<VerifyJob *>(reinterpret_cast<uint8_t *>(d_sc) + sz_sc);
-        std::vector<uint8_t> host(pbytes + 32 * nl + sz_idx + sz_ch + sz_sc, 0);
+        // the proofs are staged in pinned memory by the pool and go up in one
+        // copy; the small arrays through another
+        ctx->verify_pin.ensure(pbytes);
+        uint8_t *stage = (uint8_t *)ctx->verify_pin.p;
+        parallel_for(nl, [&](size_t a) { memcpy(stage + poff[a], proofs[live[a]], proof_lens[live[a]]); });
+        hip_check(hipMemcpyAsync(d_proofs, stage, pbytes, hipMemcpyHostToDevice, st), "copy proofs");
+        std::vector<uint8_t> host(32 * nl + sz_idx + sz_ch + sz_sc, 0);
         std::vector<VerifyJob> jobs(nl);
         for (size_t a = 0; a < nl; a++) {
             const size_t p = live[a];
-            memcpy(host.data() + poff[a], proofs[p], proof_lens[p]);
-            memcpy(host.data() + pbytes + 32 * a, roots + 32 * p, 32);
+            memcpy(host.data() + 32 * a, roots + 32 * p, 32);
             VerifyJob &J = jobs[a];
             J.proof = d_proofs + poff[a];
             J.root = d_roots + 32 * a;
@@ -185,10 +201,10 @@ int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t co
             J.rec_bytes = parsed[p].rec_bytes;
             J.roots_off = parsed[p].roots_off;
         }
-        memcpy(host.data() + pbytes + 32 * nl, idx.data(), 8 * idx.size());
-        memcpy(host.data() + pbytes + 32 * nl + sz_idx, chal.data(), sz_ch);
-        memcpy(host.data() + pbytes + 32 * nl + sz_idx + sz_ch, scal.data(), sz_sc);
-        hip_check(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, st), "copy proofs");
+        memcpy(host.data() + 32 * nl, idx.data(), 8 * idx.size());
+        memcpy(host.data() + 32 * nl + sz_idx, chal.data(), sz_ch);
+        memcpy(host.data() + 32 * nl + sz_idx + sz_ch, scal.data(), sz_sc);
+        hip_check(hipMemcpyAsync(d_roots, host.data(), host.size(), hipMemcpyHostToDevice, st), "copy inputs");
         hip_check(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(VerifyJob) * nl, hipMemcpyHostToDevice, st), "copy jobs");
         uint8_t *d_fold = dok.as<uint8_t>(), *d_path = d_fold + nl * q;
         ctx->timer.run(st, "k_verify_fold", [&] {
