@@ -71,13 +71,20 @@ int bfd_challenger_sample_bits(bfd_challenger *c, uint32_t bits, uint64_t *out);
 /* ---- PolyCommitScheme ---- */
 /* Basefold::commit (basefold.rs:112-120): evals is the height x width
  * row-major matrix of evaluations, one polynomial per column (width 1 ..
- * 2^24), height = 2^d; each Merkle leaf hashes one 16*width-byte row. */
+ * 65535), height = 2^d; each Merkle leaf hashes one 16*width-byte row. */
 int bfd_commit(bfd_ctx *ctx, const bfd_code *code, const uint8_t *evals, size_t height, size_t width,
                bfd_commitment **out, uint8_t root[32]);
 /* Same, with the evaluations already resident in device memory (not copied
  * out of; the commitment keeps its own copy of the data). */
 int bfd_commit_device(bfd_ctx *ctx, const bfd_code *code, const void *d_evals, size_t height, size_t width,
                       bfd_commitment **out, uint8_t root[32]);
+/* Basefold::commit followed by Basefold::eval at `point` (d elements) in one
+ * call: values (width x 16 bytes) = the MLE of every column, computed in one
+ * pass over the evaluations (sum of eq(point, x) * data[x]) in the same
+ * stream pass as the commitment, one synchronisation for both.  No reference
+ * counterpart as one call (basefold.rs:112-131 are two). */
+int bfd_commit_eval(bfd_ctx *ctx, const bfd_code *code, const void *evals, int evals_on_device, size_t height,
+                    size_t width, const uint8_t *point, bfd_commitment **out, uint8_t root[32], uint8_t *values);
 int bfd_commitment_root(const bfd_commitment *comm, uint8_t root[32]);
 int bfd_commitment_destroy(bfd_commitment *comm);
 

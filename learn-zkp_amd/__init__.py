@@ -105,6 +105,8 @@ def lib():
                                        ctypes.POINTER(u8p), ctypes.POINTER(u8p), ctypes.POINTER(u8p),
                                        ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_void_p),
                                        ctypes.POINTER(ctypes.c_int)]
+        L.bfd_commit_eval.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t,
+                                      ctypes.c_size_t, u8p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, u8p]
         L.bfd_field_mul.argtypes = [u8p, u8p, u8p]
         L.bfd_field_inv.argtypes = [u8p, u8p]
         L.bfd_random_vec.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u8p]
@@ -330,6 +332,23 @@ class Basefold:
         _check(lib().bfd_commit_device(self.ctx.h, self.code_h, ctypes.c_void_p(ptr), height, width,
                                        ctypes.byref(h), root))
         return BasefoldCommitmentData(bytes(root), h, None, self, width)
+
+    def commit_eval(self, data, point, width: Optional[int] = None):
+        """commit followed by eval at ``point`` in one call (the claims come from
+        the same stream pass). Returns (commitment data, values (width, 2))."""
+        arr = np.ascontiguousarray(data, dtype=np.uint64)
+        if width is None:
+            width = arr.shape[1] if arr.ndim == 3 else 1
+        flat = arr.reshape(-1, 2)
+        if width < 1 or len(flat) % width:
+            raise BfdError(BFD_ERR_ARG, "data does not hold whole rows of the given width")
+        pt = _elems(point) if len(point) else np.zeros((1, 2), dtype=np.uint64)
+        root = (ctypes.c_uint8 * 32)()
+        h = ctypes.c_void_p()
+        vals = np.zeros((width, 2), dtype=np.uint64)
+        _check(lib().bfd_commit_eval(self.ctx.h, self.code_h, _p(flat), 0, len(flat) // width, width, _p(pt),
+                                     ctypes.byref(h), root, _p(vals)))
+        return BasefoldCommitmentData(bytes(root), h, flat, self, width), vals
 
     def data(self, comm_data: BasefoldCommitmentData) -> np.ndarray:
         return comm_data.data

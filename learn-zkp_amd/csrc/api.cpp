@@ -314,6 +314,13 @@ int bfd_commit_device(bfd_ctx *ctx, const bfd_code *code, const void *d_evals, s
                       bfd_commitment **out, uint8_t root[32]) {
     return guarded([&] { do_commit(ctx, code, d_evals, true, height, width, out, root); });
 }
+int bfd_commit_eval(bfd_ctx *ctx, const bfd_code *code, const void *evals, int evals_on_device, size_t height,
+                    size_t width, const uint8_t *point, bfd_commitment **out, uint8_t root[32], uint8_t *values) {
+    return guarded([&] {
+        if (!values || (!point && code && code->d)) fail(BFD_ERR_ARG, "null argument");
+        do_commit(ctx, code, evals, evals_on_device != 0, height, width, out, root, point, values);
+    });
+}
 int bfd_commitment_root(const bfd_commitment *comm, uint8_t root[32]) {
     return guarded([&] {
         if (!comm || !root) fail(BFD_ERR_ARG, "null argument");
@@ -337,26 +344,12 @@ int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uin
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);
         hipStream_t st = ctx->stream;
-        const int n = (int)comm->n;
-        const size_t w = comm->w;
-        std::vector<gf> x(n);
-        for (int i = 0; i < n; i++) x[i] = load_elem(point + 16 * i);
-        if (n == 0) {
-            hip_check(hipMemcpyAsync(values, comm->data.p, 16 * w, hipMemcpyDeviceToHost, st), "copy");
-            hip_check(hipStreamSynchronize(st), "eval");
-            return;
-        }
-        // evaluate every column (basefold.rs:126-131, multilinear.rs:200-214):
-        // fix the top variable repeatedly; on the row-major matrix that is one
-        // flat fix over 2^i rows of w elements
-        DevBuf tmp = ctx_buf(ctx, 16 * (comm->N / 2) * w);
-        const gf *cur = comm->data.as<gf>();
-        for (int i = n - 1; i >= 0; i--) {
-            hip_check(launch::fix(st, cur, tmp.as<gf>(), ((size_t)1 << i) * w, x[i]), "fix");
-            cur = tmp.as<gf>();
-        }
-        hip_check(hipMemcpyAsync(values, cur, 16 * w, hipMemcpyDeviceToHost, st), "copy");
+        // one pass over the data (k_mle_dot) instead of n fix passes
+        ColumnEval ev;
+        ev.enqueue(ctx, comm->data.as<gf>(), comm->N, comm->w, point, (int)comm->n);
         hip_check(hipStreamSynchronize(st), "eval");
+        ctx->timer.collect();
+        ev.finish(ctx, values);
     });
 }
 

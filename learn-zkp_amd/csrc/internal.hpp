@@ -298,6 +298,10 @@ bfd_code *create_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d
 // cached code for the batched and serving paths
 bfd_code *cached_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d);
 
+// widest commitment (columns); the per-column launches put columns on the
+// grid's y dimension
+constexpr size_t kMaxWidth = 65535;
+
 inline void set_device(bfd_ctx *ctx) { hip_check(hipSetDevice(ctx->device), "hipSetDevice"); }
 
 // a device buffer on the context stream, drawn from and returned to its cache
@@ -399,6 +403,61 @@ inline void tree_steps(size_t leaves, std::vector<TreeStep> &out, size_t from = 
     }
 }
 
+// MLE value of every column of a rows x w row-major matrix at `point`
+// (basefold.rs:126-131) in one pass over the data (k_mle_dot): enqueue()
+// stages the eq factors through ctx->eq_pin and the partial sums back into
+// ctx->pinned on the context stream; after the caller's synchronisation,
+// finish() adds them up.  Shares device_eq's invariant: the staging buffers
+// are idle whenever a call starts, because every call synchronises before it
+// returns.
+struct ColumnEval {
+    size_t w = 1;
+    unsigned nb = 0;
+    int n = 0;
+    DevBuf dev;
+    const gf *single = nullptr;  // n == 0: the values are the data row itself
+
+    void enqueue(bfd_ctx *ctx, const gf *data, size_t rows, size_t w_, const uint8_t *point, int n_) {
+        hipStream_t st = ctx->stream;
+        w = w_;
+        n = n_;
+        const unsigned max_blocks = std::max(1u, std::min(512u, (unsigned)(4096 / w)));
+        ctx->pinned.ensure(16 * (size_t)max_blocks * w + 64);
+        if (n == 0) {
+            hip_check(hipMemcpyAsync(ctx->pinned.p, data, 16 * w, hipMemcpyDeviceToHost, st), "copy values");
+            return;
+        }
+        std::vector<gf> x(n);
+        for (int i = 0; i < n; i++) x[i] = load_elem(point + 16 * i);
+        const int lo = (n + 1) / 2, hi = n - lo;
+        std::vector<gf> A = host_eq(x.data(), lo), B = host_eq(x.data() + lo, hi);
+        const size_t na = A.size(), nbq = B.size();
+        ctx->eq_pin.ensure(16 * (na + nbq));
+        gf *h = (gf *)ctx->eq_pin.p;
+        memcpy(h, A.data(), 16 * na);
+        memcpy(h + na, B.data(), 16 * nbq);
+        dev = ctx_buf(ctx, 16 * (na + nbq) + 16 * (size_t)max_blocks * w);
+        gf *dA = dev.as<gf>(), *dpart = dA + na + nbq;
+        hip_check(hipMemcpyAsync(dA, h, 16 * (na + nbq), hipMemcpyHostToDevice, st), "copy eq factors");
+        ctx->timer.run(st, "k_mle_dot", [&] {
+            return launch::mle_dot(st, data, w, dA, lo, dA + na, rows, dpart, max_blocks, &nb);
+        });
+        hip_check(hipMemcpyAsync(ctx->pinned.p, dpart, 16 * (size_t)nb * w, hipMemcpyDeviceToHost, st), "copy sums");
+    }
+    void finish(bfd_ctx *ctx, uint8_t *out) const {
+        const uint8_t *pin = (const uint8_t *)ctx->pinned.p;
+        if (n == 0) {
+            memcpy(out, pin, 16 * w);
+            return;
+        }
+        for (size_t c = 0; c < w; c++) {
+            gf s = gf_zero();
+            for (unsigned u = 0; u < nb; u++) s = gf_add(s, load_elem(pin + 16 * (c * nb + u)));
+            store_elem(out + 16 * c, s);
+        }
+    }
+};
+
 // all Merkle levels above `leaves` digests at layer0 (layers contiguous)
 inline void tree_nodes(bfd_ctx *ctx, uint64_t *layer0, size_t leaves) {
     hipStream_t st = ctx->stream;
@@ -450,10 +509,13 @@ inline void encode(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, i
     }
 }
 
+// Basefold::commit; with a point, also the claims of every column
+// (Basefold::eval) from the same stream pass and synchronisation.
 inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, bool on_device, size_t height,
-               size_t width, bfd_commitment **out, uint8_t root[32]) {
+               size_t width, bfd_commitment **out, uint8_t root[32], const uint8_t *point = nullptr,
+               uint8_t *values = nullptr) {
     if (!ctx || !code || !out || !evals) fail(BFD_ERR_ARG, "null argument");
-    if (width < 1 || width > ((size_t)1 << 24)) fail(BFD_ERR_ARG, "width must be 1..2^24");
+    if (width < 1 || width > kMaxWidth) fail(BFD_ERR_ARG, "width must be 1..65535");
     if (height != ((size_t)1 << code->d)) fail(BFD_ERR_ARG, "height must equal 2^d of the code");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     set_device(ctx);
@@ -489,8 +551,11 @@ inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, boo
     }
     tree_nodes(ctx, tree, cm->M);
     hip_check(hipMemcpyAsync(cm->root, tree + 4 * (2 * cm->M - 2), 32, hipMemcpyDeviceToHost, st), "copy root");
+    ColumnEval ev;
+    if (values) ev.enqueue(ctx, data, cm->N, w, point, (int)cm->n);
     hip_check(hipStreamSynchronize(st), "commit");
     ctx->timer.collect();
+    if (values) ev.finish(ctx, values);
     memcpy(root, cm->root, 32);
     ctx->resident += cm->bytes;
     *out = cm.release();

@@ -358,6 +358,30 @@ __global__ __launch_bounds__(CB) void k_fix(const gf *__restrict__ in, gf *__res
     }
 }
 
+// MLE evaluation of every column of a rows x w row-major matrix in one pass
+// (basefold.rs:126-131 as the sum over x of eq(point, x) * data[x]):
+// column blockIdx.y, eq(point, x) = A[x mod 2^m] * B[x >> m] rebuilt per
+// element from the two host-built factors; partial[col * gridDim.x + block]
+__global__ __launch_bounds__(CB) void k_mle_dot(const gf *__restrict__ data, size_t w, const gf *__restrict__ A,
+                                                int m, const gf *__restrict__ B, size_t rows,
+                                                gf *__restrict__ partial) {
+    __shared__ gf red[CB / 64];
+    const size_t col = blockIdx.y, mask = ((size_t)1 << m) - 1;
+    gf acc = gf_zero();
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < rows; x += (size_t)gridDim.x * blockDim.x) {
+        const gf e = gf_mul_reg(gf_ld(A + (x & mask)), gf_ld(B + (x >> m)));
+        acc = gf_add(acc, gf_mul_reg(e, gf_ld(data + x * w + col)));
+    }
+    acc = wave_xor(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        gf s = gf_zero();
+        for (int v = 0; v < CB / 64; v++) s = gf_add(s, red[v]);
+        gf_st(partial + col * gridDim.x + blockIdx.x, s);
+    }
+}
+
 // eq table as the tensor product of two host-built factors:
 // out[lo + (hi << m)] = A[lo] * B[hi]
 __global__ __launch_bounds__(CB) void k_outer(const gf *__restrict__ A, int m, const gf *__restrict__ B,
@@ -947,6 +971,14 @@ hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, 
     unsigned gsz = grid_for(q, CB, 512);
     k_prod_fold_round<<<gsz, CB, 0, st>>>(f, g, T, q, x, partial);
     *nblocks = gsz;
+    return hipGetLastError();
+}
+hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,
+                   unsigned max_blocks, unsigned *nblocks) {
+    if (w == 0 || w > 65535 || max_blocks == 0) return hipErrorInvalidValue;
+    const unsigned gx = grid_for(rows, CB, max_blocks);
+    k_mle_dot<<<dim3(gx, (unsigned)w), CB, 0, st>>>(data, w, A, m, B, rows, partial);
+    *nblocks = gx;
     return hipGetLastError();
 }
 hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,

@@ -81,6 +81,10 @@ hipError_t prod_round(hipStream_t st, const gf *f, const gf *g, const gf *T, siz
                       unsigned *nblocks);
 hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, gf x, gf *partial,
                            unsigned *nblocks);
+// MLE value of every column of a rows x w matrix at a point whose eq table is
+// A (2^m entries) x B: partial[col * nblocks + block], one pass over the data
+hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,
+                   unsigned max_blocks, unsigned *nblocks);
 // Quadratic sumcheck: per pair (blockIdx.y) and block, the three sums
 // A0, A1, A2 over halves of h entries at partial[3 * (pair * nblocks + block)]
 hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,

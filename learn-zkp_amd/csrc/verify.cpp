@@ -83,7 +83,7 @@ int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t co
     return guarded([&] {
         if (!ctx || !code || !results || (count && (!roots || !values || !proofs || !proof_lens || !challengers)))
             fail(BFD_ERR_ARG, "null argument");
-        if (width < 1 || width > ((size_t)1 << 24)) fail(BFD_ERR_ARG, "width must be 1..2^24");
+        if (width < 1 || width > kMaxWidth) fail(BFD_ERR_ARG, "width must be 1..65535");
         if (code->d && count && !points) fail(BFD_ERR_ARG, "null points");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);

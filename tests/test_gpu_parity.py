@@ -265,6 +265,19 @@ def test_verifier_batch(bfd, oracle, ctx):
     assert want.count(0) == 9
 
 
+@pytest.mark.parametrize("n,log2_c,w", [(0, 1, 1), (0, 2, 5), (1, 1, 1), (13, 1, 1), (12, 2, 7), (17, 1, 3)])
+def test_commit_eval_fused(bfd, oracle, ctx, n, log2_c, w):
+    """commit + eval in one call (the claims from the one-pass reduction) equal the restatement's
+    root and column MLEs, and bfd_eval (same reduction) agrees on the same commitment."""
+    _say(f"case commit_eval n={n} log2_c={log2_c} w={w}")
+    evals, point = bfd.random_vec(1300 + n + w, (1 << n) * w), bfd.random_vec(1400 + n, n)
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, n)), ctx)
+    cd, vals = pcs.commit_eval(evals, point, w)
+    root_o, claims_o, _ = oracle.basefold_prove_w(128, log2_c, evals, w, point)
+    assert cd.comm == root_o and (vals == claims_o).all()
+    assert (pcs.eval(cd, point) == claims_o).all()
+
+
 def test_encode_matches_oracle_codeword_root(bfd, oracle, ctx):
     """The main-tree root pins the whole codeword (every leaf)."""
     for n, log2_c in [(5, 0), (6, 1), (7, 2), (12, 2)]:
