@@ -118,7 +118,8 @@ uint64_t bfd_proof_size_width(uint32_t lambda, uint32_t log2_c, uint32_t d, uint
  * evals[p]: 2^log_sizes[p] evaluations (device pointers if evals_on_device);
  * points[p]: log_sizes[p] coordinates; claims[p]: the evaluation at the point.
  * With observe_prefix set, each challenger first observes the root and the
- * claim (the harness order of poly-commit/src/lib.rs:77-89).
+ * claim (the harness order of poly-commit/src/lib.rs:77-89).  Every
+ * polynomial needs its own challenger (BFD_ERR_ARG if two entries alias).
  * Outputs count x 32 roots and count proofs: a non-NULL proofs[p] on input is
  * a caller buffer of proof_lens[p] bytes (at least bfd_proof_size), otherwise
  * the library allocates it (free with bfd_proof_free). */

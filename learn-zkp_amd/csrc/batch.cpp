@@ -7,6 +7,8 @@
 // continuous-batching engine together: one commit tick, then one open tick
 // per round for the whole batch (segmented launches, one host round trip per
 // tick), instead of one launch sequence and round trip per polynomial.
+#include <algorithm>
+
 #include "engine.hpp"
 
 using namespace bfd::rt;
@@ -108,6 +110,11 @@ extern "C" int bfd_commit_open_batch(bfd_ctx *ctx, uint32_t lambda, uint32_t log
             if (!challengers[p] || !evals[p] || !points[p] || (claims && !claims[p])) fail(BFD_ERR_ARG, "null argument");
             chs[p] = &challengers[p]->c;
         }
+        // each polynomial's transcript advances on its own (concurrently)
+        std::vector<Challenger *> sorted(chs);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            fail(BFD_ERR_ARG, "every polynomial needs its own challenger");
         batch_commit_open(ctx, lambda, log2_c, count, log_sizes, evals, evals_on_device != 0, points, claims,
                           chs.data(), observe_prefix != 0, roots, proofs, proof_lens);
     });

@@ -121,14 +121,24 @@ static void job_alloc(Job &j, hipStream_t st) {
 
 void LaneDeleter::operator()(Lane *l) const { delete l; }
 
-void Lane::reset_ring() { seg_used_ = 0; }
+void Lane::reset_ring() {
+    flush();
+    seg_used_ = 0;
+    flushed_ = 0;
+}
 
+// Records a segmented launch: its table goes into the pinned ring now, the
+// upload and the launch itself wait for flush(), so a tick's tables travel in
+// one copy instead of one per launch.
 uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial, uint32_t max_blocks_per_seg) {
     if (segs.empty()) return 0;
     if (seg_used_ + segs.size() > seg_cap_) {
-        // the tick outgrew its staging ring: drain the stream, then start over
+        // the tick outgrew its staging ring: run what it holds, drain the
+        // stream, then start over
+        flush();
         hip_check(hipStreamSynchronize(st_), "segment ring sync");
         seg_used_ = 0;
+        flushed_ = 0;
         if (segs.size() > seg_cap_) {
             seg_cap_ = 2 * segs.size() + 256;
             seg_pin_.ensure(sizeof(Seg) * seg_cap_);
@@ -146,15 +156,28 @@ uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *pa
         g.nblk = (uint32_t)want;
         blocks += g.nblk;
     }
-    Seg *h = (Seg *)seg_pin_.p + seg_used_;
-    memcpy(h, segs.data(), sizeof(Seg) * segs.size());
-    Seg *d = seg_dev_.as<Seg>() + seg_used_;
-    hip_check(hipMemcpyAsync(d, h, sizeof(Seg) * segs.size(), hipMemcpyHostToDevice, st_), "copy segs");
+    memcpy((Seg *)seg_pin_.p + seg_used_, segs.data(), sizeof(Seg) * segs.size());
+    pending_.push_back(Pending{k, seg_used_, (int)segs.size(), blocks, partial, name});
     seg_used_ += segs.size();
-    auto go = [&] { return launch::seg_launch(st_, k, d, (int)segs.size(), blocks, partial); };
-    if (timer_) timer_->run(st_, name, go);
-    else hip_check(go(), name);
     return blocks;
+}
+
+// Uploads the ring entries recorded since the last flush in one copy and
+// enqueues their launches in order.  Every stream operation that depends on
+// a recorded launch (read-backs, gathers) comes after a flush.
+void Lane::flush() {
+    if (pending_.empty()) return;
+    hip_check(hipMemcpyAsync(seg_dev_.as<Seg>() + flushed_, (Seg *)seg_pin_.p + flushed_,
+                             sizeof(Seg) * (seg_used_ - flushed_), hipMemcpyHostToDevice, st_),
+              "copy segs");
+    flushed_ = seg_used_;
+    for (const Pending &p : pending_) {
+        const Seg *d = seg_dev_.as<Seg>() + p.off;
+        auto go = [&] { return launch::seg_launch(st_, p.k, d, p.nseg, p.blocks, p.partial); };
+        if (timer_) timer_->run(st_, p.name, go);
+        else hip_check(go(), p.name);
+    }
+    pending_.clear();
 }
 
 // Node levels of every tree in trees_ (leaf layer, leaf count).  The single
@@ -257,6 +280,7 @@ void Lane::commit_tick(std::vector<Job *> &jobs) {
         }
         seg(SegKernel::Fix, segs, "ks_fix");
     }
+    flush();
     // read back roots (and claims)
     const size_t m = jobs.size();
     std::vector<uint64_t> addr(2 * m);
@@ -395,6 +419,7 @@ void Lane::open_tick(std::vector<Job *> &jobs) {
         dotted[a]->dot_blk0 = segs[a].blk0;
         dotted[a]->dot_nblk = segs[a].nblk;
     }
+    flush();
     // (D) small read-backs: this round's pi root, or the final polynomial
     std::vector<uint64_t> addr32(m, 0), addr16(m, 0);
     for (size_t a = 0; a < m; a++) {
@@ -492,11 +517,11 @@ void Lane::open_finish(std::vector<Job *> &jobs) {
         jobs[a]->stage = Job::Done;
         jobs[a]->t_done = now_ms();
     }
-    // rounds: transcripts advance (a few field operations each: sequential);
-    // finished jobs sample their queries
-    for (size_t a = 0; a < m; a++) {
+    // rounds: every job's transcript advances (jobs own distinct
+    // challengers); finished jobs sample their queries
+    parallel_for(m, [&](size_t a) {
         Job &j = *jobs[a];
-        if (j.stage != Job::Open) continue;
+        if (j.stage != Job::Open) return;
         if (j.k < j.n) {
             gf S = gf_zero();
             for (uint32_t t = 0; t < j.dot_nblk; t++) S = gf_add(S, load_elem(rb + 48 * m + 16 * (j.dot_blk0 + t)));
@@ -507,7 +532,7 @@ void Lane::open_finish(std::vector<Job *> &jobs) {
             gather_counts(j.o, j.ne, j.nd);
             j.stage = Job::Gather;
         }
-    }
+    });
 }
 
 }  // namespace rt

@@ -96,6 +96,7 @@ class Lane {
     uint32_t seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial = nullptr,
                  uint32_t max_blocks_per_seg = 2048);
     void reset_ring();
+    void flush();
     void tree_levels();
     hipStream_t st_;
     KernelTimer *timer_;
@@ -103,7 +104,16 @@ class Lane {
     // previous one has completed on the stream)
     PinnedBuf seg_pin_, rb_pin_, gather_pin_, stage_pin_, plan_pin_;
     DevBuf seg_dev_, rb_dev_, addr_dev_, partials_, gather_dev_, plan_dev_, eq_dev_;
-    size_t seg_cap_ = 0, seg_used_ = 0;
+    size_t seg_cap_ = 0, seg_used_ = 0, flushed_ = 0;
+    struct Pending {  // a recorded segmented launch (ring entries at off)
+        SegKernel k;
+        size_t off;
+        int nseg;
+        uint32_t blocks;
+        gf *partial;
+        const char *name;
+    };
+    std::vector<Pending> pending_;
     std::vector<Seg> segs_;
     std::vector<std::pair<uint64_t *, size_t>> trees_;  // leaf layer, leaf count
     std::vector<std::vector<TreeStep>> plans_;

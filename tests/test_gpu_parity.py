@@ -387,6 +387,18 @@ def test_batch_equals_plain_and_oracle(bfd, oracle, ctx):
             assert proof_p == proofs[i].tobytes()
 
 
+def test_batch_rejects_shared_challenger(bfd, ctx):
+    """Transcripts advance concurrently, so two polynomials may not share one challenger."""
+    sizes = [4, 5]
+    evals = [bfd.random_vec(1 + i, 1 << n) for i, n in enumerate(sizes)]
+    points = [bfd.random_vec(3 + i, n) for i, n in enumerate(sizes)]
+    ch = bfd.GenericChallenger.keccak256()
+    with pytest.raises(bfd.BfdError) as e:
+        claims = [np.zeros((1, 2), np.uint64)] * 2
+        bfd.commit_open_batch(128, 1, evals, points, claims, [ch, ch], True, ctx=ctx)
+    assert e.value.code == bfd.BFD_ERR_ARG
+
+
 def test_batch_uneven_mix_verifies(bfd, oracle, ctx):
     """The benchmark mix shape (sizes 12..22) at reduced count: roots equal the
     plain commitments and every proof verifies on the CPU."""
