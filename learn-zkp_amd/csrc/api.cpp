@@ -69,7 +69,9 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     lay.pis = pis.as<gf>();
     lay.pitrees = trees.as<uint64_t>();
     const unsigned kMaxPartials = 512;
-    DevBuf partials(16 * kMaxPartials, st);
+    // [pi-tree root (32 bytes) | inner-product block sums], read back together
+    DevBuf partials = ctx_buf(ctx, 32 + 16 * kMaxPartials);
+    gf *sums = partials.as<gf>() + 2;
     ctx->pinned.ensure(32 + 16 * kMaxPartials + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
 
@@ -84,11 +86,12 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
         // pi tree over rows (pi[j], pi[j+half])  (basefold.rs:350-355)
         ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half); });
         tree_nodes(ctx, tk, half);
-        hip_check(hipMemcpyAsync(pin, tk + 4 * (2 * half - 2), 32, hipMemcpyDeviceToHost, st), "copy pi root");
-        // inner product of the halved eq table with the low half of poly (eval_impr.rs:187-197)
+        // inner product of the halved eq table with the low half of poly
+        // (eval_impr.rs:187-197); k_dot also copies the pi-tree root in front
         unsigned nb = 0;
-        ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, partials.as<gf>(), &nb); });
-        hip_check(hipMemcpyAsync(pin + 32, partials.p, 16 * nb, hipMemcpyDeviceToHost, st), "copy partials");
+        const uint64_t *root_k = tk + 4 * (2 * half - 2);
+        ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, sums, &nb, root_k); });
+        hip_check(hipMemcpyAsync(pin, partials.p, 32 + 16 * nb, hipMemcpyDeviceToHost, st), "copy root and sums");
         hip_check(hipStreamSynchronize(st), "round sync");
         const double th = now_ms();
         gf S = gf_zero();
@@ -98,9 +101,9 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
         // fold pi (binary_reed_solomon.rs:88-96) and fix the top variable of poly
         gf *pnext = pis.as<gf>() + lay.pi_off[k + 1];
         const gf *ftl = code->ft.as<gf>() + code->ft_off[i];
-        ctx->timer.run(st, "k_fold", [&] { return launch::fold(st, pik, pnext, ftl, half, x); });
         gf *pout = poly.as<gf>();
-        ctx->timer.run(st, "k_fix", [&] { return launch::fix(st, poly_cur, pout, ph, x); });
+        ctx->timer.run(st, "k_fold_fix",
+                       [&] { return launch::fold_fix(st, pik, pnext, ftl, half, poly_cur, pout, ph, x); });
         poly_cur = pout;
     }
     ctx->timer.host("round_transcript", t_host);

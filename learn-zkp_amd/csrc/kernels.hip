@@ -317,9 +317,15 @@ __device__ __forceinline__ gf wave_xor(gf v) {
     return v;
 }
 
-// partial[block] = XOR_k a[k] * b[k]
+// partial[block] = XOR_k a[k] * b[k]; with `root` set, block 0 also copies
+// that 32-byte digest in front of the sums (partial[-2..-1]), so a round's
+// pi-tree root and inner-product sums come back in one read-back
 __global__ __launch_bounds__(CB) void k_dot(const gf *__restrict__ a, const gf *__restrict__ b, size_t n,
-                                            gf *__restrict__ partial) {
+                                            gf *__restrict__ partial, const uint64_t *__restrict__ root) {
+    if (root && blockIdx.x == 0 && threadIdx.x < 2) {
+        const ulonglong2 v = reinterpret_cast<const ulonglong2 *>(root)[threadIdx.x];
+        reinterpret_cast<ulonglong2 *>(partial - 2)[threadIdx.x] = v;
+    }
     __shared__ gf red[CB / 64];
     gf acc = gf_zero();
     for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
@@ -345,6 +351,29 @@ __global__ __launch_bounds__(CB) void k_fold(const gf *__restrict__ w, gf *__res
         hi = gf_add(hi, lo);
         lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
         gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), pr)));
+    }
+}
+
+// fold of pi and fix_last_var of the table in one launch (both take the
+// round's challenge): blocks [0, nfold) fold, the others fix; the fix may run
+// in place (out == in), each thread reading only the j and j + half it owns
+__global__ __launch_bounds__(CB) void k_fold_fix(const gf *__restrict__ w, gf *__restrict__ wout,
+                                                 const gf *__restrict__ ft, size_t mid, const gf *in, gf *out,
+                                                 size_t half, gf x, unsigned nfold) {
+    const gf_pre px = gf_prepare(x);
+    if (blockIdx.x < nfold) {
+        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < mid; j += (size_t)nfold * blockDim.x) {
+            gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
+            hi = gf_add(hi, lo);
+            lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
+            gf_st(wout + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+        }
+    } else {
+        const size_t nb = gridDim.x - nfold;
+        for (size_t j = (blockIdx.x - nfold) * (size_t)blockDim.x + threadIdx.x; j < half; j += nb * blockDim.x) {
+            const gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
+            gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+        }
     }
 }
 
@@ -941,14 +970,21 @@ hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv) {
     return hipGetLastError();
 }
 
-hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks) {
+hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks,
+               const uint64_t *root) {
     unsigned g = grid_for(n, CB, 512);
-    k_dot<<<g, CB, 0, st>>>(a, b, n, partial);
+    k_dot<<<g, CB, 0, st>>>(a, b, n, partial, root);
     *nblocks = g;
     return hipGetLastError();
 }
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r) {
     k_fold<<<grid_for(mid, CB, kCapMul), CB, 0, st>>>(w, out, ft, mid, r);
+    return hipGetLastError();
+}
+hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t mid, const gf *in, gf *out,
+                    size_t half, gf x) {
+    const unsigned nfold = grid_for(mid, CB, kCapMul), nfix = grid_for(half, CB, kCapMul);
+    k_fold_fix<<<nfold + nfix, CB, 0, st>>>(w, wout, ft, mid, in, out, half, x, nfold);
     return hipGetLastError();
 }
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x) {

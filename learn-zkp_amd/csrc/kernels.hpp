@@ -73,7 +73,14 @@ hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t co
 hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv);
 // largest lv subtree() and SegKernel::Subtree accept
 int subtree_max_levels();
-hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks);
+// partial[block] = sum of a[k] b[k]; with root set, the 32 bytes at root are
+// also copied to partial - 2 (two elements in front of the sums)
+hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks,
+               const uint64_t *root = nullptr);
+// fold (binary_reed_solomon.rs:88-96) and fix_last_var with the same
+// challenge in one launch; the fix may be in place
+hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t mid, const gf *in, gf *out,
+                    size_t half, gf x);
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r);
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x);
 hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_t n);
