@@ -80,6 +80,25 @@ def traffic_ratio(kernel: str):
     return None
 
 
+def valu_per_node(kernel: str):
+    """VALU instructions per node hash per lane (SQ_INSTS_VALU x 64 / nodes)
+    from the committed rocprof PMC summary of this kernel, when one exists."""
+    path = os.path.join(ROOT, "profiles", "basefold_2e24_rate4_kernels.md")
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 5 and parts[0] == "valu_issue" and parts[1] == kernel:
+                return float(parts[4])
+    except OSError:
+        pass
+    return None
+
+
+# one wave64 VALU instruction per CU per cycle (4 SIMDs x 16 lanes, 4 cycles
+# per instruction) on 256 CUs at 2.4 GHz
+VALU_ISSUE_PEAK = 256 * 2.4e9
+
+
 def cpu_baseline(args) -> dict:
     """The CPU restatement (oracle/) with OpenMP on this host's cores, on a
     bounded sample: commit+open of 2^SAMPLE evaluations at the same rate."""
@@ -431,6 +450,11 @@ def main() -> None:
     if perms_per_step:
         rate = perms_per_step * args.steps / (kms * 1e-3)
         valu = {"keccak_perm_per_s": rate, "isolated_keccak_perm_per_s": r_k, "frac": rate / r_k}
+        ipn = valu_per_node(name) if name == "k_nodes" else None
+        if ipn:
+            # wave instructions per second against the nominal issue rate
+            valu["instr_per_node"] = ipn
+            valu["issue_frac_nominal"] = rate * ipn / 64 / VALU_ISSUE_PEAK
     # whole-step VALU position: time the step's Keccak-f permutations and
     # GF(2^128) multiplications would take at the isolated kernels' rates,
     # over the measured step time

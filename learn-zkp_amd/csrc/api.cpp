@@ -2,8 +2,9 @@
 //
 // The host drives one HIP stream per context.  Bulk work (encode, Merkle
 // hashing, folds, sumcheck reductions, query gathers) runs as kernels; the
-// Fiat-Shamir transcript stays on the host and needs one 48-byte read-back
-// per round (the pi-tree root and the sumcheck partial sums).
+// rounds' Fiat-Shamir steps run on the device too (a one-wave kernel per
+// round), so an opening reads back once after its last round, samples the
+// queries on the host and gathers the openings.
 #include "internal.hpp"
 
 using namespace bfd::rt;
@@ -68,16 +69,33 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     lay.tree = cm->tree.as<uint64_t>();
     lay.pis = pis.as<gf>();
     lay.pitrees = trees.as<uint64_t>();
-    const unsigned kMaxPartials = 512;
-    // [pi-tree root (32 bytes) | inner-product block sums], read back together
-    DevBuf partials = ctx_buf(ctx, 32 + 16 * kMaxPartials);
-    gf *sums = partials.as<gf>() + 2;
-    ctx->pinned.ensure(32 + 16 * kMaxPartials + 64);
+    // inner-product block sums of the current round (dot_blocks() <= 512)
+    DevBuf partials = ctx_buf(ctx, 16 * 512);
+    gf *sums = partials.as<gf>();
+    // the transcript runs on the device: the starting image and the round
+    // kernel's segment table go up in one copy, the d rounds are enqueued
+    // without a read-back, and one read-back after the last round returns
+    // the round polynomials, the pi-tree roots and the challenger state
+    const uint32_t in_cap = fs_in_cap(ch.input.size());
+    const size_t fsb = (fs_bytes((uint32_t)d, in_cap) + 255) & ~(size_t)255;
+    DevBuf fs = ctx_buf(ctx, fsb + sizeof(Seg) * (size_t)(d ? d : 1));
+    Seg *d_segs = (Seg *)(fs.as<uint8_t>() + fsb);
+    ctx->pinned.ensure(fsb + sizeof(Seg) * (size_t)(d ? d : 1) + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
+    if (d > 0) {
+        fs_write(o, ch, in_cap, pin);
+        Seg *hs = (Seg *)(pin + fsb);
+        for (int k = 0; k < d; k++) {
+            const size_t half = (M >> k) / 2;
+            const uint64_t *root_k = trees.as<uint64_t>() + 4 * lay.tree_off[k] + 4 * (2 * half - 2);
+            hs[k] = fs_round_seg(root_k, fs.p, k, 0, launch::dot_blocks((size_t)1 << (d - 1 - k)));
+        }
+        hip_check(hipMemcpyAsync(fs.p, pin, fsb + sizeof(Seg) * d, hipMemcpyHostToDevice, st), "copy transcript");
+    }
+    const gf *x_dev = fs_challenge(fs.p);
 
     const gf *poly_cur = poly0;
     ctx->timer.host("open_setup", now_ms() - t_setup);
-    double t_host = 0;
     for (int k = 0; k < d; k++) {
         const int i = d - 1 - k;
         const size_t half = (M >> k) / 2, ph = (size_t)1 << i;
@@ -87,29 +105,28 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
         ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half); });
         tree_nodes(ctx, tk, half);
         // inner product of the halved eq table with the low half of poly
-        // (eval_impr.rs:187-197); k_dot also copies the pi-tree root in front
+        // (eval_impr.rs:187-197)
         unsigned nb = 0;
-        const uint64_t *root_k = tk + 4 * (2 * half - 2);
-        ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, sums, &nb, root_k); });
-        hip_check(hipMemcpyAsync(pin, partials.p, 32 + 16 * nb, hipMemcpyDeviceToHost, st), "copy root and sums");
-        hip_check(hipStreamSynchronize(st), "round sync");
-        const double th = now_ms();
-        gf S = gf_zero();
-        for (unsigned t = 0; t < nb; t++) S = gf_add(S, load_elem(pin + 32 + 16 * t));
-        const gf x = o.round(k, pin, S, ch);
-        t_host += now_ms() - th;
+        ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, sums, &nb); });
+        // the round's transcript step: observe root and round poly, sample x
+        ctx->timer.run(st, "ks_round", [&] { return launch::seg_launch(st, SegKernel::Round, d_segs + k, 1, 1, sums); });
         // fold pi (binary_reed_solomon.rs:88-96) and fix the top variable of poly
         gf *pnext = pis.as<gf>() + lay.pi_off[k + 1];
         const gf *ftl = code->ft.as<gf>() + code->ft_off[i];
         gf *pout = poly.as<gf>();
         ctx->timer.run(st, "k_fold_fix",
-                       [&] { return launch::fold_fix(st, pik, pnext, ftl, half, poly_cur, pout, ph, x); });
+                       [&] { return launch::fold_fix(st, pik, pnext, ftl, half, poly_cur, pout, ph, x_dev); });
         poly_cur = pout;
     }
-    ctx->timer.host("round_transcript", t_host);
-    hip_check(hipMemcpyAsync(pin, poly_cur, 16, hipMemcpyDeviceToHost, st), "copy final");
+    const size_t rbb = d > 0 ? fs_readback_bytes((uint32_t)d) : 0;
+    if (d > 0) hip_check(hipMemcpyAsync(pin, fs.p, rbb, hipMemcpyDeviceToHost, st), "copy transcript back");
+    hip_check(hipMemcpyAsync(pin + ((rbb + 15) & ~(size_t)15), poly_cur, 16, hipMemcpyDeviceToHost, st),
+              "copy final");
     hip_check(hipStreamSynchronize(st), "final sync");
-    o.finish(load_elem(pin), ch);
+    const double th = now_ms();
+    if (d > 0) fs_read(pin, o, ch);
+    o.finish(load_elem(pin + ((rbb + 15) & ~(size_t)15)), ch);
+    ctx->timer.host("round_transcript", now_ms() - th);
 
     // openings: one gather of elements, one of digests
     double t0 = now_ms();

@@ -5,9 +5,11 @@
 // Basefold proofs, each with its own transcript; every proof equals the one
 // the plain path produces for the same polynomial.  All jobs go through the
 // continuous-batching engine together: one commit tick, then one open tick
-// per round for the whole batch (segmented launches, one host round trip per
-// tick), instead of one launch sequence and round trip per polynomial.
+// per round for the whole batch (segmented launches, transcripts on the
+// device), with the next tick enqueued while the previous one runs, instead
+// of one launch sequence per polynomial.
 #include <algorithm>
+#include <deque>
 
 #include "engine.hpp"
 
@@ -28,6 +30,13 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     set_device(ctx);
     hipStream_t st = ctx->stream;
+    double t = now_ms();
+    // host phases, reported as host:batch_* when the context's timer is on
+    auto phase = [&](const char *name) {
+        const double t1 = now_ms();
+        ctx->timer.host(name, t1 - t);
+        t = t1;
+    };
     std::vector<std::unique_ptr<Job>> jobs(count);
     std::vector<Job *> all(count);
     size_t bytes = 0, staged = 0;
@@ -47,9 +56,8 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
             hip_check(hipMemcpyAsync((void *)d, evals[p], nb, hipMemcpyHostToDevice, st), "copy evaluations");
             off += nb;
         }
-        job_prepare(ctx, j, lambda, log2_c, (int)log_sizes[p], d, points[p], claims ? claims[p] : nullptr);
-        j.ch = chs[p];
-        j.observe_prefix = observe_prefix;
+        job_prepare(ctx, j, lambda, log2_c, (int)log_sizes[p], d, points[p], claims ? claims[p] : nullptr, chs[p],
+                    observe_prefix);
         j.proof = proofs[p];
         j.proof_cap = proofs[p] ? proof_lens[p] : 0;
         bytes += j.bytes;
@@ -69,21 +77,61 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
         }
     }
     Lane &lane = ctx_lane(ctx);
+    phase("batch_prepare");
     lane.commit_tick(all);
+    phase("batch_enqueue");
+    // The proofs are written in the last ticks, when the device has little
+    // left to do; touching their pages now, while the device runs the commit
+    // tick, keeps the page faults of freshly allocated output buffers out of
+    // those ticks.
+    {
+        std::vector<std::pair<uint8_t *, size_t>> chunks;
+        for (Job *j : all) {
+            const size_t sz = proof_size(j->b, j->n, j->code->q);
+            if (j->proof && j->proof_cap < sz) fail(BFD_ERR_ARG, "proof buffer too small");
+            if (!j->proof) {
+                j->proof = (uint8_t *)malloc(sz ? sz : 1);
+                if (!j->proof) fail(BFD_ERR_ARG, "host allocation failed");
+                j->proof_owned = true;
+                j->proof_cap = sz;
+            }
+            for (size_t o = 0; o < sz; o += (size_t)1 << 20)
+                chunks.push_back({j->proof + o, std::min(sz - o, (size_t)1 << 20)});
+        }
+        parallel_for(chunks.size(), [&](size_t c) { memset(chunks[c].first, 0, chunks[c].second); });
+        phase("batch_prefault");
+    }
     hip_check(hipStreamSynchronize(st), "commit tick");
+    phase("batch_wait");
     ctx->timer.collect();
     lane.commit_finish(all);
-    std::vector<Job *> active = all;
-    while (!active.empty()) {
-        lane.open_tick(active);
-        hip_check(hipStreamSynchronize(st), "open tick");
+    phase("batch_finish");
+    // open ticks, two in flight: while one runs, the next is enqueued, so
+    // the host's share of a tick (sampling queries, assembling proofs)
+    // overlaps the device's
+    std::deque<OpenTick> inflight;
+    std::vector<Job *> work;
+    for (;;) {
+        if (inflight.size() < 2) {
+            work.clear();
+            for (Job *j : all)
+                if (j->stage == Job::Open || j->stage == Job::Gather) work.push_back(j);
+            if (!work.empty()) {
+                inflight.push_back(lane.open_tick(work));
+                phase("batch_enqueue");
+                continue;
+            }
+        }
+        if (inflight.empty()) break;
+        hip_check(hipEventSynchronize(inflight.front().done), "open tick");
+        phase("batch_wait");
         ctx->timer.collect();
-        lane.open_finish(active);
-        std::vector<Job *> next;
-        for (Job *j : active)
-            if (j->stage != Job::Done) next.push_back(j);
-        active.swap(next);
+        lane.open_finish(inflight.front());
+        inflight.pop_front();
+        phase("batch_finish");
     }
+    for (Job *j : all)
+        if (j->stage != Job::Done) fail(BFD_ERR_HIP, "batched opening did not complete");
     for (size_t p = 0; p < count; p++) {
         Job &j = *jobs[p];
         memcpy(roots + 32 * p, j.root, 32);

@@ -31,11 +31,12 @@ void ensure(DevBuf &b, size_t bytes, hipStream_t st) {
 size_t job_bytes(const Job &j) {
     const size_t c = (size_t)1 << j.b;
     return align_up(16 * j.M) + align_up(32 * (2 * j.M - 1)) + align_up(16 * ((size_t)1 << (j.n - 1))) +
-           align_up(16 * (j.N / 2)) + align_up(16 * j.lay.pis_elems(j.M, c)) + align_up(32 * j.lay.pitree_digests());
+           align_up(16 * (j.N / 2)) + align_up(16 * j.lay.pis_elems(j.M, c)) + align_up(32 * j.lay.pitree_digests()) +
+           align_up(fs_bytes((uint32_t)j.n, j.fs_cap));
 }
 
 void job_prepare(bfd_ctx *ctx, Job &j, uint32_t lambda, uint32_t log2_c, int n, const void *d_evals,
-                 const uint8_t *point, const uint8_t *claim) {
+                 const uint8_t *point, const uint8_t *claim, Challenger *ch, bool observe_prefix) {
     if (n < 1 || n > 30) fail(BFD_ERR_ARG, "batched and served polynomials need 1..30 variables");
     if (log2_c > 8) fail(BFD_ERR_ARG, "log2_c too large");
     j.n = n;
@@ -53,6 +54,11 @@ void job_prepare(bfd_ctx *ctx, Job &j, uint32_t lambda, uint32_t log2_c, int n, 
     }
     j.have_claim = claim != nullptr;
     if (claim) j.claim = load_elem(claim);
+    j.ch = ch;
+    j.observe_prefix = observe_prefix;
+    // the open starts from the transcript's current input plus the prefix
+    // (32-byte root, 16-byte claim)
+    j.fs_cap = fs_in_cap(ch->input.size() + (observe_prefix ? 48 : 0));
     j.lay.plan(j.M, n);
     j.bytes = job_bytes(j);
 }
@@ -113,6 +119,7 @@ static void job_alloc(Job &j, hipStream_t st) {
     j.poly = (gf *)take(16 * (j.N / 2));
     j.pis = (gf *)take(16 * j.lay.pis_elems(j.M, c));
     j.pitrees = (uint64_t *)take(32 * j.lay.pitree_digests());
+    j.fs = take(fs_bytes((uint32_t)j.n, j.fs_cap));
     j.lay.cw = j.cw;
     j.lay.tree = j.tree;
     j.lay.pis = j.pis;
@@ -121,10 +128,18 @@ static void job_alloc(Job &j, hipStream_t st) {
 
 void LaneDeleter::operator()(Lane *l) const { delete l; }
 
+Lane::Lane(hipStream_t st, KernelTimer *timer) : st_(st), timer_(timer) {
+    for (auto &e : ev_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+}
+Lane::~Lane() {
+    for (auto &e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
 void Lane::reset_ring() {
     flush();
-    seg_used_ = 0;
-    flushed_ = 0;
+    seg_used_[cur_] = 0;
+    flushed_[cur_] = 0;
 }
 
 // Records a segmented launch: its table goes into the pinned ring now, the
@@ -132,33 +147,35 @@ void Lane::reset_ring() {
 // one copy instead of one per launch.
 uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial, uint32_t max_blocks_per_seg) {
     if (segs.empty()) return 0;
-    if (seg_used_ + segs.size() > seg_cap_) {
+    const int c = cur_;
+    if (seg_used_[c] + segs.size() > seg_cap_[c]) {
         // the tick outgrew its staging ring: run what it holds, drain the
         // stream, then start over
         flush();
         hip_check(hipStreamSynchronize(st_), "segment ring sync");
-        seg_used_ = 0;
-        flushed_ = 0;
-        if (segs.size() > seg_cap_) {
-            seg_cap_ = 2 * segs.size() + 256;
-            seg_pin_.ensure(sizeof(Seg) * seg_cap_);
-            seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
+        seg_used_[c] = 0;
+        flushed_[c] = 0;
+        if (segs.size() > seg_cap_[c]) {
+            seg_cap_[c] = 2 * segs.size() + 256;
+            seg_pin_[c].ensure(sizeof(Seg) * seg_cap_[c]);
+            seg_dev_[c] = DevBuf(sizeof(Seg) * seg_cap_[c], st_);
         }
     }
-    // items per block: one subtree per block for Subtree, else one per thread
+    // items per block: one subtree per block for Subtree, one opening per
+    // block for Round, else one per thread
     const int bs = k == SegKernel::Subtree ? 1 : launch::seg_block_size(k);
     uint32_t blocks = 0;
     for (auto &g : segs) {
         uint64_t want = (g.count + bs - 1) / bs;
-        if (want < 1) want = 1;
+        if (want < 1 || k == SegKernel::Round) want = 1;
         if (want > max_blocks_per_seg) want = max_blocks_per_seg;
         g.blk0 = blocks;
         g.nblk = (uint32_t)want;
         blocks += g.nblk;
     }
-    memcpy((Seg *)seg_pin_.p + seg_used_, segs.data(), sizeof(Seg) * segs.size());
-    pending_.push_back(Pending{k, seg_used_, (int)segs.size(), blocks, partial, name});
-    seg_used_ += segs.size();
+    memcpy((Seg *)seg_pin_[c].p + seg_used_[c], segs.data(), sizeof(Seg) * segs.size());
+    pending_.push_back(Pending{k, seg_used_[c], (int)segs.size(), blocks, partial, name});
+    seg_used_[c] += segs.size();
     return blocks;
 }
 
@@ -167,12 +184,13 @@ uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *pa
 // a recorded launch (read-backs, gathers) comes after a flush.
 void Lane::flush() {
     if (pending_.empty()) return;
-    hip_check(hipMemcpyAsync(seg_dev_.as<Seg>() + flushed_, (Seg *)seg_pin_.p + flushed_,
-                             sizeof(Seg) * (seg_used_ - flushed_), hipMemcpyHostToDevice, st_),
+    const int c = cur_;
+    hip_check(hipMemcpyAsync(seg_dev_[c].as<Seg>() + flushed_[c], (Seg *)seg_pin_[c].p + flushed_[c],
+                             sizeof(Seg) * (seg_used_[c] - flushed_[c]), hipMemcpyHostToDevice, st_),
               "copy segs");
-    flushed_ = seg_used_;
+    flushed_[c] = seg_used_[c];
     for (const Pending &p : pending_) {
-        const Seg *d = seg_dev_.as<Seg>() + p.off;
+        const Seg *d = seg_dev_[c].as<Seg>() + p.off;
         auto go = [&] { return launch::seg_launch(st_, p.k, d, p.nseg, p.blocks, p.partial); };
         if (timer_) timer_->run(st_, p.name, go);
         else hip_check(go(), p.name);
@@ -226,11 +244,12 @@ void Lane::tree_levels() {
 void Lane::commit_tick(std::vector<Job *> &jobs) {
     if (jobs.empty()) return;
     reset_ring();
+    const int cr = cur_;
     const size_t need = 160 * jobs.size() + 256;
-    if (seg_cap_ < need) {
-        seg_cap_ = need;
-        seg_pin_.ensure(sizeof(Seg) * seg_cap_);
-        seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
+    if (seg_cap_[cr] < need) {
+        seg_cap_[cr] = need;
+        seg_pin_[cr].ensure(sizeof(Seg) * seg_cap_[cr]);
+        seg_dev_[cr] = DevBuf(sizeof(Seg) * seg_cap_[cr], st_);
     }
     int max_n = 0;
     for (size_t a = 0; a < jobs.size(); a++) {
@@ -289,22 +308,22 @@ void Lane::commit_tick(std::vector<Job *> &jobs) {
         addr[a] = (uint64_t)(j.tree + 4 * (2 * j.M - 2));
         addr[m + a] = (uint64_t)(j.have_claim ? j.data : j.poly);
     }
-    ensure(addr_dev_, 16 * m, st_);
-    ensure(rb_dev_, 64 * m, st_);
-    rb_pin_.ensure(64 * m + 64);
+    ensure(addr_dev_[cr], 16 * m, st_);
+    ensure(rb_dev_[cr], 64 * m, st_);
+    rb_pin_[cr].ensure(64 * m + 64);
     // the address list lives in pinned memory until the copy has run
-    uint64_t *pa = (uint64_t *)((uint8_t *)rb_pin_.p + 48 * m);
+    uint64_t *pa = (uint64_t *)((uint8_t *)rb_pin_[cr].p + 48 * m);
     memcpy(pa, addr.data(), 16 * m);
-    hip_check(hipMemcpyAsync(addr_dev_.p, pa, 16 * m, hipMemcpyHostToDevice, st_), "copy addr");
-    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>(), rb_dev_.p, m, 32), "gather roots");
-    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>() + m, rb_dev_.as<uint8_t>() + 32 * m, m, 16),
+    hip_check(hipMemcpyAsync(addr_dev_[cr].p, pa, 16 * m, hipMemcpyHostToDevice, st_), "copy addr");
+    hip_check(launch::gather_abs(st_, addr_dev_[cr].as<uint64_t>(), rb_dev_[cr].p, m, 32), "gather roots");
+    hip_check(launch::gather_abs(st_, addr_dev_[cr].as<uint64_t>() + m, rb_dev_[cr].as<uint8_t>() + 32 * m, m, 16),
               "gather claims");
-    hip_check(hipMemcpyAsync(rb_pin_.p, rb_dev_.p, 48 * m, hipMemcpyDeviceToHost, st_), "copy roots");
+    hip_check(hipMemcpyAsync(rb_pin_[cr].p, rb_dev_[cr].p, 48 * m, hipMemcpyDeviceToHost, st_), "copy roots");
 }
 
 void Lane::commit_finish(std::vector<Job *> &jobs) {
     const size_t m = jobs.size();
-    const uint8_t *rb = (const uint8_t *)rb_pin_.p;
+    const uint8_t *rb = (const uint8_t *)rb_pin_[cur_].p;
     parallel_for(m, [&](size_t a) {
         Job &j = *jobs[a];
         memcpy(j.root, rb + 32 * a, 32);
@@ -327,48 +346,64 @@ void Lane::commit_finish(std::vector<Job *> &jobs) {
 }
 
 // ---------------------------------------------------------------------------
-// open tick: every job in the Open stage advances one round; jobs in the
-// Gather stage gather their openings
+// open tick: every job in the Open stage advances one round (or, after its
+// last round, does the final fold and reads its transcript back); jobs in
+// the Gather stage gather their openings
 // ---------------------------------------------------------------------------
-void Lane::open_tick(std::vector<Job *> &jobs) {
-    if (jobs.empty()) return;
+OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
+    OpenTick t;
+    cur_ ^= 1;
+    const int c = cur_;
+    t.par = c;
+    t.done = ev_[c];
     reset_ring();
-    const size_t need = 64 * jobs.size() + 256;
-    if (seg_cap_ < need) {
-        seg_cap_ = need;
-        seg_pin_.ensure(sizeof(Seg) * seg_cap_);
-        seg_dev_ = DevBuf(sizeof(Seg) * seg_cap_, st_);
-    }
     auto &segs = segs_;
     const size_t m = jobs.size();
+    const size_t need = 64 * m + 256;
+    if (seg_cap_[c] < need) {
+        seg_cap_[c] = need;
+        seg_pin_[c].ensure(sizeof(Seg) * seg_cap_[c]);
+        seg_dev_[c] = DevBuf(sizeof(Seg) * seg_cap_[c], st_);
+    }
     for (size_t a = 0; a < m; a++) jobs[a]->slot = a;
 
-    // (A) halved eq tables of jobs entering their first round
+    // (A) jobs entering their first round: halved eq tables and the starting
+    // image of their device transcript
     {
-        size_t elems = 0;
+        size_t elems = 0, img = 0;
         for (Job *jp : jobs)
-            if (jp->stage == Job::Open && jp->k == 0) elems += jp->eqA.size() + jp->eqB.size();
-        if (elems) {
-            ensure(eq_dev_, 16 * elems, st_);
-            std::vector<gf> h;
-            h.reserve(elems);
+            if (jp->stage == Job::Open && jp->k == 0) {
+                elems += jp->eqA.size() + jp->eqB.size();
+                img += align_up(fs_bytes((uint32_t)jp->n, jp->fs_cap));
+            }
+        if (elems + img) {
+            ensure(eq_dev_[c], 16 * elems + 16, st_);
+            stage_pin_[c].ensure(16 * elems + img + 64);
+            gf *h = (gf *)stage_pin_[c].p;
+            uint8_t *im = (uint8_t *)stage_pin_[c].p + 16 * elems;
+            size_t e = 0;
             segs.clear();
             for (Job *jp : jobs) {
                 Job &j = *jp;
                 if (j.stage != Job::Open || j.k != 0) continue;
-                const gf *A = eq_dev_.as<gf>() + h.size();
-                h.insert(h.end(), j.eqA.begin(), j.eqA.end());
-                const gf *B = eq_dev_.as<gf>() + h.size();
-                h.insert(h.end(), j.eqB.begin(), j.eqB.end());
+                const gf *A = eq_dev_[c].as<gf>() + e;
+                memcpy(h + e, j.eqA.data(), 16 * j.eqA.size());
+                e += j.eqA.size();
+                const gf *B = eq_dev_[c].as<gf>() + e;
+                memcpy(h + e, j.eqB.data(), 16 * j.eqB.size());
+                e += j.eqB.size();
                 segs.push_back(make_seg(A, j.half_eq, B, (size_t)1 << j.o.half_eq_vars(), (uint64_t)j.eq_lo));
+                const size_t nb = fs_write(j.o, *j.ch, j.fs_cap, im);
+                hip_check(hipMemcpyAsync(j.fs, im, nb, hipMemcpyHostToDevice, st_), "copy transcript");
+                im += align_up(fs_bytes((uint32_t)j.n, j.fs_cap));
             }
-            stage_pin_.ensure(16 * elems);
-            memcpy(stage_pin_.p, h.data(), 16 * elems);
-            hip_check(hipMemcpyAsync(eq_dev_.p, stage_pin_.p, 16 * elems, hipMemcpyHostToDevice, st_), "copy eq halves");
+            if (elems)
+                hip_check(hipMemcpyAsync(eq_dev_[c].p, h, 16 * elems, hipMemcpyHostToDevice, st_), "copy eq halves");
             seg(SegKernel::Outer, segs, "ks_outer");
         }
     }
-    // (B) fold pi and fix poly with the previous round's challenge
+    // (B) fold pi and fix poly with the previous round's challenge, which
+    // the previous tick's transcript step left in the job's device region
     segs.clear();
     for (Job *jp : jobs) {
         Job &j = *jp;
@@ -376,7 +411,7 @@ void Lane::open_tick(std::vector<Job *> &jobs) {
         const int kp = j.k - 1, i = j.n - 1 - kp;
         const size_t half = (j.M >> kp) / 2;
         segs.push_back(make_seg(j.lay.pi(kp), (void *)j.lay.pi(kp + 1), j.code->ft.as<gf>() + j.code->ft_off[i], half,
-                                0, j.x_prev));
+                                (uint64_t)fs_challenge(j.fs)));
     }
     seg(SegKernel::Fold, segs, "ks_fold");
     segs.clear();
@@ -384,11 +419,11 @@ void Lane::open_tick(std::vector<Job *> &jobs) {
         Job &j = *jp;
         if (j.stage != Job::Open || j.k == 0) continue;
         const int i = j.n - j.k;
-        segs.push_back(make_seg(j.poly_cur, j.poly, nullptr, (size_t)1 << i, 0, j.x_prev));
+        segs.push_back(make_seg(j.poly_cur, j.poly, nullptr, (size_t)1 << i, (uint64_t)fs_challenge(j.fs)));
         j.poly_cur = j.poly;
     }
     seg(SegKernel::Fix, segs, "ks_fix");
-    // (C) this round's pi tree and inner product
+    // (C) this round's pi tree, inner product and transcript step
     segs.clear();
     for (Job *jp : jobs) {
         Job &j = *jp;
@@ -414,71 +449,70 @@ void Lane::open_tick(std::vector<Job *> &jobs) {
         dotted.push_back(jp);
     }
     ensure(partials_, 16 * 64 * m + 16, st_);
-    const uint32_t nblk = seg(SegKernel::Dot, segs, "ks_dot", partials_.as<gf>(), 64);
+    seg(SegKernel::Dot, segs, "ks_dot", partials_.as<gf>(), 64);
+    std::vector<Seg> rsegs;
     for (size_t a = 0; a < dotted.size(); a++) {
-        dotted[a]->dot_blk0 = segs[a].blk0;
-        dotted[a]->dot_nblk = segs[a].nblk;
+        const Job &j = *dotted[a];
+        const size_t half = (j.M >> j.k) / 2;
+        const uint64_t *root = j.lay.pitree(j.k) + 4 * (2 * half - 2);
+        rsegs.push_back(fs_round_seg(root, j.fs, j.k, segs[a].blk0, segs[a].nblk));
     }
+    seg(SegKernel::Round, rsegs, "ks_round", partials_.as<gf>());
     flush();
-    // (D) small read-backs: this round's pi root, or the final polynomial
-    std::vector<uint64_t> addr32(m, 0), addr16(m, 0);
-    for (size_t a = 0; a < m; a++) {
-        const Job &j = *jobs[a];
-        if (j.stage != Job::Open) continue;
-        if (j.k < j.n) {
-            const size_t half = (j.M >> j.k) / 2;
-            addr32[a] = (uint64_t)(j.lay.pitree(j.k) + 4 * (2 * half - 2));
-        } else {
-            addr16[a] = (uint64_t)j.poly_cur;
+    // (D) jobs past their last round: read back the transcript region and
+    // the final polynomial, as 16-byte records gathered into one buffer
+    size_t nrec = 0;
+    for (Job *jp : jobs)
+        if (jp->stage == Job::Open && jp->k == jp->n) nrec += fs_readback_bytes((uint32_t)jp->n) / 16 + 1;
+    if (nrec) {
+        rb_pin_[c].ensure(16 * nrec + 8 * nrec + 64);
+        uint64_t *addr = (uint64_t *)((uint8_t *)rb_pin_[c].p + 16 * nrec);
+        size_t r = 0;
+        for (Job *jp : jobs) {
+            Job &j = *jp;
+            if (j.stage != Job::Open || j.k != j.n) continue;
+            t.finals.push_back({jp, 16 * r});
+            const size_t nb = fs_readback_bytes((uint32_t)j.n) / 16;
+            for (size_t q = 0; q < nb; q++) addr[r++] = (uint64_t)(j.fs + 16 * q);
+            addr[r++] = (uint64_t)j.poly_cur;
         }
+        ensure(addr_dev_[c], 8 * nrec, st_);
+        ensure(rb_dev_[c], 16 * nrec, st_);
+        hip_check(hipMemcpyAsync(addr_dev_[c].p, addr, 8 * nrec, hipMemcpyHostToDevice, st_), "copy addr");
+        hip_check(launch::gather_abs(st_, addr_dev_[c].as<uint64_t>(), rb_dev_[c].p, nrec, 16), "gather finals");
+        hip_check(hipMemcpyAsync(rb_pin_[c].p, rb_dev_[c].p, 16 * nrec, hipMemcpyDeviceToHost, st_), "copy finals");
     }
-    // unused slots read a harmless address (the job's own root)
-    for (size_t a = 0; a < m; a++) {
-        if (!addr32[a]) addr32[a] = (uint64_t)(jobs[a]->tree + 4 * (2 * jobs[a]->M - 2));
-        if (!addr16[a]) addr16[a] = (uint64_t)jobs[a]->cw;
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        if (j.stage != Job::Open) continue;
+        if (j.k == j.n) j.stage = Job::Final;
+        j.k++;
     }
-    const size_t rb_bytes = 48 * m + 16 * (size_t)nblk;
-    rb_pin_.ensure(rb_bytes + 16 * m + 64);
-    ensure(addr_dev_, 16 * m, st_);
-    ensure(rb_dev_, 48 * m + 64, st_);
-    uint64_t *pa = (uint64_t *)((uint8_t *)rb_pin_.p + rb_bytes);
-    memcpy(pa, addr32.data(), 8 * m);
-    memcpy(pa + m, addr16.data(), 8 * m);
-    hip_check(hipMemcpyAsync(addr_dev_.p, pa, 16 * m, hipMemcpyHostToDevice, st_), "copy addr");
-    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>(), rb_dev_.p, m, 32), "gather roots");
-    hip_check(launch::gather_abs(st_, addr_dev_.as<uint64_t>() + m, rb_dev_.as<uint8_t>() + 32 * m, m, 16),
-              "gather finals");
-    hip_check(hipMemcpyAsync(rb_pin_.p, rb_dev_.p, 48 * m, hipMemcpyDeviceToHost, st_), "copy roots");
-    if (nblk)
-        hip_check(hipMemcpyAsync((uint8_t *)rb_pin_.p + 48 * m, partials_.p, 16 * (size_t)nblk,
-                                 hipMemcpyDeviceToHost, st_),
-                  "copy partials");
     // (E) openings of jobs whose queries are known
     size_t NE = 0, ND = 0;
-    gather_at_.assign(m, {0, 0});
-    for (size_t a = 0; a < m; a++) {
-        Job &j = *jobs[a];
+    for (Job *jp : jobs) {
+        Job &j = *jp;
         if (j.stage != Job::Gather) continue;
-        gather_at_[a] = {NE, ND};
+        t.gathered.push_back(jp);
+        t.gather_at.push_back({NE, ND});
         NE += j.ne;
         ND += j.nd;
     }
-    gather_NE_ = NE;
+    t.NE = NE;
     if (NE + ND) {
-        plan_pin_.ensure(8 * (NE + ND));
-        uint64_t *plan = (uint64_t *)plan_pin_.p;
-        parallel_for(m, [&](size_t a) {
-            Job &j = *jobs[a];
-            if (j.stage != Job::Gather) return;
-            plan_gathers(j.o, j.lay, plan + gather_at_[a].first, plan + NE + gather_at_[a].second);
+        plan_pin_[c].ensure(8 * (NE + ND));
+        uint64_t *plan = (uint64_t *)plan_pin_[c].p;
+        parallel_for(t.gathered.size(), [&](size_t g) {
+            const Job &j = *t.gathered[g];
+            plan_gathers(j.o, j.lay, plan + t.gather_at[g].first, plan + NE + t.gather_at[g].second);
         });
-        ensure(plan_dev_, 8 * (NE + ND), st_);
-        ensure(gather_dev_, 16 * NE + 32 * ND + 64, st_);
-        gather_pin_.ensure(16 * NE + 32 * ND + 64);
-        hip_check(hipMemcpyAsync(plan_dev_.p, plan, 8 * (NE + ND), hipMemcpyHostToDevice, st_), "copy plan");
-        uint8_t *oe = gather_dev_.as<uint8_t>(), *od = oe + 16 * NE;
-        auto g1 = [&] { return launch::gather_abs(st_, plan_dev_.as<uint64_t>(), oe, NE, 16); };
-        auto g2 = [&] { return launch::gather_abs(st_, plan_dev_.as<uint64_t>() + NE, od, ND, 32); };
+        ensure(plan_dev_[c], 8 * (NE + ND), st_);
+        ensure(gather_dev_[c], 16 * NE + 32 * ND + 64, st_);
+        gather_pin_[c].ensure(16 * NE + 32 * ND + 64);
+        hip_check(hipMemcpyAsync(plan_dev_[c].p, plan, 8 * (NE + ND), hipMemcpyHostToDevice, st_), "copy plan");
+        uint8_t *oe = gather_dev_[c].as<uint8_t>(), *od = oe + 16 * NE;
+        auto g1 = [&] { return launch::gather_abs(st_, plan_dev_[c].as<uint64_t>(), oe, NE, 16); };
+        auto g2 = [&] { return launch::gather_abs(st_, plan_dev_[c].as<uint64_t>() + NE, od, ND, 32); };
         if (timer_) {
             timer_->run(st_, "k_gather_abs", g1);
             timer_->run(st_, "k_gather_abs", g2);
@@ -486,20 +520,20 @@ void Lane::open_tick(std::vector<Job *> &jobs) {
             hip_check(g1(), "gather");
             hip_check(g2(), "gather");
         }
-        hip_check(hipMemcpyAsync(gather_pin_.p, oe, 16 * NE + 32 * ND, hipMemcpyDeviceToHost, st_), "copy gather");
+        hip_check(hipMemcpyAsync(gather_pin_[c].p, oe, 16 * NE + 32 * ND, hipMemcpyDeviceToHost, st_),
+                  "copy gather");
+        for (Job *jp : t.gathered) jp->stage = Job::Gathering;
     }
+    hip_check(hipEventRecord(t.done, st_), "hipEventRecord");
+    return t;
 }
 
-void Lane::open_finish(std::vector<Job *> &jobs) {
-    const size_t m = jobs.size();
-    const uint8_t *rb = (const uint8_t *)rb_pin_.p;
-    // proofs of jobs that gathered this tick
-    const uint8_t *he = (const uint8_t *)gather_pin_.p, *hd = he + 16 * gather_NE_;
-    std::vector<size_t> gidx;
-    for (size_t a = 0; a < m; a++)
-        if (jobs[a]->stage == Job::Gather) gidx.push_back(a);
-    for (size_t a : gidx) {
-        Job &j = *jobs[a];
+void Lane::open_finish(OpenTick &t) {
+    const int c = t.par;
+    // proofs of the jobs this tick gathered
+    const uint8_t *he = (const uint8_t *)gather_pin_[c].p, *hd = he + 16 * t.NE;
+    for (Job *jp : t.gathered) {
+        Job &j = *jp;
         const size_t sz = proof_size(j.o.b, j.o.d, j.o.q);
         if (j.proof && j.proof_cap < sz) fail(BFD_ERR_ARG, "proof buffer too small");
         if (!j.proof) {
@@ -509,29 +543,26 @@ void Lane::open_finish(std::vector<Job *> &jobs) {
         }
         j.proof_len = sz;
     }
-    parallel_for(gidx.size(), [&](size_t g) {
-        Job &j = *jobs[gidx[g]];
-        assemble_proof(j.o, he + 16 * gather_at_[gidx[g]].first, hd + 32 * gather_at_[gidx[g]].second, j.proof);
+    parallel_for(t.gathered.size(), [&](size_t g) {
+        Job &j = *t.gathered[g];
+        assemble_proof(j.o, he + 16 * t.gather_at[g].first, hd + 32 * t.gather_at[g].second, j.proof);
     });
-    for (size_t a : gidx) {
-        jobs[a]->stage = Job::Done;
-        jobs[a]->t_done = now_ms();
+    const double now = now_ms();
+    for (Job *jp : t.gathered) {
+        jp->stage = Job::Done;
+        jp->t_done = now;
     }
-    // rounds: every job's transcript advances (jobs own distinct
-    // challengers); finished jobs sample their queries
-    parallel_for(m, [&](size_t a) {
-        Job &j = *jobs[a];
-        if (j.stage != Job::Open) return;
-        if (j.k < j.n) {
-            gf S = gf_zero();
-            for (uint32_t t = 0; t < j.dot_nblk; t++) S = gf_add(S, load_elem(rb + 48 * m + 16 * (j.dot_blk0 + t)));
-            j.x_prev = j.o.round(j.k, rb + 32 * a, S, *j.ch);
-            j.k++;
-        } else {
-            j.o.finish(load_elem(rb + 32 * m + 16 * a), *j.ch);
-            gather_counts(j.o, j.ne, j.nd);
-            j.stage = Job::Gather;
-        }
+    // jobs past their last round: the round data and the challenger state
+    // come from the read-back; the queries are sampled here
+    // (basefold.rs:194-203); every job owns a distinct challenger
+    const uint8_t *rb = (const uint8_t *)rb_pin_[c].p;
+    parallel_for(t.finals.size(), [&](size_t a) {
+        Job &j = *t.finals[a].first;
+        const uint8_t *img = rb + t.finals[a].second;
+        fs_read(img, j.o, *j.ch);
+        j.o.finish(load_elem(img + fs_readback_bytes((uint32_t)j.n)), *j.ch);
+        gather_counts(j.o, j.ne, j.nd);
+        j.stage = Job::Gather;
     });
 }
 

@@ -4,9 +4,12 @@
 // matrix height per instance, poly-commit/src/basefold.rs:94).  Jobs advance
 // in ticks: one tick launches, for every job it holds, the next step of that
 // job (the whole commitment; or one open round: fold/fix with the previous
-// challenge, the next pi tree and inner product; or the final gather), each
-// step as one segmented launch per kernel shared by all jobs, then a single
-// stream synchronisation hands roots and sums to the per-job transcripts.
+// challenge, the next pi tree, inner product and transcript step; or the
+// final gather), each step as one segmented launch per kernel shared by all
+// jobs.  The rounds' transcripts live on the device, so an open tick needs
+// nothing from the host and the next one can be enqueued before it ends;
+// the host steps in once per job, to sample the queries after its last
+// round and to assemble its proof after the gather.
 #pragma once
 #include "internal.hpp"
 
@@ -25,6 +28,7 @@ struct Job {
     bool observe_prefix = true;        // observe(root), observe(claim) first
     Challenger *ch = nullptr;          // transcript (owned by the caller or the job)
     Challenger own_ch;
+    uint32_t fs_cap = 0;               // input capacity of the device transcript
     // device buffers: carved from `base` (an arena or a batch slab) when set,
     // else from an allocation of its own
     DevBuf slab;
@@ -33,17 +37,19 @@ struct Job {
     size_t bytes = 0;
     gf *cw = nullptr, *half_eq = nullptr, *poly = nullptr, *pis = nullptr;
     uint64_t *tree = nullptr, *pitrees = nullptr;
+    uint8_t *fs = nullptr;             // device transcript region (FsState)
     OpenLayout lay;
     OpenRounds o;
     // progress
-    enum Stage { Commit, Open, Gather, Done } stage = Commit;
-    int k = 0;                         // next open round
-    gf x_prev{};
+    // Open: rounds on the device; Final: its last round's read-back is in
+    // flight; Gather: queries sampled, openings to gather; Gathering: the
+    // gather is in flight
+    enum Stage { Commit, Open, Final, Gather, Gathering, Done } stage = Commit;
+    int k = 0;                         // next open round to enqueue (n: the final fold)
     const gf *poly_cur = nullptr;
     std::vector<gf> eqA, eqB;
     int eq_lo = 0;
     // per-tick bookkeeping
-    uint32_t dot_blk0 = 0, dot_nblk = 0;
     size_t slot = 0;
     size_t ne = 0, nd = 0;
     // results
@@ -54,13 +60,21 @@ struct Job {
     bool finished = false;             // set under the server mutex once Done
     // timestamps (ms, steady clock)
     double t_submit = 0, t_admit = 0, t_done = 0;
+    Job() = default;
+    Job(const Job &) = delete;
+    Job &operator=(const Job &) = delete;
+    // a library-allocated proof not handed to the caller dies with the job
+    ~Job() {
+        if (proof_owned && proof) free(proof);
+    }
 };
 
 // Device bytes a job holds while it is resident.
 size_t job_bytes(const Job &j);
-// Fills the request-derived fields of a job.
+// Fills the request-derived fields of a job; ch is its transcript (its
+// current input sizes the device transcript region).
 void job_prepare(bfd_ctx *ctx, Job &j, uint32_t lambda, uint32_t log2_c, int n, const void *d_evals,
-                 const uint8_t *point, const uint8_t *claim);
+                 const uint8_t *point, const uint8_t *claim, Challenger *ch, bool observe_prefix);
 
 // First-fit sub-allocator over one device allocation (serving: the HBM
 // budget is reserved once instead of mapping memory per request).
@@ -80,17 +94,31 @@ class Arena {
     std::map<size_t, size_t> free_;  // offset -> length, coalesced
 };
 
+// What the host does after an open tick has completed on the stream.
+struct OpenTick {
+    int par = 0;                                       // staging half the tick used
+    hipEvent_t done = nullptr;                         // recorded after its last operation
+    std::vector<std::pair<Job *, size_t>> finals;      // job, offset of its read-back in the staging
+    std::vector<Job *> gathered;                       // jobs whose openings it gathered
+    std::vector<std::pair<size_t, size_t>> gather_at;  // per gathered job: first element / digest record
+    size_t NE = 0;                                     // element records of the gather
+};
+
 // One execution lane: a stream with its own staging buffers.
 class Lane {
   public:
-    Lane(hipStream_t st, KernelTimer *timer) : st_(st), timer_(timer) {}
+    Lane(hipStream_t st, KernelTimer *timer);
+    ~Lane();
     hipStream_t stream() const { return st_; }
-    // enqueue a commit tick / an open tick for `jobs`; the caller then waits
-    // for the stream and calls the matching finish
+    // enqueue a commit tick for `jobs`; the caller then waits for the stream
+    // and calls commit_finish
     void commit_tick(std::vector<Job *> &jobs);
     void commit_finish(std::vector<Job *> &jobs);
-    void open_tick(std::vector<Job *> &jobs);
-    void open_finish(std::vector<Job *> &jobs);
+    // enqueue an open tick for the jobs with device work (stage Open or
+    // Gather); at most two ticks may be in flight, finished in order with
+    // open_finish once t.done has completed
+    OpenTick open_tick(std::vector<Job *> &jobs);
+    void open_finish(OpenTick &t);
 
   private:
     uint32_t seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial = nullptr,
@@ -100,11 +128,15 @@ class Lane {
     void tree_levels();
     hipStream_t st_;
     KernelTimer *timer_;
-    // pinned staging; reused from tick to tick (a tick starts only after the
-    // previous one has completed on the stream)
-    PinnedBuf seg_pin_, rb_pin_, gather_pin_, stage_pin_, plan_pin_;
-    DevBuf seg_dev_, rb_dev_, addr_dev_, partials_, gather_dev_, plan_dev_, eq_dev_;
-    size_t seg_cap_ = 0, seg_used_ = 0, flushed_ = 0;
+    // staging in two halves, alternating per tick: a tick may be enqueued
+    // while the previous one still runs, so it must not overwrite what that
+    // one's copies and kernels have yet to read
+    int cur_ = 0;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    PinnedBuf seg_pin_[2], rb_pin_[2], gather_pin_[2], stage_pin_[2], plan_pin_[2];
+    DevBuf seg_dev_[2], rb_dev_[2], addr_dev_[2], gather_dev_[2], plan_dev_[2], eq_dev_[2];
+    DevBuf partials_;  // inner-product sums (stream-ordered: one tick at a time uses them)
+    size_t seg_cap_[2] = {0, 0}, seg_used_[2] = {0, 0}, flushed_[2] = {0, 0};
     struct Pending {  // a recorded segmented launch (ring entries at off)
         SegKernel k;
         size_t off;
@@ -118,8 +150,6 @@ class Lane {
     std::vector<std::pair<uint64_t *, size_t>> trees_;  // leaf layer, leaf count
     std::vector<std::vector<TreeStep>> plans_;
     std::vector<size_t> singles_;  // leading k_nodes-form steps of each plan
-    std::vector<std::pair<size_t, size_t>> gather_at_;  // per job: first element / digest record
-    size_t gather_NE_ = 0;
 };
 
 }  // namespace rt

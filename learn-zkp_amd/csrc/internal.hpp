@@ -208,9 +208,15 @@ struct KernelTimer {
         t.first += ms;
         t.second += 1;
     }
-    // call after a stream synchronisation
+    // adds the launches that have completed; launches still in flight (work
+    // enqueued after the synchronisation the caller waited for) stay pending
     void collect() {
+        std::vector<Rec> later;
         for (auto &r : pending) {
+            if (hipEventQuery(r.b) != hipSuccess) {
+                later.push_back(r);
+                continue;
+            }
             float ms = 0;
             hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
             auto &t = totals[r.name];
@@ -219,7 +225,7 @@ struct KernelTimer {
             pool.push_back(r.a);
             pool.push_back(r.b);
         }
-        pending.clear();
+        pending.swap(later);
     }
     ~KernelTimer() {
         for (auto &r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -665,6 +671,80 @@ struct OpenRounds {
         for (size_t t = 0; t < q; t++) idx[t] = ch.sample_bits(L);
     }
 };
+
+// ---------------------------------------------------------------------------
+// Device transcript of an opening (FsState region, kernels.hpp): the host
+// writes the starting image (the challenger's bytes and the Eval prover's
+// constants), the round kernel advances it on the device, and after the
+// last round the host reads back the round data and the challenger state.
+// ---------------------------------------------------------------------------
+// input capacity: the starting input, or the 32-byte digest that replaces
+// it at the first sample, plus the 64 bytes a round observes; a multiple of 8
+inline uint32_t fs_in_cap(size_t in_len) { return (uint32_t)(((in_len > 32 ? in_len : 32) + 64 + 7) & ~(size_t)7); }
+
+// Writes the starting image to dst; returns the bytes to upload.
+inline size_t fs_write(const OpenRounds &o, const Challenger &ch, uint32_t in_cap, uint8_t *dst) {
+    if (ch.input.size() + 64 > in_cap || ch.output.size() > 32) fail(BFD_ERR_ARG, "transcript state too large");
+    FsState h;
+    memset(&h, 0, sizeof h);
+    h.in_len = (uint32_t)ch.input.size();
+    h.out_len = (uint32_t)ch.output.size();
+    h.in_cap = in_cap;
+    h.n = (uint32_t)o.n;
+    h.sub = o.subclaim;
+    h.inner = o.inner_sub;
+    h.s_delta = o.s_delta;
+    if (!ch.output.empty()) memcpy(h.out, ch.output.data(), ch.output.size());
+    memcpy(dst, &h, sizeof h);
+    gf *c = (gf *)(dst + fs_consts_off());
+    for (int i = 0; i < o.n; i++) {
+        c[i] = o.r[i];
+        c[o.n + i] = o.rinv[i];
+    }
+    for (int i = 0; i <= o.n; i++) c[2 * o.n + i] = o.cf[i];
+    const size_t in_off = fs_input_off((uint32_t)o.n);
+    memset(dst + fs_rounds_off((uint32_t)o.n), 0, in_off - fs_rounds_off((uint32_t)o.n));
+    if (!ch.input.empty()) memcpy(dst + in_off, ch.input.data(), ch.input.size());
+    return in_off + ch.input.size();
+}
+
+// bytes to read back after the last round (the input is then one digest)
+inline size_t fs_readback_bytes(uint32_t n) { return fs_input_off(n) + 32; }
+
+// Takes the round polynomials, the pi-tree roots and the challenger state
+// from a read-back image into o and ch (the state OpenRounds::round would
+// have left on the host).
+inline void fs_read(const uint8_t *img, OpenRounds &o, Challenger &ch) {
+    FsState h;
+    memcpy(&h, img, sizeof h);
+    if ((int)h.n != o.n || h.in_len > 32 || h.out_len > 32) fail(BFD_ERR_HIP, "device transcript state is corrupt");
+    const FsRound *rd = (const FsRound *)(img + fs_rounds_off(h.n));
+    for (int k = 0; k < o.d; k++) {
+        o.comp[k] = {rd[k].rp0, rd[k].rp2};
+        memcpy(o.pi_roots[k].data(), rd[k].root, 32);
+    }
+    const uint8_t *in = img + fs_input_off(h.n);
+    ch.input.assign(in, in + h.in_len);
+    ch.output.assign(h.out, h.out + h.out_len);
+    o.subclaim = h.sub;
+    o.inner_sub = h.inner;
+    o.s_delta = h.s_delta;
+}
+
+// Segment of the round kernel for round k of one opening: its pi-tree root,
+// its transcript region and its inner-product sums partial[blk0, blk0 + nb).
+inline Seg fs_round_seg(const uint64_t *root, void *fs, int k, uint32_t blk0, uint32_t nb) {
+    Seg g;
+    memset(&g, 0, sizeof g);
+    g.in = root;
+    g.out = fs;
+    g.count = (uint64_t)k;
+    g.x = gf{blk0, nb};
+    return g;
+}
+
+// Device address of the challenge slot of a transcript region.
+inline const gf *fs_challenge(const void *fs) { return (const gf *)((const uint8_t *)fs + offsetof(FsState, x)); }
 
 // Device addresses of one polynomial's opening data.
 struct OpenLayout {

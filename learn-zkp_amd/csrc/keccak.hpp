@@ -159,6 +159,39 @@ __device__ __forceinline__ void keccak256_var(const uint64_t *m, size_t nw, uint
     for (int k = 0; k < 4; k++) out[k] = (uint64_t)s[k].lo | ((uint64_t)s[k].hi << 32);
 }
 
+// Keccak-256 of len bytes at m, for the one-lane transcript sponge.  m is
+// 8-byte aligned and readable up to len rounded up to a multiple of 8 (the
+// bytes past len are masked off); the padding (0x01 after the message, 0x80
+// in the last rate byte) goes into the block that holds the end of the
+// message, a block of its own when len is a multiple of 136.
+__device__ inline void keccak256_bytes(const uint8_t *m, uint32_t len, uint64_t out[4]) {
+    const uint64_t *m64 = reinterpret_cast<const uint64_t *>(m);
+    lane s[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) s[k] = lane{0u, 0u};
+    for (uint32_t off = 0;; off += 136) {
+        const uint32_t rest = len - off;
+        const bool last = rest < 136;
+#pragma unroll
+        for (int k = 0; k < 17; k++) {
+            const uint32_t wo = 8 * k;
+            uint64_t v = 0;
+            if (wo < rest) {
+                v = m64[(off + wo) >> 3];
+                if (rest - wo < 8) v &= (1ull << (8 * (rest - wo))) - 1;
+            }
+            if (last && (rest >> 3) == (uint32_t)k) v ^= 1ull << (8 * (rest & 7));
+            if (last && k == 16) v ^= 0x8000000000000000ull;
+            s[k].lo ^= (uint32_t)v;
+            s[k].hi ^= (uint32_t)(v >> 32);
+        }
+        keccak_f(s);
+        if (last) break;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[k] = (uint64_t)s[k].lo | ((uint64_t)s[k].hi << 32);
+}
+
 // permutation on 64-bit words (used by the micro-benchmark)
 __device__ __forceinline__ void keccak_f(uint64_t w[25]) {
     lane s[25];

@@ -359,8 +359,8 @@ __global__ __launch_bounds__(CB) void k_fold(const gf *__restrict__ w, gf *__res
 // in place (out == in), each thread reading only the j and j + half it owns
 __global__ __launch_bounds__(CB) void k_fold_fix(const gf *__restrict__ w, gf *__restrict__ wout,
                                                  const gf *__restrict__ ft, size_t mid, const gf *in, gf *out,
-                                                 size_t half, gf x, unsigned nfold) {
-    const gf_pre px = gf_prepare(x);
+                                                 size_t half, const gf *__restrict__ x, unsigned nfold) {
+    const gf_pre px = gf_prepare(gf_ld(x));
     if (blockIdx.x < nfold) {
         for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < mid; j += (size_t)nfold * blockDim.x) {
             gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
@@ -765,13 +765,18 @@ __global__ __launch_bounds__(CB) void ks_dot(const Seg *segs, int nseg, gf *__re
     }
 }
 
-// in = pi (2*count), out = next pi (count), aux = level twiddles, x = r
+// the segment's challenge: at device address arg when arg is set (written
+// there by the round's transcript kernel), else the value x
+__device__ __forceinline__ gf seg_challenge(const Seg &g) { return g.arg ? gf_ld((const gf *)g.arg) : g.x; }
+
+// in = pi (2*count), out = next pi (count), aux = level twiddles, challenge
+// per seg_challenge
 __global__ __launch_bounds__(CB) void ks_fold(const Seg *segs, int nseg) {
     const Seg &g = my_seg(segs, nseg);
     const gf *w = (const gf *)g.in, *ft = (const gf *)g.aux;
     gf *out = (gf *)g.out;
     const size_t mid = g.count;
-    const gf_pre pr = gf_prepare(g.x);
+    const gf_pre pr = gf_prepare(seg_challenge(g));
     SEG_LOOP(g, j) {
         gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
         hi = gf_add(hi, lo);
@@ -780,17 +785,111 @@ __global__ __launch_bounds__(CB) void ks_fold(const Seg *segs, int nseg) {
     }
 }
 
-// in = table (2*count), out = fixed table (count), x = challenge
+// in = table (2*count), out = fixed table (count), challenge per
+// seg_challenge
 __global__ __launch_bounds__(CB) void ks_fix(const Seg *segs, int nseg) {
     const Seg &g = my_seg(segs, nseg);
     const gf *in = (const gf *)g.in;
     gf *out = (gf *)g.out;
     const size_t half = g.count;
-    const gf_pre px = gf_prepare(g.x);
+    const gf_pre px = gf_prepare(seg_challenge(g));
     SEG_LOOP(g, j) {
         gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
         gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
     }
+}
+
+__device__ __forceinline__ gf lane_gf(gf v, int src) { return gf{(uint64_t)__shfl((long long)v.lo, src), (uint64_t)__shfl((long long)v.hi, src)}; }
+
+// One round of the Eval sumcheck's transcript for one opening, run by one
+// wave (OpenRounds::round on the device; basefold.rs:356, eval.rs:64-110,
+// eval_impr.rs:41-58, sumcheck/src/lib.rs:94-100): turn the inner product S
+// into the compressed round polynomial, observe the pi-tree root and
+// [rp0, rp2], sample the challenge and advance the subclaims.  Independent
+// products run on different lanes at once (lane t takes operand pair t), so
+// the round costs six multiplication latencies instead of twelve; lane 0
+// writes the results.
+__device__ void fs_round(FsState *st, int k, gf S, const uint64_t *root) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = st->n;
+    const int i = (int)n - 1 - k;
+    uint8_t *base = (uint8_t *)st;
+    const gf *r = (const gf *)(base + fs_consts_off()), *rinv = r + n, *cf = rinv + n;
+    FsRound *rounds = (FsRound *)(base + fs_rounds_off(n));
+    uint8_t *in = base + fs_input_off(n);
+    uint64_t *in64 = reinterpret_cast<uint64_t *>(in);
+    const gf sub = (i == (int)n - 1) ? st->sub : st->inner;
+    const gf one_r = gf_add(gf_one(), gf_ld(r + i)), sd = st->s_delta;
+    // coeff0 = cf_i S (eval_impr.rs:187-197 scaled by the correcting factor)
+    // and d0 = s_delta (1 + r_i) at once
+    gf p = gf_mul_reg(lane == 1 ? sd : gf_ld(cf + i), lane == 1 ? one_r : S);
+    const gf coeff0 = lane_gf(p, 0), d0 = lane_gf(p, 1);
+    // eval_1 = (sub + (1 + r_i) coeff0) / r_i  (eval_impr.rs:41-43)
+    const gf eval1 = gf_mul_reg(gf_add(sub, gf_mul_reg(one_r, coeff0)), gf_ld(rinv + i));
+    const gf coeff1 = gf_add(eval1, coeff0);
+    // the four products of (coeff0 + coeff1 X)(d0 + s_delta X)
+    p = gf_mul_reg((lane & 2) ? coeff1 : coeff0, (lane & 1) ? sd : d0);
+    const gf rp0 = lane_gf(p, 0), rp1 = gf_add(lane_gf(p, 1), lane_gf(p, 2)), rp2 = lane_gf(p, 3);
+    // observe root || rp0 || rp2 (little-endian words; observing drops the
+    // unread output) and hash the whole input for the sample
+    uint64_t d[4];
+    if (st->in_len == 32) {
+        // every round after the first: the input is the previous digest
+        const uint64_t m[12] = {in64[0], in64[1], in64[2], in64[3], root[0], root[1],
+                                root[2],  root[3],  rp0.lo, rp0.hi, rp2.lo, rp2.hi};
+        kk::keccak256_words<12>(m, d);
+    } else {
+        if (lane == 0) {
+            const uint64_t w[8] = {root[0], root[1], root[2], root[3], rp0.lo, rp0.hi, rp2.lo, rp2.hi};
+            const uint32_t len = st->in_len;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+#pragma unroll 1
+                for (int t = 0; t < 8; t++) in[len + 8 * q + t] = (uint8_t)(w[q] >> (8 * t));
+            }
+            kk::keccak256_bytes(in, len + 64, d);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) d[q] = (uint64_t)__shfl((long long)d[q], 0);
+    }
+    // 16 bytes are taken from the end of the digest (byte j of the challenge
+    // = digest byte 31 - j); the digest becomes the input and the output,
+    // whose bytes 0..15 stay unread
+    const gf x{__builtin_bswap64(d[3]), __builtin_bswap64(d[2])};
+    // x rp2, coeff1 x and s_delta (1 + r_i + x) at once (eval.rs:110,
+    // lib.rs:100, eval_impr.rs:45-58)
+    p = gf_mul_reg(lane == 0 ? x : (lane == 1 ? coeff1 : sd), lane == 0 ? rp2 : (lane == 1 ? x : gf_add(one_r, x)));
+    const gf xrp2 = lane_gf(p, 0), cx = lane_gf(p, 1), sd_next = lane_gf(p, 2);
+    const gf sub_next = gf_add(rp0, gf_mul_reg(x, gf_add(rp1, xrp2)));
+    if (lane != 0) return;
+    uint64_t *out64 = reinterpret_cast<uint64_t *>(st->out);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        in64[q] = d[q];
+        out64[q] = d[q];
+    }
+    st->in_len = 32;
+    st->out_len = 16;
+    st->sub = sub_next;
+    st->inner = gf_add(coeff0, cx);
+    st->s_delta = sd_next;
+    st->x = x;
+    FsRound &rec = rounds[k];
+    rec.rp0 = rp0;
+    rec.rp2 = rp2;
+    for (int t = 0; t < 4; t++) rec.root[t] = root[t];
+}
+
+// One block (one wave) per opening: in = the round's pi-tree root, out =
+// FsState, count = round k, x = (first block, blocks) of the round's inner
+// product sums in `partial`
+__global__ __launch_bounds__(64) void ks_round(const Seg *segs, int nseg, const gf *__restrict__ partial) {
+    const Seg &g = my_seg(segs, nseg);
+    const uint32_t b0 = (uint32_t)g.x.lo, nb = (uint32_t)g.x.hi;
+    gf acc = gf_zero();
+    for (uint32_t t = threadIdx.x; t < nb; t += 64) acc = gf_add(acc, gf_ld(partial + b0 + t));
+    acc = wave_xor(acc);
+    fs_round((FsState *)g.out, (int)g.count, acc, (const uint64_t *)g.in);
 }
 
 // in = A (2^m), aux = B, out = table, arg = m
@@ -972,17 +1071,18 @@ hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv) {
 
 hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks,
                const uint64_t *root) {
-    unsigned g = grid_for(n, CB, 512);
+    const unsigned g = dot_blocks(n);
     k_dot<<<g, CB, 0, st>>>(a, b, n, partial, root);
     *nblocks = g;
     return hipGetLastError();
 }
+unsigned dot_blocks(size_t n) { return grid_for(n, CB, 512); }
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r) {
     k_fold<<<grid_for(mid, CB, kCapMul), CB, 0, st>>>(w, out, ft, mid, r);
     return hipGetLastError();
 }
 hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t mid, const gf *in, gf *out,
-                    size_t half, gf x) {
+                    size_t half, const gf *x) {
     const unsigned nfold = grid_for(mid, CB, kCapMul), nfix = grid_for(half, CB, kCapMul);
     k_fold_fix<<<nfold + nfix, CB, 0, st>>>(w, wout, ft, mid, in, out, half, x, nfold);
     return hipGetLastError();
@@ -1068,6 +1168,7 @@ hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int ns
     case SegKernel::Fold: ks_fold<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Fix: ks_fix<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Outer: ks_outer<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::Round: ks_round<<<nblocks, 64, 0, st>>>(d_segs, nseg, partial); break;
     }
     return hipGetLastError();
 }
@@ -1081,6 +1182,7 @@ int seg_block_size(SegKernel which) {
     case SegKernel::LeavesPair:
     case SegKernel::Nodes: return KB;
     case SegKernel::Subtree: return kSubThreads;
+    case SegKernel::Round: return 64;
     default: return CB;
     }
 }

@@ -35,12 +35,40 @@ struct VerifyJob {
     uint64_t roots_off;     // byte offset of the pi-tree roots
 };
 
-enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer };
+// Device-resident Fiat-Shamir state of one opening: the byte buffers of
+// GenericChallenger<_, Keccak256> (p3/src/challenger.rs:24-90 over plonky3's
+// HashChallenger<u8, _, 32>) and the Eval-sumcheck scalars that change from
+// round to round (sumcheck/src/function/eval.rs:90-110, eval_impr.rs:41-58).
+// One region per opening, laid out as
+//   FsState | gf consts[3n + 1] (r, r^-1, correcting factors) |
+//   FsRound rounds[n] | uint8_t input[in_cap]
+struct FsState {
+    uint32_t in_len, out_len, in_cap, n;
+    gf sub;      // subclaim of the Eval sumcheck (used in the first round)
+    gf inner;    // inner subclaim (eval.rs:110)
+    gf s_delta;  // eq(r[i+1..], x[i+1..]) of the delta polynomial
+    gf x;        // last sampled challenge (read by the fold and fix kernels)
+    uint8_t out[32];  // sampled digest; bytes out[0, out_len) are still unread
+};
+// What one round leaves for the proof: the compressed round polynomial
+// (eval.rs:64-66) and the pi-tree root observed before it.
+struct FsRound {
+    gf rp0, rp2;
+    uint64_t root[4];
+};
+__host__ __device__ inline size_t fs_consts_off() { return sizeof(FsState); }
+__host__ __device__ inline size_t fs_rounds_off(uint32_t n) { return sizeof(FsState) + 16 * (3 * (size_t)n + 1); }
+__host__ __device__ inline size_t fs_input_off(uint32_t n) { return fs_rounds_off(n) + sizeof(FsRound) * (size_t)n; }
+__host__ __device__ inline size_t fs_bytes(uint32_t n, uint32_t in_cap) { return fs_input_off(n) + in_cap; }
+
+enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer,
+                       Round };
 
 namespace launch {
 
 // segmented launch over nseg segments occupying nblocks blocks in total;
-// `partial` receives one element per block for SegKernel::Dot
+// `partial` receives one element per block for SegKernel::Dot and holds the
+// sums SegKernel::Round reduces (one block per segment)
 hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int nseg, uint32_t nblocks, gf *partial);
 int seg_block_size(SegKernel which);
 // out[i] = rb (16 or 32) bytes from absolute device address src[i]
@@ -77,10 +105,13 @@ int subtree_max_levels();
 // also copied to partial - 2 (two elements in front of the sums)
 hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks,
                const uint64_t *root = nullptr);
+// the number of blocks (partial sums) dot() uses for n elements
+unsigned dot_blocks(size_t n);
 // fold (binary_reed_solomon.rs:88-96) and fix_last_var with the same
-// challenge in one launch; the fix may be in place
+// challenge in one launch; the fix may be in place.  The challenge is read
+// from device memory (*x), where the round's transcript kernel wrote it.
 hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t mid, const gf *in, gf *out,
-                    size_t half, gf x);
+                    size_t half, const gf *x);
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r);
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x);
 hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_t n);

@@ -6,7 +6,9 @@
 // requests admitted since its last tick, the open lane runs one open tick
 // (one round, or the final gather, of every committed request).  Both ticks
 // are in flight together, so newly arriving work overlaps the rounds of
-// earlier requests on the device.  A request is admitted when its resident
+// earlier requests on the device; and since the rounds' transcripts run on
+// the device, the open lane keeps a second tick enqueued behind the running
+// one.  A request is admitted when its resident
 // footprint (codeword, all Merkle layers, pi vectors and trees) fits the
 // server's HBM budget; its memory is released when its proof is done.  Every
 // request uses a fresh transcript with the harness prefix (observe root,
@@ -48,12 +50,15 @@ namespace {
 void worker_loop(bfd_server *S) {
     try {
         hip_check(hipSetDevice(S->ctx->device), "hipSetDevice");
-        hipEvent_t ev_commit, ev_open;
+        hipEvent_t ev_commit;
         hip_check(hipEventCreateWithFlags(&ev_commit, hipEventDisableTiming), "event");
-        hip_check(hipEventCreateWithFlags(&ev_open, hipEventDisableTiming), "event");
-        std::vector<Job *> committing, opening, in_tick;
-        bool commit_busy = false, open_busy = false;
-        double commit_t0 = 0, open_t0 = 0;
+        std::vector<Job *> committing, opening, work;
+        // open ticks in flight (at most two: the rounds need nothing from
+        // the host, so the next tick is enqueued while one runs) and the
+        // time each was enqueued
+        std::deque<std::pair<OpenTick, double>> open_q;
+        bool commit_busy = false;
+        double commit_t0 = 0;
         for (;;) {
             // completions
             if (commit_busy && hipEventQuery(ev_commit) == hipSuccess) {
@@ -65,14 +70,13 @@ void worker_loop(bfd_server *S) {
                 committing.clear();
                 commit_busy = false;
             }
-            if (open_busy && hipEventQuery(ev_open) == hipSuccess) {
+            while (!open_q.empty() && hipEventQuery(open_q.front().first.done) == hipSuccess) {
+                OpenTick &ot = open_q.front().first;
                 const double t1 = now_ms();
-                S->t_open_busy += t1 - open_t0;
-                S->open_lane->open_finish(in_tick);
+                S->t_open_busy += t1 - open_q.front().second;
+                S->open_lane->open_finish(ot);
                 S->t_finish += now_ms() - t1;
-                std::vector<Job *> done;
-                for (Job *j : in_tick)
-                    if (j->stage == Job::Done) done.push_back(j);
+                std::vector<Job *> done = ot.gathered;  // a tick's gathers finish their jobs
                 if (!done.empty()) {
                     std::vector<Job *> keep;
                     for (Job *j : opening)
@@ -89,8 +93,7 @@ void worker_loop(bfd_server *S) {
                     S->cv_done.notify_all();
                     S->cv_work.notify_all();
                 }
-                in_tick.clear();
-                open_busy = false;
+                open_q.pop_front();
             }
             // admission and launches
             if (!commit_busy) {
@@ -120,17 +123,19 @@ void worker_loop(bfd_server *S) {
                     commit_busy = true;
                 }
             }
-            if (!open_busy && !opening.empty()) {
-                in_tick = opening;
-                open_t0 = now_ms();
-                S->open_lane->open_tick(in_tick);
-                hip_check(hipEventRecord(ev_open, S->s_open), "record");
-                S->t_enqueue += now_ms() - open_t0;
-                S->n_open_ticks++;
-                S->n_open_jobs += in_tick.size();
-                open_busy = true;
+            if (open_q.size() < 2) {
+                work.clear();
+                for (Job *j : opening)
+                    if (j->stage == Job::Open || j->stage == Job::Gather) work.push_back(j);
+                if (!work.empty()) {
+                    const double t0 = now_ms();
+                    open_q.push_back({S->open_lane->open_tick(work), t0});
+                    S->t_enqueue += now_ms() - t0;
+                    S->n_open_ticks++;
+                    S->n_open_jobs += work.size();
+                }
             }
-            if (!commit_busy && !open_busy) {
+            if (!commit_busy && open_q.empty()) {
                 std::unique_lock<std::mutex> lk(S->mu);
                 if (S->stopping && S->pending.empty()) break;
                 // nothing in flight: everything admitted has finished, so the
@@ -142,7 +147,6 @@ void worker_loop(bfd_server *S) {
             }
         }
         (void)hipEventDestroy(ev_commit);
-        (void)hipEventDestroy(ev_open);
     } catch (const Error &e) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->fatal = e.msg;
@@ -196,9 +200,7 @@ int bfd_server_submit(bfd_server *S, uint32_t lambda, uint32_t log2_c, uint32_t 
     return guarded([&] {
         if (!S || !d_evals || !point || !ticket) fail(BFD_ERR_ARG, "null argument");
         auto j = std::make_unique<Job>();
-        job_prepare(S->ctx, *j, lambda, log2_c, (int)n, d_evals, point, nullptr);
-        j->ch = &j->own_ch;
-        j->observe_prefix = true;
+        job_prepare(S->ctx, *j, lambda, log2_c, (int)n, d_evals, point, nullptr, &j->own_ch, true);
         if (S->budget && j->bytes > S->budget) fail(BFD_ERR_BUDGET, "request larger than the server's HBM budget");
         j->t_submit = now_ms();
         std::lock_guard<std::mutex> lk(S->mu);
@@ -276,9 +278,7 @@ int bfd_server_destroy(bfd_server *S) {
         (void)hipSetDevice(S->ctx->device);
         (void)hipStreamSynchronize(S->s_open);
         (void)hipStreamSynchronize(S->s_commit);
-        for (auto &kv : S->jobs)
-            if (kv.second->proof_owned && kv.second->proof) free(kv.second->proof);
-        S->jobs.clear();
+        S->jobs.clear();  // frees proofs never collected
         S->commit_lane.reset();
         S->open_lane.reset();
         (void)hipStreamSynchronize(S->s_open);
