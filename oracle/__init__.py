@@ -255,9 +255,10 @@ def basefold_verify(lam: int, log2_c: int, root: bytes, point, claim, proof: byt
                                          ctypes.c_size_t(len(proof))))
 
 
-def basefold_prove_w(lam: int, log2_c: int, evals, width: int, point):
+def basefold_prove_w(lam: int, log2_c: int, evals, width: int, point, pre: bytes = b""):
     """Width-w form: evals is the row-major 2^n x width matrix (one polynomial
-    per column). Returns (root, claims (width elements), proof bytes)."""
+    per column). `pre`: bytes the transcript observed before the harness
+    prefix. Returns (root, claims (width elements), proof bytes)."""
     e, p = as_elems(evals), as_elems(point)
     n = len(p)
     assert len(e) == (1 << n) * width
@@ -265,8 +266,10 @@ def basefold_prove_w(lam: int, log2_c: int, evals, width: int, point):
     claims = elems(width)
     proof = ctypes.POINTER(ctypes.c_uint8)()
     plen = ctypes.c_size_t()
-    rc = lib().bfo_basefold_prove_w(lam, log2_c, n, ctypes.c_size_t(width), _p(e), _p(p), root, _p(claims),
-                                    ctypes.byref(proof), ctypes.byref(plen))
+    pb = (ctypes.c_uint8 * max(1, len(pre))).from_buffer_copy(bytes(pre) or b"\0")
+    rc = lib().bfo_basefold_prove_px(lam, log2_c, n, ctypes.c_size_t(width), _p(e), _p(p), pb,
+                                     ctypes.c_size_t(len(pre)), root, _p(claims), ctypes.byref(proof),
+                                     ctypes.byref(plen))
     if rc != 0:
         raise RuntimeError(f"oracle prove failed ({rc})")
     data = ctypes.string_at(proof, plen.value)
@@ -274,13 +277,15 @@ def basefold_prove_w(lam: int, log2_c: int, evals, width: int, point):
     return bytes(root), claims, data
 
 
-def basefold_verify_w(lam: int, log2_c: int, width: int, root: bytes, point, claims, proof: bytes) -> int:
+def basefold_verify_w(lam: int, log2_c: int, width: int, root: bytes, point, claims, proof: bytes,
+                      pre: bytes = b"") -> int:
     p, c = as_elems(point), as_elems(claims)
     assert len(c) == width
     rb = (ctypes.c_uint8 * 32).from_buffer_copy(root)
     pb = (ctypes.c_uint8 * max(1, len(proof))).from_buffer_copy(proof or b"\0")
-    return int(lib().bfo_basefold_verify_w(lam, log2_c, len(p), ctypes.c_size_t(width), rb, _p(p), _p(c), pb,
-                                           ctypes.c_size_t(len(proof))))
+    qb = (ctypes.c_uint8 * max(1, len(pre))).from_buffer_copy(bytes(pre) or b"\0")
+    return int(lib().bfo_basefold_verify_px(lam, log2_c, len(p), ctypes.c_size_t(width), rb, _p(p), _p(c), qb,
+                                            ctypes.c_size_t(len(pre)), pb, ctypes.c_size_t(len(proof))))
 
 
 def product_sumcheck(f, g, r):

@@ -804,6 +804,13 @@ int bfo_basefold_prove(int lambda, int log2_c, int n, const uint8_t *evals_b,
 int bfo_basefold_prove_w(int lambda, int log2_c, int n, size_t width, const uint8_t *evals_b,
                          const uint8_t *point_b, uint8_t root_out[32], uint8_t *claims_out,
                          uint8_t **proof_out, size_t *proof_len) {
+    return bfo_basefold_prove_px(lambda, log2_c, n, width, evals_b, point_b, NULL, 0, root_out, claims_out,
+                                 proof_out, proof_len);
+}
+
+int bfo_basefold_prove_px(int lambda, int log2_c, int n, size_t width, const uint8_t *evals_b,
+                          const uint8_t *point_b, const uint8_t *pre, size_t pre_len, uint8_t root_out[32],
+                          uint8_t *claims_out, uint8_t **proof_out, size_t *proof_len) {
     if (n < 0 || n > 40 || log2_c < 0 || log2_c > 8 || width < 1) return -1;
     const size_t w = width;
     prover P;
@@ -828,6 +835,7 @@ int bfo_basefold_prove_w(int lambda, int log2_c, int n, size_t width, const uint
     /* harness transcript prefix: observe(root), observe_ext_slice(values) (poly-commit/src/lib.rs:82-92);
      * values = one MLE evaluation per column (basefold.rs:126-131) */
     bfo_challenger *ch = bfo_challenger_new();
+    if (pre_len) bfo_challenger_observe(ch, pre, pre_len); /* whatever the caller observed first */
     bfo_challenger_observe(ch, merkle_root_ptr(&P.main), 32);
     u128 *claims = malloc(sizeof(u128) * w), *col = malloc(sizeof(u128) * N);
     for (size_t t = 0; t < w; t++) {
@@ -959,6 +967,12 @@ int bfo_basefold_verify(int lambda, int log2_c, int n, const uint8_t root[32],
 int bfo_basefold_verify_w(int lambda, int log2_c, int n, size_t width, const uint8_t root[32],
                           const uint8_t *point_b, const uint8_t *claims_b,
                           const uint8_t *proof, size_t proof_len) {
+    return bfo_basefold_verify_px(lambda, log2_c, n, width, root, point_b, claims_b, NULL, 0, proof, proof_len);
+}
+
+int bfo_basefold_verify_px(int lambda, int log2_c, int n, size_t width, const uint8_t root[32],
+                           const uint8_t *point_b, const uint8_t *claims_b, const uint8_t *pre, size_t pre_len,
+                           const uint8_t *proof, size_t proof_len) {
     if (width < 1 || width > ((size_t)1 << 24)) return 4;
     const size_t w = width;
     int d = n, b = log2_c, L = b + d;
@@ -979,6 +993,7 @@ int bfo_basefold_verify_w(int lambda, int log2_c, int n, size_t width, const uin
     if (rd.err) return 4;
 
     bfo_challenger *ch = bfo_challenger_new();
+    if (pre_len) bfo_challenger_observe(ch, pre, pre_len);
     bfo_challenger_observe(ch, root, 32);
     for (size_t t = 0; t < w; t++) ch_observe_elem(ch, ld(claims_b + 16 * t));
     /* scalars and claim (basefold.rs:252-258) */

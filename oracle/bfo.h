@@ -94,6 +94,16 @@ int bfo_basefold_prove_w(int lambda, int log2_c, int n, size_t width, const uint
 int bfo_basefold_verify_w(int lambda, int log2_c, int n, size_t width, const uint8_t root[32],
                           const uint8_t *point, const uint8_t *claims,
                           const uint8_t *proof, size_t proof_len);
+/*
+ * The same with a transcript that has already observed pre_len bytes `pre`
+ * before the harness prefix (a caller's own earlier observations).
+ */
+int bfo_basefold_prove_px(int lambda, int log2_c, int n, size_t width, const uint8_t *evals,
+                          const uint8_t *point, const uint8_t *pre, size_t pre_len, uint8_t root[32],
+                          uint8_t *claims, uint8_t **proof, size_t *proof_len);
+int bfo_basefold_verify_px(int lambda, int log2_c, int n, size_t width, const uint8_t root[32],
+                           const uint8_t *point, const uint8_t *claims, const uint8_t *pre, size_t pre_len,
+                           const uint8_t *proof, size_t proof_len);
 void bfo_free(void *p);
 
 /*

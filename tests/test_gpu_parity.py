@@ -387,6 +387,40 @@ def test_batch_equals_plain_and_oracle(bfd, oracle, ctx):
             assert proof_p == proofs[i].tobytes()
 
 
+# bytes a caller's transcript observed before the harness prefix: the first
+# round then hashes len + 48 + 64 bytes, which ends at an odd length, one
+# byte short of the 136-byte rate, on it, one past it, and two or three
+# blocks in
+PREFIX_LENS = [5, 23, 24, 25, 160, 300]
+
+
+def test_transcript_prefix_lengths(bfd, oracle, ctx):
+    """The device transcript starts from any challenger state: plain and
+    batched openings after a caller prefix equal the oracle's, and both
+    leave the challenger in the same state."""
+    n, log2_c = 6, 1
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, n)), ctx)
+    for plen in PREFIX_LENS:
+        _say(f"prefix {plen} bytes")
+        pre = bytes((7 * t + plen) & 0xFF for t in range(plen))
+        evals, point = bfd.random_vec(50 + plen, 1 << n), bfd.random_vec(60 + plen, n)
+        root_o, claims_o, proof_o = oracle.basefold_prove_w(128, log2_c, evals, 1, point, pre=pre)
+        assert oracle.basefold_verify_w(128, log2_c, 1, root_o, point, claims_o, proof_o, pre=pre) == 0
+        cd = pcs.commit(evals)
+        claim = pcs.eval(cd, point)
+        assert cd.comm == root_o and (claim == claims_o).all()
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(pre)
+        ch.observe(cd.comm)
+        ch.observe_ext_slice(claim)
+        assert pcs.open(cd, [bfd.PolyEvals(claim, point)], ch) == proof_o, plen
+        ch2 = bfd.GenericChallenger.keccak256()
+        ch2.observe(pre)
+        _, proofs = bfd.commit_open_batch(128, log2_c, [evals], [point], [claim], [ch2], True, ctx=ctx)
+        assert proofs[0].tobytes() == proof_o, plen
+        assert ch.sample_bits(64) == ch2.sample_bits(64), plen
+
+
 def test_batch_rejects_shared_challenger(bfd, ctx):
     """Transcripts advance concurrently, so two polynomials may not share one challenger."""
     sizes = [4, 5]

@@ -172,6 +172,25 @@ def test_basefold_prove_verify_round_trip(oracle, log2_c):
             assert O.basefold_verify(128, log2_c, root, p, claim, bytes(bad)) == 3
 
 
+def test_transcript_prefix_binds_proof(oracle):
+    """Bytes observed before the harness prefix change every challenge: the
+    proof differs from the plain one, verifies under the same prefix, and
+    fails under none or another one (the transcript is bound to its whole
+    history)."""
+    O = oracle
+    n, log2_c = 5, 1
+    e, p = rnd(O, 1 << n, 90), rnd(O, n, 91)
+    root0, claims0, proof0 = O.basefold_prove_w(128, log2_c, e, 1, p)
+    for plen in (1, 23, 137, 300):
+        pre = bytes((3 * t + 1) & 0xFF for t in range(plen))
+        root, claims, proof = O.basefold_prove_w(128, log2_c, e, 1, p, pre=pre)
+        assert root == root0 and (claims == claims0).all()
+        assert len(proof) == len(proof0) and proof != proof0
+        assert O.basefold_verify_w(128, log2_c, 1, root, p, claims, proof, pre=pre) == 0
+        assert O.basefold_verify_w(128, log2_c, 1, root, p, claims, proof) != 0
+        assert O.basefold_verify_w(128, log2_c, 1, root, p, claims, proof, pre=pre[:-1] + b"\xff") != 0
+
+
 def test_proof_size_formula(oracle):
     """bincode size of BasefoldProof at 2^n, rate 1/2 matches the wire-format arithmetic."""
     O = oracle
