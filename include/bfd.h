@@ -180,6 +180,30 @@ int bfd_sumcheck_quadratic(bfd_ctx *ctx, uint32_t n, uint32_t m, const uint8_t *
                            uint8_t claim[16], uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals,
                            uint8_t subclaim[16]);
 
+/* ---- QuadraticProver, step by step (sumcheck/src/function/quadratic.rs:
+ * 31-136): the SumcheckFunctionProver a caller's own prove_sumcheck driver
+ * (sumcheck/src/lib.rs:56-103) calls once per round, with the tables
+ * resident in device memory.  Same tables, pairs and scalars as
+ * bfd_sumcheck_quadratic; polys is on the host or, with polys_on_device, in
+ * device memory (copied either way).
+ *   compute_sum        -> bfd_sc_compute_sum        (quadratic.rs:86-99)
+ *   compute_round_poly -> bfd_sc_round_poly: [c0, c1, c2] with c1 from the
+ *                         subclaim as decompress_round_poly does (:101-124)
+ *   fix_last_var       -> bfd_sc_fix_last_var       (:126-130)
+ *   evaluations        -> bfd_sc_evaluations: the m values once every
+ *                         variable is fixed, else BFD_ERR_STATE (None) (:132-135)
+ * compress_round_poly ([c0, c2]) and decompress_round_poly stay with the
+ * caller; they touch no table. ---- */
+typedef struct bfd_sc bfd_sc;
+int bfd_sc_quadratic_create(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys, int polys_on_device,
+                            uint32_t npairs, const uint8_t *scalars, const uint32_t *pair_idx, bfd_sc **out);
+int bfd_sc_num_vars(const bfd_sc *h, uint32_t *num_vars);  /* variables left */
+int bfd_sc_compute_sum(bfd_sc *h, uint8_t out[16]);
+int bfd_sc_round_poly(bfd_sc *h, const uint8_t subclaim[16], uint8_t coeffs[48]);
+int bfd_sc_fix_last_var(bfd_sc *h, const uint8_t x[16]);
+int bfd_sc_evaluations(bfd_sc *h, uint8_t *out);
+int bfd_sc_destroy(bfd_sc *h);
+
 /* ---- Basefold::verify (basefold.rs:234-325) of `count` proofs made with
  * `code`, each over a commitment of `width` columns: roots (count x 32
  * bytes), points[p] (d elements), values[p] (width elements), proofs[p] of

@@ -565,6 +565,100 @@ def prove_sumcheck_quadratic(f: Quadratic, challenger: GenericChallenger, ctx: O
     return claim, polys[:2 * n].reshape(n, 2, 2), chal[:n], fin, sub
 
 
+class QuadraticProver:
+    """QuadraticProver (sumcheck/src/function/quadratic.rs:31-136) with its
+    tables resident in device memory (bfd_sc_*): the SumcheckFunctionProver a
+    prove_sumcheck driver calls once per round.  compress / decompress touch no
+    table and run here."""
+
+    def __init__(self, f: Quadratic, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        P = np.ascontiguousarray(f.polys, dtype=np.uint64)
+        m, size = P.shape[0], P.shape[1]
+        n = size.bit_length() - 1
+        if size != 1 << n:
+            raise BfdError(BFD_ERR_ARG, "tables must hold 2^n evaluations")
+        sc = np.ascontiguousarray(np.array([np.asarray(p[0], dtype=np.uint64).reshape(2) for p in f.pairs]))
+        idx = np.ascontiguousarray(np.array([[p[1], p[2]] for p in f.pairs], dtype=np.uint32).reshape(-1))
+        self.m = m
+        h = ctypes.c_void_p()
+        _check(lib().bfd_sc_quadratic_create(self.ctx.h, n, m, _p(P.reshape(-1, 2)), 0, len(sc), _p(sc),
+                                             idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bfd_sc_destroy(self.h)
+        self.h = None
+
+    @property
+    def num_vars(self) -> int:
+        v = ctypes.c_uint32()
+        _check(lib().bfd_sc_num_vars(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def compute_sum(self, round: int = 0) -> np.ndarray:
+        out = np.zeros((1, 2), dtype=np.uint64)
+        _check(lib().bfd_sc_compute_sum(self.h, _p(out)))
+        return out
+
+    def compute_round_poly(self, round: int, subclaim) -> np.ndarray:
+        out = np.zeros((3, 2), dtype=np.uint64)
+        _check(lib().bfd_sc_round_poly(self.h, _p(_elems(subclaim)), _p(out)))
+        return out
+
+    @staticmethod
+    def compress_round_poly(round_poly) -> np.ndarray:
+        rp = _elems(round_poly)
+        return np.ascontiguousarray(rp[[0, 2]])
+
+    @staticmethod
+    def decompress_round_poly(subclaim, compressed) -> np.ndarray:
+        c = _elems(compressed)
+        return np.ascontiguousarray(np.stack([c[0], _elems(subclaim)[0] ^ c[1], c[1]]))
+
+    def fix_last_var(self, x) -> None:
+        _check(lib().bfd_sc_fix_last_var(self.h, _p(_elems(x))))
+
+    def evaluations(self) -> Optional[np.ndarray]:
+        if self.num_vars != 0:
+            return None
+        out = np.zeros((self.m, 2), dtype=np.uint64)
+        _check(lib().bfd_sc_evaluations(self.h, _p(out)))
+        return out
+
+
+def _horner(coeffs: np.ndarray, x: np.ndarray) -> np.ndarray:
+    acc = np.zeros((1, 2), dtype=np.uint64)
+    for c in coeffs[::-1]:
+        acc = _mul_host(acc, x) ^ c.reshape(1, 2)
+    return acc
+
+
+def prove_sumcheck(prover, claim, challenger: GenericChallenger):
+    """prove_sumcheck (sumcheck/src/lib.rs:56-103) over any prover with the
+    SumcheckFunctionProver methods: observe the claim, then per round observe
+    the compressed round poly, sample r_i, fix the last variable and reduce
+    the subclaim (horner, lib.rs:39-45).  Returns (subclaim, compressed round
+    polys, challenges in round order)."""
+    subclaim = np.array(_elems(claim), dtype=np.uint64).reshape(1, 2)
+    n = prover.num_vars
+    if n == 0:
+        return subclaim, [], []
+    challenger.observe_ext_slice(subclaim)
+    polys, chal = [], []
+    for rnd in reversed(range(n)):
+        rp = prover.compute_round_poly(rnd, subclaim)
+        comp = prover.compress_round_poly(rp)
+        challenger.observe_ext_slice(comp)
+        x = challenger.sample_ext_element()
+        prover.fix_last_var(x)
+        subclaim = _horner(rp, x)
+        polys.append(comp)
+        chal.append(x)
+    return subclaim, polys, chal
+
+
 def batch_quadratics(fs: Sequence[Quadratic], alpha) -> Quadratic:
     """Batch::new(fs, claims, alpha) (sumcheck/src/function/batch.rs:15-34) for
     Quadratics of equal num_vars, the case that needs neither E::TWO powers nor

@@ -147,6 +147,31 @@ def test_quadratic_sumcheck_bit_exact(bfd, oracle, ctx):
             assert (np.asarray(g).reshape(-1, 2) == np.asarray(w).reshape(-1, 2)).all(), n
 
 
+def test_quadratic_prover_stepwise(bfd, oracle, ctx):
+    """The step-wise QuadraticProver (bfd_sc_*) under a host prove_sumcheck driver
+    gives the restatement's claim, round polys, challenges, evaluations and
+    subclaim; evaluations() is None until every variable is fixed, and the
+    sum at the final point equals the subclaim."""
+    for n in (0, 1, 2, 9, 13):
+        _say(f"case stepwise quadratic n={n}")
+        f = quad_case(bfd, n, 970 + n)
+        want = oracle.quadratic_sumcheck(f.polys, [p[0] for p in f.pairs], [(p[1], p[2]) for p in f.pairs])
+        prover = bfd.QuadraticProver(f, ctx)
+        assert prover.num_vars == n
+        if n:
+            assert prover.evaluations() is None
+        claim = prover.compute_sum()
+        ch = bfd.GenericChallenger.keccak256()
+        sub, polys, chal = bfd.prove_sumcheck(prover, claim, ch)
+        got = (claim, np.array(polys).reshape(-1, 2), np.array(chal).reshape(-1, 2), prover.evaluations(), sub)
+        for g, w in zip(got, want):
+            assert (np.asarray(g).reshape(-1, 2) == np.asarray(w).reshape(-1, 2)).all(), n
+        assert (prover.compute_sum() == sub).all(), n
+        ch2 = bfd.GenericChallenger.keccak256()
+        bfd.prove_sumcheck_quadratic(f, ch2, ctx)
+        assert ch.sample_bits(64) == ch2.sample_bits(64), n
+
+
 def test_batch_quadratic_bit_exact(bfd, oracle, ctx):
     """Batch of three equal-size Quadratics (batch.rs) as one merged call, against the restatement."""
     n = 9
