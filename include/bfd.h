@@ -180,6 +180,23 @@ int bfd_sumcheck_quadratic(bfd_ctx *ctx, uint32_t n, uint32_t m, const uint8_t *
                            uint8_t claim[16], uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals,
                            uint8_t subclaim[16]);
 
+/* ---- RandomFoldableCode<Polyval> for the binary RS code (basefold/code.rs:
+ * 16-84, code/binary_reed_solomon.rs:38-111) on a code's device tables.
+ * bfd_code_encode = encode_batch: the 2^d x width row-major message (k_d rows)
+ * to the 2^(log2_c + d) x width codeword in the reference's row order
+ * (bit-reversed coset NTTs, :67-82); host or device buffers, which may not
+ * overlap.  bfd_code_fold = fold(i, w, r): w of 2^(log2_c + i + 1) elements
+ * to its first 2^(log2_c + i) (on the device, out may equal w; on the host,
+ * out receives the folded half).  bfd_code_interpolate(i, j, lo, hi, r) is
+ * interpolate (:103-109) at position j of level i, on the host.  encode0 is
+ * c copies of the message (:61-65) and needs no call. ---- */
+int bfd_code_encode(bfd_ctx *ctx, const bfd_code *code, const void *m, int m_on_device, size_t height, size_t width,
+                    void *out, int out_on_device);
+int bfd_code_fold(bfd_ctx *ctx, const bfd_code *code, uint32_t i, const void *w, int on_device, const uint8_t r[16],
+                  void *out);
+int bfd_code_interpolate(const bfd_code *code, uint32_t i, uint64_t j, const uint8_t lo[16], const uint8_t hi[16],
+                         const uint8_t r[16], uint8_t out[16]);
+
 /* ---- QuadraticProver, step by step (sumcheck/src/function/quadratic.rs:
  * 31-136): the SumcheckFunctionProver a caller's own prove_sumcheck driver
  * (sumcheck/src/lib.rs:56-103) calls once per round, with the tables

@@ -286,6 +286,67 @@ class BasefoldCommitmentData:
         self.handle = None
 
 
+class DeviceCode:
+    """RandomFoldableCode<Polyval> of BinaryReedSolomonCode
+    (code/binary_reed_solomon.rs:38-111) on the code's device tables:
+    encode_batch / encode / encode0 / fold / interpolate / num_queries."""
+
+    def __init__(self, owner: "Basefold"):
+        self._owner = owner  # keeps the code handle alive
+        c = owner.config.code
+        self.lambda_, self.log2_c, self.d = c.lambda_, c.log2_c, c.d
+
+    def c(self) -> int:
+        return 1 << self.log2_c
+
+    def k_d(self) -> int:
+        return 1 << self.d
+
+    def n_d(self) -> int:
+        return 1 << (self.log2_c + self.d)
+
+    def num_queries(self) -> int:
+        return self._owner.num_queries
+
+    def encode0(self, m) -> np.ndarray:
+        m = _elems(m)
+        return np.ascontiguousarray(np.repeat(m[:1], self.c(), axis=0))
+
+    def encode_batch(self, m, width: int = 1) -> np.ndarray:
+        """k_d x width row-major message -> n_d x width codeword (flat (rows*width, 2))."""
+        arr = _elems(m)
+        if len(arr) != self.k_d() * width:
+            raise BfdError(BFD_ERR_ARG, "message must hold k_d x width elements")
+        out = np.zeros((self.n_d() * width, 2), dtype=np.uint64)
+        o = self._owner
+        _check(lib().bfd_code_encode(o.ctx.h, o.code_h, _p(arr), 0, self.k_d(), width, _p(out), 0))
+        return out
+
+    def encode(self, data) -> np.ndarray:
+        return self.encode_batch(data, 1)
+
+    def encode_batch_device(self, m_ptr: int, out_ptr: int, width: int = 1) -> None:
+        o = self._owner
+        _check(lib().bfd_code_encode(o.ctx.h, o.code_h, ctypes.c_void_p(m_ptr), 1, self.k_d(), width,
+                                     ctypes.c_void_p(out_ptr), 1))
+
+    def fold(self, i: int, w, r) -> np.ndarray:
+        """fold(i, w, r): the first half of w after the inverse butterflies and interpolation."""
+        w = _elems(w)
+        if len(w) != 1 << (self.log2_c + i + 1):
+            raise BfdError(BFD_ERR_ARG, "w must hold 2^(log2_c + i + 1) elements")
+        out = np.zeros((len(w) // 2, 2), dtype=np.uint64)
+        o = self._owner
+        _check(lib().bfd_code_fold(o.ctx.h, o.code_h, i, _p(w), 0, _p(_elems(r)), _p(out)))
+        return out
+
+    def interpolate(self, i: int, j: int, lo, hi, r) -> np.ndarray:
+        out = np.zeros((1, 2), dtype=np.uint64)
+        _check(lib().bfd_code_interpolate(self._owner.code_h, i, ctypes.c_uint64(j), _p(_elems(lo)),
+                                          _p(_elems(hi)), _p(_elems(r)), _p(out)))
+        return out
+
+
 class Basefold:
     """PolyCommitScheme<Polyval, Polyval> for Basefold over the binary RS code."""
 
@@ -309,6 +370,11 @@ class Basefold:
     @property
     def num_queries(self) -> int:
         return int(lib().bfd_code_num_queries(self.code_h))
+
+    @property
+    def code(self) -> "DeviceCode":
+        """The RandomFoldableCode (basefold/code.rs:16-84) behind this scheme."""
+        return DeviceCode(self)
 
     def commit(self, data, width: Optional[int] = None) -> BasefoldCommitmentData:
         """RowMajorMatrix commit (basefold.rs:112-120).  ``data`` is the

@@ -199,6 +199,8 @@ bfd_code *create_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d
                   "xor_span");
     }
     hip_check(hipStreamSynchronize(st), "code setup");
+    code->bt = std::move(bt);
+    code->bt_off = std::move(bt_off);
     return code.release();
 }
 
@@ -588,6 +590,86 @@ int bfd_sumcheck_quadratic(bfd_ctx *ctx, uint32_t n, uint32_t m, const uint8_t *
         hip_check(hipStreamSynchronize(st), "final sync");
         ctx->timer.collect();
         store_elem(subclaim_out, sub);
+    });
+}
+
+int bfd_code_encode(bfd_ctx *ctx, const bfd_code *code, const void *m, int m_on_device, size_t height, size_t width,
+                    void *out, int out_on_device) {
+    return guarded([&] {
+        if (!ctx || !code || !m || !out) fail(BFD_ERR_ARG, "null argument");
+        if (width < 1 || width > kMaxWidth) fail(BFD_ERR_ARG, "width must be 1..65535");
+        if (height != ((size_t)1 << code->d)) fail(BFD_ERR_ARG, "height must equal k_d = 2^d of the code");
+        const int b = (int)code->log2_c, n = (int)code->d;
+        const size_t N = height, M = height << b, w = width;
+        if (m_on_device && out_on_device) {
+            const uint8_t *a0 = (const uint8_t *)m, *o0 = (const uint8_t *)out;
+            if (a0 < o0 + 16 * M * w && o0 < a0 + 16 * N * w) fail(BFD_ERR_ARG, "input and codeword overlap");
+        }
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        DevBuf din, dout;
+        const gf *data = (const gf *)m;
+        gf *cw = (gf *)out;
+        if (!m_on_device) {
+            din = ctx_buf(ctx, 16 * N * w);
+            hip_check(hipMemcpyAsync(din.p, m, 16 * N * w, hipMemcpyHostToDevice, st), "copy message");
+            data = din.as<gf>();
+        }
+        if (!out_on_device) {
+            dout = ctx_buf(ctx, 16 * M * w);
+            cw = dout.as<gf>();
+        }
+        if (w == 1) encode(ctx, code, data, cw, b, n);
+        else encode_w(ctx, code, data, cw, b, n, w);
+        if (!out_on_device) hip_check(hipMemcpyAsync(out, cw, 16 * M * w, hipMemcpyDeviceToHost, st), "copy codeword");
+        hip_check(hipStreamSynchronize(st), "encode");
+        ctx->timer.collect();
+    });
+}
+
+int bfd_code_fold(bfd_ctx *ctx, const bfd_code *code, uint32_t i, const void *w, int on_device, const uint8_t r[16],
+                  void *out) {
+    return guarded([&] {
+        if (!ctx || !code || !w || !r || !out) fail(BFD_ERR_ARG, "null argument");
+        if (i >= code->d) fail(BFD_ERR_ARG, "fold level must be below d");
+        const size_t mid = (size_t)1 << (code->log2_c + i);
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        DevBuf dw;
+        const gf *src = (const gf *)w;
+        gf *dst = (gf *)out;
+        if (!on_device) {
+            dw = ctx_buf(ctx, 16 * 2 * mid);
+            hip_check(hipMemcpyAsync(dw.p, w, 32 * mid, hipMemcpyHostToDevice, st), "copy word");
+            src = dw.as<gf>();
+            dst = dw.as<gf>();  // in place, as the reference truncates w
+        }
+        const gf *ft = code->ft.as<gf>() + code->ft_off[i];
+        const gf rv = load_elem(r);
+        ctx->timer.run(st, "k_fold", [&] { return launch::fold(st, src, dst, ft, mid, rv); });
+        if (!on_device) hip_check(hipMemcpyAsync(out, dst, 16 * mid, hipMemcpyDeviceToHost, st), "copy fold");
+        hip_check(hipStreamSynchronize(st), "fold");
+        ctx->timer.collect();
+    });
+}
+
+int bfd_code_interpolate(const bfd_code *code, uint32_t i, uint64_t j, const uint8_t lo[16], const uint8_t hi[16],
+                         const uint8_t r[16], uint8_t out[16]) {
+    return guarded([&] {
+        if (!code || !lo || !hi || !r || !out) fail(BFD_ERR_ARG, "null argument");
+        if (i >= code->d) fail(BFD_ERR_ARG, "fold level must be below d");
+        const int K = (int)(code->log2_c + i);
+        if (j >> K) fail(BFD_ERR_ARG, "position beyond the level");
+        gf t = gf_zero();
+        for (int p = 0; p < K; p++)
+            if ((j >> p) & 1) t = gf_add(t, code->bt[code->bt_off[i] + p]);
+        // binary_reed_solomon.rs:103-109: hi += lo; lo += hi t; lo + (hi - lo) r
+        gf l = load_elem(lo), h = load_elem(hi);
+        h = gf_add(h, l);
+        l = gf_add(l, hmul(h, t));
+        store_elem(out, gf_add(l, hmul(gf_add(h, l), load_elem(r))));
     });
 }
 

@@ -277,6 +277,10 @@ struct bfd_code {
     uint64_t q = 0;
     bfd::rt::DevBuf ft;                     // fold-order twiddles, level s at ft_off[s]
     std::vector<size_t> ft_off;
+    // host copy of the XOR basis the table is built from: ft[s][j] = XOR over
+    // the set bits p of j of bt[bt_off[s] + p]
+    std::vector<gf> bt;
+    std::vector<size_t> bt_off;
 };
 
 struct bfd_commitment {

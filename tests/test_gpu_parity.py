@@ -313,6 +313,49 @@ def test_encode_matches_oracle_codeword_root(bfd, oracle, ctx):
         assert cd.comm == oracle.merkle_root(cw.tobytes(), len(cw), 16)
 
 
+def test_code_encode_fold_interpolate(bfd, oracle, ctx):
+    """RandomFoldableCode on the device (bfd_code_encode / fold / interpolate):
+    encode_batch at widths 1 and 5 (host and device buffers), fold at every
+    level and interpolate at sampled positions equal the restatement's; and,
+    independent of it, folding the codeword of m through levels d-1..0 while
+    fixing m's last variable with the same challenges ends at c copies of the
+    fixed value (encode0(final_poly) == pi, basefold.rs:198)."""
+    import torch
+    for log2_c, d in [(0, 4), (1, 6), (2, 9), (1, 14)]:
+        _say(f"code log2_c={log2_c} d={d}")
+        pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, d)), ctx)
+        code = pcs.code
+        m = bfd.random_vec(1300 + d, 1 << d)
+        cw = code.encode(m)
+        assert (cw == oracle.rs_encode(log2_c, d, m)).all()
+        assert (code.encode0(m[:1]) == np.repeat(m[:1], 1 << log2_c, axis=0)).all()
+        mw = bfd.random_vec(1400 + d, 5 << d)
+        cw5 = code.encode_batch(mw, 5)
+        assert (cw5 == oracle.rs_encode(log2_c, d, mw, 5)).all()
+        dm = torch.from_numpy(mw.view(np.int64).reshape(-1)).to("cuda:0")
+        dout = torch.zeros(2 * len(cw5), dtype=torch.int64, device="cuda:0")
+        torch.cuda.synchronize()
+        code.encode_batch_device(dm.data_ptr(), dout.data_ptr(), 5)
+        assert (dout.cpu().numpy().view(np.uint64).reshape(-1, 2) == cw5).all()
+        for i in range(d):
+            w = bfd.random_vec(1500 + 31 * d + i, 2 << (log2_c + i))
+            r = bfd.random_vec(1600 + 31 * d + i, 1)
+            f = code.fold(i, w, r)
+            assert (f == oracle.rs_fold(log2_c, d, i, w, r)).all(), (d, i)
+            mid = len(w) // 2
+            for j in sorted({0, 1 % mid, mid - 1, (7 * i + 3) % mid}):
+                assert (code.interpolate(i, j, w[j], w[j + mid], r) == f[j]).all(), (d, i, j)
+        if d <= 9:
+            pi, poly = cw, m
+            for i in reversed(range(d)):
+                r = bfd.random_vec(1700 + 31 * d + i, 1)
+                pi = code.fold(i, pi, r)
+                half = len(poly) // 2
+                poly = np.concatenate([poly[j:j + 1] ^ bfd._mul_host(r, poly[j + half:j + half + 1] ^ poly[j:j + 1])
+                                       for j in range(half)])
+            assert (pi == code.encode0(poly)).all(), d
+
+
 def test_reopen_same_commitment(bfd, oracle, ctx):
     """open borrows the commitment immutably: two openings at different points."""
     n, log2_c = 8, 1
