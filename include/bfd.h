@@ -67,6 +67,16 @@ int bfd_challenger_destroy(bfd_challenger *c);
 int bfd_challenger_observe(bfd_challenger *c, const uint8_t *bytes, size_t len); /* CanObserve, :51-60 */
 int bfd_challenger_sample(bfd_challenger *c, uint8_t out[16]);                     /* CanSample, :62-74 */
 int bfd_challenger_sample_bits(bfd_challenger *c, uint32_t bits, uint64_t *out);   /* :76-83 */
+/* The challenger's state as the two byte buffers of plonky3's
+ * HashChallenger<u8, _, 32> (input_buffer, output_buffer; the bytes still to
+ * be sampled are output[0, output_len), taken from the end), so a transcript
+ * can cross the boundary either way.  export writes *input_len bytes when
+ * input_cap suffices (else BFD_ERR_ARG with *input_len set); import makes a
+ * new challenger in that state. */
+int bfd_challenger_export(const bfd_challenger *c, uint8_t *input, size_t input_cap, size_t *input_len,
+                          uint8_t output[32], size_t *output_len);
+int bfd_challenger_import(const uint8_t *input, size_t input_len, const uint8_t *output, size_t output_len,
+                          bfd_challenger **out);
 
 /* ---- PolyCommitScheme ---- */
 /* Basefold::commit (basefold.rs:112-120): evals is the height x width

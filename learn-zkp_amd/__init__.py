@@ -23,7 +23,7 @@ import ctypes
 import pathlib
 import subprocess
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -252,6 +252,25 @@ class GenericChallenger:
         v = ctypes.c_uint64()
         _check(lib().bfd_challenger_sample_bits(self.h, bits, ctypes.byref(v)))
         return int(v.value)
+
+    def export_state(self) -> Tuple[bytes, bytes]:
+        """(input buffer, output buffer) of the underlying HashChallenger."""
+        n_in, n_out = ctypes.c_size_t(), ctypes.c_size_t()
+        out = (ctypes.c_uint8 * 32)()
+        rc = lib().bfd_challenger_export(self.h, None, 0, ctypes.byref(n_in), out, ctypes.byref(n_out))
+        if rc != 0 and n_in.value == 0:
+            _check(rc)
+        buf = (ctypes.c_uint8 * max(1, n_in.value))()
+        _check(lib().bfd_challenger_export(self.h, buf, n_in.value, ctypes.byref(n_in), out, ctypes.byref(n_out)))
+        return bytes(buf)[:n_in.value], bytes(out)[:n_out.value]
+
+    @classmethod
+    def import_state(cls, input_buf: bytes, output_buf: bytes) -> "GenericChallenger":
+        h = ctypes.c_void_p()
+        ib = (ctypes.c_uint8 * max(1, len(input_buf))).from_buffer_copy(bytes(input_buf) or b"\0")
+        ob = (ctypes.c_uint8 * max(1, len(output_buf))).from_buffer_copy(bytes(output_buf) or b"\0")
+        _check(lib().bfd_challenger_import(ib, len(input_buf), ob, len(output_buf), ctypes.byref(h)))
+        return cls(h)
 
 
 @dataclass

@@ -328,6 +328,33 @@ int bfd_challenger_sample_bits(bfd_challenger *c, uint32_t bits, uint64_t *out) 
     });
 }
 
+int bfd_challenger_export(const bfd_challenger *c, uint8_t *input, size_t input_cap, size_t *input_len,
+                          uint8_t output[32], size_t *output_len) {
+    return guarded([&] {
+        if (!c || !input_len || !output_len || (input_cap && !input)) fail(BFD_ERR_ARG, "null argument");
+        *input_len = c->c.input.size();
+        *output_len = c->c.output.size();
+        if (input_cap < c->c.input.size()) fail(BFD_ERR_ARG, "input buffer too small");
+        if (c->c.output.size() > 32) fail(BFD_ERR_STATE, "output buffer holds more than one digest");
+        if (!c->c.input.empty()) memcpy(input, c->c.input.data(), c->c.input.size());
+        if (!c->c.output.empty()) {
+            if (!output) fail(BFD_ERR_ARG, "null argument");
+            memcpy(output, c->c.output.data(), c->c.output.size());
+        }
+    });
+}
+int bfd_challenger_import(const uint8_t *input, size_t input_len, const uint8_t *output, size_t output_len,
+                          bfd_challenger **out) {
+    return guarded([&] {
+        if (!out || (input_len && !input) || (output_len && !output)) fail(BFD_ERR_ARG, "null argument");
+        if (output_len > 32) fail(BFD_ERR_ARG, "the output buffer holds at most one 32-byte digest");
+        auto c = std::make_unique<bfd_challenger>();
+        c->c.input.assign(input, input + input_len);
+        c->c.output.assign(output, output + output_len);
+        *out = c.release();
+    });
+}
+
 int bfd_commit(bfd_ctx *ctx, const bfd_code *code, const uint8_t *evals, size_t height, size_t width,
                bfd_commitment **out, uint8_t root[32]) {
     return guarded([&] { do_commit(ctx, code, evals, false, height, width, out, root); });

@@ -4,6 +4,7 @@ challenger) agrees with the CPU restatement.  No GPU compute here."""
 import ctypes
 
 import numpy as np
+import pytest
 
 
 def test_library_exports_header_symbols(bfd):
@@ -37,6 +38,40 @@ def test_challenger_matches_oracle(bfd, oracle):
             c = a.clone()
             assert (c.sample_ext_element() == a.sample_ext_element()).all()
             b.sample()
+
+
+def test_challenger_state_export_import(bfd, oracle):
+    """The two byte buffers round-trip: a challenger rebuilt from another's
+    exported state continues exactly as the original and as the oracle's; the
+    input buffer is the observed bytes since the last digest, the output
+    buffer at most one digest, consumed from the end."""
+    rng = np.random.default_rng(11)
+    a, b = bfd.GenericChallenger.keccak256(), oracle.Challenger()
+    pending = b""
+    for step in range(200):
+        op = rng.integers(0, 3)
+        if op == 0:
+            msg = bytes(rng.integers(0, 256, rng.integers(0, 300), dtype=np.uint8))
+            a.observe(msg)
+            b.observe(msg)
+            pending += msg
+            inp, out = a.export_state()
+            assert out == b"" and inp.endswith(pending) and len(inp) in (len(pending), len(pending) + 32)
+        elif op == 1:
+            assert (a.sample_ext_element() == b.sample()).all()
+            inp, out = a.export_state()
+            assert len(inp) == 32 and len(out) <= 32 and out == inp[:len(out)]
+            pending = b""
+        else:
+            inp, out = a.export_state()
+            a = bfd.GenericChallenger.import_state(inp, out)
+            assert a.export_state() == (inp, out)
+        bits = int(rng.integers(1, 60))
+        c = bfd.GenericChallenger.import_state(*a.export_state())
+        assert c.sample_bits(bits) == a.clone().sample_bits(bits)
+    assert a.sample_bits(64) == b.sample_bits(64)
+    with pytest.raises(bfd.BfdError):
+        bfd.GenericChallenger.import_state(b"", bytes(33))
 
 
 def test_errors_without_handles(bfd):
