@@ -7,7 +7,7 @@
 //
 // CDNA4 has no carry-less multiply.  The device multiply used by every kernel
 // is gf_mul_reg / gf_mul_pre (register Karatsuba over 32x32 carry-less
-// products built from masked integer multiply-adds, below); about 520 VALU
+// products built from masked integer multiply-adds, below); about 466 VALU
 // instructions per product.  The 256-bit product is reduced with the two-step
 // Montgomery fold of the reference backends (x86_64_avx2.rs:41-68).
 //
@@ -152,20 +152,35 @@ struct u64p {
     uint32_t lo, hi;
 };
 
+// Class c collects the products x.p[i] * y.p[j] with i + j = c (mod 4).  Two
+// products of one class whose i + j differ by 4 may be ADDED (one multiply-
+// add) instead of XORed: at position 4u + c the first contributes at most
+// min(u + 1, 15 - u) terms and the second at most min(u, 16 - u), together at
+// most 15, so every count still fits its four-bit field and no carry reaches
+// the next position of the class.  That merges (0,0)+(1,3), (0,1)+(2,3),
+// (1,0)+(3,2) and (0,2)+(3,3) and leaves 10 XORs per product instead of 16.
 BFD_D u64p clmul32p(const pieces &x, const pieces &y) {
-    uint64_t pr[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) pr[i][j] = (uint64_t)x.p[i] * y.p[j];
-    // class c collects products with i + j = c (mod 4)
+    const uint64_t a00 = (uint64_t)x.p[1] * y.p[3] + (uint64_t)x.p[0] * y.p[0];
+    const uint64_t a01 = (uint64_t)x.p[2] * y.p[3] + (uint64_t)x.p[0] * y.p[1];
+    const uint64_t a10 = (uint64_t)x.p[3] * y.p[2] + (uint64_t)x.p[1] * y.p[0];
+    const uint64_t a02 = (uint64_t)x.p[3] * y.p[3] + (uint64_t)x.p[0] * y.p[2];
+    const uint64_t p22 = (uint64_t)x.p[2] * y.p[2], p31 = (uint64_t)x.p[3] * y.p[1];
+    const uint64_t p11 = (uint64_t)x.p[1] * y.p[1], p20 = (uint64_t)x.p[2] * y.p[0];
+    const uint64_t p03 = (uint64_t)x.p[0] * y.p[3], p12 = (uint64_t)x.p[1] * y.p[2];
+    const uint64_t p21 = (uint64_t)x.p[2] * y.p[1], p30 = (uint64_t)x.p[3] * y.p[0];
+#define LO_(v) ((uint32_t)(v))
+#define HI_(v) ((uint32_t)((v) >> 32))
     uint32_t zl[4], zh[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        uint64_t a = pr[0][c], b = pr[1][(c + 3) & 3], d = pr[2][(c + 2) & 3], e = pr[3][(c + 1) & 3];
-        zl[c] = x3(x3((uint32_t)a, (uint32_t)b, (uint32_t)d), (uint32_t)e, 0u);
-        zh[c] = x3(x3((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(d >> 32)), (uint32_t)(e >> 32), 0u);
-    }
+    zl[0] = x3(LO_(a00), LO_(p22), LO_(p31));
+    zh[0] = x3(HI_(a00), HI_(p22), HI_(p31));
+    zl[1] = LO_(a01) ^ LO_(a10);
+    zh[1] = HI_(a01) ^ HI_(a10);
+    zl[2] = x3(LO_(a02), LO_(p11), LO_(p20));
+    zh[2] = x3(HI_(a02), HI_(p11), HI_(p20));
+    zl[3] = x3(x3(LO_(p03), LO_(p12), LO_(p21)), LO_(p30), 0u);
+    zh[3] = x3(x3(HI_(p03), HI_(p12), HI_(p21)), HI_(p30), 0u);
+#undef LO_
+#undef HI_
     const uint32_t m0 = 0x11111111u, m1 = m0 << 1, m2 = m0 << 2;
     // disjoint masks: bit-field inserts pick each class's positions
     uint32_t lo = (zl[0] & m0) | (~m0 & ((zl[1] & m1) | (~m1 & ((zl[2] & m2) | (~m2 & zl[3])))));
