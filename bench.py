@@ -58,11 +58,12 @@ def valu_work(n: int, log2_c: int) -> tuple:
 SUBTREE_FROM = 1 << 17
 
 
-def single_level_nodes(leaves: int) -> int:
+def single_level_nodes(leaves: int, subtree_from: int = SUBTREE_FROM) -> int:
     """Nodes hashed by k_nodes launches in a tree over `leaves` digests: every
-    level whose input layer holds more than SUBTREE_FROM digests."""
+    level whose input layer holds more than `subtree_from` digests (the
+    shape's threshold)."""
     nodes, h = 0, leaves
-    while h > SUBTREE_FROM:
+    while h > subtree_from:
         nodes += h // 2
         h //= 2
     return nodes
@@ -142,11 +143,12 @@ def cpu_model() -> str:
     return "unknown CPU"
 
 
-def batched_vs_plain(bfd, ctx, torch, reps: int) -> dict:
+def batched_vs_plain(bfd, ctx, torch, reps: int, tune_reps: int = 0) -> dict:
     """Config 4: 64 polynomials, log2 sizes uniform in {12..22} (numpy seed 0),
     rate 1/2, evaluations and points seeded per polynomial, resident in HBM.
     Times one batched call against 64 plain commit+open calls; both produce
-    the same 64 proofs."""
+    the same 64 proofs.  With tune_reps > 0 the plain path is first tuned for
+    every size of the mix and the engine for the mix itself."""
     rng = np.random.default_rng(0)
     sizes = [int(v) for v in rng.integers(12, 23, size=64)]
     log2_c = 1
@@ -162,6 +164,14 @@ def batched_vs_plain(bfd, ctx, torch, reps: int) -> dict:
         cd = pcs[n].commit_device(ptr, 1 << n)
         claims.append(pcs[n].eval(cd, pt))
         del cd
+    tuning = None
+    if tune_reps > 0:
+        t_tune = time.perf_counter()
+        for n in sorted(set(sizes)):
+            ctx.tune(pcs[n], reps=tune_reps)
+        chosen, ms_def, ms_tuned = ctx.tune_batch(sizes, log2_c=log2_c, lambda_=LAMBDA, reps=tune_reps)
+        tuning = {"engine_shape": chosen.as_dict(), "batch_ms_default": ms_def, "batch_ms_tuned": ms_tuned,
+                  "tune_s": time.perf_counter() - t_tune}
 
     def plain():
         out = []
@@ -195,7 +205,8 @@ def batched_vs_plain(bfd, ctx, torch, reps: int) -> dict:
     t_p, t_b = statistics.median(tp), statistics.median(tb)
     return {"workload": "64 polys, log2 sizes U{12..22} seed 0, rate 1/2", "total_evals": total,
             "plain_ms": 1e3 * t_p, "batched_ms": 1e3 * t_b, "plain_elems_per_s": total / t_p,
-            "batched_elems_per_s": total / t_b, "speedup": t_p / t_b, "proofs_identical": True}
+            "batched_elems_per_s": total / t_b, "speedup": t_p / t_b, "proofs_identical": True,
+            "tuning": tuning}
 
 
 def reference_shapes(bfd, ctx, torch, reps: int) -> list:
@@ -356,6 +367,9 @@ def main() -> None:
                     help="proofs in the batched-verification figure (0: skip)")
     ap.add_argument("--copy-ref", action="store_true",
                     help="run one k_copy16 over 2^24 elements first (HBM byte-counter calibration)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the built-in launch shapes (default: autotune before timing)")
+    ap.add_argument("--tune-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -381,6 +395,15 @@ def main() -> None:
     claim = pcs.eval(cd0, point)
     root0 = cd0.comm
     del cd0
+    # launch shapes: autotuned for this size before anything is timed (the
+    # tuner's own runs are not part of the timed steps)
+    tuning = None
+    if not args.no_tune:
+        t_tune = time.perf_counter()
+        chosen, ms_def, ms_tuned = ctx.tune(pcs, reps=args.tune_reps)
+        tuning = {"shape": chosen.as_dict(), "default_shape": bfd.Tuning.default().as_dict(),
+                  "commit_open_ms_default": ms_def, "commit_open_ms_tuned": ms_tuned,
+                  "tune_s": time.perf_counter() - t_tune}
 
     def step():
         cd = pcs.commit_device(ptr, N)
@@ -416,9 +439,10 @@ def main() -> None:
     # library's launch plan for one tree).  Trees per step: the main tree over
     # M leaves and the pi tree of every round k over (M >> k) / 2 leaves.
     trees = [M] + [(M >> k) // 2 for k in range(n)]
-    node_launches = sum(max(0, t.bit_length() - 1 - (SUBTREE_FROM.bit_length() - 1)) for t in trees)
+    sub_from = 1 << ctx.tuning(n + LOG2_C).subtree_from_log2
+    node_launches = sum(max(0, t.bit_length() - 1 - (sub_from.bit_length() - 1)) for t in trees)
     per_step_units = {
-        "k_nodes": sum(single_level_nodes(t) for t in trees),
+        "k_nodes": sum(single_level_nodes(t, sub_from) for t in trees),
         "k_leaves_elem": M,
         "k_leaves_pair": sum((M >> k) // 2 for k in range(n)),
     }
@@ -488,11 +512,13 @@ def main() -> None:
         "valu_step": valu_step,
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(ktimes.items())},
         "proof_bytes": len(proof),
+        "tuning": tuning,
     }
     if not args.no_ref_shapes:
         result["reference_shapes"] = reference_shapes(bfd, ctx, torch, 3)
     if not args.no_batch:
-        result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps)
+        result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps,
+                                             0 if args.no_tune else args.tune_reps)
     if args.verify_proofs > 0:
         result["verify"] = verification(bfd, ctx, torch, args.verify_proofs)
     if args.serve_requests > 0:

@@ -258,6 +258,52 @@ int bfd_field_inv(const uint8_t a[16], uint8_t out[16]);
  * (p3/src/field/from_uniform_bytes.rs:5-27) ---- */
 int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out);
 
+/* ---- launch shapes and their autotuner.  No reference counterpart: the
+ * reference runs on the CPU, where there is nothing to shape.  A shape
+ * changes only how the work is cut into launches, never a result: every
+ * shape gives the same commitments and proofs.  The context keeps one shape
+ * per codeword size 2^log2_m (log2_m = d + log2_c, 0..47) for the plain path
+ * and one (BFD_TUNING_BATCH) for the batched and serving engine. ---- */
+#define BFD_TUNING_BATCH 0xfffffffeu /* log2_m selector of the engine's shape */
+#define BFD_TUNING_ALL 0xffffffffu   /* bfd_tuning_set only: every plain-path size */
+typedef struct bfd_tuning {
+    uint32_t ntt_order;         /* NTT pass plan over layers 1..n-1: 0 three layers per pass, the shorter pass
+                                   last; 1 the shorter pass first; 2 two layers per pass */
+    uint32_t ntt_block;         /* threads per block of the NTT passes: 64, 128 or 256 */
+    uint32_t ntt_waves_per_cu;  /* grid cap of the NTT passes, waves per compute unit: 4..1024 */
+    uint32_t ntt_occupancy;     /* build of the three-layer pass: 2 or 3 waves per SIMD */
+    uint32_t hash_block;        /* threads per block of the leaf and node kernels: 64, 128 or 256 */
+    uint32_t hash_waves_per_cu; /* grid cap of the leaf and node kernels, waves per compute unit: 4..1024 */
+    uint32_t subtree_from_log2; /* Merkle levels whose input holds more than 2^this digests get a launch each
+                                   (1..40); the levels above run subtree_levels per launch through LDS.  The
+                                   engine divides the threshold by the number of trees hashed together */
+    uint32_t subtree_levels;    /* 1..9 */
+    uint32_t seg_blocks_per_cu; /* engine only: most blocks one polynomial takes in a segmented launch, per
+                                   compute unit: 1..64 */
+    uint32_t reserved[7];       /* zero */
+} bfd_tuning;
+/* the built-in shape */
+int bfd_tuning_default(bfd_tuning *out);
+int bfd_tuning_get(const bfd_ctx *ctx, uint32_t log2_m, bfd_tuning *out);
+int bfd_tuning_set(bfd_ctx *ctx, uint32_t log2_m, const bfd_tuning *t);
+/* Tunes the plain path for `code`'s shape: times commit + open of a
+ * synthetic polynomial of 2^d evaluations under candidate shapes, one field
+ * at a time from the current shape (median of `reps` runs each; a candidate
+ * must win by more than 0.5%), installs the fastest for log2_m = d + log2_c
+ * and returns it with the medians of the built-in shape and of the chosen
+ * one, re-measured alternately at the end. */
+int bfd_tune(bfd_ctx *ctx, const bfd_code *code, uint32_t reps, bfd_tuning *chosen, double *ms_default,
+             double *ms_tuned);
+/* The same for the engine's shape, timing bfd_commit_open_batch on
+ * synthetic polynomials of the given log2 sizes at rate 1/2^log2_c. */
+int bfd_tune_batch(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t count, const uint32_t *log_sizes,
+                   uint32_t reps, bfd_tuning *chosen, double *ms_default, double *ms_tuned);
+/* Text file of the shapes that differ from the built-in one, one line per
+ * size ("log2_m=26 ntt_order=0 ..."; "batch" for the engine's); load
+ * applies every line of a file that save wrote. */
+int bfd_tuning_save(const bfd_ctx *ctx, const char *path);
+int bfd_tuning_load(bfd_ctx *ctx, const char *path);
+
 /* ---- measurement helpers ---- */
 /* Times `iters` dependent multiplies (variant 0: LDS comb, 1: register
  * Karatsuba), `iters` chained Keccak-256 hashes of 64-byte messages (2, the

@@ -37,6 +37,39 @@ u8p = ctypes.POINTER(ctypes.c_uint8)
 
 # status codes of include/bfd.h
 BFD_OK, BFD_ERR_ARG, BFD_ERR_HIP, BFD_ERR_BUDGET, BFD_ERR_STATE = 0, 1, 2, 3, 4
+# log2_m selectors of the launch-shape calls
+BFD_TUNING_BATCH, BFD_TUNING_ALL = 0xFFFFFFFE, 0xFFFFFFFF
+
+
+class Tuning(ctypes.Structure):
+    """bfd_tuning: how the plain path (per codeword size) or the engine cuts
+    its NTT and Merkle work into launches.  No reference counterpart; every
+    shape gives the same results."""
+    _fields_ = [("ntt_order", ctypes.c_uint32), ("ntt_block", ctypes.c_uint32),
+                ("ntt_waves_per_cu", ctypes.c_uint32), ("ntt_occupancy", ctypes.c_uint32),
+                ("hash_block", ctypes.c_uint32), ("hash_waves_per_cu", ctypes.c_uint32),
+                ("subtree_from_log2", ctypes.c_uint32), ("subtree_levels", ctypes.c_uint32),
+                ("seg_blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
+    FIELDS = ("ntt_order", "ntt_block", "ntt_waves_per_cu", "ntt_occupancy", "hash_block", "hash_waves_per_cu",
+              "subtree_from_log2", "subtree_levels", "seg_blocks_per_cu")
+
+    @classmethod
+    def default(cls) -> "Tuning":
+        t = cls()
+        _check(lib().bfd_tuning_default(ctypes.byref(t)))
+        return t
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k in self.FIELDS}
+
+    def replace(self, **kw) -> "Tuning":
+        t = Tuning()
+        ctypes.pointer(t)[0] = self
+        for k, v in kw.items():
+            if k not in self.FIELDS:
+                raise KeyError(k)
+            setattr(t, k, v)
+        return t
 
 
 class BfdError(RuntimeError):
@@ -130,6 +163,17 @@ def lib():
         L.bfd_ctx_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32),
                                            ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        tp = ctypes.c_void_p  # bfd_tuning *
+        L.bfd_tuning_default.argtypes = [tp]
+        L.bfd_tuning_get.argtypes = [ctypes.c_void_p, ctypes.c_uint32, tp]
+        L.bfd_tuning_set.argtypes = [ctypes.c_void_p, ctypes.c_uint32, tp]
+        L.bfd_tune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, tp,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.bfd_tune_batch.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
+                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, tp,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.bfd_tuning_save.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.bfd_tuning_load.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -201,6 +245,41 @@ class Context:
             nm = names.raw[i * nl:(i + 1) * nl].split(b"\0", 1)[0].decode()
             out[nm] = (ms[i], ln[i])
         return out
+
+
+    # -- launch shapes (bfd_tuning_*, bfd_tune*) --
+    def tuning(self, log2_m: int) -> Tuning:
+        """Shape of the plain path for codewords of 2^log2_m elements, or the
+        engine's (log2_m = BFD_TUNING_BATCH)."""
+        t = Tuning()
+        _check(lib().bfd_tuning_get(self.h, log2_m, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, log2_m: int, t: Tuning) -> None:
+        _check(lib().bfd_tuning_set(self.h, log2_m, ctypes.byref(t)))
+
+    def tune(self, pcs: "Basefold", reps: int = 3) -> Tuple[Tuning, float, float]:
+        """Tunes the plain path for pcs's code; returns (chosen shape, ms of
+        commit+open under the built-in shape, ms under the chosen one)."""
+        t, a, b = Tuning(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().bfd_tune(self.h, pcs.code_h, reps, ctypes.byref(t), ctypes.byref(a), ctypes.byref(b)))
+        return t, a.value, b.value
+
+    def tune_batch(self, log_sizes: Sequence[int], log2_c: int = 1, lambda_: int = 128,
+                   reps: int = 3) -> Tuple[Tuning, float, float]:
+        """Tunes the engine on a batched call of polynomials of these log2
+        sizes; returns (chosen shape, ms built-in, ms chosen)."""
+        ls = (ctypes.c_uint32 * len(log_sizes))(*log_sizes)
+        t, a, b = Tuning(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().bfd_tune_batch(self.h, lambda_, log2_c, len(log_sizes), ls, reps, ctypes.byref(t),
+                                    ctypes.byref(a), ctypes.byref(b)))
+        return t, a.value, b.value
+
+    def save_tuning(self, path: str) -> None:
+        _check(lib().bfd_tuning_save(self.h, str(path).encode()))
+
+    def load_tuning(self, path: str) -> None:
+        _check(lib().bfd_tuning_load(self.h, str(path).encode()))
 
 
 _default_ctx: Optional[Context] = None

@@ -95,6 +95,8 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     const gf *x_dev = fs_challenge(fs.p);
 
     const gf *poly_cur = poly0;
+    const bfd_tuning tn = ctx->tuning_for(M);
+    const launch::Shape sh = shape_of(tn, ctx->cus);
     ctx->timer.host("open_setup", now_ms() - t_setup);
     for (int k = 0; k < d; k++) {
         const int i = d - 1 - k;
@@ -102,8 +104,8 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
         const gf *pik = lay.pi(k);
         uint64_t *tk = trees.as<uint64_t>() + 4 * lay.tree_off[k];
         // pi tree over rows (pi[j], pi[j+half])  (basefold.rs:350-355)
-        ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half); });
-        tree_nodes(ctx, tk, half);
+        ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half, sh); });
+        tree_nodes(ctx, tk, half, tn);
         // inner product of the halved eq table with the low half of poly
         // (eval_impr.rs:187-197)
         unsigned nb = 0;
@@ -238,6 +240,10 @@ int bfd_ctx_create(int device, uint64_t hbm_budget_bytes, bfd_ctx **out) {
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         ctx->cache.st = ctx->stream;
+        ctx->tuning.fill(default_tuning());
+        ctx->batch_tuning = default_tuning();
+        hip_check(hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+        if (ctx->cus < 1) ctx->cus = 1;
         ctx->cache.cap = total_b / 4;
         if (hbm_budget_bytes && hbm_budget_bytes < ctx->cache.cap) ctx->cache.cap = hbm_budget_bytes;
         hipMemPool_t pool;
@@ -647,8 +653,9 @@ int bfd_code_encode(bfd_ctx *ctx, const bfd_code *code, const void *m, int m_on_
             dout = ctx_buf(ctx, 16 * M * w);
             cw = dout.as<gf>();
         }
-        if (w == 1) encode(ctx, code, data, cw, b, n);
-        else encode_w(ctx, code, data, cw, b, n, w);
+        const bfd_tuning tn = ctx->tuning_for(M);
+        if (w == 1) encode(ctx, code, data, cw, b, n, tn);
+        else encode_w(ctx, code, data, cw, b, n, w, tn);
         if (!out_on_device) hip_check(hipMemcpyAsync(out, cw, 16 * M * w, hipMemcpyDeviceToHost, st), "copy codeword");
         hip_check(hipStreamSynchronize(st), "encode");
         ctx->timer.collect();

@@ -77,6 +77,7 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
         }
     }
     Lane &lane = ctx_lane(ctx);
+    lane.set_tuning(ctx->batch_tuning, ctx->cus);
     phase("batch_prepare");
     lane.commit_tick(all);
     phase("batch_enqueue");

@@ -145,7 +145,8 @@ void Lane::reset_ring() {
 // Records a segmented launch: its table goes into the pinned ring now, the
 // upload and the launch itself wait for flush(), so a tick's tables travel in
 // one copy instead of one per launch.
-uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial, uint32_t max_blocks_per_seg) {
+uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial, uint32_t max_blocks) {
+    const uint64_t max_blocks_per_seg = max_blocks ? max_blocks : (uint64_t)tune_.seg_blocks_per_cu * (uint64_t)cus_;
     if (segs.empty()) return 0;
     const int c = cur_;
     if (seg_used_[c] + segs.size() > seg_cap_[c]) {
@@ -204,12 +205,12 @@ void Lane::flush() {
 // together, so the latency-bound tops of every tree share two or three
 // launches instead of being spread over the tick.
 void Lane::tree_levels() {
-    const size_t from = subtree_from(trees_.size());
+    const size_t from = subtree_from(trees_.size(), (size_t)1 << tune_.subtree_from_log2);
     plans_.resize(trees_.size());
     singles_.assign(trees_.size(), 0);
     size_t max_single = 0, max_sub = 0;
     for (size_t a = 0; a < trees_.size(); a++) {
-        tree_steps(trees_[a].second, plans_[a], from);
+        tree_steps(trees_[a].second, plans_[a], from, (int)tune_.subtree_levels);
         size_t k = 0;
         while (k < plans_[a].size() && plans_[a][k].lv == 0) k++;
         singles_[a] = k;
@@ -265,15 +266,22 @@ void Lane::commit_tick(std::vector<Job *> &jobs) {
         segs.push_back(make_seg(j.data, j.cw, j.code->ft.as<gf>() + j.code->ft_off[0], (j.N / 2) << j.b, (uint64_t)j.b));
     }
     seg(SegKernel::NttFirst, segs, "ks_ntt_first");
-    // layers 1.. in passes of up to three (k_ntt_multi's grouping); a job
-    // with fewer layers left takes the two- or one-layer form
+    // layers 1.. in the passes of the shape's plan (up to three layers per
+    // pass, k_ntt_multi's grouping): the t-th pass of every job goes into one
+    // launch per pass width, whatever layer it starts at
     std::vector<Seg> by_r[4];
-    for (int s = 1; s < max_n; s += 3) {
+    std::vector<std::vector<std::pair<int, int>>> plan(jobs.size());
+    size_t max_passes = 0;
+    for (size_t a = 0; a < jobs.size(); a++) {
+        ntt_passes(jobs[a]->n, tune_.ntt_order, plan[a]);
+        max_passes = std::max(max_passes, plan[a].size());
+    }
+    for (size_t t = 0; t < max_passes; t++) {
         for (auto &v : by_r) v.clear();
-        for (Job *jp : jobs) {
-            Job &j = *jp;
-            if (j.n <= s) continue;
-            const int R = std::min(3, j.n - s);
+        for (size_t a = 0; a < jobs.size(); a++) {
+            if (t >= plan[a].size()) continue;
+            Job &j = *jobs[a];
+            const int s = plan[a][t].first, R = plan[a][t].second;
             const gf *ft = j.code->ft.as<gf>() + j.code->ft_off[s];
             by_r[R].push_back(make_seg(nullptr, j.cw, ft, j.M >> (R == 1 ? 1 : R), (uint64_t)(s + j.b)));
         }

@@ -119,15 +119,24 @@ class Lane {
     // open_finish once t.done has completed
     OpenTick open_tick(std::vector<Job *> &jobs);
     void open_finish(OpenTick &t);
+    // launch shape of the ticks enqueued from now on (its ntt_order,
+    // subtree_from_log2, subtree_levels and seg_blocks_per_cu apply)
+    void set_tuning(const bfd_tuning &t, int cus) {
+        tune_ = t;
+        cus_ = cus;
+    }
 
   private:
+    // max_blocks_per_seg = 0: the shape's seg_blocks_per_cu per compute unit
     uint32_t seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *partial = nullptr,
-                 uint32_t max_blocks_per_seg = 2048);
+                 uint32_t max_blocks_per_seg = 0);
     void reset_ring();
     void flush();
     void tree_levels();
     hipStream_t st_;
     KernelTimer *timer_;
+    bfd_tuning tune_ = default_tuning();
+    int cus_ = 256;
     // staging in two halves, alternating per tick: a tick may be enqueued
     // while the previous one still runs, so it must not overwrite what that
     // one's copies and kernels have yet to read

@@ -233,6 +233,79 @@ struct KernelTimer {
     }
 };
 
+// ---------------------------------------------------------------------------
+// launch shapes (bfd_tuning; chosen per codeword size by tune.cpp)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kTuneSizes = 48;  // log2_m = 0..47
+
+inline bfd_tuning default_tuning() {
+    bfd_tuning t;
+    memset(&t, 0, sizeof t);
+    t.ntt_order = 0;
+    t.ntt_block = 256;
+    t.ntt_waves_per_cu = 32;  // 8 blocks of 256 per compute unit
+    t.ntt_occupancy = 2;
+    t.hash_block = 256;
+    t.hash_waves_per_cu = 64;  // 16 blocks of 256 per compute unit
+    t.subtree_from_log2 = 17;
+    t.subtree_levels = 9;
+    t.seg_blocks_per_cu = 8;
+    return t;
+}
+
+// empty string when t is a valid shape, else what is wrong with it
+inline std::string tuning_problem(const bfd_tuning &t) {
+    const auto blk = [](uint32_t b) { return b == 64 || b == 128 || b == 256; };
+    if (t.ntt_order > 2) return "ntt_order must be 0, 1 or 2";
+    if (!blk(t.ntt_block)) return "ntt_block must be 64, 128 or 256";
+    if (t.ntt_waves_per_cu < 4 || t.ntt_waves_per_cu > 1024) return "ntt_waves_per_cu must be 4..1024";
+    if (t.ntt_occupancy != 2 && t.ntt_occupancy != 3) return "ntt_occupancy must be 2 or 3";
+    if (!blk(t.hash_block)) return "hash_block must be 64, 128 or 256";
+    if (t.hash_waves_per_cu < 4 || t.hash_waves_per_cu > 1024) return "hash_waves_per_cu must be 4..1024";
+    if (t.subtree_from_log2 < 1 || t.subtree_from_log2 > 40) return "subtree_from_log2 must be 1..40";
+    if (t.subtree_levels < 1 || t.subtree_levels > (uint32_t)launch::subtree_max_levels())
+        return "subtree_levels must be 1..9";
+    if (t.seg_blocks_per_cu < 1 || t.seg_blocks_per_cu > 64) return "seg_blocks_per_cu must be 1..64";
+    for (uint32_t r : t.reserved)
+        if (r) return "reserved fields must be zero";
+    return "";
+}
+
+inline bool tuning_equal(const bfd_tuning &a, const bfd_tuning &b) { return memcmp(&a, &b, sizeof a) == 0; }
+
+// grid caps in blocks for a device of `cus` compute units
+inline launch::Shape shape_of(const bfd_tuning &t, int cus) {
+    launch::Shape sh;
+    sh.ntt_block = (int)t.ntt_block;
+    sh.ntt_cap = std::max<size_t>(1, (size_t)cus * t.ntt_waves_per_cu * 64 / t.ntt_block);
+    sh.ntt_waves = (int)t.ntt_occupancy;
+    sh.hash_block = (int)t.hash_block;
+    sh.hash_cap = std::max<size_t>(1, (size_t)cus * t.hash_waves_per_cu * 64 / t.hash_block);
+    return sh;
+}
+
+// The NTT passes over layers 1..n-1 as (first layer, layers in the pass)
+// pairs: order 0 runs three layers per pass with the shorter remainder pass
+// last, order 1 the remainder first, order 2 two layers per pass (and one
+// single layer when n-1 is odd).  Any order applies every layer once, in
+// increasing order, which is all the data-order NTT needs.
+inline void ntt_passes(int n, uint32_t order, std::vector<std::pair<int, int>> &out) {
+    out.clear();
+    const int L = n > 1 ? n - 1 : 0;
+    if (order == 2) {
+        for (int s = 1; s <= L; s += 2) out.push_back({s, std::min(2, L - s + 1)});
+        return;
+    }
+    const int rem = L % 3;
+    int s = 1;
+    if (order == 1 && rem) {
+        out.push_back({s, rem});
+        s += rem;
+    }
+    for (; s + 3 <= L + 1; s += 3) out.push_back({s, 3});
+    if (order != 1 && rem) out.push_back({s, rem});
+}
+
 }  // namespace rt
 }  // namespace bfd
 
@@ -269,6 +342,15 @@ struct bfd_ctx {
     bfd::rt::BlockCache cache;
     // engine lane on `stream` for batched calls (made on first use)
     std::unique_ptr<bfd::rt::Lane, bfd::rt::LaneDeleter> lane;
+    // launch shapes: tuning[log2 of the codeword length] for the plain path,
+    // batch_tuning for the engine (batched calls and servers made later)
+    int cus = 256;
+    std::array<bfd_tuning, bfd::rt::kTuneSizes> tuning;
+    bfd_tuning batch_tuning;
+    const bfd_tuning &tuning_for(size_t M) const {
+        const int l = M ? 63 - __builtin_clzll((unsigned long long)M) : 0;
+        return tuning[(size_t)l < tuning.size() ? (size_t)l : tuning.size() - 1];
+    }
 };
 
 struct bfd_code {
@@ -384,16 +466,18 @@ struct TreeStep {
     int lv;       // levels hashed; 0 = one k_nodes level
 };
 constexpr size_t kSubtreeFrom = (size_t)1 << 17;
-inline size_t subtree_from(size_t trees) {
-    size_t from = kSubtreeFrom;
+inline size_t subtree_from(size_t trees, size_t base = kSubtreeFrom) {
+    size_t from = base;
     while (trees > 1 && from > 2) {
         from >>= 1;
         trees = (trees + 1) >> 1;
     }
     return from;
 }
-// steps with lv == 0 come first, then the subtree steps
-inline void tree_steps(size_t leaves, std::vector<TreeStep> &out, size_t from = kSubtreeFrom) {
+// steps with lv == 0 come first, then the subtree steps of at most max_lv
+// levels each
+inline void tree_steps(size_t leaves, std::vector<TreeStep> &out, size_t from = kSubtreeFrom,
+                       int max_lv = launch::subtree_max_levels()) {
     if (leaves & (leaves - 1)) fail(BFD_ERR_ARG, "Merkle leaf count must be a power of two");
     out.clear();
     size_t off = 0, h = leaves;
@@ -404,7 +488,7 @@ inline void tree_steps(size_t leaves, std::vector<TreeStep> &out, size_t from = 
             h >>= 1;
             continue;
         }
-        const int lv = std::min(launch::subtree_max_levels(), __builtin_ctzll(h));
+        const int lv = std::min(std::min(max_lv, launch::subtree_max_levels()), __builtin_ctzll(h));
         out.push_back({off, h, lv});
         for (int i = 0; i < lv; i++) {
             off += h;
@@ -468,23 +552,26 @@ struct ColumnEval {
     }
 };
 
-// all Merkle levels above `leaves` digests at layer0 (layers contiguous)
-inline void tree_nodes(bfd_ctx *ctx, uint64_t *layer0, size_t leaves) {
+// all Merkle levels above `leaves` digests at layer0 (layers contiguous),
+// cut into launches by the shape tn
+inline void tree_nodes(bfd_ctx *ctx, uint64_t *layer0, size_t leaves, const bfd_tuning &tn) {
     hipStream_t st = ctx->stream;
     std::vector<TreeStep> steps;
-    tree_steps(leaves, steps);
+    tree_steps(leaves, steps, (size_t)1 << tn.subtree_from_log2, (int)tn.subtree_levels);
+    const launch::Shape sh = shape_of(tn, ctx->cus);
     for (const TreeStep &s : steps) {
         uint64_t *prev = layer0 + 4 * s.prev;
         if (s.lv == 0)
-            ctx->timer.run(st, "k_nodes", [&] { return launch::nodes(st, prev, prev + 4 * s.h, s.h / 2); });
+            ctx->timer.run(st, "k_nodes", [&] { return launch::nodes(st, prev, prev + 4 * s.h, s.h / 2, sh); });
         else
             ctx->timer.run(st, "k_subtree", [&] { return launch::subtree(st, prev, s.h, s.lv); });
     }
 }
 
 // RS encode of a width-w row-major matrix (binary_reed_solomon.rs:67-82):
-// layer 0, then the remaining layers three per pass
-inline void encode_w(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, int b, int n, size_t w) {
+// layer 0, then the remaining layers in the passes of the shape's plan
+inline void encode_w(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, int b, int n, size_t w,
+                     const bfd_tuning &tn) {
     hipStream_t st = ctx->stream;
     const gf *ft = code->ft.as<gf>();
     if (n == 0) {  // c copies of the single row
@@ -492,30 +579,37 @@ inline void encode_w(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw,
             hip_check(hipMemcpyAsync(cw + u * w, data, 16 * w, hipMemcpyDeviceToDevice, st), "replicate");
         return;
     }
-    ctx->timer.run(st, "k_ntt_first_w", [&] { return launch::ntt_first_w(st, data, cw, ft + code->ft_off[0], b, n, w); });
-    for (int s = 1; s < n; s += 3) {
-        const int R = std::min(3, n - s);
+    const launch::Shape sh = shape_of(tn, ctx->cus);
+    ctx->timer.run(st, "k_ntt_first_w",
+                   [&] { return launch::ntt_first_w(st, data, cw, ft + code->ft_off[0], b, n, w, sh); });
+    std::vector<std::pair<int, int>> passes;
+    ntt_passes(n, tn.ntt_order, passes);
+    for (const auto &p : passes) {
+        const int s = p.first, R = p.second;
         const gf *t0 = ft + code->ft_off[s];
         const gf *t1 = R >= 2 ? ft + code->ft_off[s + 1] : nullptr, *t2 = R >= 3 ? ft + code->ft_off[s + 2] : nullptr;
-        ctx->timer.run(st, "k_ntt_multi_w", [&] { return launch::ntt_multi_w(st, cw, t0, t1, t2, R, s, b, n, w); });
+        ctx->timer.run(st, "k_ntt_multi_w",
+                       [&] { return launch::ntt_multi_w(st, cw, t0, t1, t2, R, s, b, n, w, sh); });
     }
 }
 
-inline void encode(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, int b, int n) {
+inline void encode(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, int b, int n, const bfd_tuning &tn) {
     hipStream_t st = ctx->stream;
     const gf *ft = code->ft.as<gf>();
-    ctx->timer.run(st, "k_ntt_first", [&] { return launch::ntt_first(st, data, cw, ft + code->ft_off[0], b, n); });
-    // remaining layers three at a time (one pass over the codeword per three)
-    int s = 1;
-    for (; s + 3 <= n; s += 3) {
-        const gf *t0 = ft + code->ft_off[s], *t1 = ft + code->ft_off[s + 1], *t2 = ft + code->ft_off[s + 2];
-        ctx->timer.run(st, "k_ntt_multi", [&] { return launch::ntt_multi(st, cw, t0, t1, t2, 3, s, b, n); });
-    }
-    if (s + 2 == n) {
-        const gf *t0 = ft + code->ft_off[s], *t1 = ft + code->ft_off[s + 1];
-        ctx->timer.run(st, "k_ntt_multi", [&] { return launch::ntt_multi(st, cw, t0, t1, nullptr, 2, s, b, n); });
-    } else if (s + 1 == n) {
-        ctx->timer.run(st, "k_ntt_layer", [&] { return launch::ntt_layer(st, cw, ft + code->ft_off[s], s, b, n); });
+    const launch::Shape sh = shape_of(tn, ctx->cus);
+    ctx->timer.run(st, "k_ntt_first", [&] { return launch::ntt_first(st, data, cw, ft + code->ft_off[0], b, n, sh); });
+    // the remaining layers in passes of up to three (one pass over the
+    // codeword per pass)
+    std::vector<std::pair<int, int>> passes;
+    ntt_passes(n, tn.ntt_order, passes);
+    for (const auto &p : passes) {
+        const int s = p.first, R = p.second;
+        const gf *t0 = ft + code->ft_off[s];
+        const gf *t1 = R >= 2 ? ft + code->ft_off[s + 1] : nullptr, *t2 = R >= 3 ? ft + code->ft_off[s + 2] : nullptr;
+        if (R == 1)
+            ctx->timer.run(st, "k_ntt_layer", [&] { return launch::ntt_layer(st, cw, t0, s, b, n, sh); });
+        else
+            ctx->timer.run(st, "k_ntt_multi", [&] { return launch::ntt_multi(st, cw, t0, t1, t2, R, s, b, n, sh); });
     }
 }
 
@@ -552,14 +646,16 @@ inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, boo
               "copy evaluations");
     gf *data = cm->data.as<gf>(), *cw = cm->cw.as<gf>();
     uint64_t *tree = cm->tree.as<uint64_t>();
+    const bfd_tuning tn = ctx->tuning_for(cm->M);
+    const launch::Shape sh = shape_of(tn, ctx->cus);
     if (w == 1) {
-        encode(ctx, code, data, cw, (int)cm->b, (int)cm->n);
-        ctx->timer.run(st, "k_leaves_elem", [&] { return launch::leaves_elem(st, cw, tree, cm->M); });
+        encode(ctx, code, data, cw, (int)cm->b, (int)cm->n, tn);
+        ctx->timer.run(st, "k_leaves_elem", [&] { return launch::leaves_elem(st, cw, tree, cm->M, sh); });
     } else {
-        encode_w(ctx, code, data, cw, (int)cm->b, (int)cm->n, w);
-        ctx->timer.run(st, "k_leaves_rows", [&] { return launch::leaves_rows(st, cw, w, tree, cm->M); });
+        encode_w(ctx, code, data, cw, (int)cm->b, (int)cm->n, w, tn);
+        ctx->timer.run(st, "k_leaves_rows", [&] { return launch::leaves_rows(st, cw, w, tree, cm->M, sh); });
     }
-    tree_nodes(ctx, tree, cm->M);
+    tree_nodes(ctx, tree, cm->M, tn);
     hip_check(hipMemcpyAsync(cm->root, tree + 4 * (2 * cm->M - 2), 32, hipMemcpyDeviceToHost, st), "copy root");
     ColumnEval ev;
     if (values) ev.enqueue(ctx, data, cm->N, w, point, (int)cm->n);

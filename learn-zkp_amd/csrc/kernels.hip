@@ -143,6 +143,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         NTT_GROUP(R, cw, t0, t1, t2, K, mK, stride, g, 1, 0);
 }
 
+// The three-layer pass built for three waves per SIMD (at most 168 VGPRs):
+// more loads in flight per SIMD, against whatever the tighter register
+// budget costs; the autotuner picks between the two builds.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ntt_multi3_w3(
+    gf *__restrict__ cw, const gf *__restrict__ t0, const gf *__restrict__ t1, const gf *__restrict__ t2, int K,
+    size_t groups) {
+    const size_t mK = ((size_t)1 << K) - 1, stride = (size_t)1 << K;
+    for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < groups;
+         g += (size_t)gridDim.x * blockDim.x)
+        NTT_GROUP(3, cw, t0, t1, t2, K, mK, stride, g, 1, 0);
+}
+
 // Width-w codeword (row-major, w elements per row): the same passes with one
 // thread per (group, column); every column of a row uses the row's twiddles.
 template <int R>
@@ -998,69 +1010,86 @@ namespace launch {
 static const unsigned kCapMul = 256 * 8;
 static const unsigned kCapHash = 256 * 16;
 
+static bool shape_ok(const Shape &sh) {
+    const auto blk = [](int b) { return b == 64 || b == 128 || b == 256; };
+    return blk(sh.ntt_block) && blk(sh.hash_block) && sh.ntt_cap > 0 && sh.hash_cap > 0 &&
+           (sh.ntt_waves == 2 || sh.ntt_waves == 3);
+}
+
 hipError_t xor_span(hipStream_t st, gf *ft, const gf *bt, int bits, size_t count) {
     k_xor_span<<<grid_for(count, 256, 65535), 256, 0, st>>>(ft, bt, bits, count);
     return hipGetLastError();
 }
 
-hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n) {
+hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, const Shape &sh) {
+    if (!shape_ok(sh)) return hipErrorInvalidValue;
     const int c = 1 << b;
     if (n == 0) {
         k_replicate<<<1, 64, 0, st>>>(data, cw, c, 1);
         return hipGetLastError();
     }
     const size_t work = ((size_t)1 << n) / 2 * c;
-    k_ntt_first<<<grid_for(work, 256, kCapMul), 256, 0, st>>>(data, cw, ft0, b, work);
+    k_ntt_first<<<grid_for(work, sh.ntt_block, sh.ntt_cap), sh.ntt_block, 0, st>>>(data, cw, ft0, b, work);
     return hipGetLastError();
 }
-hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n) {
+hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n, const Shape &sh) {
+    if (!shape_ok(sh)) return hipErrorInvalidValue;
     const size_t work = ((size_t)1 << n) / 2 << b;
-    k_ntt_layer<<<grid_for(work, 256, kCapMul), 256, 0, st>>>(cw, fts, s, b, work);
+    k_ntt_layer<<<grid_for(work, sh.ntt_block, sh.ntt_cap), sh.ntt_block, 0, st>>>(cw, fts, s, b, work);
     return hipGetLastError();
 }
-hipError_t ntt_multi(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n) {
-    if (R < 2 || R > 3 || s + R > n) return hipErrorInvalidValue;
+hipError_t ntt_multi(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n,
+                     const Shape &sh) {
+    if (R < 2 || R > 3 || s + R > n || !shape_ok(sh)) return hipErrorInvalidValue;
     const size_t groups = ((size_t)1 << (n + b)) >> R;
-    const unsigned g = grid_for(groups, 256, kCapMul);
-    if (R == 2) k_ntt_multi<2><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups);
-    else k_ntt_multi<3><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups);
+    const int bs = sh.ntt_block;
+    const unsigned g = grid_for(groups, bs, sh.ntt_cap);
+    if (R == 2) k_ntt_multi<2><<<g, bs, 0, st>>>(cw, t0, t1, t2, s + b, groups);
+    else if (sh.ntt_waves == 3) k_ntt_multi3_w3<<<g, bs, 0, st>>>(cw, t0, t1, t2, s + b, groups);
+    else k_ntt_multi<3><<<g, bs, 0, st>>>(cw, t0, t1, t2, s + b, groups);
     return hipGetLastError();
 }
 
-hipError_t ntt_first_w(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, size_t w) {
-    if (n < 1 || w < 1) return hipErrorInvalidValue;
+hipError_t ntt_first_w(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, size_t w,
+                       const Shape &sh) {
+    if (n < 1 || w < 1 || !shape_ok(sh)) return hipErrorInvalidValue;
     const size_t work = ((size_t)1 << n) / 2 * ((size_t)1 << b);
-    k_ntt_first_w<<<grid_for(work * w, 256, kCapMul), 256, 0, st>>>(data, cw, ft0, b, work, w);
+    k_ntt_first_w<<<grid_for(work * w, sh.ntt_block, sh.ntt_cap), sh.ntt_block, 0, st>>>(data, cw, ft0, b, work, w);
     return hipGetLastError();
 }
 hipError_t ntt_multi_w(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n,
-                       size_t w) {
-    if (R < 1 || R > 3 || s + R > n || w < 1) return hipErrorInvalidValue;
+                       size_t w, const Shape &sh) {
+    if (R < 1 || R > 3 || s + R > n || w < 1 || !shape_ok(sh)) return hipErrorInvalidValue;
     const size_t groups = ((size_t)1 << (n + b)) >> R;
-    const unsigned g = grid_for(groups * w, 256, kCapMul);
-    if (R == 1) k_ntt_multi_w<1><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
-    else if (R == 2) k_ntt_multi_w<2><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
-    else k_ntt_multi_w<3><<<g, 256, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
+    const int bs = sh.ntt_block;
+    const unsigned g = grid_for(groups * w, bs, sh.ntt_cap);
+    if (R == 1) k_ntt_multi_w<1><<<g, bs, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
+    else if (R == 2) k_ntt_multi_w<2><<<g, bs, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
+    else k_ntt_multi_w<3><<<g, bs, 0, st>>>(cw, t0, t1, t2, s + b, groups, w);
     return hipGetLastError();
 }
-hipError_t leaves_rows(hipStream_t st, const gf *rows, size_t w, uint64_t *out, size_t count) {
-    k_leaves_rows<<<grid_for(count, KB, kCapHash), KB, 0, st>>>(rows, w, out, count);
+hipError_t leaves_rows(hipStream_t st, const gf *rows, size_t w, uint64_t *out, size_t count, const Shape &sh) {
+    if (!shape_ok(sh)) return hipErrorInvalidValue;
+    k_leaves_rows<<<grid_for(count, sh.hash_block, sh.hash_cap), sh.hash_block, 0, st>>>(rows, w, out, count);
     return hipGetLastError();
 }
 hipError_t rowdot(hipStream_t st, const gf *m, size_t w, const gf *scalars, gf *out, size_t rows) {
     k_rowdot<<<grid_for(rows, CB, kCapMul), CB, 0, st>>>(m, w, scalars, out, rows);
     return hipGetLastError();
 }
-hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count) {
-    k_leaves_elem<<<grid_for(count, KB, kCapHash), KB, 0, st>>>(rows, out, count);
+hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count, const Shape &sh) {
+    if (!shape_ok(sh)) return hipErrorInvalidValue;
+    k_leaves_elem<<<grid_for(count, sh.hash_block, sh.hash_cap), sh.hash_block, 0, st>>>(rows, out, count);
     return hipGetLastError();
 }
-hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half) {
-    k_leaves_pair<<<grid_for(half, KB, kCapHash), KB, 0, st>>>(pi, out, half);
+hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half, const Shape &sh) {
+    if (!shape_ok(sh)) return hipErrorInvalidValue;
+    k_leaves_pair<<<grid_for(half, sh.hash_block, sh.hash_cap), sh.hash_block, 0, st>>>(pi, out, half);
     return hipGetLastError();
 }
-hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count) {
-    k_nodes<<<grid_for(count, KB, kCapHash), KB, 0, st>>>(prev, next, count);
+hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count, const Shape &sh) {
+    if (!shape_ok(sh)) return hipErrorInvalidValue;
+    k_nodes<<<grid_for(count, sh.hash_block, sh.hash_cap), sh.hash_block, 0, st>>>(prev, next, count);
     return hipGetLastError();
 }
 hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv) {

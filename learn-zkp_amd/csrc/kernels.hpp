@@ -66,6 +66,19 @@ enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, Lea
 
 namespace launch {
 
+// Launch shapes of the plain path's NTT and hashing kernels, chosen per
+// codeword size by the autotuner (tune.cpp).  Grid caps are in blocks; the
+// kernels loop over their work with a grid stride, so every shape computes
+// the same result.  ntt_waves selects the build of the three-layer pass: 2
+// (at most 256 VGPRs) or 3 (at most 168) waves per SIMD.
+struct Shape {
+    int ntt_block = 256;
+    unsigned ntt_cap = 256 * 8;
+    int ntt_waves = 2;
+    int hash_block = 256;
+    unsigned hash_cap = 256 * 16;
+};
+
 // segmented launch over nseg segments occupying nblocks blocks in total;
 // `partial` receives one element per block for SegKernel::Dot and holds the
 // sums SegKernel::Round reduces (one block per segment)
@@ -79,23 +92,25 @@ hipError_t gather_abs(hipStream_t st, const uint64_t *src, void *out, size_t n, 
 hipError_t xor_span(hipStream_t st, gf *ft, const gf *bt, int bits, size_t count);
 // RS encode: layer 0 (reads the 2^n evaluations, replicates into the c slots)
 // then layers s = 1..n-1 in place on the slot-interleaved codeword
-hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n);
-hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n);
+hipError_t ntt_first(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, const Shape &sh = Shape{});
+hipError_t ntt_layer(hipStream_t st, gf *cw, const gf *fts, int s, int b, int n, const Shape &sh = Shape{});
 // R = 2 or 3 layers s .. s+R-1 in one pass; t_r is the table of layer s + r
-hipError_t ntt_multi(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n);
+hipError_t ntt_multi(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n,
+                     const Shape &sh = Shape{});
 // width-w forms (row-major matrix, w elements per row): layer 0, and R = 1..3
 // layers per pass on the codeword matrix
-hipError_t ntt_first_w(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, size_t w);
+hipError_t ntt_first_w(hipStream_t st, const gf *data, gf *cw, const gf *ft0, int b, int n, size_t w,
+                       const Shape &sh = Shape{});
 hipError_t ntt_multi_w(hipStream_t st, gf *cw, const gf *t0, const gf *t1, const gf *t2, int R, int s, int b, int n,
-                       size_t w);
+                       size_t w, const Shape &sh = Shape{});
 // leaf i = Keccak256 of the 16w bytes of row i
-hipError_t leaves_rows(hipStream_t st, const gf *rows, size_t w, uint64_t *out, size_t count);
+hipError_t leaves_rows(hipStream_t st, const gf *rows, size_t w, uint64_t *out, size_t count, const Shape &sh = Shape{});
 // out[j] = sum_t scalars[t] * m[j*w + t]
 hipError_t rowdot(hipStream_t st, const gf *m, size_t w, const gf *scalars, gf *out, size_t rows);
-hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count);
-hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half);
+hipError_t leaves_elem(hipStream_t st, const gf *rows, uint64_t *out, size_t count, const Shape &sh = Shape{});
+hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half, const Shape &sh = Shape{});
 // one Merkle level: next[i] = H(prev[2i] || prev[2i+1])
-hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count);
+hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count, const Shape &sh = Shape{});
 // lv Merkle levels above the layer `prev` of h digests (h a power of two,
 // 1 <= lv <= subtree_max_levels(), lv <= log2 h), written to the layers that follow it
 hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv);

@@ -16,8 +16,10 @@ ROOT = Path(__file__).resolve().parents[1]
 SRC = ROOT / "learn-zkp_amd" / "csrc" / "kernels.hip"
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
-# bytes of scratch per lane allowed; every other kernel must use none
-ALLOWED = {"k_ntt_multiILi3": 32, "ks_ntt_multiILi3": 48}
+# bytes of scratch per lane allowed; every other kernel must use none.  The
+# three-wave build of the radix-8 pass trades a small spill for occupancy and
+# is used only where the autotuner measures it faster.
+ALLOWED = {"k_ntt_multi3_w3": 96}
 
 
 @pytest.fixture(scope="module")

@@ -130,3 +130,23 @@ def test_batch_quadratics_is_the_alpha_combination(bfd, oracle):
         expect ^= oracle.mul(c, power)
         power = oracle.mul(power, alpha)
     assert (mc == expect).all()
+
+
+def test_tuning_struct_and_default(bfd):
+    """bfd_tuning is 16 uint32 fields in the header's order; the built-in
+    shape needs no device, and a null context is an argument error."""
+    import re
+    text = bfd.HEADER.read_text()
+    body = re.search(r"typedef struct bfd_tuning \{(.*?)\} bfd_tuning;", text, re.S).group(1)
+    names = re.findall(r"uint32_t (\w+)(?:\[\d+\])?;", body)
+    assert tuple(names[:-1]) == bfd.Tuning.FIELDS and names[-1] == "reserved"
+    assert ctypes.sizeof(bfd.Tuning) == 64
+    t = bfd.Tuning.default()
+    assert t.as_dict() == {"ntt_order": 0, "ntt_block": 256, "ntt_waves_per_cu": 32, "ntt_occupancy": 2,
+                           "hash_block": 256, "hash_waves_per_cu": 64, "subtree_from_log2": 17,
+                           "subtree_levels": 9, "seg_blocks_per_cu": 8}
+    L = bfd.lib()
+    assert L.bfd_tuning_set(None, 0, ctypes.byref(t)) == bfd.BFD_ERR_ARG
+    assert L.bfd_tuning_get(None, 0, ctypes.byref(t)) == bfd.BFD_ERR_ARG
+    assert L.bfd_tuning_default(None) == bfd.BFD_ERR_ARG
+    assert L.bfd_tuning_load(None, b"/nonexistent") == bfd.BFD_ERR_ARG
