@@ -14,6 +14,7 @@ Prints one JSON line.  `value` = evaluations committed+opened per second.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import statistics
@@ -379,6 +380,14 @@ def main() -> None:
     import torch
     from __graft_entry__ import load_package
     bfd = load_package()
+    # Python's cyclic collector: with torch loaded a full collection walks
+    # ~170k objects and pauses the calling thread ~50 ms, which lands inside a
+    # timed step now and then.  The library itself runs no Python; objects
+    # made so far are frozen out of the collector and collection stays off
+    # for the measurements (reference counting still frees everything here).
+    gc.collect()
+    gc.freeze()
+    gc.disable()
 
     n = args.log_n
     N = 1 << n
@@ -405,21 +414,40 @@ def main() -> None:
                   "commit_open_ms_default": ms_def, "commit_open_ms_tuned": ms_tuned,
                   "tune_s": time.perf_counter() - t_tune}
 
+    phases = []  # (commit, transcript, open) wall times of the timed steps
+
     def step():
+        ta = time.perf_counter()
         cd = pcs.commit_device(ptr, N)
+        tb = time.perf_counter()
         ch = bfd.GenericChallenger.keccak256()
         ch.observe(cd.comm)
         ch.observe_ext_slice(claim)
+        tc = time.perf_counter()
         proof = pcs.open(cd, [bfd.PolyEvals(claim, point)], ch)
+        phases.append((tb - ta, tc - tb, time.perf_counter() - tc))
         return cd, proof
 
     for _ in range(args.warmup):
         cd, proof = step()
         assert cd.comm == root0
         del cd
+    # one untimed step with every launch bracketed by events: the per-kernel
+    # breakdown, and which kernel dominates
+    ctx.synchronize()
+    ctx.kernel_times(enable=True)
+    cd, proof = step()
+    del cd
+    ctx.synchronize()
+    kall = ctx.kernel_times(enable=False)
+    name = max((kv for kv in kall.items() if not kv[0].startswith("host:")), key=lambda kv: kv[1][0])[0]
+    # the timed steps bracket only the dominant kernel's launches with events
+    # (events between launches cost the stream a few microseconds each)
+    ctx.kernel_filter(name)
     ctx.synchronize()
     torch.cuda.synchronize()
-    ctx.kernel_times(enable=True)  # reset and start per-kernel timing
+    ctx.kernel_times(enable=True)
+    phases.clear()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         cd, proof = step()
@@ -428,10 +456,11 @@ def main() -> None:
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     ktimes = ctx.kernel_times(enable=False)
+    ctx.kernel_filter(None)
     ms_per_step = 1e3 * t / args.steps
 
     # dominant kernel family and its roofline position
-    name, (kms, launches) = max(ktimes.items(), key=lambda kv: kv[1][0])
+    kms, launches = ktimes[name]
     M = N << LOG2_C
     c = 1 << LOG2_C
     # Merkle levels whose input layer holds more than SUBTREE_FROM digests run
@@ -510,7 +539,9 @@ def main() -> None:
                      "traffic": traffic, "avg_launch_ms": kms / launches, "launches_per_step": launches / args.steps},
         "valu": valu,
         "valu_step": valu_step,
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in sorted(ktimes.items())},
+        "kernel_ms_per_step": {k: v[0] for k, v in sorted(kall.items())},
+        "step_phases_ms": {k: [round(1e3 * p[i], 3) for p in phases]
+                           for i, k in enumerate(("commit", "transcript", "open"))},
         "proof_bytes": len(proof),
         "tuning": tuning,
     }

@@ -163,6 +163,7 @@ def lib():
         L.bfd_ctx_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32),
                                            ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.bfd_ctx_kernel_filter.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         tp = ctypes.c_void_p  # bfd_tuning *
         L.bfd_tuning_default.argtypes = [tp]
         L.bfd_tuning_get.argtypes = [ctypes.c_void_p, ctypes.c_uint32, tp]
@@ -230,6 +231,10 @@ class Context:
         ms = ctypes.c_double()
         _check(lib().bfd_microbench(self.h, what, n, iters, ctypes.byref(ms)))
         return ms.value
+
+    def kernel_filter(self, only: Optional[str] = None) -> None:
+        """Time only launches named `only` (None: every kernel)."""
+        _check(lib().bfd_ctx_kernel_filter(self.h, only.encode() if only else None))
 
     def kernel_times(self, enable: bool = True) -> dict:
         """Returns {name: (total_ms, launches)} collected since the last call

@@ -791,4 +791,12 @@ int bfd_ctx_kernel_times(bfd_ctx *ctx, int enable, char *names, size_t name_len,
     });
 }
 
+int bfd_ctx_kernel_filter(bfd_ctx *ctx, const char *only) {
+    return guarded([&] {
+        if (!ctx) fail(BFD_ERR_ARG, "null ctx");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        ctx->timer.only = only ? only : "";
+    });
+}
+
 }  // extern "C"

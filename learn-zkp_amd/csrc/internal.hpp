@@ -172,6 +172,10 @@ inline double now_ms() {
 // Optional per-kernel timing with HIP events on the context's stream.
 struct KernelTimer {
     bool enabled = false;
+    // when set, only launches of this name are timed: events between
+    // launches cost the stream a few microseconds each, so a timed run that
+    // needs one kernel's duration brackets that kernel alone
+    std::string only;
     struct Rec {
         std::string name;
         hipEvent_t a, b;
@@ -191,7 +195,7 @@ struct KernelTimer {
     }
     template <class F>
     void run(hipStream_t st, const char *name, F &&launch) {
-        if (!enabled) {
+        if (!enabled || (!only.empty() && only != name)) {
             hip_check(launch(), name);
             return;
         }

@@ -995,6 +995,18 @@ __global__ __launch_bounds__(KB) void k_bench_keccak(uint64_t *io, size_t n, int
 }
 
 // plain 16-byte-per-thread copy: the reference point for HBM byte counters
+// p[i] = splitmix64(seed + i): synthetic input with the bit statistics of
+// random field elements (timing runs; the switching activity of the data
+// moves the clock under a power limit)
+__global__ void k_fill_mix(uint64_t *__restrict__ p, size_t n, uint64_t seed) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        p[i] = z ^ (z >> 31);
+    }
+}
+
 __global__ void k_copy16(const gf *__restrict__ src, gf *__restrict__ dst, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         gf_st(dst + i, gf_ld(src + i));
@@ -1230,6 +1242,10 @@ hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters) {
 }
 hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n) {
     k_copy16<<<grid_for(n, 256, 1 << 16), 256, 0, st>>>(src, dst, n);
+    return hipGetLastError();
+}
+hipError_t fill_mix(hipStream_t st, uint64_t *p, size_t n, uint64_t seed) {
+    k_fill_mix<<<grid_for(n, 256, 1 << 14), 256, 0, st>>>(p, n, seed);
     return hipGetLastError();
 }
 hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters) {

@@ -155,6 +155,8 @@ hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters);
 hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n);
 hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters);
+// p[i] = splitmix64 of (seed, i) for n 64-bit words
+hipError_t fill_mix(hipStream_t st, uint64_t *p, size_t n, uint64_t seed);
 
 }  // namespace launch
 }  // namespace bfd
