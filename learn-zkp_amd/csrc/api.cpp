@@ -76,11 +76,15 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     // kernel's segment table go up in one copy, the d rounds are enqueued
     // without a read-back, and one read-back after the last round returns
     // the round polynomials, the pi-tree roots and the challenger state
+    // region: transcript | d round segments | d zeroed completion counters
+    // of the rounds' fused top-of-tree launches (k_top_round)
     const uint32_t in_cap = fs_in_cap(ch.input.size());
     const size_t fsb = (fs_bytes((uint32_t)d, in_cap) + 255) & ~(size_t)255;
-    DevBuf fs = ctx_buf(ctx, fsb + sizeof(Seg) * (size_t)(d ? d : 1));
+    const size_t nseg = (size_t)(d ? d : 1), img = fsb + sizeof(Seg) * nseg + 4 * nseg;
+    DevBuf fs = ctx_buf(ctx, img);
     Seg *d_segs = (Seg *)(fs.as<uint8_t>() + fsb);
-    ctx->pinned.ensure(fsb + sizeof(Seg) * (size_t)(d ? d : 1) + 64);
+    unsigned *d_counters = (unsigned *)(fs.as<uint8_t>() + fsb + sizeof(Seg) * nseg);
+    ctx->pinned.ensure(img + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
     if (d > 0) {
         fs_write(o, ch, in_cap, pin);
@@ -90,11 +94,13 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
             const uint64_t *root_k = trees.as<uint64_t>() + 4 * lay.tree_off[k] + 4 * (2 * half - 2);
             hs[k] = fs_round_seg(root_k, fs.p, k, 0, launch::dot_blocks((size_t)1 << (d - 1 - k)));
         }
-        hip_check(hipMemcpyAsync(fs.p, pin, fsb + sizeof(Seg) * d, hipMemcpyHostToDevice, st), "copy transcript");
+        memset(pin + fsb + sizeof(Seg) * nseg, 0, 4 * nseg);
+        hip_check(hipMemcpyAsync(fs.p, pin, img, hipMemcpyHostToDevice, st), "copy transcript");
     }
     const gf *x_dev = fs_challenge(fs.p);
 
     const gf *poly_cur = poly0;
+    std::vector<TreeStep> steps;
     const bfd_tuning tn = ctx->tuning_for(M);
     const launch::Shape sh = shape_of(tn, ctx->cus);
     ctx->timer.host("open_setup", now_ms() - t_setup);
@@ -105,13 +111,24 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
         uint64_t *tk = trees.as<uint64_t>() + 4 * lay.tree_off[k];
         // pi tree over rows (pi[j], pi[j+half])  (basefold.rs:350-355)
         ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half, sh); });
-        tree_nodes(ctx, tk, half, tn);
-        // inner product of the halved eq table with the low half of poly
-        // (eval_impr.rs:187-197)
-        unsigned nb = 0;
-        ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, sums, &nb); });
-        // the round's transcript step: observe root and round poly, sample x
-        ctx->timer.run(st, "ks_round", [&] { return launch::seg_launch(st, SegKernel::Round, d_segs + k, 1, 1, sums); });
+        tree_steps(half, steps, (size_t)1 << tn.subtree_from_log2, (int)tn.subtree_levels);
+        // the inner product of the halved eq table with the low half of poly
+        // (eval_impr.rs:187-197) and the round's transcript step (observe
+        // root and round poly, sample x) ride on the tree's last launch
+        const bool fused = !steps.empty() && steps.back().lv > 0;
+        tree_step_launches(ctx, tk, steps, sh, fused ? steps.size() - 1 : steps.size());
+        if (fused) {
+            const TreeStep &ts = steps.back();
+            ctx->timer.run(st, "k_top_round", [&] {
+                return launch::top_round(st, tk + 4 * ts.prev, ts.h, ts.lv, half_eq.as<gf>(), poly_cur, ph, sums,
+                                         d_segs + k, d_counters + k);
+            });
+        } else {
+            unsigned nb = 0;
+            ctx->timer.run(st, "k_dot", [&] { return launch::dot(st, half_eq.as<gf>(), poly_cur, ph, sums, &nb); });
+            ctx->timer.run(st, "ks_round",
+                           [&] { return launch::seg_launch(st, SegKernel::Round, d_segs + k, 1, 1, sums); });
+        }
         // fold pi (binary_reed_solomon.rs:88-96) and fix the top variable of poly
         gf *pnext = pis.as<gf>() + lay.pi_off[k + 1];
         const gf *ftl = code->ft.as<gf>() + code->ft_off[i];

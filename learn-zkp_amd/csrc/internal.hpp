@@ -556,20 +556,26 @@ struct ColumnEval {
     }
 };
 
-// all Merkle levels above `leaves` digests at layer0 (layers contiguous),
-// cut into launches by the shape tn
-inline void tree_nodes(bfd_ctx *ctx, uint64_t *layer0, size_t leaves, const bfd_tuning &tn) {
+// the first `count` of a tree's steps (tree_steps) as launches
+inline void tree_step_launches(bfd_ctx *ctx, uint64_t *layer0, const std::vector<TreeStep> &steps,
+                               const launch::Shape &sh, size_t count) {
     hipStream_t st = ctx->stream;
-    std::vector<TreeStep> steps;
-    tree_steps(leaves, steps, (size_t)1 << tn.subtree_from_log2, (int)tn.subtree_levels);
-    const launch::Shape sh = shape_of(tn, ctx->cus);
-    for (const TreeStep &s : steps) {
+    for (size_t q = 0; q < count && q < steps.size(); q++) {
+        const TreeStep &s = steps[q];
         uint64_t *prev = layer0 + 4 * s.prev;
         if (s.lv == 0)
             ctx->timer.run(st, "k_nodes", [&] { return launch::nodes(st, prev, prev + 4 * s.h, s.h / 2, sh); });
         else
             ctx->timer.run(st, "k_subtree", [&] { return launch::subtree(st, prev, s.h, s.lv); });
     }
+}
+
+// all Merkle levels above `leaves` digests at layer0 (layers contiguous),
+// cut into launches by the shape tn
+inline void tree_nodes(bfd_ctx *ctx, uint64_t *layer0, size_t leaves, const bfd_tuning &tn) {
+    std::vector<TreeStep> steps;
+    tree_steps(leaves, steps, (size_t)1 << tn.subtree_from_log2, (int)tn.subtree_levels);
+    tree_step_launches(ctx, layer0, steps, shape_of(tn, ctx->cus), steps.size());
 }
 
 // RS encode of a width-w row-major matrix (binary_reed_solomon.rs:67-82):

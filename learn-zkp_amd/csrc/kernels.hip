@@ -904,6 +904,54 @@ __global__ __launch_bounds__(64) void ks_round(const Seg *segs, int nseg, const 
     fs_round((FsState *)g.out, (int)g.count, acc, (const uint64_t *)g.in);
 }
 
+// The last Merkle launch of a round's pi tree, the round's inner product and
+// its Fiat-Shamir step in one launch.  Blocks [0, nsub) hash the top levels
+// exactly as k_subtree; blocks [nsub, nsub + ndot) each sum a slice of
+// a[k] b[k] (the halved eq table against the low half of the polynomial,
+// eval_impr.rs:187-197) into partial[block - nsub].  The top of a tree is a
+// chain of Keccak-f latencies on a few CUs, so the inner product runs on the
+// idle ones for nothing.  Every block then counts itself done; the block
+// that counts last runs the transcript step (fs_round) with its first wave
+// on the root and the reduced sums.  Each block publishes its writes with a
+// fence before it counts, and the last one fences before it reads them; no
+// block waits for another.  *counter is zero at launch.
+__global__ __launch_bounds__(kSubThreads) void k_top_round(uint64_t *__restrict__ prev, size_t h, int lv,
+                                                            unsigned nsub, const gf *__restrict__ a,
+                                                            const gf *__restrict__ b, size_t n, gf *partial,
+                                                            unsigned ndot, const Seg *round, unsigned *counter) {
+    __shared__ ulonglong2 buf[kSubLds][2];
+    __shared__ gf red[kSubThreads / 64];
+    __shared__ unsigned last;
+    const unsigned blk = blockIdx.x;
+    if (blk < nsub) {
+        subtree(prev, h, lv, blk, buf);
+    } else {
+        const unsigned db = blk - nsub;
+        gf acc = gf_zero();
+        for (size_t k = db * (size_t)kSubThreads + threadIdx.x; k < n; k += (size_t)ndot * kSubThreads)
+            acc = gf_add(acc, gf_mul_reg(gf_ld(a + k), gf_ld(b + k)));
+        acc = wave_xor(acc);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            gf sum = gf_zero();
+            for (int w = 0; w < kSubThreads / 64; w++) sum = gf_add(sum, red[w]);
+            gf_st(partial + db, sum);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == nsub + ndot - 1;
+    __syncthreads();
+    if (!last || threadIdx.x >= 64) return;
+    __threadfence();
+    const Seg &g = *round;
+    gf acc = gf_zero();
+    for (unsigned t = threadIdx.x; t < ndot; t += 64) acc = gf_add(acc, gf_ld(partial + t));
+    acc = wave_xor(acc);
+    fs_round((FsState *)g.out, (int)g.count, acc, (const uint64_t *)g.in);
+}
+
 // in = A (2^m), aux = B, out = table, arg = m
 __global__ __launch_bounds__(CB) void ks_outer(const Seg *segs, int nseg) {
     const Seg &g = my_seg(segs, nseg);
@@ -1107,6 +1155,15 @@ hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t co
 hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv) {
     if (lv < 1 || lv > kSubMaxLevels || (h & (h - 1)) || (h >> lv) == 0) return hipErrorInvalidValue;
     k_subtree<<<grid_for(h >> lv, 1, kCapHash), kSubThreads, 0, st>>>(prev, h, lv);
+    return hipGetLastError();
+}
+
+unsigned top_round_dot_blocks(size_t n) { return grid_for(n, kSubThreads, 512); }
+hipError_t top_round(hipStream_t st, uint64_t *prev, size_t h, int lv, const gf *a, const gf *b, size_t n,
+                     gf *partial, const Seg *round, unsigned *counter) {
+    if (lv < 1 || lv > kSubMaxLevels || (h & (h - 1)) || (h >> lv) == 0 || (h >> lv) > 4096) return hipErrorInvalidValue;
+    const unsigned nsub = (unsigned)(h >> lv), ndot = top_round_dot_blocks(n);
+    k_top_round<<<nsub + ndot, kSubThreads, 0, st>>>(prev, h, lv, nsub, a, b, n, partial, ndot, round, counter);
     return hipGetLastError();
 }
 

@@ -122,6 +122,14 @@ hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, 
                const uint64_t *root = nullptr);
 // the number of blocks (partial sums) dot() uses for n elements
 unsigned dot_blocks(size_t n);
+// The last subtree step of a round's pi tree (as subtree()) with the round's
+// inner product sum a[k] b[k], k < n (partial sums in partial[0,
+// top_round_dot_blocks(n))), and the round's transcript step (the
+// SegKernel::Round work of *round, on the reduced sum) by whichever block
+// finishes last.  *counter must be zero.
+hipError_t top_round(hipStream_t st, uint64_t *prev, size_t h, int lv, const gf *a, const gf *b, size_t n,
+                     gf *partial, const Seg *round, unsigned *counter);
+unsigned top_round_dot_blocks(size_t n);
 // fold (binary_reed_solomon.rs:88-96) and fix_last_var with the same
 // challenge in one launch; the fix may be in place.  The challenge is read
 // from device memory (*x), where the round's transcript kernel wrote it.
