@@ -174,8 +174,11 @@ int bfd_server_create(bfd_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t max_acti
         hip_check(hipStreamCreateWithFlags(&S->s_open, hipStreamNonBlocking), "hipStreamCreate");
         S->commit_lane.reset(new Lane(S->s_commit, nullptr));
         S->open_lane.reset(new Lane(S->s_open, nullptr));
-        S->commit_lane->set_tuning(ctx->batch_tuning, ctx->cus);
-        S->open_lane->set_tuning(ctx->batch_tuning, ctx->cus);
+        {
+            std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+            S->commit_lane->set_tuning(ctx->batch_tuning, ctx->cus);
+            S->open_lane->set_tuning(ctx->batch_tuning, ctx->cus);
+        }
         // hand memory held for the plain path back to the device first
         {
             std::lock_guard<std::recursive_mutex> lk(ctx->mu);
