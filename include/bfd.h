@@ -174,6 +174,11 @@ int bfd_server_destroy(bfd_server *srv);
 int bfd_sumcheck_product(bfd_ctx *ctx, uint32_t n, const uint8_t *f, const uint8_t *g, const uint8_t *r,
                          bfd_challenger *challenger, uint8_t claim[16], uint8_t *round_polys,
                          uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim[16]);
+/* Same, with f and g (2^n elements each) already in device memory; they are
+ * copied into the working tables, not modified. */
+int bfd_sumcheck_product_device(bfd_ctx *ctx, uint32_t n, const void *d_f, const void *d_g, const uint8_t *r,
+                                bfd_challenger *challenger, uint8_t claim[16], uint8_t *round_polys,
+                                uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim[16]);
 
 /* ---- Quadratic sumcheck  sum_b sum_k s_k P_{a_k}(b) P_{b_k}(b)
  * (sumcheck/src/function/quadratic.rs:13-134) driven by prove_sumcheck

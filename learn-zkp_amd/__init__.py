@@ -131,6 +131,8 @@ def lib():
                                     u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.bfd_sumcheck_product.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u8p, u8p, u8p, ctypes.c_void_p,
                                            u8p, u8p, u8p, u8p, u8p]
+        L.bfd_sumcheck_product_device.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                                  u8p, ctypes.c_void_p, u8p, u8p, u8p, u8p, u8p]
         L.bfd_sumcheck_quadratic.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, u8p, ctypes.c_uint32,
                                              u8p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p, u8p, u8p, u8p,
                                              u8p, u8p]
@@ -682,13 +684,15 @@ class Server:
             pass
 
 
-def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional[Context] = None):
+def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional[Context] = None,
+                           device_ptrs: Optional[Tuple[int, int]] = None):
     """Degree-3 product sumcheck sum_b f(b) g(b) eq(r, b).
 
-    Returns (claim, compressed round polys (n, 3, 2), challenges (n, 2),
-    final [f, g] (2, 2), subclaim)."""
+    With device_ptrs = (f_ptr, g_ptr) the tables are read from device memory
+    (2^n elements each) and f, g are ignored.  Returns (claim, compressed
+    round polys (n, 3, 2), challenges (n, 2), final [f, g] (2, 2), subclaim)."""
     ctx = ctx or default_context()
-    f, g, r = _elems(f), _elems(g), _elems(r)
+    r = _elems(r)
     n = len(r)
     claim = np.zeros((1, 2), dtype=np.uint64)
     polys = np.zeros((max(1, 3 * n), 2), dtype=np.uint64)
@@ -696,8 +700,14 @@ def prove_sumcheck_product(f, g, r, challenger: GenericChallenger, ctx: Optional
     fin = np.zeros((2, 2), dtype=np.uint64)
     sub = np.zeros((1, 2), dtype=np.uint64)
     rr = r if n else np.zeros((1, 2), dtype=np.uint64)
-    _check(lib().bfd_sumcheck_product(ctx.h, n, _p(f), _p(g), _p(rr), challenger.h, _p(claim), _p(polys),
-                                      _p(chal), _p(fin), _p(sub)))
+    if device_ptrs is not None:
+        _check(lib().bfd_sumcheck_product_device(ctx.h, n, ctypes.c_void_p(device_ptrs[0]),
+                                                 ctypes.c_void_p(device_ptrs[1]), _p(rr), challenger.h, _p(claim),
+                                                 _p(polys), _p(chal), _p(fin), _p(sub)))
+    else:
+        f, g = _elems(f), _elems(g)
+        _check(lib().bfd_sumcheck_product(ctx.h, n, _p(f), _p(g), _p(rr), challenger.h, _p(claim), _p(polys),
+                                          _p(chal), _p(fin), _p(sub)))
     return claim, polys[:3 * n].reshape(n, 3, 2), chal[:n], fin, sub
 
 

@@ -463,6 +463,114 @@ uint64_t bfd_proof_size_width(uint32_t lambda, uint32_t log2_c, uint32_t d, uint
 
 void bfd_proof_free(uint8_t *proof) { free(proof); }
 
+}  // extern "C"
+
+namespace {
+
+// The degree-3 product sumcheck with its transcript on the device: one
+// launch per round (fold, pair sums and, in the block that finishes last,
+// the round's Fiat-Shamir step), one upload of the transcript image before
+// the rounds and one read-back after them.  f and g are copied into working
+// tables first (host or device source).
+void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g_src, bool on_device,
+                      const uint8_t *r_b, Challenger &ch, uint8_t claim_out[16], uint8_t *round_polys,
+                      uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim_out[16]) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    set_device(ctx);
+    hipStream_t st = ctx->stream;
+    const size_t N = (size_t)1 << n;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (n == 0) {  // prove_sumcheck returns before observing (sumcheck/src/lib.rs:63-65)
+        uint8_t fg[32];
+        if (on_device) {
+            hip_check(hipMemcpyAsync(fg, f_src, 16, hipMemcpyDeviceToHost, st), "copy f");
+            hip_check(hipMemcpyAsync(fg + 16, g_src, 16, hipMemcpyDeviceToHost, st), "copy g");
+            hip_check(hipStreamSynchronize(st), "sync");
+        } else {
+            memcpy(fg, f_src, 16);
+            memcpy(fg + 16, g_src, 16);
+        }
+        const gf f0 = load_elem(fg), g0 = load_elem(fg + 16), cl = hmul(f0, g0);
+        store_elem(claim_out, cl);
+        memcpy(final_fg, fg, 32);
+        store_elem(subclaim_out, cl);
+        return;
+    }
+    std::vector<gf> r(n);
+    for (uint32_t i = 0; i < n; i++) r[i] = load_elem(r_b + 16 * i);
+    const int hn = (int)n - 1;
+    // correcting factors of the halved eq table (eval_impr.rs:81-111)
+    std::vector<gf> cf(n, gf_one());
+    for (int j = hn - 1; j >= 0; j--) {
+        gf t = gf_add(gf_one(), r[j]);
+        if (gf_eq(t, gf_zero())) fail(BFD_ERR_ARG, "r_j = 1 makes the halved eq table singular");
+        cf[j] = hmul(cf[j + 1], hinv(t));
+    }
+    if (ch.output.size() > 32) fail(BFD_ERR_ARG, "transcript state too large");
+    DevBuf df = ctx_buf(ctx, 16 * N), dg = ctx_buf(ctx, 16 * N), T = ctx_buf(ctx, 16 * ((size_t)1 << hn));
+    hip_check(hipMemcpyAsync(df.p, f_src, 16 * N, kind, st), "copy f");
+    hip_check(hipMemcpyAsync(dg.p, g_src, 16 * N, kind, st), "copy g");
+    device_eq(ctx, r.data(), hn, T.as<gf>());
+    // transcript image: ProdFs | cf | e0 | polys | chal | input | n counters
+    const uint32_t in_cap = (uint32_t)(((ch.input.size() > 32 ? ch.input.size() : 32) + 64 + 7) & ~(size_t)7);
+    const size_t in_off = prod_input_off(n), cnt_off = (in_off + in_cap + 7) & ~(size_t)7;
+    const size_t img = cnt_off + 4 * (size_t)n;
+    DevBuf fs = ctx_buf(ctx, img), partials = ctx_buf(ctx, 16 * 3 * 512);
+    ctx->pinned.ensure(img + 64);
+    uint8_t *pin = (uint8_t *)ctx->pinned.p;
+    memset(pin, 0, img);
+    ProdFs h;
+    memset(&h, 0, sizeof h);
+    h.in_len = (uint32_t)ch.input.size();
+    h.out_len = (uint32_t)ch.output.size();
+    h.in_cap = in_cap;
+    h.n = n;
+    h.s = gf_one();
+    if (!ch.output.empty()) memcpy(h.out, ch.output.data(), ch.output.size());
+    memcpy(pin, &h, sizeof h);
+    gf *c = (gf *)(pin + prod_cf_off());
+    for (uint32_t i = 0; i < n; i++) {
+        c[i] = cf[i];
+        c[n + i] = gf_add(gf_one(), r[i]);  // eq(r_i, X) = (1 + r_i) + X
+    }
+    if (!ch.input.empty()) memcpy(pin + in_off, ch.input.data(), ch.input.size());
+    hip_check(hipMemcpyAsync(fs.p, pin, img, hipMemcpyHostToDevice, st), "copy transcript");
+    ProdFs *dfs = (ProdFs *)fs.p;
+    unsigned *counters = (unsigned *)(fs.as<uint8_t>() + cnt_off);
+    for (uint32_t k = 0; k < n; k++) {
+        const size_t half = (size_t)1 << (n - 1 - k);
+        ctx->timer.run(st, "k_prod_step", [&] {
+            return launch::prod_step(st, df.as<gf>(), dg.as<gf>(), T.as<gf>(), half, (int)k, dfs, partials.as<gf>(),
+                                     counters + k);
+        });
+    }
+    // read back the region up to the digest and the last two entries of f, g
+    const size_t rb = in_off + 32, fo = (rb + 15) & ~(size_t)15;
+    hip_check(hipMemcpyAsync(pin, fs.p, rb, hipMemcpyDeviceToHost, st), "copy transcript back");
+    hip_check(hipMemcpyAsync(pin + fo, df.p, 32, hipMemcpyDeviceToHost, st), "copy f");
+    hip_check(hipMemcpyAsync(pin + fo + 32, dg.p, 32, hipMemcpyDeviceToHost, st), "copy g");
+    hip_check(hipStreamSynchronize(st), "sumcheck");
+    ctx->timer.collect();
+    memcpy(&h, pin, sizeof h);
+    if (h.n != n || h.in_len != 32 || h.out_len != 16) fail(BFD_ERR_HIP, "device transcript state is corrupt");
+    store_elem(claim_out, h.claim);
+    memcpy(round_polys, pin + prod_polys_off(n), 48 * (size_t)n);
+    memcpy(challenges, pin + prod_chal_off(n), 16 * (size_t)n);
+    ch.input.assign(pin + in_off, pin + in_off + 32);
+    ch.output.assign(h.out, h.out + h.out_len);
+    // the last fold of the two remaining entries (fix_last_var with x)
+    const gf x = h.x;
+    const gf f0 = load_elem(pin + fo), f1 = load_elem(pin + fo + 16);
+    const gf g0 = load_elem(pin + fo + 32), g1 = load_elem(pin + fo + 48);
+    store_elem(final_fg, gf_add(f0, hmul(x, gf_add(f1, f0))));
+    store_elem(final_fg + 16, gf_add(g0, hmul(x, gf_add(g1, g0))));
+    store_elem(subclaim_out, h.sub);
+}
+
+}  // namespace
+
+extern "C" {
+
 int bfd_sumcheck_product(bfd_ctx *ctx, uint32_t n, const uint8_t *f_b, const uint8_t *g_b, const uint8_t *r_b,
                          bfd_challenger *challenger, uint8_t claim_out[16], uint8_t *round_polys,
                          uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim_out[16]) {
@@ -471,90 +579,21 @@ int bfd_sumcheck_product(bfd_ctx *ctx, uint32_t n, const uint8_t *f_b, const uin
             (n && (!r_b || !round_polys || !challenges)))
             fail(BFD_ERR_ARG, "null argument");
         if (n > 32) fail(BFD_ERR_ARG, "too many variables");
-        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-        set_device(ctx);
-        hipStream_t st = ctx->stream;
-        Challenger &ch = challenger->c;
-        const size_t N = (size_t)1 << n;
-        if (n == 0) {  // prove_sumcheck returns before observing (sumcheck/src/lib.rs:63-65)
-            gf f0 = load_elem(f_b), g0 = load_elem(g_b), cl = hmul(f0, g0);
-            store_elem(claim_out, cl);
-            store_elem(final_fg, f0);
-            store_elem(final_fg + 16, g0);
-            store_elem(subclaim_out, cl);
-            return;
-        }
-        std::vector<gf> r(n);
-        for (uint32_t i = 0; i < n; i++) r[i] = load_elem(r_b + 16 * i);
-        const int hn = (int)n - 1;
-        std::vector<gf> cf(n + 1, gf_one());
-        for (int j = hn - 1; j >= 0; j--) {
-            gf t = gf_add(gf_one(), r[j]);
-            if (gf_eq(t, gf_zero())) fail(BFD_ERR_ARG, "r_j = 1 makes the halved eq table singular");
-            cf[j] = hmul(cf[j + 1], hinv(t));
-        }
-        DevBuf df = ctx_buf(ctx, 16 * N), dg = ctx_buf(ctx, 16 * N), T = ctx_buf(ctx, 16 * ((size_t)1 << hn));
-        hip_check(hipMemcpyAsync(df.p, f_b, 16 * N, hipMemcpyHostToDevice, st), "copy f");
-        hip_check(hipMemcpyAsync(dg.p, g_b, 16 * N, hipMemcpyHostToDevice, st), "copy g");
-        device_eq(ctx, r.data(), hn, T.as<gf>());
-        const unsigned kMaxPartials = 512;
-        DevBuf partials(16 * 3 * kMaxPartials, st);
-        ctx->pinned.ensure(16 * 3 * kMaxPartials + 64);
-        uint8_t *pin = (uint8_t *)ctx->pinned.p;
-        gf s = gf_one(), sub = gf_zero(), x = gf_zero();
-        for (int i = (int)n - 1; i >= 0; i--) {
-            const int k = (int)n - 1 - i;
-            unsigned nb = 0;
-            if (i == (int)n - 1)
-                ctx->timer.run(st, "k_prod_round", [&] {
-                    return launch::prod_round(st, df.as<gf>(), dg.as<gf>(), T.as<gf>(), (size_t)1 << i,
-                                              partials.as<gf>(), &nb);
-                });
-            else
-                ctx->timer.run(st, "k_prod_fold_round", [&] {
-                    return launch::prod_fold_round(st, df.as<gf>(), dg.as<gf>(), T.as<gf>(), (size_t)1 << i, x,
-                                                   partials.as<gf>(), &nb);
-                });
-            hip_check(hipMemcpyAsync(pin, partials.p, 48 * nb, hipMemcpyDeviceToHost, st), "copy partials");
-            hip_check(hipStreamSynchronize(st), "round sync");
-            gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
-            for (unsigned t = 0; t < nb; t++) {
-                A0 = gf_add(A0, load_elem(pin + 48 * t));
-                A1 = gf_add(A1, load_elem(pin + 48 * t + 16));
-                A2 = gf_add(A2, load_elem(pin + 48 * t + 32));
-            }
-            // q(X) = q0 + q1 X + q2 X^2 with q1 = A0 + A1 + A2 (characteristic 2)
-            const gf q0 = hmul(cf[i], A0), q2 = hmul(cf[i], A2), q1 = hmul(cf[i], gf_add(A0, gf_add(A1, A2)));
-            const gf e0 = gf_add(gf_one(), r[i]);  // eq(r_i, X) = (1 + r_i) + X
-            const gf c0 = hmul(s, hmul(e0, q0));
-            const gf c1 = hmul(s, gf_add(hmul(e0, q1), q0));
-            const gf c2 = hmul(s, gf_add(hmul(e0, q2), q1));
-            const gf c3 = hmul(s, q2);
-            if (i == (int)n - 1) {
-                sub = gf_add(c1, gf_add(c2, c3));  // claim = p(0) + p(1)
-                store_elem(claim_out, sub);
-                ch.observe_elem(sub);                // sumcheck/src/lib.rs:67
-            }
-            store_elem(round_polys + 48 * k, c0);
-            store_elem(round_polys + 48 * k + 16, c2);
-            store_elem(round_polys + 48 * k + 32, c3);
-            ch.observe_elem(c0);
-            ch.observe_elem(c2);
-            ch.observe_elem(c3);
-            x = ch.sample();
-            store_elem(challenges + 16 * k, x);
-            sub = gf_add(c0, hmul(x, gf_add(c1, hmul(x, gf_add(c2, hmul(x, c3))))));
-            s = hmul(s, gf_add(e0, x));
-        }
-        // last fold of the two remaining entries happens on the host
-        hip_check(hipMemcpyAsync(pin, df.p, 32, hipMemcpyDeviceToHost, st), "copy f");
-        hip_check(hipMemcpyAsync(pin + 32, dg.p, 32, hipMemcpyDeviceToHost, st), "copy g");
-        hip_check(hipStreamSynchronize(st), "final sync");
-        ctx->timer.collect();
-        gf f0 = load_elem(pin), f1 = load_elem(pin + 16), g0 = load_elem(pin + 32), g1 = load_elem(pin + 48);
-        store_elem(final_fg, gf_add(f0, hmul(x, gf_add(f1, f0))));
-        store_elem(final_fg + 16, gf_add(g0, hmul(x, gf_add(g1, g0))));
-        store_elem(subclaim_out, sub);
+        product_sumcheck(ctx, n, f_b, g_b, false, r_b, challenger->c, claim_out, round_polys, challenges, final_fg,
+                         subclaim_out);
+    });
+}
+
+int bfd_sumcheck_product_device(bfd_ctx *ctx, uint32_t n, const void *d_f, const void *d_g, const uint8_t *r_b,
+                                bfd_challenger *challenger, uint8_t claim_out[16], uint8_t *round_polys,
+                                uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim_out[16]) {
+    return guarded([&] {
+        if (!ctx || !d_f || !d_g || !challenger || !claim_out || !final_fg || !subclaim_out ||
+            (n && (!r_b || !round_polys || !challenges)))
+            fail(BFD_ERR_ARG, "null argument");
+        if (n > 32) fail(BFD_ERR_ARG, "too many variables");
+        product_sumcheck(ctx, n, d_f, d_g, true, r_b, challenger->c, claim_out, round_polys, challenges, final_fg,
+                         subclaim_out);
     });
 }
 

@@ -460,6 +460,9 @@ __device__ __forceinline__ void block_xor3(gf a0, gf a1, gf a2, gf *partial) {
     }
 }
 
+// lane src's value of v, on every lane of the wave
+__device__ __forceinline__ gf lane_gf(gf v, int src) { return gf{(uint64_t)__shfl((long long)v.lo, src), (uint64_t)__shfl((long long)v.hi, src)}; }
+
 __device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf &a0, gf &a1, gf &a2) {
     gf p0 = gf_mul_reg(f0, g0);
     gf p1 = gf_mul_reg(f1, g1);
@@ -471,39 +474,128 @@ __device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf 
 }
 
 // sums of the first round over tables of 2*half entries
-__global__ __launch_bounds__(CB) void k_prod_round(const gf *__restrict__ f, const gf *__restrict__ g,
-                                                   const gf *__restrict__ T, size_t half, gf *__restrict__ partial) {
-    gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
-         j += (size_t)gridDim.x * blockDim.x)
-        prod_terms(gf_ld(f + j), gf_ld(f + j + half), gf_ld(g + j), gf_ld(g + j + half), gf_ld(T + j), a0, a1,
-                   a2);
-    block_xor3(a0, a1, a2, partial);
+// One round's transcript step of the product sumcheck, run by the first
+// wave of the block that finishes the round (prove_sumcheck_round,
+// sumcheck/src/lib.rs:81-103): the round polynomial p(X) = s q(X) eq(r_i, X)
+// from the reduced sums, with q(X) = q0 + q1 X + q2 X^2 the pair sums scaled
+// by the correcting factor (q1 = A0 + A1 + A2 in characteristic 2) and
+// eq(r_i, X) = (1 + r_i) + X; then observe [claim,] c0, c2, c3, sample x and
+// advance the subclaim (Horner) and s.  Independent products run on
+// different lanes; lane 0 writes the results.
+__device__ void prod_fs_step(ProdFs *st, int k, gf A0, gf A1, gf A2) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = st->n;
+    const int i = (int)n - 1 - k;
+    uint8_t *base = (uint8_t *)st;
+    const gf *cf = (const gf *)(base + prod_cf_off()), *e0s = cf + n;
+    gf *polys = (gf *)(base + prod_polys_off(n)), *chal = (gf *)(base + prod_chal_off(n));
+    uint8_t *in = base + prod_input_off(n);
+    uint64_t *in64 = reinterpret_cast<uint64_t *>(in);
+    const gf c = gf_ld(cf + i), e0 = gf_ld(e0s + i), s = st->s;
+    gf p = gf_mul_reg(c, lane == 0 ? A0 : (lane == 1 ? A2 : gf_add(A0, gf_add(A1, A2))));
+    const gf q0 = lane_gf(p, 0), q2 = lane_gf(p, 1), q1 = lane_gf(p, 2);
+    p = gf_mul_reg(e0, lane == 0 ? q0 : (lane == 1 ? q1 : q2));
+    const gf t0 = lane_gf(p, 0), t1 = lane_gf(p, 1), t2 = lane_gf(p, 2);
+    p = gf_mul_reg(s, lane == 0 ? t0 : (lane == 1 ? gf_add(t1, q0) : (lane == 2 ? gf_add(t2, q1) : q2)));
+    const gf c0 = lane_gf(p, 0), c1 = lane_gf(p, 1), c2 = lane_gf(p, 2), c3 = lane_gf(p, 3);
+    const gf claim = gf_add(c1, gf_add(c2, c3));  // p(0) + p(1), used in round 0
+    uint64_t d[4];
+    if (k > 0 && st->in_len == 32) {
+        const uint64_t m[10] = {in64[0], in64[1], in64[2], in64[3], c0.lo, c0.hi, c2.lo, c2.hi, c3.lo, c3.hi};
+        kk::keccak256_words<10>(m, d);
+    } else {
+        if (lane == 0) {
+            uint32_t len = st->in_len;
+            const auto put = [&](uint64_t v) {
+#pragma unroll 1
+                for (int t = 0; t < 8; t++) in[len + t] = (uint8_t)(v >> (8 * t));
+                len += 8;
+            };
+            if (k == 0) {  // the claim is observed first (sumcheck/src/lib.rs:67)
+                put(claim.lo);
+                put(claim.hi);
+            }
+            put(c0.lo);
+            put(c0.hi);
+            put(c2.lo);
+            put(c2.hi);
+            put(c3.lo);
+            put(c3.hi);
+            kk::keccak256_bytes(in, len, d);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) d[q] = (uint64_t)__shfl((long long)d[q], 0);
+    }
+    // the challenge is the last 16 digest bytes, last byte first
+    const gf x{__builtin_bswap64(d[3]), __builtin_bswap64(d[2])};
+    p = gf_mul_reg(lane == 0 ? x : s, lane == 0 ? c3 : gf_add(e0, x));
+    const gf h1 = lane_gf(p, 0), s_next = lane_gf(p, 1);
+    const gf h2 = gf_mul_reg(x, gf_add(c2, h1));
+    const gf sub_next = gf_add(c0, gf_mul_reg(x, gf_add(c1, h2)));
+    if (lane != 0) return;
+    uint64_t *out64 = reinterpret_cast<uint64_t *>(st->out);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        in64[q] = d[q];
+        out64[q] = d[q];
+    }
+    st->in_len = 32;
+    st->out_len = 16;
+    if (k == 0) st->claim = claim;
+    st->sub = sub_next;
+    st->s = s_next;
+    st->x = x;
+    polys[3 * k] = c0;
+    polys[3 * k + 1] = c2;
+    polys[3 * k + 2] = c3;
+    chal[k] = x;
 }
 
-// fixes the top variable of f and g (tables of 4*q entries -> 2*q) with x,
-// in place, and accumulates the next round's sums over the new tables
-__global__ __launch_bounds__(CB) void k_prod_fold_round(gf *__restrict__ f, gf *__restrict__ g,
-                                                        const gf *__restrict__ T, size_t q, gf x,
-                                                        gf *__restrict__ partial) {
+// Round k of the product sumcheck (see launch::prod_step).  Rounds after the
+// first fold f and g on their top variable with the last challenge while
+// they form the pair sums of prod_terms (one pass over the tables).
+template <bool FOLD>
+__global__ __launch_bounds__(CB) void k_prod_step(gf *__restrict__ f, gf *__restrict__ g, const gf *__restrict__ T,
+                                                  size_t half, int k, ProdFs *fs, gf *partial, unsigned *counter) {
     gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
-    const size_t h = 2 * q;
-    const gf_pre px = gf_prepare(x);
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < q;
-         j += (size_t)gridDim.x * blockDim.x) {
-        gf fa = gf_ld(f + j), fb = gf_ld(f + j + q), fc = gf_ld(f + j + h), fd = gf_ld(f + j + h + q);
-        gf ga = gf_ld(g + j), gb = gf_ld(g + j + q), gc = gf_ld(g + j + h), gd = gf_ld(g + j + h + q);
-        gf f0 = gf_add(fa, gf_mul_pre(gf_add(fc, fa), px));
-        gf f1 = gf_add(fb, gf_mul_pre(gf_add(fd, fb), px));
-        gf g0 = gf_add(ga, gf_mul_pre(gf_add(gc, ga), px));
-        gf g1 = gf_add(gb, gf_mul_pre(gf_add(gd, gb), px));
-        gf_st(f + j, f0);
-        gf_st(f + j + q, f1);
-        gf_st(g + j, g0);
-        gf_st(g + j + q, g1);
-        prod_terms(f0, f1, g0, g1, gf_ld(T + j), a0, a1, a2);
+    if constexpr (FOLD) {
+        const size_t q = half, h = 2 * q;
+        const gf_pre px = gf_prepare(fs->x);
+        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < q;
+             j += (size_t)gridDim.x * blockDim.x) {
+            gf fa = gf_ld(f + j), fb = gf_ld(f + j + q), fc = gf_ld(f + j + h), fd = gf_ld(f + j + h + q);
+            gf ga = gf_ld(g + j), gb = gf_ld(g + j + q), gc = gf_ld(g + j + h), gd = gf_ld(g + j + h + q);
+            gf f0 = gf_add(fa, gf_mul_pre(gf_add(fc, fa), px));
+            gf f1 = gf_add(fb, gf_mul_pre(gf_add(fd, fb), px));
+            gf g0 = gf_add(ga, gf_mul_pre(gf_add(gc, ga), px));
+            gf g1 = gf_add(gb, gf_mul_pre(gf_add(gd, gb), px));
+            gf_st(f + j, f0);
+            gf_st(f + j + q, f1);
+            gf_st(g + j, g0);
+            gf_st(g + j + q, g1);
+            prod_terms(f0, f1, g0, g1, gf_ld(T + j), a0, a1, a2);
+        }
+    } else {
+        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
+             j += (size_t)gridDim.x * blockDim.x)
+            prod_terms(gf_ld(f + j), gf_ld(f + j + half), gf_ld(g + j), gf_ld(g + j + half), gf_ld(T + j), a0, a1,
+                       a2);
     }
     block_xor3(a0, a1, a2, partial);
+    __shared__ unsigned last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last || threadIdx.x >= 64) return;
+    __threadfence();
+    gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
+    for (unsigned t = threadIdx.x; t < gridDim.x; t += 64) {
+        A0 = gf_add(A0, gf_ld(partial + 3 * t));
+        A1 = gf_add(A1, gf_ld(partial + 3 * t + 1));
+        A2 = gf_add(A2, gf_ld(partial + 3 * t + 2));
+    }
+    prod_fs_step(fs, k, wave_xor(A0), wave_xor(A1), wave_xor(A2));
 }
 
 // ---------------------------------------------------------------------------
@@ -811,7 +903,6 @@ __global__ __launch_bounds__(CB) void ks_fix(const Seg *segs, int nseg) {
     }
 }
 
-__device__ __forceinline__ gf lane_gf(gf v, int src) { return gf{(uint64_t)__shfl((long long)v.lo, src), (uint64_t)__shfl((long long)v.hi, src)}; }
 
 // One round of the Eval sumcheck's transcript for one opening, run by one
 // wave (OpenRounds::round on the device; basefold.rs:356, eval.rs:64-110,
@@ -1193,18 +1284,11 @@ hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_
     k_outer<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(A, m, B, out, n);
     return hipGetLastError();
 }
-hipError_t prod_round(hipStream_t st, const gf *f, const gf *g, const gf *T, size_t half, gf *partial,
-                      unsigned *nblocks) {
-    unsigned gsz = grid_for(half, CB, 512);
-    k_prod_round<<<gsz, CB, 0, st>>>(f, g, T, half, partial);
-    *nblocks = gsz;
-    return hipGetLastError();
-}
-hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, gf x, gf *partial,
-                           unsigned *nblocks) {
-    unsigned gsz = grid_for(q, CB, 512);
-    k_prod_fold_round<<<gsz, CB, 0, st>>>(f, g, T, q, x, partial);
-    *nblocks = gsz;
+hipError_t prod_step(hipStream_t st, gf *f, gf *g, const gf *T, size_t half, int k, ProdFs *fs, gf *partial,
+                     unsigned *counter) {
+    const unsigned gsz = grid_for(half, CB, 512);
+    if (k == 0) k_prod_step<false><<<gsz, CB, 0, st>>>(f, g, T, half, k, fs, partial, counter);
+    else k_prod_step<true><<<gsz, CB, 0, st>>>(f, g, T, half, k, fs, partial, counter);
     return hipGetLastError();
 }
 hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,

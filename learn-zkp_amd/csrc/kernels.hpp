@@ -61,6 +61,26 @@ __host__ __device__ inline size_t fs_rounds_off(uint32_t n) { return sizeof(FsSt
 __host__ __device__ inline size_t fs_input_off(uint32_t n) { return fs_rounds_off(n) + sizeof(FsRound) * (size_t)n; }
 __host__ __device__ inline size_t fs_bytes(uint32_t n, uint32_t in_cap) { return fs_input_off(n) + in_cap; }
 
+// Device-resident transcript of a degree-3 product sumcheck (the
+// prove_sumcheck driver, sumcheck/src/lib.rs:56-103, over f g eq(r, .)): the
+// challenger's byte buffers and the running scalars.  One region laid out as
+//   ProdFs | gf cf[n] | gf e0[n] | gf polys[3n] | gf chal[n] | uint8_t input[in_cap]
+// cf[i] is the correcting factor of the halved eq table at round variable i
+// and e0[i] = 1 + r_i; polys and chal receive each round's compressed
+// polynomial [c0, c2, c3] and challenge.
+struct ProdFs {
+    uint32_t in_len, out_len, in_cap, n;
+    gf s;      // product of eq(r_j, x_j) over the rounds so far
+    gf sub;    // subclaim
+    gf x;      // last sampled challenge (read by the next round's fold)
+    gf claim;  // sum over the cube, observed before the first round
+    uint8_t out[32];
+};
+__host__ __device__ inline size_t prod_cf_off() { return sizeof(ProdFs); }
+__host__ __device__ inline size_t prod_polys_off(uint32_t n) { return sizeof(ProdFs) + 32 * (size_t)n; }
+__host__ __device__ inline size_t prod_chal_off(uint32_t n) { return prod_polys_off(n) + 48 * (size_t)n; }
+__host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_off(n) + 16 * (size_t)n; }
+
 enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer,
                        Round };
 
@@ -138,10 +158,16 @@ hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t 
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r);
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x);
 hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_t n);
-hipError_t prod_round(hipStream_t st, const gf *f, const gf *g, const gf *T, size_t half, gf *partial,
-                      unsigned *nblocks);
-hipError_t prod_fold_round(hipStream_t st, gf *f, gf *g, const gf *T, size_t q, gf x, gf *partial,
-                           unsigned *nblocks);
+// Round k of a product sumcheck with its transcript step on the device:
+// round 0 reads f, g over 2 * half entries; later rounds first fold both
+// tables on the top variable with the previous round's challenge (read from
+// the ProdFs region) down to 2 * half entries.  Each block leaves its sums
+// in partial[3 * block, +3); the block that finishes last (counted in
+// *counter, zero at launch) reduces them and runs the round's transcript
+// step, which writes the round's polynomial and challenge into the region.
+// Every block and partial sum fits: at most 512 blocks.
+hipError_t prod_step(hipStream_t st, gf *f, gf *g, const gf *T, size_t half, int k, ProdFs *fs, gf *partial,
+                     unsigned *counter);
 // MLE value of every column of a rows x w matrix at a point whose eq table is
 // A (2^m entries) x B: partial[col * nblocks + block], one pass over the data
 hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,

@@ -413,6 +413,74 @@ def test_product_sumcheck_full_size_invariants(bfd, oracle, ctx):
     assert oracle.to_int(oracle.mul(oracle.mul(fx, gx), eq)) == s
 
 
+def _replay_product(oracle, ch, n, claim, polys, chal):
+    """Replays the product sumcheck's transcript on the oracle challenger ch
+    (already in the caller's starting state): every challenge must equal the
+    sampled one and every round polynomial satisfy p(0) + p(1) = subclaim.
+    Returns the final subclaim as an integer."""
+    ch.observe(claim.tobytes())
+    s = oracle.to_int(claim)
+    for k in range(n):
+        c0, c2, c3 = polys[k, 0:1], polys[k, 1:2], polys[k, 2:3]
+        c1 = oracle.from_int(s) ^ c2 ^ c3
+        ch.observe(c0.tobytes() + c2.tobytes() + c3.tobytes())
+        x = ch.sample()
+        assert (x == chal[k:k + 1]).all(), k
+        acc = c3
+        for c in (c2, c1, c0):
+            acc = oracle.mul(acc, x) ^ c
+        s = oracle.to_int(acc)
+    return s
+
+
+def test_product_sumcheck_transcript_prefixes(bfd, oracle, ctx):
+    """The product sumcheck's transcript runs on the device from any
+    challenger state: after caller prefixes of 5, 32 (one digest's length),
+    100 and 200 bytes (the first round then hashes two blocks), and after a
+    prefix followed by a sample (unread output pending), the challenges replay
+    on the oracle's challenger and both challengers end in the same state."""
+    n = 6
+    f, g, r = bfd.random_vec(700, 1 << n), bfd.random_vec(701, 1 << n), bfd.random_vec(702, n)
+    fresh = oracle.product_sumcheck(f, g, r)
+    rng = np.random.default_rng(3)
+    for plen, sample_after in [(5, False), (32, False), (100, False), (200, False), (40, True)]:
+        _say(f"product sumcheck prefix {plen} sample={sample_after}")
+        prefix = bytes(rng.integers(0, 256, plen, dtype=np.uint8))
+        ch, och = bfd.GenericChallenger.keccak256(), oracle.Challenger()
+        ch.observe(prefix)
+        och.observe(prefix)
+        if sample_after:
+            assert (ch.sample_ext_element() == och.sample()).all()
+        claim, polys, chal, fin, sub = bfd.prove_sumcheck_product(f, g, r, ch, ctx)
+        assert (claim == fresh[0]).all()
+        s = _replay_product(oracle, och, n, claim, polys, chal)
+        assert s == oracle.to_int(sub)
+        assert (ch.sample_ext_element() == och.sample()).all()
+        assert ch.sample_bits(20) == och.sample_bits(20)
+
+
+def test_product_sumcheck_device_inputs(bfd, oracle, ctx):
+    """Tables already in device memory give the same outputs as host tables,
+    leave the caller's buffers unchanged, at 2^12 and at 2^22."""
+    import torch
+    for n in (12, 22):
+        _say(f"product sumcheck device inputs n={n}")
+        f, g, r = bfd.random_vec(800 + n, 1 << n), bfd.random_vec(900 + n, 1 << n), bfd.random_vec(1000 + n, n)
+        df = torch.from_numpy(f.view(np.int64).reshape(-1)).to("cuda:0")
+        dg = torch.from_numpy(g.view(np.int64).reshape(-1)).to("cuda:0")
+        torch.cuda.synchronize()
+        got = bfd.prove_sumcheck_product(None, None, r, bfd.GenericChallenger.keccak256(), ctx,
+                                         device_ptrs=(df.data_ptr(), dg.data_ptr()))
+        exp = bfd.prove_sumcheck_product(f, g, r, bfd.GenericChallenger.keccak256(), ctx)
+        for a, b in zip(got, exp):
+            assert (np.asarray(a) == np.asarray(b)).all(), n
+        if n == 12:
+            for a, b in zip(got, oracle.product_sumcheck(f, g, r)):
+                assert (np.asarray(a) == np.asarray(b)).all()
+        assert (df.cpu().numpy() == f.view(np.int64).reshape(-1)).all()
+        assert (dg.cpu().numpy() == g.view(np.int64).reshape(-1)).all()
+
+
 @pytest.mark.skipif(os.environ.get("BFD_SKIP_FULL") == "1", reason="full-size case disabled")
 def test_basefold_full_size_verifies(bfd, oracle, ctx):
     """2^24 evaluations, rate 1/4: the claim agrees with the CPU MLE and the
