@@ -172,7 +172,25 @@ def product_sumcheck(bfd, ctx, torch, reps: int) -> dict:
             "evals_per_s": (1 << n) / t}
 
 
-def batched_vs_plain(bfd, ctx, torch, reps: int, tune_reps: int = 0) -> dict:
+def mix_position(jobs, seconds: float, rates) -> dict:
+    """Roofline position of a set of commit+open jobs [(log2 n, log2 c)]
+    completed in `seconds`: algorithmic HBM GB/s (and its fraction of 8 TB/s)
+    and the VALU fraction (K/R_K + G/R_G)/t at the isolated kernels' rates."""
+    B = sum(algorithmic_bytes(n, c)["total"] for n, c in jobs)
+    K = G = 0
+    for n, c in jobs:
+        k, g = valu_work(n, c)
+        K += k
+        G += g
+    out = {"hbm_gbs_algorithmic": B / seconds / 1e9, "hbm_frac": B / seconds / 1e9 / HBM_PEAK_GBS,
+           "keccak_perms": K, "gf_muls": G}
+    if rates:
+        out["valu_floor_ms"] = 1e3 * (K / rates[0] + G / rates[1])
+        out["valu_frac"] = (K / rates[0] + G / rates[1]) / seconds
+    return out
+
+
+def batched_vs_plain(bfd, ctx, torch, reps: int, tune_reps: int = 0, rates=None) -> dict:
     """Config 4: 64 polynomials, log2 sizes uniform in {12..22} (numpy seed 0),
     rate 1/2, evaluations and points seeded per polynomial, resident in HBM.
     Times one batched call against 64 plain commit+open calls; both produce
@@ -235,6 +253,8 @@ def batched_vs_plain(bfd, ctx, torch, reps: int, tune_reps: int = 0) -> dict:
     return {"workload": "64 polys, log2 sizes U{12..22} seed 0, rate 1/2", "total_evals": total,
             "plain_ms": 1e3 * t_p, "batched_ms": 1e3 * t_b, "plain_elems_per_s": total / t_p,
             "batched_elems_per_s": total / t_b, "speedup": t_p / t_b, "proofs_identical": True,
+            "batched_position": mix_position([(n, log2_c) for n in sizes], t_b, rates),
+            "plain_position": mix_position([(n, log2_c) for n in sizes], t_p, rates),
             "tuning": tuning}
 
 
@@ -316,7 +336,7 @@ def verification(bfd, ctx, torch, count: int) -> dict:
             "proofs_per_s": count / t}
 
 
-def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int) -> dict:
+def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int, rates_k=None) -> dict:
     """Config 5: Poisson arrivals (mean `rate` req/s, numpy seed 0) of
     commit-then-open requests, log2 size uniform over {14..24}, rate 1/2 or 1/4
     chosen 50/50, evaluations from a seeded pool resident in HBM (2 per size),
@@ -371,6 +391,7 @@ def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int) -> dict
             "budget_bytes": budget, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "elems_per_s": total / makespan, "makespan_s": makespan,
             "arrival_span_s": float(gaps.sum()), "requests_per_s_served": n_req / makespan,
+            "position": mix_position([(int(n), int(c)) for n, c in zip(sizes, rates)], makespan, rates_k),
             "peak_resident_sampled_bytes": peak_resident, "server_timing": timing}
 
 
@@ -578,18 +599,19 @@ def main() -> None:
     result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 5)
     if not args.no_batch:
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps,
-                                             0 if args.no_tune else args.tune_reps)
+                                             0 if args.no_tune else args.tune_reps, (r_k, r_g))
     if args.verify_proofs > 0:
         result["verify"] = verification(bfd, ctx, torch, args.verify_proofs)
     if args.serve_requests > 0:
         result["serving"] = serving_trace(bfd, ctx, torch, args.serve_requests, args.serve_rate,
-                                          int(args.serve_budget_gb * 1e9))
+                                          int(args.serve_budget_gb * 1e9), (r_k, r_g))
         # at 200 req/s the offered work exceeds what one GPU serves, so
         # latency grows with the trace; `requests_per_s_served` there is the
         # saturation throughput, and this trace shows latency below it
         if args.serve_sustain_rate > 0:
             result["serving_sustainable"] = serving_trace(bfd, ctx, torch, args.serve_requests,
-                                                          args.serve_sustain_rate, int(args.serve_budget_gb * 1e9))
+                                                          args.serve_sustain_rate, int(args.serve_budget_gb * 1e9),
+                                                          (r_k, r_g))
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(result))
