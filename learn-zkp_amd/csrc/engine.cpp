@@ -421,8 +421,7 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         segs.push_back(make_seg(j.lay.pi(kp), (void *)j.lay.pi(kp + 1), j.code->ft.as<gf>() + j.code->ft_off[i], half,
                                 (uint64_t)fs_challenge(j.fs)));
     }
-    seg(SegKernel::Fold, segs, "ks_fold");
-    segs.clear();
+    // ... and in the same launch (segments without a twiddle table fix)
     for (Job *jp : jobs) {
         Job &j = *jp;
         if (j.stage != Job::Open || j.k == 0) continue;
@@ -430,7 +429,7 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         segs.push_back(make_seg(j.poly_cur, j.poly, nullptr, (size_t)1 << i, (uint64_t)fs_challenge(j.fs)));
         j.poly_cur = j.poly;
     }
-    seg(SegKernel::Fix, segs, "ks_fix");
+    seg(SegKernel::FoldFix, segs, "ks_fold_fix");
     // (C) this round's pi tree, inner product and transcript step
     segs.clear();
     for (Job *jp : jobs) {

@@ -903,6 +903,34 @@ __global__ __launch_bounds__(CB) void ks_fix(const Seg *segs, int nseg) {
     }
 }
 
+// ks_fold and ks_fix segments in one launch: a segment with a twiddle table
+// (aux) folds, one without fixes; a block serves one segment, so the two
+// kinds share the device instead of running one after the other
+__global__ __launch_bounds__(CB) void ks_fold_fix(const Seg *segs, int nseg) {
+    const Seg &g = my_seg(segs, nseg);
+    if (g.aux) {
+        const gf *w = (const gf *)g.in, *ft = (const gf *)g.aux;
+        gf *out = (gf *)g.out;
+        const size_t mid = g.count;
+        const gf_pre pr = gf_prepare(seg_challenge(g));
+        SEG_LOOP(g, j) {
+            gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
+            hi = gf_add(hi, lo);
+            lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
+            gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), pr)));
+        }
+    } else {
+        const gf *in = (const gf *)g.in;
+        gf *out = (gf *)g.out;
+        const size_t half = g.count;
+        const gf_pre px = gf_prepare(seg_challenge(g));
+        SEG_LOOP(g, j) {
+            gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
+            gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+        }
+    }
+}
+
 
 // One round of the Eval sumcheck's transcript for one opening, run by one
 // wave (OpenRounds::round on the device; basefold.rs:356, eval.rs:64-110,
@@ -1349,6 +1377,7 @@ hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int ns
     case SegKernel::Dot: ks_dot<<<nblocks, CB, 0, st>>>(d_segs, nseg, partial); break;
     case SegKernel::Fold: ks_fold<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Fix: ks_fix<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::FoldFix: ks_fold_fix<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Outer: ks_outer<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Round: ks_round<<<nblocks, 64, 0, st>>>(d_segs, nseg, partial); break;
     }

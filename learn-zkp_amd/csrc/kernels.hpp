@@ -82,7 +82,7 @@ __host__ __device__ inline size_t prod_chal_off(uint32_t n) { return prod_polys_
 __host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_off(n) + 16 * (size_t)n; }
 
 enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer,
-                       Round };
+                       Round, FoldFix };
 
 namespace launch {
 
