@@ -170,6 +170,7 @@ uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *pa
         uint64_t want = (g.count + bs - 1) / bs;
         if (want < 1 || k == SegKernel::Round) want = 1;
         if (want > max_blocks_per_seg) want = max_blocks_per_seg;
+        if (g.nblk && want > g.nblk) want = g.nblk;  // the segment's own cap
         g.blk0 = blocks;
         g.nblk = (uint32_t)want;
         blocks += g.nblk;
@@ -438,7 +439,22 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         const size_t half = (j.M >> j.k) / 2;
         segs.push_back(make_seg(j.lay.pi(j.k), (void *)j.lay.pitree(j.k), nullptr, half, half));
     }
-    seg(SegKernel::LeavesPair, segs, "ks_leaves_pair");
+    // the inner products ride in the same launch, at most 64 blocks (partial
+    // sums) per job, job a's sums at partials_[64 a, 64 a + 64)
+    const size_t nleaf = segs.size();
+    std::vector<Job *> dotted;
+    for (Job *jp : jobs) {
+        Job &j = *jp;
+        if (j.stage != Job::Open || j.k >= j.n) continue;
+        const int i = j.n - 1 - j.k;
+        Seg g = make_seg(j.half_eq, nullptr, j.poly_cur, (size_t)1 << i, 0, gf{64 * (uint64_t)dotted.size(), 0});
+        g.nblk = 64;  // cap; seg() sets the count
+        segs.push_back(g);
+        dotted.push_back(jp);
+    }
+    ensure(partials_, 16 * 64 * m + 16, st_);
+    seg(SegKernel::LeavesPairDot, segs, "ks_leaves_pair_dot", partials_.as<gf>());
+    std::vector<Seg> dsegs(segs.begin() + nleaf, segs.end());
     trees_.clear();
     for (Job *jp : jobs) {
         Job &j = *jp;
@@ -446,23 +462,12 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         trees_.push_back({(uint64_t *)j.lay.pitree(j.k), (j.M >> j.k) / 2});
     }
     tree_levels();
-    segs.clear();
-    std::vector<Job *> dotted;
-    for (Job *jp : jobs) {
-        Job &j = *jp;
-        if (j.stage != Job::Open || j.k >= j.n) continue;
-        const int i = j.n - 1 - j.k;
-        segs.push_back(make_seg(j.half_eq, nullptr, j.poly_cur, (size_t)1 << i, 0));
-        dotted.push_back(jp);
-    }
-    ensure(partials_, 16 * 64 * m + 16, st_);
-    seg(SegKernel::Dot, segs, "ks_dot", partials_.as<gf>(), 64);
     std::vector<Seg> rsegs;
     for (size_t a = 0; a < dotted.size(); a++) {
         const Job &j = *dotted[a];
         const size_t half = (j.M >> j.k) / 2;
         const uint64_t *root = j.lay.pitree(j.k) + 4 * (2 * half - 2);
-        rsegs.push_back(fs_round_seg(root, j.fs, j.k, segs[a].blk0, segs[a].nblk));
+        rsegs.push_back(fs_round_seg(root, j.fs, j.k, (uint32_t)(64 * a), dsegs[a].nblk));
     }
     seg(SegKernel::Round, rsegs, "ks_round", partials_.as<gf>());
     flush();

@@ -815,19 +815,6 @@ __global__ __launch_bounds__(KB) void ks_leaves_elem(const Seg *segs, int nseg) 
     }
 }
 
-// arg = half
-__global__ __launch_bounds__(KB) void ks_leaves_pair(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
-    const gf *pi = (const gf *)g.in;
-    uint64_t *out = (uint64_t *)g.out;
-    const size_t half = g.arg;
-    SEG_LOOP(g, j) {
-        gf a = gf_ld(pi + j), b = gf_ld(pi + j + half);
-        uint64_t m[4] = {a.lo, a.hi, b.lo, b.hi}, d[4];
-        kk::keccak256_words<4>(m, d);
-        st_digest(out + 4 * j, d);
-    }
-}
 
 __global__ __launch_bounds__(KB) void ks_nodes(const Seg *segs, int nseg) {
     const Seg &g = my_seg(segs, nseg);
@@ -853,9 +840,26 @@ __global__ __launch_bounds__(kSubThreads) void ks_subtree(const Seg *segs, int n
 }
 
 // in = halved eq table, aux = poly; partial[blockIdx.x] = block sum
-__global__ __launch_bounds__(CB) void ks_dot(const Seg *segs, int nseg, gf *__restrict__ partial) {
+// A round's pi-tree leaves (segments with an output: leaf j = Keccak-256 of
+// pi[j] || pi[j + arg]) and its inner products (segments without one: in =
+// halved eq table, aux = polynomial, count elements; block b of the segment
+// leaves its sum in partial[x.lo + b]) in one launch, so the small inner
+// products run beside the leaf hashing instead of as a launch of their own.
+__global__ __launch_bounds__(KB) void ks_leaves_pair_dot(const Seg *segs, int nseg, gf *__restrict__ partial) {
     const Seg &g = my_seg(segs, nseg);
-    __shared__ gf red[CB / 64];
+    if (g.out) {
+        const gf *pi = (const gf *)g.in;
+        uint64_t *out = (uint64_t *)g.out;
+        const size_t half = g.arg;
+        SEG_LOOP(g, j) {
+            gf a = gf_ld(pi + j), b = gf_ld(pi + j + half);
+            uint64_t m[4] = {a.lo, a.hi, b.lo, b.hi}, d[4];
+            kk::keccak256_words<4>(m, d);
+            st_digest(out + 4 * j, d);
+        }
+        return;
+    }
+    __shared__ gf red[KB / 64];
     const gf *a = (const gf *)g.in, *b = (const gf *)g.aux;
     gf acc = gf_zero();
     SEG_LOOP(g, k) acc = gf_add(acc, gf_mul_reg(gf_ld(a + k), gf_ld(b + k)));
@@ -863,31 +867,16 @@ __global__ __launch_bounds__(CB) void ks_dot(const Seg *segs, int nseg, gf *__re
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        gf s = gf_zero();
-        for (int w = 0; w < CB / 64; w++) s = gf_add(s, red[w]);
-        gf_st(partial + blockIdx.x, s);
+        gf sum = gf_zero();
+        for (int w = 0; w < KB / 64; w++) sum = gf_add(sum, red[w]);
+        gf_st(partial + g.x.lo + (blockIdx.x - g.blk0), sum);
     }
 }
+
 
 // the segment's challenge: at device address arg when arg is set (written
 // there by the round's transcript kernel), else the value x
 __device__ __forceinline__ gf seg_challenge(const Seg &g) { return g.arg ? gf_ld((const gf *)g.arg) : g.x; }
-
-// in = pi (2*count), out = next pi (count), aux = level twiddles, challenge
-// per seg_challenge
-__global__ __launch_bounds__(CB) void ks_fold(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
-    const gf *w = (const gf *)g.in, *ft = (const gf *)g.aux;
-    gf *out = (gf *)g.out;
-    const size_t mid = g.count;
-    const gf_pre pr = gf_prepare(seg_challenge(g));
-    SEG_LOOP(g, j) {
-        gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
-        hi = gf_add(hi, lo);
-        lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
-        gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), pr)));
-    }
-}
 
 // in = table (2*count), out = fixed table (count), challenge per
 // seg_challenge
@@ -903,9 +892,10 @@ __global__ __launch_bounds__(CB) void ks_fix(const Seg *segs, int nseg) {
     }
 }
 
-// ks_fold and ks_fix segments in one launch: a segment with a twiddle table
-// (aux) folds, one without fixes; a block serves one segment, so the two
-// kinds share the device instead of running one after the other
+// fold (binary_reed_solomon.rs:88-96: in = pi, out = next pi, aux = level
+// twiddles) and fix segments (as ks_fix) in one launch: a segment with a
+// twiddle table (aux) folds, one without fixes.  A block serves one segment,
+// so the two kinds share the device instead of running one after the other.
 __global__ __launch_bounds__(CB) void ks_fold_fix(const Seg *segs, int nseg) {
     const Seg &g = my_seg(segs, nseg);
     if (g.aux) {
@@ -1371,11 +1361,9 @@ hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int ns
     case SegKernel::NttMulti2: ks_ntt_multi<2><<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
     case SegKernel::NttMulti3: ks_ntt_multi<3><<<nblocks, 256, 0, st>>>(d_segs, nseg); break;
     case SegKernel::LeavesElem: ks_leaves_elem<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
-    case SegKernel::LeavesPair: ks_leaves_pair<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
+    case SegKernel::LeavesPairDot: ks_leaves_pair_dot<<<nblocks, KB, 0, st>>>(d_segs, nseg, partial); break;
     case SegKernel::Nodes: ks_nodes<<<nblocks, KB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Subtree: ks_subtree<<<nblocks, kSubThreads, 0, st>>>(d_segs, nseg); break;
-    case SegKernel::Dot: ks_dot<<<nblocks, CB, 0, st>>>(d_segs, nseg, partial); break;
-    case SegKernel::Fold: ks_fold<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Fix: ks_fix<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::FoldFix: ks_fold_fix<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
     case SegKernel::Outer: ks_outer<<<nblocks, CB, 0, st>>>(d_segs, nseg); break;
@@ -1390,7 +1378,7 @@ int seg_block_size(SegKernel which) {
     case SegKernel::NttMulti2:
     case SegKernel::NttMulti3: return 256;
     case SegKernel::LeavesElem:
-    case SegKernel::LeavesPair:
+    case SegKernel::LeavesPairDot:
     case SegKernel::Nodes: return KB;
     case SegKernel::Subtree: return kSubThreads;
     case SegKernel::Round: return 64;

@@ -81,8 +81,8 @@ __host__ __device__ inline size_t prod_polys_off(uint32_t n) { return sizeof(Pro
 __host__ __device__ inline size_t prod_chal_off(uint32_t n) { return prod_polys_off(n) + 48 * (size_t)n; }
 __host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_off(n) + 16 * (size_t)n; }
 
-enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPair, Nodes, Subtree, Dot, Fold, Fix, Outer,
-                       Round, FoldFix };
+enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPairDot, Nodes, Subtree, Fix, FoldFix,
+                       Outer, Round };
 
 namespace launch {
 
@@ -100,7 +100,7 @@ struct Shape {
 };
 
 // segmented launch over nseg segments occupying nblocks blocks in total;
-// `partial` receives one element per block for SegKernel::Dot and holds the
+// `partial` receives the inner-product sums of SegKernel::LeavesPairDot and holds the
 // sums SegKernel::Round reduces (one block per segment)
 hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int nseg, uint32_t nblocks, gf *partial);
 int seg_block_size(SegKernel which);
