@@ -190,6 +190,37 @@ def mix_position(jobs, seconds: float, rates) -> dict:
     return out
 
 
+def quadratic_sumcheck(bfd, ctx, torch, reps: int) -> dict:
+    """prove_sumcheck of a Quadratic (the reference's general sumcheck
+    function, quadratic.rs) over 8 tables of 2^20 evaluations with 8 random
+    pairs and scalars (numpy seed 5), tables resident, fresh challenger."""
+    n, m, npairs = 20, 8, 8
+    rng = np.random.default_rng(5)
+    polys = bfd.random_vec(50, m << n).reshape(m, 1 << n, 2)
+    scalars = bfd.random_vec(51, npairs)
+    pairs = [(scalars[k], int(rng.integers(0, m)), int(rng.integers(0, m))) for k in range(npairs)]
+    f = bfd.Quadratic(polys, pairs)
+    dP = torch.from_numpy(np.ascontiguousarray(polys).view(np.int64).reshape(-1)).to("cuda:0")
+    torch.cuda.synchronize()
+
+    def run():
+        return bfd.prove_sumcheck_quadratic(f, bfd.GenericChallenger.keccak256(), ctx, device_ptr=dP.data_ptr(),
+                                            shape=(m, 1 << n))
+
+    ref = run()
+    ts = []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        out = run()
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+    assert all((np.asarray(a) == np.asarray(b)).all() for a, b in zip(out, ref)), "quadratic sumcheck not deterministic"
+    t = statistics.median(ts)
+    return {"workload": f"{m} tables of 2^{n}, {npairs} pairs, tables resident, fresh challenger", "ms": 1e3 * t,
+            "table_evals_per_s": m * (1 << n) / t}
+
+
 def batched_vs_plain(bfd, ctx, torch, reps: int, tune_reps: int = 0, rates=None) -> dict:
     """Config 4: 64 polynomials, log2 sizes uniform in {12..22} (numpy seed 0),
     rate 1/2, evaluations and points seeded per polynomial, resident in HBM.
@@ -597,6 +628,7 @@ def main() -> None:
     if not args.no_ref_shapes:
         result["reference_shapes"] = reference_shapes(bfd, ctx, torch, 3)
     result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 5)
+    result["quadratic_sumcheck"] = quadratic_sumcheck(bfd, ctx, torch, 5)
     if not args.no_batch:
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps,
                                              0 if args.no_tune else args.tune_reps, (r_k, r_g))

@@ -194,6 +194,12 @@ int bfd_sumcheck_quadratic(bfd_ctx *ctx, uint32_t n, uint32_t m, const uint8_t *
                            const uint8_t *scalars, const uint32_t *pair_idx, bfd_challenger *challenger,
                            uint8_t claim[16], uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals,
                            uint8_t subclaim[16]);
+/* Same, with the m tables already in device memory (table-major, m x 2^n
+ * elements); they are copied into the working tables, not modified. */
+int bfd_sumcheck_quadratic_device(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *d_polys, uint32_t npairs,
+                                  const uint8_t *scalars, const uint32_t *pair_idx, bfd_challenger *challenger,
+                                  uint8_t claim[16], uint8_t *round_polys, uint8_t *challenges,
+                                  uint8_t *final_evals, uint8_t subclaim[16]);
 
 /* ---- RandomFoldableCode<Polyval> for the binary RS code (basefold/code.rs:
  * 16-84, code/binary_reed_solomon.rs:38-111) on a code's device tables.

@@ -136,6 +136,9 @@ def lib():
         L.bfd_sumcheck_quadratic.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, u8p, ctypes.c_uint32,
                                              u8p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p, u8p, u8p, u8p,
                                              u8p, u8p]
+        L.bfd_sumcheck_quadratic_device.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p,
+                                                    ctypes.c_uint32, u8p, ctypes.POINTER(ctypes.c_uint32),
+                                                    ctypes.c_void_p, u8p, u8p, u8p, u8p, u8p]
         L.bfd_verify_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, u8p,
                                        ctypes.POINTER(u8p), ctypes.POINTER(u8p), ctypes.POINTER(u8p),
                                        ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_void_p),
@@ -719,15 +722,22 @@ class Quadratic:
     pairs: Sequence                         # [(scalar (2,) uint64, a, b), ...]
 
 
-def prove_sumcheck_quadratic(f: Quadratic, challenger: GenericChallenger, ctx: Optional[Context] = None):
+def prove_sumcheck_quadratic(f: Quadratic, challenger: GenericChallenger, ctx: Optional[Context] = None,
+                             device_ptr: Optional[int] = None, shape: Optional[Tuple[int, int]] = None):
     """prove_sumcheck (sumcheck/src/lib.rs:56-103) of a Quadratic function; the
     claim is the function's sum (compute_sum).
 
+    With device_ptr the tables are read from device memory (table-major,
+    shape = (m, 2^n)) and f.polys is ignored; f.pairs still gives the pairs.
     Returns (claim, compressed round polys (n, 2, 2), challenges (n, 2),
     final evaluations of every table (m, 2), subclaim)."""
     ctx = ctx or default_context()
-    P = np.ascontiguousarray(f.polys, dtype=np.uint64)
-    m, size = P.shape[0], P.shape[1]
+    if device_ptr is not None:
+        m, size = shape
+        P = None
+    else:
+        P = np.ascontiguousarray(f.polys, dtype=np.uint64)
+        m, size = P.shape[0], P.shape[1]
     n = size.bit_length() - 1
     if size != 1 << n:
         raise BfdError(BFD_ERR_ARG, "tables must hold 2^n evaluations")
@@ -738,9 +748,13 @@ def prove_sumcheck_quadratic(f: Quadratic, challenger: GenericChallenger, ctx: O
     chal = np.zeros((max(1, n), 2), dtype=np.uint64)
     fin = np.zeros((m, 2), dtype=np.uint64)
     sub = np.zeros((1, 2), dtype=np.uint64)
-    _check(lib().bfd_sumcheck_quadratic(ctx.h, n, m, _p(P.reshape(-1, 2)), len(sc), _p(sc),
-                                        idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), challenger.h,
-                                        _p(claim), _p(polys), _p(chal), _p(fin), _p(sub)))
+    ip = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    if device_ptr is not None:
+        _check(lib().bfd_sumcheck_quadratic_device(ctx.h, n, m, ctypes.c_void_p(device_ptr), len(sc), _p(sc), ip,
+                                                   challenger.h, _p(claim), _p(polys), _p(chal), _p(fin), _p(sub)))
+    else:
+        _check(lib().bfd_sumcheck_quadratic(ctx.h, n, m, _p(P.reshape(-1, 2)), len(sc), _p(sc), ip, challenger.h,
+                                            _p(claim), _p(polys), _p(chal), _p(fin), _p(sub)))
     return claim, polys[:2 * n].reshape(n, 2, 2), chal[:n], fin, sub
 
 

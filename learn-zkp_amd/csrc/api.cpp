@@ -597,88 +597,133 @@ int bfd_sumcheck_product_device(bfd_ctx *ctx, uint32_t n, const void *d_f, const
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// prove_sumcheck for a Quadratic (quadratic.rs:84-136) with its transcript on
+// the device: per round one launch for the pair sums (whose last block runs
+// the round's Fiat-Shamir step) and one for the fix of every table with the
+// challenge that step left in device memory; one upload of the transcript
+// image before the rounds and one read-back after them.
+void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys, bool on_device, uint32_t npairs,
+                        const uint8_t *scalars, const uint32_t *pair_idx, Challenger &ch, uint8_t claim_out[16],
+                        uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals, uint8_t subclaim_out[16]) {
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    set_device(ctx);
+    hipStream_t st = ctx->stream;
+    const size_t N = (size_t)1 << n;
+    std::vector<gf> sc(npairs);
+    for (uint32_t k = 0; k < npairs; k++) sc[k] = load_elem(scalars + 16 * k);
+    if (n == 0) {  // prove_sumcheck returns before observing (sumcheck/src/lib.rs:63-65)
+        std::vector<uint8_t> host(16 * (size_t)m);
+        if (on_device) {
+            hip_check(hipMemcpyAsync(host.data(), polys, 16 * (size_t)m, hipMemcpyDeviceToHost, st), "copy tables");
+            hip_check(hipStreamSynchronize(st), "sync");
+        } else {
+            memcpy(host.data(), polys, 16 * (size_t)m);
+        }
+        gf cl = gf_zero();
+        for (uint32_t k = 0; k < npairs; k++)
+            cl = gf_add(cl, hmul(sc[k], hmul(load_elem(host.data() + 16 * pair_idx[2 * k]),
+                                             load_elem(host.data() + 16 * pair_idx[2 * k + 1]))));
+        store_elem(claim_out, cl);
+        memcpy(final_evals, host.data(), 16 * (size_t)m);
+        store_elem(subclaim_out, cl);
+        return;
+    }
+    if (ch.output.size() > 32) fail(BFD_ERR_ARG, "transcript state too large");
+    DevBuf dP = ctx_buf(ctx, 16 * N * m), dpairs = ctx_buf(ctx, 8 * (size_t)npairs);
+    hip_check(hipMemcpyAsync(dP.p, polys, 16 * N * m, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
+              "copy tables");
+    hip_check(hipMemcpyAsync(dpairs.p, pair_idx, 8 * (size_t)npairs, hipMemcpyHostToDevice, st), "copy pairs");
+    // blocks per pair: enough for the device at the first round
+    const unsigned max_blocks = std::max(1u, std::min(512u, 4096u / npairs));
+    DevBuf partials = ctx_buf(ctx, 48 * (size_t)max_blocks * npairs);
+    // transcript image: QuadFs | scalars | polys | chal | input | n counters
+    const uint32_t in_cap = (uint32_t)(((ch.input.size() > 32 ? ch.input.size() : 32) + 64 + 7) & ~(size_t)7);
+    const size_t in_off = quad_input_off(npairs, n), cnt_off = (in_off + in_cap + 7) & ~(size_t)7;
+    const size_t img = cnt_off + 4 * (size_t)n;
+    DevBuf fs = ctx_buf(ctx, img);
+    ctx->pinned.ensure(img + 64);
+    uint8_t *pin = (uint8_t *)ctx->pinned.p;
+    memset(pin, 0, img);
+    QuadFs h;
+    memset(&h, 0, sizeof h);
+    h.in_len = (uint32_t)ch.input.size();
+    h.out_len = (uint32_t)ch.output.size();
+    h.in_cap = in_cap;
+    h.n = n;
+    h.npairs = npairs;
+    if (!ch.output.empty()) memcpy(h.out, ch.output.data(), ch.output.size());
+    memcpy(pin, &h, sizeof h);
+    memcpy(pin + quad_scalars_off(), scalars, 16 * (size_t)npairs);
+    if (!ch.input.empty()) memcpy(pin + in_off, ch.input.data(), ch.input.size());
+    hip_check(hipMemcpyAsync(fs.p, pin, img, hipMemcpyHostToDevice, st), "copy transcript");
+    QuadFs *dfs = (QuadFs *)fs.p;
+    unsigned *counters = (unsigned *)(fs.as<uint8_t>() + cnt_off);
+    const gf *x_dev = (const gf *)(fs.as<uint8_t>() + offsetof(QuadFs, x));
+    for (uint32_t k = 0; k < n; k++) {
+        const size_t half = (size_t)1 << (n - 1 - k);
+        ctx->timer.run(st, "k_quad_step", [&] {
+            return launch::quad_step(st, dP.as<gf>(), N, dpairs.as<uint32_t>(), npairs, half, (int)k, dfs,
+                                     partials.as<gf>(), max_blocks, counters + k);
+        });
+        ctx->timer.run(st, "k_fix_multi", [&] { return launch::fix_multi_dev(st, dP.as<gf>(), N, m, half, x_dev); });
+    }
+    const size_t rb = in_off + 32;
+    hip_check(hipMemcpyAsync(pin, fs.p, rb, hipMemcpyDeviceToHost, st), "copy transcript back");
+    // the m final evaluations P_p[0]
+    hip_check(hipMemcpy2DAsync(final_evals, 16, dP.p, 16 * N, 16, m, hipMemcpyDeviceToHost, st), "copy finals");
+    hip_check(hipStreamSynchronize(st), "sumcheck");
+    ctx->timer.collect();
+    memcpy(&h, pin, sizeof h);
+    if (h.n != n || h.in_len != 32 || h.out_len != 16) fail(BFD_ERR_HIP, "device transcript state is corrupt");
+    store_elem(claim_out, h.claim);
+    memcpy(round_polys, pin + quad_polys_off(npairs), 32 * (size_t)n);
+    memcpy(challenges, pin + quad_chal_off(npairs, n), 16 * (size_t)n);
+    ch.input.assign(pin + in_off, pin + in_off + 32);
+    ch.output.assign(h.out, h.out + h.out_len);
+    store_elem(subclaim_out, h.sub);
+}
+
+void quadratic_args(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys, uint32_t npairs, const uint8_t *scalars,
+                    const uint32_t *pair_idx, bfd_challenger *challenger, uint8_t *claim_out, uint8_t *round_polys,
+                    uint8_t *challenges, uint8_t *final_evals, uint8_t *subclaim_out) {
+    if (!ctx || !polys || !scalars || !pair_idx || !challenger || !claim_out || !final_evals || !subclaim_out ||
+        (n && (!round_polys || !challenges)))
+        fail(BFD_ERR_ARG, "null argument");
+    if (n > 32) fail(BFD_ERR_ARG, "too many variables");
+    if (m == 0 || m > 65535 || npairs == 0 || npairs > 65535) fail(BFD_ERR_ARG, "need 1..65535 tables and pairs");
+    for (uint32_t k = 0; k < 2 * npairs; k++)
+        if (pair_idx[k] >= m) fail(BFD_ERR_ARG, "pair index out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
 int bfd_sumcheck_quadratic(bfd_ctx *ctx, uint32_t n, uint32_t m, const uint8_t *polys, uint32_t npairs,
                            const uint8_t *scalars, const uint32_t *pair_idx, bfd_challenger *challenger,
                            uint8_t claim_out[16], uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals,
                            uint8_t subclaim_out[16]) {
     return guarded([&] {
-        if (!ctx || !polys || !scalars || !pair_idx || !challenger || !claim_out || !final_evals || !subclaim_out ||
-            (n && (!round_polys || !challenges)))
-            fail(BFD_ERR_ARG, "null argument");
-        if (n > 32) fail(BFD_ERR_ARG, "too many variables");
-        if (m == 0 || m > 65535 || npairs == 0 || npairs > 65535) fail(BFD_ERR_ARG, "need 1..65535 tables and pairs");
-        for (uint32_t k = 0; k < 2 * npairs; k++)
-            if (pair_idx[k] >= m) fail(BFD_ERR_ARG, "pair index out of range");
-        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-        set_device(ctx);
-        hipStream_t st = ctx->stream;
-        Challenger &ch = challenger->c;
-        const size_t N = (size_t)1 << n;
-        std::vector<gf> sc(npairs);
-        for (uint32_t k = 0; k < npairs; k++) sc[k] = load_elem(scalars + 16 * k);
-        if (n == 0) {  // prove_sumcheck returns before observing (sumcheck/src/lib.rs:63-65)
-            gf cl = gf_zero();
-            for (uint32_t k = 0; k < npairs; k++)
-                cl = gf_add(cl, hmul(sc[k], hmul(load_elem(polys + 16 * pair_idx[2 * k]),
-                                                 load_elem(polys + 16 * pair_idx[2 * k + 1]))));
-            store_elem(claim_out, cl);
-            memcpy(final_evals, polys, 16 * (size_t)m);
-            store_elem(subclaim_out, cl);
-            return;
-        }
-        DevBuf dP = ctx_buf(ctx, 16 * N * m), dpairs = ctx_buf(ctx, 8 * (size_t)npairs);
-        hip_check(hipMemcpyAsync(dP.p, polys, 16 * N * m, hipMemcpyHostToDevice, st), "copy tables");
-        hip_check(hipMemcpyAsync(dpairs.p, pair_idx, 8 * (size_t)npairs, hipMemcpyHostToDevice, st), "copy pairs");
-        // blocks per pair: enough for the device at the first round, bounded read-back
-        const unsigned max_blocks = std::max(1u, std::min(512u, 4096u / npairs));
-        DevBuf partials = ctx_buf(ctx, 48 * (size_t)max_blocks * npairs);
-        ctx->pinned.ensure(48 * (size_t)max_blocks * npairs + 64);
-        uint8_t *pin = (uint8_t *)ctx->pinned.p;
-        gf sub = gf_zero();
-        for (int i = (int)n - 1; i >= 0; i--) {
-            const int k = (int)n - 1 - i;
-            const size_t h = (size_t)1 << i;
-            unsigned nb = 0;
-            ctx->timer.run(st, "k_quad_sums", [&] {
-                return launch::quad_sums(st, dP.as<gf>(), N, dpairs.as<uint32_t>(), npairs, h, partials.as<gf>(),
-                                         max_blocks, &nb);
-            });
-            hip_check(hipMemcpyAsync(pin, partials.p, 48 * (size_t)nb * npairs, hipMemcpyDeviceToHost, st),
-                      "copy partials");
-            hip_check(hipStreamSynchronize(st), "round sync");
-            gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
-            for (uint32_t t = 0; t < npairs; t++) {
-                gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
-                const uint8_t *q = pin + 48 * (size_t)nb * t;
-                for (unsigned u = 0; u < nb; u++) {
-                    A0 = gf_add(A0, load_elem(q + 48 * u));
-                    A1 = gf_add(A1, load_elem(q + 48 * u + 16));
-                    A2 = gf_add(A2, load_elem(q + 48 * u + 32));
-                }
-                c0 = gf_add(c0, hmul(sc[t], A0));
-                c2 = gf_add(c2, hmul(sc[t], A2));
-                both = gf_add(both, hmul(sc[t], gf_add(A0, A1)));  // p(0) + p(1)
-            }
-            if (k == 0) {  // claim = compute_sum (quadratic.rs:86-99), observed first (lib.rs:67)
-                sub = both;
-                store_elem(claim_out, sub);
-                ch.observe_elem(sub);
-            }
-            const gf c1 = gf_add(sub, c2);  // decompress_round_poly (quadratic.rs:70-80), 2 c0 = 0
-            store_elem(round_polys + 32 * k, c0);
-            store_elem(round_polys + 32 * k + 16, c2);
-            ch.observe_elem(c0);  // compress_round_poly = [c0, c2] (quadratic.rs:62-64)
-            ch.observe_elem(c2);
-            const gf x = ch.sample();
-            store_elem(challenges + 16 * k, x);
-            ctx->timer.run(st, "k_fix_multi", [&] { return launch::fix_multi(st, dP.as<gf>(), N, m, h, x); });
-            sub = gf_add(c0, hmul(x, gf_add(c1, hmul(x, c2))));
-        }
-        // the m final evaluations P_p[0]
-        hip_check(hipMemcpy2DAsync(final_evals, 16, dP.p, 16 * N, 16, m, hipMemcpyDeviceToHost, st), "copy finals");
-        hip_check(hipStreamSynchronize(st), "final sync");
-        ctx->timer.collect();
-        store_elem(subclaim_out, sub);
+        quadratic_args(ctx, n, m, polys, npairs, scalars, pair_idx, challenger, claim_out, round_polys, challenges,
+                       final_evals, subclaim_out);
+        quadratic_sumcheck(ctx, n, m, polys, false, npairs, scalars, pair_idx, challenger->c, claim_out, round_polys,
+                           challenges, final_evals, subclaim_out);
+    });
+}
+
+int bfd_sumcheck_quadratic_device(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *d_polys, uint32_t npairs,
+                                  const uint8_t *scalars, const uint32_t *pair_idx, bfd_challenger *challenger,
+                                  uint8_t claim_out[16], uint8_t *round_polys, uint8_t *challenges,
+                                  uint8_t *final_evals, uint8_t subclaim_out[16]) {
+    return guarded([&] {
+        quadratic_args(ctx, n, m, d_polys, npairs, scalars, pair_idx, challenger, claim_out, round_polys, challenges,
+                       final_evals, subclaim_out);
+        quadratic_sumcheck(ctx, n, m, d_polys, true, npairs, scalars, pair_idx, challenger->c, claim_out, round_polys,
+                           challenges, final_evals, subclaim_out);
     });
 }
 

@@ -622,6 +622,121 @@ __global__ __launch_bounds__(CB) void k_quad_sums(const gf *__restrict__ P, size
 }
 
 // fix_last_var of table blockIdx.y (2*half entries -> half), in place
+// One round's transcript step of a Quadratic sumcheck, run by the first wave
+// of the block that finishes the round: per pair the three sums A0 = sum
+// f0 g0, A1 = sum f1 g1, A2 = sum (f0 + f1)(g0 + g1) over the round's
+// blocks, combined with the pair scalars into c0 = sum s A0, c2 = sum s A2
+// and the claim sum s (A0 + A1) (quadratic.rs:100-122); then observe
+// [claim,] c0, c2 (the compressed polynomial, quadratic.rs:62-64), sample x
+// and advance the subclaim with c1 = subclaim + c2 (characteristic 2,
+// quadratic.rs:70-80).  Pairs are spread over the wave's lanes.
+__device__ void quad_fs_step(QuadFs *st, int k, const gf *partial, unsigned nb) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = st->n, np = st->npairs;
+    uint8_t *base = (uint8_t *)st;
+    const gf *sc = (const gf *)(base + quad_scalars_off());
+    gf *polys = (gf *)(base + quad_polys_off(np)), *chal = (gf *)(base + quad_chal_off(np, n));
+    uint8_t *in = base + quad_input_off(np, n);
+    uint64_t *in64 = reinterpret_cast<uint64_t *>(in);
+    gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
+    for (uint32_t t = lane; t < np; t += 64) {
+        gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
+        const gf *q = partial + 3 * (size_t)nb * t;
+        for (unsigned u = 0; u < nb; u++) {
+            A0 = gf_add(A0, gf_ld(q + 3 * u));
+            A1 = gf_add(A1, gf_ld(q + 3 * u + 1));
+            A2 = gf_add(A2, gf_ld(q + 3 * u + 2));
+        }
+        const gf_pre s = gf_prepare(gf_ld(sc + t));
+        c0 = gf_add(c0, gf_mul_pre(A0, s));
+        c2 = gf_add(c2, gf_mul_pre(A2, s));
+        both = gf_add(both, gf_mul_pre(gf_add(A0, A1), s));
+    }
+    c0 = wave_xor(c0);
+    c2 = wave_xor(c2);
+    both = wave_xor(both);
+    const gf sub = k == 0 ? both : st->sub;
+    const gf c1 = gf_add(sub, c2);
+    uint64_t d[4];
+    if (k > 0 && st->in_len == 32) {
+        const uint64_t m[8] = {in64[0], in64[1], in64[2], in64[3], c0.lo, c0.hi, c2.lo, c2.hi};
+        kk::keccak256_words<8>(m, d);
+    } else {
+        if (lane == 0) {
+            uint32_t len = st->in_len;
+            const auto put = [&](uint64_t v) {
+#pragma unroll 1
+                for (int t = 0; t < 8; t++) in[len + t] = (uint8_t)(v >> (8 * t));
+                len += 8;
+            };
+            if (k == 0) {  // the claim is observed first (sumcheck/src/lib.rs:67)
+                put(both.lo);
+                put(both.hi);
+            }
+            put(c0.lo);
+            put(c0.hi);
+            put(c2.lo);
+            put(c2.hi);
+            kk::keccak256_bytes(in, len, d);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) d[q] = (uint64_t)__shfl((long long)d[q], 0);
+    }
+    // the challenge is the last 16 digest bytes, last byte first
+    const gf x{__builtin_bswap64(d[3]), __builtin_bswap64(d[2])};
+    const gf sub_next = gf_add(c0, gf_mul_reg(x, gf_add(c1, gf_mul_reg(x, c2))));
+    if (lane != 0) return;
+    uint64_t *out64 = reinterpret_cast<uint64_t *>(st->out);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        in64[q] = d[q];
+        out64[q] = d[q];
+    }
+    st->in_len = 32;
+    st->out_len = 16;
+    if (k == 0) st->claim = both;
+    st->sub = sub_next;
+    st->x = x;
+    polys[2 * k] = c0;
+    polys[2 * k + 1] = c2;
+    chal[k] = x;
+}
+
+// k_quad_sums with the round's transcript step in the last block to finish
+__global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size_t stride,
+                                                  const uint32_t *__restrict__ pairs, size_t h, int k, QuadFs *fs,
+                                                  gf *partial, unsigned *counter) {
+    const size_t a = pairs[2 * blockIdx.y], b = pairs[2 * blockIdx.y + 1];
+    const gf *f = P + a * stride, *g = P + b * stride;
+    gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < h; j += (size_t)gridDim.x * blockDim.x) {
+        const gf f0 = gf_ld(f + j), f1 = gf_ld(f + j + h), g0 = gf_ld(g + j), g1 = gf_ld(g + j + h);
+        a0 = gf_add(a0, gf_mul_reg(f0, g0));
+        a1 = gf_add(a1, gf_mul_reg(f1, g1));
+        a2 = gf_add(a2, gf_mul_reg(gf_add(f0, f1), gf_add(g0, g1)));
+    }
+    block_xor3(a0, a1, a2, partial + 3 * (size_t)gridDim.x * blockIdx.y);
+    __shared__ unsigned last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (!last || threadIdx.x >= 64) return;
+    __threadfence();
+    quad_fs_step(fs, k, partial, gridDim.x);
+}
+
+__global__ __launch_bounds__(CB) void k_fix_multi_dev(gf *__restrict__ P, size_t stride, size_t half,
+                                                      const gf *__restrict__ xp) {
+    gf *f = P + blockIdx.y * stride;
+    const gf_pre px = gf_prepare(gf_ld(xp));
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
+         j += (size_t)gridDim.x * blockDim.x) {
+        const gf lo = gf_ld(f + j), hi = gf_ld(f + j + half);
+        gf_st(f + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+    }
+}
+
 __global__ __launch_bounds__(CB) void k_fix_multi(gf *__restrict__ P, size_t stride, size_t half, gf x) {
     gf *f = P + blockIdx.y * stride;
     const gf_pre px = gf_prepare(x);
@@ -1328,6 +1443,18 @@ hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t 
 hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, gf x) {
     if (m == 0 || m > 65535) return hipErrorInvalidValue;
     k_fix_multi<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(P, stride, half, x);
+    return hipGetLastError();
+}
+hipError_t fix_multi_dev(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, const gf *x) {
+    if (m == 0 || m > 65535) return hipErrorInvalidValue;
+    k_fix_multi_dev<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(P, stride, half, x);
+    return hipGetLastError();
+}
+hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
+                     int k, QuadFs *fs, gf *partial, unsigned max_blocks, unsigned *counter) {
+    if (npairs == 0 || npairs > 65535 || max_blocks == 0) return hipErrorInvalidValue;
+    const unsigned gx = grid_for(h, CB, max_blocks);
+    k_quad_step<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, k, fs, partial, counter);
     return hipGetLastError();
 }
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,

@@ -81,6 +81,29 @@ __host__ __device__ inline size_t prod_polys_off(uint32_t n) { return sizeof(Pro
 __host__ __device__ inline size_t prod_chal_off(uint32_t n) { return prod_polys_off(n) + 48 * (size_t)n; }
 __host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_off(n) + 16 * (size_t)n; }
 
+// Device-resident transcript of a Quadratic sumcheck driven by
+// prove_sumcheck (quadratic.rs:84-136, sumcheck/src/lib.rs:56-103): the
+// challenger's byte buffers and the running subclaim.  One region laid out as
+//   QuadFs | gf scalars[npairs] | gf polys[2n] | gf chal[n] | uint8_t input[in_cap]
+// polys and chal receive each round's compressed polynomial [c0, c2] and
+// challenge.
+struct QuadFs {
+    uint32_t in_len, out_len, in_cap, n;
+    uint32_t npairs, pad_[3];
+    gf sub;    // subclaim
+    gf x;      // last sampled challenge (read by the round's fix)
+    gf claim;  // sum over the cube, observed before the first round
+    uint8_t out[32];
+};
+__host__ __device__ inline size_t quad_scalars_off() { return sizeof(QuadFs); }
+__host__ __device__ inline size_t quad_polys_off(uint32_t npairs) { return sizeof(QuadFs) + 16 * (size_t)npairs; }
+__host__ __device__ inline size_t quad_chal_off(uint32_t npairs, uint32_t n) {
+    return quad_polys_off(npairs) + 32 * (size_t)n;
+}
+__host__ __device__ inline size_t quad_input_off(uint32_t npairs, uint32_t n) {
+    return quad_chal_off(npairs, n) + 16 * (size_t)n;
+}
+
 enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPairDot, Nodes, Subtree, Fix, FoldFix,
                        Outer, Round };
 
@@ -178,6 +201,15 @@ hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t 
                      gf *partial, unsigned max_blocks, unsigned *nblocks);
 // fix the top variable of m tables (stride apart) from 2*half to half entries
 hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, gf x);
+// the same with the challenge read from device memory (*x)
+hipError_t fix_multi_dev(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, const gf *x);
+// Round k of a Quadratic sumcheck with its transcript step on the device:
+// the three sums of every pair over halves of h entries (as quad_sums), and
+// in the block that finishes last (counted in *counter, zero at launch) the
+// round's Fiat-Shamir step into the QuadFs region.  partial holds
+// 3 * npairs * max_blocks elements.
+hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
+                     int k, QuadFs *fs, gf *partial, unsigned max_blocks, unsigned *counter);
 // verifier kernels: fold chains (thread per proof x query) and Merkle paths
 // (thread per proof x query x path, d + 1 paths)
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,

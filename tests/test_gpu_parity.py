@@ -205,6 +205,49 @@ def test_quadratic_sumcheck_large_invariants(bfd, oracle, ctx):
     assert (sub == expect).all()
 
 
+def test_quadratic_sumcheck_prefixes_and_device_inputs(bfd, oracle, ctx):
+    """The Quadratic sumcheck's device transcript starts from any challenger
+    state (caller prefixes of 5, 32 and 200 bytes, a pending sample): its
+    challenges replay on the oracle's challenger and both end in the same
+    state; tables in device memory give the host path's outputs and stay
+    unmodified."""
+    import torch
+    n = 9
+    f = quad_case(bfd, n, 990)
+    fresh = oracle.quadratic_sumcheck(f.polys, [p[0] for p in f.pairs], [(p[1], p[2]) for p in f.pairs])
+    rng = np.random.default_rng(4)
+    for plen, sample_after in [(5, False), (32, False), (200, False), (40, True)]:
+        _say(f"quadratic prefix {plen} sample={sample_after}")
+        prefix = bytes(rng.integers(0, 256, plen, dtype=np.uint8))
+        ch, och = bfd.GenericChallenger.keccak256(), oracle.Challenger()
+        ch.observe(prefix)
+        och.observe(prefix)
+        if sample_after:
+            assert (ch.sample_ext_element() == och.sample()).all()
+        claim, rps, chal, fin, sub = bfd.prove_sumcheck_quadratic(f, ch, ctx)
+        assert (claim == np.asarray(fresh[0]).reshape(-1, 2)).all()
+        och.observe(claim.tobytes())
+        for k in range(n):
+            och.observe(rps[k].tobytes())
+            assert (och.sample() == chal[k]).all(), (plen, k)
+        x = chal[::-1].copy()
+        expect = np.zeros((1, 2), np.uint64)
+        for (sc, a, b) in f.pairs:
+            fa, fb = oracle.mle_evaluate(f.polys[a], x), oracle.mle_evaluate(f.polys[b], x)
+            expect ^= oracle.mul(oracle.mul(fa, fb), np.asarray(sc).reshape(1, 2))
+        assert (sub == expect).all()
+        assert (ch.sample_ext_element() == och.sample()).all()
+    P = np.ascontiguousarray(f.polys, dtype=np.uint64)
+    dP = torch.from_numpy(P.view(np.int64).reshape(-1)).to("cuda:0")
+    torch.cuda.synchronize()
+    got = bfd.prove_sumcheck_quadratic(f, bfd.GenericChallenger.keccak256(), ctx, device_ptr=dP.data_ptr(),
+                                       shape=(P.shape[0], P.shape[1]))
+    want = bfd.prove_sumcheck_quadratic(f, bfd.GenericChallenger.keccak256(), ctx)
+    for g, w in zip(got, want):
+        assert (np.asarray(g) == np.asarray(w)).all()
+    assert (dP.cpu().numpy() == P.view(np.int64).reshape(-1)).all()
+
+
 def tamper_offsets(n, log2_c, w, q):
     """Byte offsets inside a bincode BasefoldProof (basefold.rs:49-66) for d = n rounds."""
     d, L = n, n + log2_c
