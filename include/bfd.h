@@ -110,7 +110,8 @@ int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uin
 int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
              const uint8_t *values, bfd_challenger *challenger, uint8_t **proof, size_t *proof_len);
 /* Same, writing into a caller buffer of `capacity` bytes (at least
- * bfd_proof_size_width); *proof_len receives the size written. */
+ * bfd_proof_size_width); *proof_len receives the size written.  A smaller
+ * buffer is refused (BFD_ERR_ARG) before the challenger is touched. */
 int bfd_open_into(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
                   const uint8_t *values, bfd_challenger *challenger, uint8_t *proof, size_t capacity,
                   size_t *proof_len);
@@ -132,7 +133,8 @@ uint64_t bfd_proof_size_width(uint32_t lambda, uint32_t log2_c, uint32_t d, uint
  * polynomial needs its own challenger (BFD_ERR_ARG if two entries alias).
  * Outputs count x 32 roots and count proofs: a non-NULL proofs[p] on input is
  * a caller buffer of proof_lens[p] bytes (at least bfd_proof_size), otherwise
- * the library allocates it (free with bfd_proof_free). */
+ * the library allocates it (free with bfd_proof_free).  A call with a smaller
+ * buffer is refused (BFD_ERR_ARG) before any work, every challenger as it was. */
 int bfd_commit_open_batch(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t count,
                           const uint32_t *log_sizes, const void *const *evals, int evals_on_device,
                           const uint8_t *const *points, const uint8_t *const *claims,

@@ -20,6 +20,9 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     hipStream_t st = ctx->stream;
     double t_setup = now_ms();
     const size_t w = cm->w;
+    // a caller buffer is checked before the challenger is touched
+    const size_t sz = proof_size((int)cm->b, (int)cm->n, code->q, w);
+    if (*proof && *proof_len < sz) fail(BFD_ERR_ARG, "proof buffer too small");
     // width > 1 (basefold.rs:166-183): random scalars by eq combine the
     // columns of the claims, the evaluations and the codeword rows; the rest
     // of the opening runs on the combination
@@ -163,8 +166,6 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     hip_check(hipMemcpyAsync(he, oe, 16 * ne + 32 * nd, hipMemcpyDeviceToHost, st), "copy gather");
     hip_check(hipStreamSynchronize(st), "gather sync");
     ctx->timer.collect();
-    const size_t sz = proof_size(o.b, o.d, o.q, o.w);
-    if (*proof && *proof_len < sz) fail(BFD_ERR_ARG, "proof buffer too small");
     uint8_t *dst = *proof ? *proof : (uint8_t *)malloc(sz ? sz : 1);
     if (!dst) fail(BFD_ERR_ARG, "host allocation failed");
     assemble_proof(o, he, hd, dst);
@@ -413,6 +414,7 @@ int bfd_commitment_destroy(bfd_commitment *comm) {
 int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uint8_t *values) {
     return guarded([&] {
         if (!ctx || !comm || !point || !values) fail(BFD_ERR_ARG, "null argument");
+        if (comm->ctx->device != ctx->device) fail(BFD_ERR_ARG, "the commitment lives on another device");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);
         hipStream_t st = ctx->stream;
@@ -431,6 +433,7 @@ int bfd_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, con
         if (!ctx || !code || !comm || !values || !challenger || !proof || !proof_len || (!point && comm->n))
             fail(BFD_ERR_ARG, "null argument");
         if (comm->code != code) fail(BFD_ERR_ARG, "commitment was made with another code");
+        if (comm->ctx->device != ctx->device) fail(BFD_ERR_ARG, "the commitment lives on another device");
         uint8_t *p = nullptr;
         size_t n = 0;
         do_open(ctx, code, comm, point, values, challenger->c, &p, &n);
@@ -446,6 +449,7 @@ int bfd_open_into(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm
         if (!ctx || !code || !comm || !values || !challenger || !proof || !proof_len || (!point && comm->n))
             fail(BFD_ERR_ARG, "null argument");
         if (comm->code != code) fail(BFD_ERR_ARG, "commitment was made with another code");
+        if (comm->ctx->device != ctx->device) fail(BFD_ERR_ARG, "the commitment lives on another device");
         uint8_t *p = proof;
         size_t n = capacity;
         do_open(ctx, code, comm, point, values, challenger->c, &p, &n);

@@ -64,6 +64,10 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
     }
     if (ctx->budget && ctx->resident + bytes > ctx->budget)
         fail(BFD_ERR_BUDGET, "batch exceeds the context's HBM budget");
+    // caller buffers are checked before anything is enqueued, so a refused
+    // call leaves every challenger as it was
+    for (Job *j : all)
+        if (j->proof && j->proof_cap < proof_size(j->b, j->n, j->code->q)) fail(BFD_ERR_ARG, "proof buffer too small");
     // one slab for every job, from the context's block cache: a repeated
     // batch of the same shape gets the same block back without touching the
     // allocator (taking it from the pool after plain calls have split the
@@ -89,7 +93,6 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
         std::vector<std::pair<uint8_t *, size_t>> chunks;
         for (Job *j : all) {
             const size_t sz = proof_size(j->b, j->n, j->code->q);
-            if (j->proof && j->proof_cap < sz) fail(BFD_ERR_ARG, "proof buffer too small");
             if (!j->proof) {
                 j->proof = (uint8_t *)malloc(sz ? sz : 1);
                 if (!j->proof) fail(BFD_ERR_ARG, "host allocation failed");

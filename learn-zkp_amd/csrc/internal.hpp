@@ -11,6 +11,7 @@
 #include <array>
 #include <atomic>
 #include <condition_variable>
+#include <exception>
 #include <functional>
 #include <chrono>
 #include <thread>
@@ -628,7 +629,8 @@ inline void encode(bfd_ctx *ctx, const bfd_code *code, const gf *data, gf *cw, i
 inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, bool on_device, size_t height,
                size_t width, bfd_commitment **out, uint8_t root[32], const uint8_t *point = nullptr,
                uint8_t *values = nullptr) {
-    if (!ctx || !code || !out || !evals) fail(BFD_ERR_ARG, "null argument");
+    if (!ctx || !code || !out || !evals || !root) fail(BFD_ERR_ARG, "null argument");
+    if (code->ctx->device != ctx->device) fail(BFD_ERR_ARG, "the code's twiddles live on another device");
     if (width < 1 || width > kMaxWidth) fail(BFD_ERR_ARG, "width must be 1..65535");
     if (height != ((size_t)1 << code->d)) fail(BFD_ERR_ARG, "height must equal 2^d of the code");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -966,7 +968,9 @@ inline void assemble_proof(const OpenRounds &o, const uint8_t *elems, const uint
 
 // Persistent host worker pool for per-polynomial host work (proof planning
 // and assembly, transcript set-up).  run() hands out indices through an
-// atomic counter; the calling thread works too.  One run at a time.
+// atomic counter; the calling thread works too.  One run at a time.  An
+// exception thrown by f stops the hand-out; run() waits for every worker and
+// then rethrows the first one in the calling thread.
 class HostPool {
   public:
     static HostPool &get() {
@@ -991,10 +995,15 @@ class HostPool {
             gen_++;
         }
         cv_.notify_all();
-        for (size_t i; (i = next_.fetch_add(1)) < n;) fn(i);
+        work(fn, n);
         std::unique_lock<std::mutex> lk(mu_);
         done_cv_.wait(lk, [&] { return busy_ == 0; });
         fn_ = nullptr;
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
     }
     ~HostPool() {
         {
@@ -1025,9 +1034,18 @@ class HostPool {
                 fn = fn_;
                 n = n_;
             }
-            for (size_t i; (i = next_.fetch_add(1)) < n;) (*fn)(i);
+            work(*fn, n);
             std::lock_guard<std::mutex> lk(mu_);
             if (--busy_ == 0) done_cv_.notify_all();
+        }
+    }
+    void work(const std::function<void(size_t)> &fn, size_t n) {
+        try {
+            for (size_t i; (i = next_.fetch_add(1)) < n;) fn(i);
+        } catch (...) {
+            next_.store(n);  // the other threads take no further index
+            std::lock_guard<std::mutex> lk(mu_);
+            if (!err_) err_ = std::current_exception();
         }
     }
     std::vector<std::thread> workers_;
@@ -1037,6 +1055,7 @@ class HostPool {
     size_t n_ = 0, busy_ = 0;
     uint64_t gen_ = 0;
     bool stop_ = false;
+    std::exception_ptr err_;
     std::atomic<size_t> next_{0};
 };
 

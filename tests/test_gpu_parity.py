@@ -416,6 +416,35 @@ def test_reopen_same_commitment(bfd, oracle, ctx):
         assert proof == proof_o
 
 
+def test_open_into_caller_buffer(bfd, oracle, ctx):
+    """bfd_open_into: a buffer one byte short is refused before the
+    challenger moves; a buffer of exactly bfd_proof_size gets the proof."""
+    import ctypes
+    n, log2_c = 9, 2
+    evals, point = bfd.random_vec(12, 1 << n), bfd.random_vec(13, n)
+    pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(128, log2_c, n)), ctx)
+    cd = pcs.commit(evals)
+    claim = pcs.eval(cd, point)
+    ch = bfd.GenericChallenger.keccak256()
+    ch.observe(cd.comm)
+    ch.observe_ext_slice(claim)
+    before = ch.export_state()
+    size = bfd.lib().bfd_proof_size(128, log2_c, n)
+    p = np.ascontiguousarray(bfd._elems(point))
+    v = np.ascontiguousarray(bfd._elems(claim))
+    buf = (ctypes.c_uint8 * size)()
+    plen = ctypes.c_size_t()
+    rc = bfd.lib().bfd_open_into(ctx.h, pcs.code_h, cd.handle, bfd._p(p), bfd._p(v), ch.h, buf, size - 1,
+                                 ctypes.byref(plen))
+    assert rc == bfd.BFD_ERR_ARG
+    assert ch.export_state() == before
+    bfd._check(bfd.lib().bfd_open_into(ctx.h, pcs.code_h, cd.handle, bfd._p(p), bfd._p(v), ch.h, buf, size,
+                                       ctypes.byref(plen)))
+    assert plen.value == size
+    _, _, proof_o = oracle.basefold_prove(128, log2_c, evals, point)
+    assert bytes(buf) == proof_o
+
+
 def test_product_sumcheck_bit_exact(bfd, oracle, ctx):
     for n in [1, 2, 3, 5, 8, 12]:
         _say(f"product sumcheck n={n}")
