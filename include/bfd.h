@@ -248,7 +248,8 @@ int bfd_sc_destroy(bfd_sc *h);
  * `code`, each over a commitment of `width` columns: roots (count x 32
  * bytes), points[p] (d elements), values[p] (width elements), proofs[p] of
  * proof_lens[p] bytes, challengers[p] in the state open started from (the
- * harness prefix observed; advanced in place).  The proof bytes are decoded
+ * harness prefix observed; advanced in place; BFD_ERR_ARG if two entries
+ * alias).  The proof bytes are decoded
  * and the transcripts replayed on the host; every query's fold chain and
  * Merkle paths of all proofs are checked on the device in two launches.
  * results[p]: 0 accepted, 1 InvalidFinalPoly, 2 InvalidQuery, 3 Mmcs (a

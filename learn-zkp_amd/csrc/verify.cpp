@@ -13,6 +13,8 @@
 // sequence.  The results map back to the reference's first error in query
 // order: a main-tree or pi-tree path (Mmcs), then the folded value
 // (InvalidQuery).
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace {
@@ -91,8 +93,16 @@ int bfd_verify_batch(bfd_ctx *ctx, const bfd_code *code, size_t width, size_t co
         const int b = (int)code->log2_c, d = (int)code->d, L = b + d;
         const size_t q = code->q, w = width;
         std::vector<Parsed> parsed(count);
+        if (code->ctx->device != ctx->device) fail(BFD_ERR_ARG, "the code's twiddles live on another device");
         for (size_t p = 0; p < count; p++)
             if (!proofs[p] || !challengers[p] || !values[p] || (d && !points[p])) fail(BFD_ERR_ARG, "null argument");
+        // the transcripts are replayed in parallel, one challenger each
+        {
+            std::vector<const bfd_challenger *> seen(challengers, challengers + count);
+            std::sort(seen.begin(), seen.end());
+            if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+                fail(BFD_ERR_ARG, "every proof needs its own challenger");
+        }
         // decode and replay every transcript, one proof per pool task
         std::vector<std::vector<uint64_t>> idx_p(count);
         std::vector<std::vector<gf>> chal_p(count), scal_p(count);

@@ -328,6 +328,11 @@ def test_verifier_batch(bfd, oracle, ctx):
         proofs.append(proof)
         chs.append(ch)
         want.append(oracle.basefold_verify_w(128, log2_c, w, root, pt, claims, proof))
+    before = [c.export_state() for c in chs]
+    with pytest.raises(bfd.BfdError) as e:
+        pcs.verify_batch(roots[:2], evals[:2], proofs[:2], [chs[0], chs[0]], w)
+    assert e.value.code == bfd.BFD_ERR_ARG
+    assert [c.export_state() for c in chs] == before
     got = pcs.verify_batch(roots, evals, proofs, chs, w)
     assert got == want
     assert want.count(0) == 9
