@@ -50,6 +50,20 @@ def test_basefold_small_bit_exact(bfd, oracle, ctx, log2_c):
         assert oracle.basefold_verify(128, log2_c, root, point, claim, proof) == 0
 
 
+@pytest.mark.parametrize("lam,log2_c,n", [(128, 3, 6), (128, 3, 11), (100, 4, 5), (128, 8, 2), (80, 1, 7),
+                                          (128, 5, 9)])
+def test_basefold_other_rates_and_security(bfd, oracle, ctx, lam, log2_c, n):
+    """Blowups 8..256 and lambda 80/100 (num_queries, binary_reed_solomon.rs:55-59, differs from 128):
+    the layer-0 slots, twiddle levels and query counts of these codes against the restatement."""
+    _say(f"case lam={lam} log2_c={log2_c} n={n}")
+    evals, point = bfd.random_vec(300 + n, 1 << n), bfd.random_vec(400 + n, n)
+    root, claim, proof = gpu_prove(bfd, ctx, lam, log2_c, evals, point)
+    root_o, claim_o, proof_o = oracle.basefold_prove(lam, log2_c, evals, point)
+    assert root == root_o and (claim == claim_o).all() and proof == proof_o
+    assert len(proof) == bfd.lib().bfd_proof_size(lam, log2_c, n)
+    assert oracle.basefold_verify(lam, log2_c, root, point, claim, proof) == 0
+
+
 def test_basefold_cfg1_bit_exact(bfd, oracle, ctx):
     """2^16 evaluations, rate 1/2, seed 0: proof bytes identical (1,645,080 B)."""
     n, log2_c = 16, 1
@@ -644,6 +658,40 @@ def test_batch_rejects_shared_challenger(bfd, ctx):
         claims = [np.zeros((1, 2), np.uint64)] * 2
         bfd.commit_open_batch(128, 1, evals, points, claims, [ch, ch], True, ctx=ctx)
     assert e.value.code == bfd.BFD_ERR_ARG
+
+
+def test_batch_short_proof_buffer_refused(bfd, oracle, ctx):
+    """A caller proof buffer one byte short is refused before any work: the
+    challengers are left as they were, and the same call with the right sizes
+    then gives the restatement's proofs."""
+    import ctypes
+    sizes = [6, 9]
+    evals = [bfd.random_vec(21 + i, 1 << n) for i, n in enumerate(sizes)]
+    points = [bfd.random_vec(23 + i, n) for i, n in enumerate(sizes)]
+    chs = [bfd.GenericChallenger.keccak256() for _ in sizes]
+    before = [c.export_state() for c in chs]
+    L = bfd.lib()
+    host = [np.ascontiguousarray(bfd._elems(e)) for e in evals]
+    pts = [np.ascontiguousarray(bfd._elems(p)) for p in points]
+    want = [int(L.bfd_proof_size(128, 1, n)) for n in sizes]
+
+    def call(caps):
+        bufs = [np.zeros(c, np.uint8) for c in caps]
+        roots = np.zeros(32 * len(sizes), np.uint8)
+        rc = L.bfd_commit_open_batch(
+            ctx.h, 128, 1, len(sizes), (ctypes.c_uint32 * 2)(*sizes),
+            (ctypes.c_void_p * 2)(*[h.ctypes.data for h in host]), 0, (bfd.u8p * 2)(*[bfd._p(p) for p in pts]),
+            None, (ctypes.c_void_p * 2)(*[c.h.value if isinstance(c.h, ctypes.c_void_p) else c.h for c in chs]),
+            1, bfd._p(roots), (bfd.u8p * 2)(*[bfd._p(b) for b in bufs]), (ctypes.c_size_t * 2)(*caps))
+        return rc, bufs
+
+    rc, _ = call([want[0], want[1] - 1])
+    assert rc == bfd.BFD_ERR_ARG
+    assert [c.export_state() for c in chs] == before
+    rc, bufs = call(want)
+    assert rc == 0
+    for i, n in enumerate(sizes):
+        assert bufs[i].tobytes() == oracle.basefold_prove(128, 1, evals[i], points[i])[2], n
 
 
 def test_batch_uneven_mix_verifies(bfd, oracle, ctx):
