@@ -124,7 +124,7 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
             const TreeStep &ts = steps.back();
             ctx->timer.run(st, "k_top_round", [&] {
                 return launch::top_round(st, tk + 4 * ts.prev, ts.h, ts.lv, half_eq.as<gf>(), poly_cur, ph, sums,
-                                         d_segs + k, d_counters + k);
+                                         d_segs + k, d_counters + k, ctx->cus);
             });
         } else {
             unsigned nb = 0;

@@ -175,6 +175,12 @@ uint32_t Lane::seg(SegKernel k, std::vector<Seg> &segs, const char *name, gf *pa
         g.nblk = (uint32_t)want;
         blocks += g.nblk;
     }
+    // subtree levels narrow enough for the whole launch to leave SIMDs idle
+    // hash on lane pairs
+    if (k == SegKernel::Subtree) {
+        const int pw = launch::pair_width(blocks, cus_);
+        for (auto &g : segs) g.x = gf{(uint64_t)pw, 0};
+    }
     memcpy((Seg *)seg_pin_[c].p + seg_used_[c], segs.data(), sizeof(Seg) * segs.size());
     pending_.push_back(Pending{k, seg_used_[c], (int)segs.size(), blocks, partial, name});
     seg_used_[c] += segs.size();

@@ -567,7 +567,7 @@ inline void tree_step_launches(bfd_ctx *ctx, uint64_t *layer0, const std::vector
         if (s.lv == 0)
             ctx->timer.run(st, "k_nodes", [&] { return launch::nodes(st, prev, prev + 4 * s.h, s.h / 2, sh); });
         else
-            ctx->timer.run(st, "k_subtree", [&] { return launch::subtree(st, prev, s.h, s.lv); });
+            ctx->timer.run(st, "k_subtree", [&] { return launch::subtree(st, prev, s.h, s.lv, ctx->cus); });
     }
 }
 

@@ -271,12 +271,16 @@ __global__ __launch_bounds__(KB) void k_nodes(const uint64_t *__restrict__ prev,
 // [0, C) and level l at [2C - 2^(lv-l+1), 2C - 2^(lv-l)).  Node t of a level
 // runs on thread t, so a narrow level occupies as few waves as possible: a
 // block's waves share SIMDs, and a level spread over all of them runs slower
-// than the same level packed into one wave.
+// than the same level packed into one wave.  A level of at most pair_w nodes
+// per block runs node t on the lane pair 2t, 2t+1 instead (keccak_f_pair):
+// its latency drops by about a third for 1.36 times the instructions, which
+// pays while the level leaves SIMDs idle (the launcher chooses pair_w so that
+// paired levels fill at most one wave per SIMD).
 constexpr int kSubThreads = 256;
 constexpr int kSubMaxLevels = 9;               // 2^(levels-1) level-1 nodes = kSubThreads
 constexpr int kSubLds = 2 << kSubMaxLevels;    // digests of LDS per block (2C - 1 used)
 __device__ __forceinline__ void subtree(uint64_t *__restrict__ prev, size_t h, int lv, size_t sub,
-                                        ulonglong2 (*buf)[2]) {
+                                        ulonglong2 (*buf)[2], int pair_w) {
     const int tid = threadIdx.x;
     const int C = 1 << lv;
     const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(prev + 4 * (sub << lv));
@@ -285,7 +289,16 @@ __device__ __forceinline__ void subtree(uint64_t *__restrict__ prev, size_t h, i
     int in = 0, out = C;
     for (int l = 1; l <= lv; l++) {
         const int w = C >> l;
-        if (tid < w) {
+        if (w <= pair_w) {  // block-uniform; both lanes of every pair are active
+            if (tid < 2 * w) {
+                const int t = tid >> 1, odd = tid & 1;
+                const ulonglong2 a = buf[in + 2 * t][0], b = buf[in + 2 * t][1];
+                const ulonglong2 c = buf[in + 2 * t + 1][0], e = buf[in + 2 * t + 1][1];
+                uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
+                kk::keccak256_words_pair<8>(m, d, odd);
+                buf[out + t][odd] = odd ? ulonglong2{d[2], d[3]} : ulonglong2{d[0], d[1]};
+            }
+        } else if (tid < w) {
             const ulonglong2 a = buf[in + 2 * tid][0], b = buf[in + 2 * tid][1];
             const ulonglong2 c = buf[in + 2 * tid + 1][0], e = buf[in + 2 * tid + 1][1];
             uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
@@ -311,10 +324,10 @@ __device__ __forceinline__ void subtree(uint64_t *__restrict__ prev, size_t h, i
     __syncthreads();  // buf is refilled by the block's next subtree
 }
 
-__global__ __launch_bounds__(kSubThreads) void k_subtree(uint64_t *__restrict__ prev, size_t h, int lv) {
+__global__ __launch_bounds__(kSubThreads) void k_subtree(uint64_t *__restrict__ prev, size_t h, int lv, int pair_w) {
     __shared__ ulonglong2 buf[kSubLds][2];
     const size_t subs = h >> lv;
-    for (size_t sub = blockIdx.x; sub < subs; sub += gridDim.x) subtree(prev, h, lv, sub, buf);
+    for (size_t sub = blockIdx.x; sub < subs; sub += gridDim.x) subtree(prev, h, lv, sub, buf, pair_w);
 }
 
 // ---------------------------------------------------------------------------
@@ -502,7 +515,7 @@ __device__ void prod_fs_step(ProdFs *st, int k, gf A0, gf A1, gf A2) {
     uint64_t d[4];
     if (k > 0 && st->in_len == 32) {
         const uint64_t m[10] = {in64[0], in64[1], in64[2], in64[3], c0.lo, c0.hi, c2.lo, c2.hi, c3.lo, c3.hi};
-        kk::keccak256_words<10>(m, d);
+        kk::keccak256_words_pair<10>(m, d, lane & 1);
     } else {
         if (lane == 0) {
             uint32_t len = st->in_len;
@@ -660,7 +673,7 @@ __device__ void quad_fs_step(QuadFs *st, int k, const gf *partial, unsigned nb) 
     uint64_t d[4];
     if (k > 0 && st->in_len == 32) {
         const uint64_t m[8] = {in64[0], in64[1], in64[2], in64[3], c0.lo, c0.hi, c2.lo, c2.hi};
-        kk::keccak256_words<8>(m, d);
+        kk::keccak256_words_pair<8>(m, d, lane & 1);
     } else {
         if (lane == 0) {
             uint32_t len = st->in_len;
@@ -945,13 +958,14 @@ __global__ __launch_bounds__(KB) void ks_nodes(const Seg *segs, int nseg) {
 }
 
 // in = layer of arg = h digests, count = h >> lv subtrees (both powers of two,
-// so lv = log2(h / count)); see subtree()
+// so lv = log2(h / count)), x.lo = pair_w; see subtree()
 __global__ __launch_bounds__(kSubThreads) void ks_subtree(const Seg *segs, int nseg) {
     __shared__ ulonglong2 buf[kSubLds][2];
     const Seg &g = my_seg(segs, nseg);
     uint64_t *prev = (uint64_t *)g.in;
     const int lv = __builtin_ctzll(g.arg) - __builtin_ctzll(g.count);
-    for (size_t sub = blockIdx.x - g.blk0; sub < g.count; sub += g.nblk) subtree(prev, g.arg, lv, sub, buf);
+    const int pair_w = (int)g.x.lo;
+    for (size_t sub = blockIdx.x - g.blk0; sub < g.count; sub += g.nblk) subtree(prev, g.arg, lv, sub, buf, pair_w);
 }
 
 // in = halved eq table, aux = poly; partial[blockIdx.x] = block sum
@@ -1073,7 +1087,7 @@ __device__ void fs_round(FsState *st, int k, gf S, const uint64_t *root) {
         // every round after the first: the input is the previous digest
         const uint64_t m[12] = {in64[0], in64[1], in64[2], in64[3], root[0], root[1],
                                 root[2],  root[3],  rp0.lo, rp0.hi, rp2.lo, rp2.hi};
-        kk::keccak256_words<12>(m, d);
+        kk::keccak256_words_pair<12>(m, d, lane & 1);
     } else {
         if (lane == 0) {
             const uint64_t w[8] = {root[0], root[1], root[2], root[3], rp0.lo, rp0.hi, rp2.lo, rp2.hi};
@@ -1142,13 +1156,14 @@ __global__ __launch_bounds__(64) void ks_round(const Seg *segs, int nseg, const 
 __global__ __launch_bounds__(kSubThreads) void k_top_round(uint64_t *__restrict__ prev, size_t h, int lv,
                                                             unsigned nsub, const gf *__restrict__ a,
                                                             const gf *__restrict__ b, size_t n, gf *partial,
-                                                            unsigned ndot, const Seg *round, unsigned *counter) {
+                                                            unsigned ndot, const Seg *round, unsigned *counter,
+                                                            int pair_w) {
     __shared__ ulonglong2 buf[kSubLds][2];
     __shared__ gf red[kSubThreads / 64];
     __shared__ unsigned last;
     const unsigned blk = blockIdx.x;
     if (blk < nsub) {
-        subtree(prev, h, lv, blk, buf);
+        subtree(prev, h, lv, blk, buf, pair_w);
     } else {
         const unsigned db = blk - nsub;
         gf acc = gf_zero();
@@ -1376,18 +1391,27 @@ hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t co
     k_nodes<<<grid_for(count, sh.hash_block, sh.hash_cap), sh.hash_block, 0, st>>>(prev, next, count);
     return hipGetLastError();
 }
-hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv) {
+// Widest level (nodes per block) that runs on lane pairs: paired levels of
+// `blocks` blocks fill at most one wave per SIMD of `cus` compute units.
+int pair_width(unsigned blocks, int cus) {
+    if (cus <= 0 || blocks == 0) return 0;
+    return (int)std::min<size_t>(kSubThreads / 2, (size_t)(kSubThreads / 2) * (size_t)cus / blocks);
+}
+
+hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv, int cus) {
     if (lv < 1 || lv > kSubMaxLevels || (h & (h - 1)) || (h >> lv) == 0) return hipErrorInvalidValue;
-    k_subtree<<<grid_for(h >> lv, 1, kCapHash), kSubThreads, 0, st>>>(prev, h, lv);
+    const unsigned g = grid_for(h >> lv, 1, kCapHash);
+    k_subtree<<<g, kSubThreads, 0, st>>>(prev, h, lv, pair_width(g, cus));
     return hipGetLastError();
 }
 
 unsigned top_round_dot_blocks(size_t n) { return grid_for(n, kSubThreads, 512); }
 hipError_t top_round(hipStream_t st, uint64_t *prev, size_t h, int lv, const gf *a, const gf *b, size_t n,
-                     gf *partial, const Seg *round, unsigned *counter) {
+                     gf *partial, const Seg *round, unsigned *counter, int cus) {
     if (lv < 1 || lv > kSubMaxLevels || (h & (h - 1)) || (h >> lv) == 0 || (h >> lv) > 4096) return hipErrorInvalidValue;
     const unsigned nsub = (unsigned)(h >> lv), ndot = top_round_dot_blocks(n);
-    k_top_round<<<nsub + ndot, kSubThreads, 0, st>>>(prev, h, lv, nsub, a, b, n, partial, ndot, round, counter);
+    k_top_round<<<nsub + ndot, kSubThreads, 0, st>>>(prev, h, lv, nsub, a, b, n, partial, ndot, round, counter,
+                                                     pair_width(nsub, cus));
     return hipGetLastError();
 }
 

@@ -155,10 +155,17 @@ hipError_t leaves_pair(hipStream_t st, const gf *pi, uint64_t *out, size_t half,
 // one Merkle level: next[i] = H(prev[2i] || prev[2i+1])
 hipError_t nodes(hipStream_t st, const uint64_t *prev, uint64_t *next, size_t count, const Shape &sh = Shape{});
 // lv Merkle levels above the layer `prev` of h digests (h a power of two,
-// 1 <= lv <= subtree_max_levels(), lv <= log2 h), written to the layers that follow it
-hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv);
+// 1 <= lv <= subtree_max_levels(), lv <= log2 h), written to the layers that
+// follow it.  With cus (the device's compute units) > 0, levels narrow enough
+// to leave SIMDs idle hash each node on a lane pair (shorter latency); 0
+// keeps one lane per node.
+hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv, int cus = 0);
 // largest lv subtree() and SegKernel::Subtree accept
 int subtree_max_levels();
+// Widest subtree level (nodes per block) hashed on lane pairs when a launch
+// of `blocks` subtree blocks runs on `cus` compute units (the x.lo of a
+// SegKernel::Subtree segment); 0 = none
+int pair_width(unsigned blocks, int cus);
 // partial[block] = sum of a[k] b[k]; with root set, the 32 bytes at root are
 // also copied to partial - 2 (two elements in front of the sums)
 hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, unsigned *nblocks,
@@ -169,9 +176,9 @@ unsigned dot_blocks(size_t n);
 // inner product sum a[k] b[k], k < n (partial sums in partial[0,
 // top_round_dot_blocks(n))), and the round's transcript step (the
 // SegKernel::Round work of *round, on the reduced sum) by whichever block
-// finishes last.  *counter must be zero.
+// finishes last.  *counter must be zero.  cus as for subtree().
 hipError_t top_round(hipStream_t st, uint64_t *prev, size_t h, int lv, const gf *a, const gf *b, size_t n,
-                     gf *partial, const Seg *round, unsigned *counter);
+                     gf *partial, const Seg *round, unsigned *counter, int cus = 0);
 unsigned top_round_dot_blocks(size_t n);
 // fold (binary_reed_solomon.rs:88-96) and fix_last_var with the same
 // challenge in one launch; the fix may be in place.  The challenge is read
