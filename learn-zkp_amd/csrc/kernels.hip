@@ -265,9 +265,9 @@ __global__ __launch_bounds__(KB) void k_nodes(const uint64_t *__restrict__ prev,
 // (h a power of two) and the next levels follow it contiguously.  Subtree
 // `sub` covers the C = 2^lv children [sub * C, (sub + 1) * C), lv <=
 // kSubMaxLevels: the block copies them into LDS, hashes level after level
-// inside LDS (one Keccak call site, so the straight-line permutation is
-// fetched into the instruction cache once), and finally writes every level's
-// digests out with coalesced 16-byte stores.  buf holds the children at
+// inside LDS (one call site per permutation form, so each straight-line
+// permutation is fetched into the instruction cache once), and finally
+// writes every level's digests out with coalesced 16-byte stores.  buf holds the children at
 // [0, C) and level l at [2C - 2^(lv-l+1), 2C - 2^(lv-l)).  Node t of a level
 // runs on thread t, so a narrow level occupies as few waves as possible: a
 // block's waves share SIMDs, and a level spread over all of them runs slower
