@@ -144,31 +144,50 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     if (d > 0) hip_check(hipMemcpyAsync(pin, fs.p, rbb, hipMemcpyDeviceToHost, st), "copy transcript back");
     hip_check(hipMemcpyAsync(pin + ((rbb + 15) & ~(size_t)15), poly_cur, 16, hipMemcpyDeviceToHost, st),
               "copy final");
+    // while the rounds run: the proof buffer, with its pages touched so that
+    // filling it later takes no page faults
+    std::unique_ptr<uint8_t, void (*)(void *)> owned(nullptr, free);
+    uint8_t *dst = *proof;
+    if (!dst) {
+        owned.reset((uint8_t *)malloc(sz ? sz : 1));
+        if (!owned) fail(BFD_ERR_ARG, "host allocation failed");
+        dst = owned.get();
+        ChunkCopy::zero(dst, sz);
+    }
     hip_check(hipStreamSynchronize(st), "final sync");
     const double th = now_ms();
     if (d > 0) fs_read(pin, o, ch);
     o.finish(load_elem(pin + ((rbb + 15) & ~(size_t)15)), ch);
     ctx->timer.host("round_transcript", now_ms() - th);
 
-    // openings: one gather of elements, one of digests
+    // openings: the query records are written on the device in their bincode
+    // layout (k_proof_records) and copied in behind the header
     double t0 = now_ms();
-    size_t ne = 0, nd = 0;
-    gather_counts(o, ne, nd);
-    std::vector<uint64_t> plan(ne + nd);
-    plan_gathers(o, lay, plan.data(), plan.data() + ne);
-    ctx->gather_pin.ensure(16 * ne + 32 * nd + 64);
-    uint8_t *he = (uint8_t *)ctx->gather_pin.p, *hd = he + 16 * ne;
-    DevBuf dix = ctx_buf(ctx, 8 * (ne + nd) + 8), dout = ctx_buf(ctx, 16 * ne + 32 * nd + 16);
-    hip_check(hipMemcpyAsync(dix.p, plan.data(), 8 * (ne + nd), hipMemcpyHostToDevice, st), "copy gather plan");
-    uint8_t *oe = dout.as<uint8_t>(), *od = oe + 16 * ne;
-    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>(), oe, ne, 16); });
-    ctx->timer.run(st, "k_gather_abs", [&] { return launch::gather_abs(st, dix.as<uint64_t>() + ne, od, nd, 32); });
-    hip_check(hipMemcpyAsync(he, oe, 16 * ne + 32 * nd, hipMemcpyDeviceToHost, st), "copy gather");
-    hip_check(hipStreamSynchronize(st), "gather sync");
-    ctx->timer.collect();
-    uint8_t *dst = *proof ? *proof : (uint8_t *)malloc(sz ? sz : 1);
-    if (!dst) fail(BFD_ERR_ARG, "host allocation failed");
-    assemble_proof(o, he, hd, dst);
+    write_proof_header(o, dst);
+    const size_t hb = proof_header_bytes(o.d), body = sz - hb;
+    if (o.q) {
+        const size_t up = sizeof(ProofJob) + 8 * o.q;
+        ctx->pinned.ensure(up);
+        uint8_t *hu = (uint8_t *)ctx->pinned.p;
+        DevBuf dj = ctx_buf(ctx, up), dout = ctx_buf(ctx, body);
+        const ProofJob J =
+            proof_job(o, lay, reinterpret_cast<const uint64_t *>(dj.as<uint8_t>() + sizeof(ProofJob)), dout.as<uint8_t>(), 0);
+        memcpy(hu, &J, sizeof J);
+        memcpy(hu + sizeof J, o.idx.data(), 8 * o.q);
+        hip_check(hipMemcpyAsync(dj.p, hu, up, hipMemcpyHostToDevice, st), "copy query indices");
+        ctx->timer.run(st, "k_proof_records",
+                       [&] { return launch::proof_records(st, dj.as<ProofJob>(), 1, (uint32_t)o.q); });
+        ctx->gather_pin.ensure(body);
+        hip_check(hipMemcpyAsync(ctx->gather_pin.p, dout.p, body, hipMemcpyDeviceToHost, st), "copy query records");
+        hip_check(hipStreamSynchronize(st), "query records");
+        ctx->timer.collect();
+        ChunkCopy cc;
+        cc.add(dst + hb, (const uint8_t *)ctx->gather_pin.p, body);
+        cc.run();
+    } else {
+        ctx->timer.collect();
+    }
+    owned.release();
     *proof = dst;
     *proof_len = sz;
     ctx->timer.host("proof", now_ms() - t0);

@@ -506,52 +506,56 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         if (j.k == j.n) j.stage = Job::Final;
         j.k++;
     }
-    // (E) openings of jobs whose queries are known
-    size_t NE = 0, ND = 0;
+    // (E) openings of jobs whose queries are known: their query records,
+    // written on the device in bincode layout (k_proof_records) and read
+    // back for open_finish to copy behind each proof's header
+    size_t body = 0, nidx = 0;
     for (Job *jp : jobs) {
         Job &j = *jp;
         if (j.stage != Job::Gather) continue;
         t.gathered.push_back(jp);
-        t.gather_at.push_back({NE, ND});
-        NE += j.ne;
-        ND += j.nd;
+        t.gather_at.push_back(body);
+        body += j.o.q * query_record_bytes(j.o.b, j.o.d, 1);
+        nidx += j.o.q;
     }
-    t.NE = NE;
-    if (NE + ND) {
-        plan_pin_[c].ensure(8 * (NE + ND));
-        uint64_t *plan = (uint64_t *)plan_pin_[c].p;
-        parallel_for(t.gathered.size(), [&](size_t g) {
+    if (body) {
+        const size_t G = t.gathered.size(), up = sizeof(ProofJob) * G + 8 * nidx;
+        plan_pin_[c].ensure(up);
+        ensure(plan_dev_[c], up, st_);
+        ensure(gather_dev_[c], body, st_);
+        gather_pin_[c].ensure(body);
+        ProofJob *hj = (ProofJob *)plan_pin_[c].p;
+        uint64_t *hidx = reinterpret_cast<uint64_t *>(hj + G);
+        const uint64_t *didx = reinterpret_cast<const uint64_t *>(plan_dev_[c].as<uint8_t>() + sizeof(ProofJob) * G);
+        size_t io = 0;
+        uint32_t rec = 0;
+        for (size_t g = 0; g < G; g++) {
             const Job &j = *t.gathered[g];
-            plan_gathers(j.o, j.lay, plan + t.gather_at[g].first, plan + NE + t.gather_at[g].second);
-        });
-        ensure(plan_dev_[c], 8 * (NE + ND), st_);
-        ensure(gather_dev_[c], 16 * NE + 32 * ND + 64, st_);
-        gather_pin_[c].ensure(16 * NE + 32 * ND + 64);
-        hip_check(hipMemcpyAsync(plan_dev_[c].p, plan, 8 * (NE + ND), hipMemcpyHostToDevice, st_), "copy plan");
-        uint8_t *oe = gather_dev_[c].as<uint8_t>(), *od = oe + 16 * NE;
-        auto g1 = [&] { return launch::gather_abs(st_, plan_dev_[c].as<uint64_t>(), oe, NE, 16); };
-        auto g2 = [&] { return launch::gather_abs(st_, plan_dev_[c].as<uint64_t>() + NE, od, ND, 32); };
-        if (timer_) {
-            timer_->run(st_, "k_gather_abs", g1);
-            timer_->run(st_, "k_gather_abs", g2);
-        } else {
-            hip_check(g1(), "gather");
-            hip_check(g2(), "gather");
+            memcpy(hidx + io, j.o.idx.data(), 8 * j.o.q);
+            hj[g] = proof_job(j.o, j.lay, didx + io, gather_dev_[c].as<uint8_t>() + t.gather_at[g], rec);
+            io += j.o.q;
+            rec += (uint32_t)j.o.q;
         }
-        hip_check(hipMemcpyAsync(gather_pin_[c].p, oe, 16 * NE + 32 * ND, hipMemcpyDeviceToHost, st_),
-                  "copy gather");
-        for (Job *jp : t.gathered) jp->stage = Job::Gathering;
+        hip_check(hipMemcpyAsync(plan_dev_[c].p, hj, up, hipMemcpyHostToDevice, st_), "copy query indices");
+        auto go = [&] { return launch::proof_records(st_, plan_dev_[c].as<ProofJob>(), (uint32_t)G, rec); };
+        if (timer_) timer_->run(st_, "k_proof_records", go);
+        else hip_check(go(), "proof records");
+        hip_check(hipMemcpyAsync(gather_pin_[c].p, gather_dev_[c].p, body, hipMemcpyDeviceToHost, st_),
+                  "copy query records");
     }
+    for (Job *jp : t.gathered) jp->stage = Job::Gathering;
     hip_check(hipEventRecord(t.done, st_), "hipEventRecord");
     return t;
 }
 
 void Lane::open_finish(OpenTick &t) {
     const int c = t.par;
-    // proofs of the jobs this tick gathered
-    const uint8_t *he = (const uint8_t *)gather_pin_[c].p, *hd = he + 16 * t.NE;
-    for (Job *jp : t.gathered) {
-        Job &j = *jp;
+    // proofs of the jobs this tick gathered: header from the host's round
+    // data, query records from the read-back
+    const uint8_t *recs = (const uint8_t *)gather_pin_[c].p;
+    ChunkCopy cc;
+    for (size_t g = 0; g < t.gathered.size(); g++) {
+        Job &j = *t.gathered[g];
         const size_t sz = proof_size(j.o.b, j.o.d, j.o.q);
         if (j.proof && j.proof_cap < sz) fail(BFD_ERR_ARG, "proof buffer too small");
         if (!j.proof) {
@@ -560,11 +564,11 @@ void Lane::open_finish(OpenTick &t) {
             j.proof_owned = true;
         }
         j.proof_len = sz;
+        write_proof_header(j.o, j.proof);
+        const size_t hb = proof_header_bytes(j.o.d);
+        cc.add(j.proof + hb, recs + t.gather_at[g], sz - hb);
     }
-    parallel_for(t.gathered.size(), [&](size_t g) {
-        Job &j = *t.gathered[g];
-        assemble_proof(j.o, he + 16 * t.gather_at[g].first, hd + 32 * t.gather_at[g].second, j.proof);
-    });
+    cc.run();
     const double now = now_ms();
     for (Job *jp : t.gathered) {
         jp->stage = Job::Done;
@@ -579,7 +583,6 @@ void Lane::open_finish(OpenTick &t) {
         const uint8_t *img = rb + t.finals[a].second;
         fs_read(img, j.o, *j.ch);
         j.o.finish(load_elem(img + fs_readback_bytes((uint32_t)j.n)), *j.ch);
-        gather_counts(j.o, j.ne, j.nd);
         j.stage = Job::Gather;
     });
 }

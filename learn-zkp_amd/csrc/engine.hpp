@@ -51,7 +51,6 @@ struct Job {
     int eq_lo = 0;
     // per-tick bookkeeping
     size_t slot = 0;
-    size_t ne = 0, nd = 0;
     // results
     uint8_t root[32] = {0};
     uint8_t *proof = nullptr;          // caller buffer or library allocation
@@ -100,8 +99,7 @@ struct OpenTick {
     hipEvent_t done = nullptr;                         // recorded after its last operation
     std::vector<std::pair<Job *, size_t>> finals;      // job, offset of its read-back in the staging
     std::vector<Job *> gathered;                       // jobs whose openings it gathered
-    std::vector<std::pair<size_t, size_t>> gather_at;  // per gathered job: first element / digest record
-    size_t NE = 0;                                     // element records of the gather
+    std::vector<size_t> gather_at;                     // per gathered job: offset of its query records
 };
 
 // One execution lane: a stream with its own staging buffers.

@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <array>
 #include <atomic>
 #include <condition_variable>
@@ -885,85 +886,69 @@ struct OpenLayout {
     const uint64_t *pitree(int k) const { return pitrees + 4 * tree_off[k]; }
 };
 
-inline size_t layer_off(size_t leaves, int t) { return 2 * leaves - 2 * (leaves >> t); }
-
-// Records one proof opens: per query the codeword row (w elements) and L
-// main-tree siblings, then per round a pi sibling value and b+i pi-tree siblings.
-inline void gather_counts(const OpenRounds &o, size_t &n_elems, size_t &n_digs) {
-    size_t per_d = (size_t)o.L;
-    for (int k = 0; k < o.d; k++) per_d += (size_t)(o.b + (o.d - 1 - k));
-    n_elems = o.q * (o.w + (size_t)o.d);
-    n_digs = o.q * per_d;
+// Bytes of one query record of BasefoldProof (basefold.rs:49-66): the
+// codeword row (w elements) and its L = b + d main-tree siblings, then per
+// round a pi sibling value and its b + i pi-tree siblings, each vector with
+// its u64 length.
+inline size_t query_record_bytes(int b, int d, size_t w) {
+    size_t r = 8 + 16 * w + 8 + 32 * (size_t)(b + d) + 8;
+    for (int k = 0; k < d; k++) r += 16 + 8 + 32 * (size_t)(b + d - 1 - k);
+    return r;
 }
 
-// Absolute addresses of every opened value, in proof order (basefold.rs:205-222).
-inline void plan_gathers(const OpenRounds &o, const OpenLayout &lay, uint64_t *elems, uint64_t *digs) {
-    for (size_t t = 0; t < o.q; t++) {
-        const uint64_t id = o.idx[t];
-        if (lay.rows)
-            for (size_t c = 0; c < lay.w; c++) *elems++ = (uint64_t)(lay.rows + id * lay.w + c);
-        else
-            *elems++ = (uint64_t)(lay.cw + id);
-        for (int u = 0; u < o.L; u++) *digs++ = (uint64_t)(lay.tree + 4 * (layer_off(o.M, u) + ((id >> u) ^ 1)));
-        for (int k = 0; k < o.d; k++) {
-            const int i = o.d - 1 - k;
-            const size_t ni = (size_t)1 << (o.b + i);
-            const size_t sib = ((id / ni) & 1) ^ 1, row = id % ni;
-            *elems++ = (uint64_t)(lay.pi(k) + row + sib * ni);
-            const uint64_t *pt = lay.pitree(k);
-            for (int u = 0; u < o.b + i; u++) *digs++ = (uint64_t)(pt + 4 * (layer_off(ni, u) + ((row >> u) ^ 1)));
-        }
-    }
-}
+// Bytes of BasefoldProof before its query records: the d compressed round
+// polynomials, the final polynomial, the d pi-tree roots and the query count.
+inline size_t proof_header_bytes(int d) { return 8 + 40 * (size_t)d + 8 + 16 + 8 + 32 * (size_t)d + 8; }
 
 // Exact bincode size of BasefoldProof for d rounds, L = b + d, q queries and
 // w elements per opened codeword row.
 inline size_t proof_size(int b, int d, size_t q, size_t w = 1) {
-    const size_t L = (size_t)(b + d);
-    size_t per_q = 8 + 16 * w + 8 + 32 * L + 8;
-    for (int k = 0; k < d; k++) per_q += 16 + 8 + 32 * (size_t)(b + (d - 1 - k));
-    return 8 + 40 * (size_t)d + 8 + 16 + 8 + 32 * (size_t)d + 8 + q * per_q;
+    return proof_header_bytes(d) + q * query_record_bytes(b, d, w);
 }
 
-struct Writer {
-    uint8_t *p;
-    void u64(uint64_t v) { memcpy(p, &v, 8); p += 8; }
-    void elem(gf v) { memcpy(p, &v.lo, 8); memcpy(p + 8, &v.hi, 8); p += 16; }
-    void bytes(const void *src, size_t n) { memcpy(p, src, n); p += n; }
-};
-
-// bincode BasefoldProof (basefold.rs:49-66) from the gathered records (same
-// order as plan_gathers) into `out`, which holds proof_size(b, d, q) bytes.
-inline void assemble_proof(const OpenRounds &o, const uint8_t *elems, const uint8_t *digs, uint8_t *out) {
-    Writer w{out};
-    w.u64((uint64_t)o.d);
+// The header of BasefoldProof (basefold.rs:49-66) into out, which holds
+// proof_header_bytes(o.d) bytes; the query records follow it.
+inline void write_proof_header(const OpenRounds &o, uint8_t *out) {
+    uint8_t *p = out;
+    const auto u64 = [&](uint64_t v) { memcpy(p, &v, 8); p += 8; };
+    const auto elem = [&](gf v) { store_elem(p, v); p += 16; };
+    u64((uint64_t)o.d);
     for (int k = 0; k < o.d; k++) {
-        w.u64(2);
-        w.elem(o.comp[k][0]);
-        w.elem(o.comp[k][1]);
+        u64(2);
+        elem(o.comp[k][0]);
+        elem(o.comp[k][1]);
     }
-    w.u64(1);
-    w.elem(o.final_poly);
-    w.u64((uint64_t)o.d);
-    for (int k = 0; k < o.d; k++) w.bytes(o.pi_roots[k].data(), 32);
-    w.u64((uint64_t)o.q);
-    for (size_t t = 0; t < o.q; t++) {
-        w.u64((uint64_t)o.w);
-        w.bytes(elems, 16 * o.w);
-        elems += 16 * o.w;
-        w.u64((uint64_t)o.L);
-        w.bytes(digs, 32 * (size_t)o.L);
-        digs += 32 * (size_t)o.L;
-        w.u64((uint64_t)o.d);
-        for (int k = 0; k < o.d; k++) {
-            const int i = o.d - 1 - k;
-            w.bytes(elems, 16);
-            elems += 16;
-            w.u64((uint64_t)(o.b + i));
-            w.bytes(digs, 32 * (size_t)(o.b + i));
-            digs += 32 * (size_t)(o.b + i);
-        }
+    u64(1);
+    elem(o.final_poly);
+    u64((uint64_t)o.d);
+    for (int k = 0; k < o.d; k++) {
+        memcpy(p, o.pi_roots[k].data(), 32);
+        p += 32;
     }
+    u64((uint64_t)o.q);
+}
+
+// k_proof_records' description of o's query records: indices at d_idx, the
+// records (o.q of query_record_bytes each) written to d_out; rec0 = the
+// job's first record in the launch.
+inline ProofJob proof_job(const OpenRounds &o, const OpenLayout &lay, const uint64_t *d_idx, uint8_t *d_out,
+                          uint32_t rec0) {
+    ProofJob J;
+    memset(&J, 0, sizeof J);
+    J.idx = d_idx;
+    J.cw = lay.cw;
+    J.rows = lay.rows;
+    J.tree = lay.tree;
+    J.pis = lay.pis;
+    J.pitrees = lay.pitrees;
+    J.out = d_out;
+    J.w = lay.rows ? lay.w : 1;
+    J.rec_bytes = query_record_bytes(o.b, o.d, J.w);
+    J.b = (uint32_t)o.b;
+    J.d = (uint32_t)o.d;
+    J.q = (uint32_t)o.q;
+    J.rec0 = rec0;
+    return J;
 }
 
 // Persistent host worker pool for per-polynomial host work (proof planning
@@ -1064,6 +1049,30 @@ template <class F>
 void parallel_for(size_t n, F &&f) {
     HostPool::get().run(n, std::forward<F>(f));
 }
+
+// Host copies cut into chunks for the pool: add() records a copy, run()
+// performs every recorded copy (one pool run, so it must not be called from
+// inside another).
+struct ChunkCopy {
+    static constexpr size_t kChunk = (size_t)256 << 10;
+    struct Piece {
+        uint8_t *dst;
+        const uint8_t *src;
+        size_t n;
+    };
+    std::vector<Piece> pieces;
+    void add(uint8_t *dst, const uint8_t *src, size_t n) {
+        for (size_t o = 0; o < n; o += kChunk) pieces.push_back({dst + o, src + o, std::min(kChunk, n - o)});
+    }
+    void run() {
+        parallel_for(pieces.size(), [&](size_t i) { memcpy(pieces[i].dst, pieces[i].src, pieces[i].n); });
+        pieces.clear();
+    }
+    // zero-fills n bytes at dst on the pool (touches fresh pages)
+    static void zero(uint8_t *dst, size_t n) {
+        parallel_for((n + kChunk - 1) / kChunk, [&](size_t c) { memset(dst + c * kChunk, 0, std::min(kChunk, n - c * kChunk)); });
+    }
+};
 
 }  // namespace rt
 }  // namespace bfd

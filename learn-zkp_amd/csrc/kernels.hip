@@ -1212,6 +1212,91 @@ __global__ void k_gather_abs(const uint64_t *__restrict__ src, uint8_t *__restri
 }
 
 // ---------------------------------------------------------------------------
+// proof records
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void st_elem8(uint8_t *o, const gf *src) {
+    const gf v = gf_ld(src);
+    uint64_t *q = reinterpret_cast<uint64_t *>(o);
+    q[0] = v.lo;
+    q[1] = v.hi;
+}
+__device__ __forceinline__ void st_digest8(uint8_t *o, const uint64_t *src) {
+    const ulonglong2 a = reinterpret_cast<const ulonglong2 *>(src)[0], c = reinterpret_cast<const ulonglong2 *>(src)[1];
+    uint64_t *q = reinterpret_cast<uint64_t *>(o);
+    q[0] = a.x;
+    q[1] = a.y;
+    q[2] = c.x;
+    q[3] = c.y;
+}
+// digests before layer t of a Merkle tree over `leaves` leaves
+__device__ __forceinline__ size_t tree_layer_off(size_t leaves, uint32_t t) { return 2 * leaves - 2 * (leaves >> t); }
+
+// The query records of BasefoldProof in their bincode layout (ProofJob),
+// read from where the commitment and the opening rounds left the data
+// (basefold.rs:205-222: the codeword row and its main-tree path, then per
+// round the pi sibling and its pi-tree path).  Blocks stride over the
+// records of all jobs; a block's threads take the items of one record (row
+// elements, main-tree siblings, then per round the sibling value and its
+// tree siblings), each written with 8-byte stores (records are 8-aligned).
+constexpr int kRecThreads = 256;
+__global__ __launch_bounds__(kRecThreads) void k_proof_records(const ProofJob *__restrict__ jobs, uint32_t njobs,
+                                                               uint32_t records) {
+    for (uint32_t r = blockIdx.x; r < records; r += gridDim.x) {
+        uint32_t lo = 0, hi = njobs - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].rec0 <= r) lo = mid;
+            else hi = mid - 1;
+        }
+        const ProofJob &J = jobs[lo];
+        const uint32_t t = r - J.rec0;
+        const uint64_t id = J.idx[t];
+        const size_t w = J.w;
+        const uint32_t b = J.b, d = J.d, L = b + d;
+        const size_t M = (size_t)1 << L;
+        uint8_t *o = J.out + (size_t)t * J.rec_bytes;
+        const size_t main_off = 16 + 16 * w, rounds_off = 24 + 16 * w + 32 * (size_t)L;
+        // items: w row elements, L main siblings, per round k one value and
+        // b + d - 1 - k siblings
+        const size_t items = w + L + d + (size_t)d * (b + d - 1) - (size_t)d * (d - 1) / 2;
+        if (threadIdx.x == 0) {
+            *reinterpret_cast<uint64_t *>(o) = w;
+            *reinterpret_cast<uint64_t *>(o + 8 + 16 * w) = L;
+            *reinterpret_cast<uint64_t *>(o + 16 + 16 * w + 32 * (size_t)L) = d;
+        }
+        for (size_t i = threadIdx.x; i < items; i += kRecThreads) {
+            if (i < w) {
+                st_elem8(o + 8 + 16 * i, J.rows ? J.rows + id * w + i : J.cw + id);
+            } else if (i < w + L) {
+                const uint32_t u = (uint32_t)(i - w);
+                st_digest8(o + main_off + 32 * (size_t)u, J.tree + 4 * (tree_layer_off(M, u) + ((id >> u) ^ 1)));
+            } else {
+                size_t j = i - w - L;
+                uint32_t k = 0;
+                while (j >= (size_t)(b + d - k)) {  // round k holds 1 + (b + d - 1 - k) items
+                    j -= b + d - k;
+                    k++;
+                }
+                const uint32_t bi = b + d - 1 - k;  // b + i for level i = d - 1 - k
+                const size_t ni = (size_t)1 << bi;
+                const size_t sib = ((id >> bi) & 1) ^ 1, row = id & (ni - 1);
+                uint8_t *ro = o + rounds_off + 24 * (size_t)k +
+                              32 * ((size_t)k * (b + d - 1) - (size_t)k * (k - 1) / 2);
+                if (j == 0) {
+                    const gf *pk = k == 0 ? J.cw : J.pis + (M - (M >> (k - 1)));
+                    st_elem8(ro, pk + row + sib * ni);
+                    *reinterpret_cast<uint64_t *>(ro + 16) = bi;
+                } else {
+                    const uint32_t u = (uint32_t)(j - 1);
+                    const uint64_t *tk = J.pitrees + (k == 0 ? 0 : 4 * (2 * M - (M >> (k - 1)) - k));
+                    st_digest8(ro + 24 + 32 * (size_t)u, tk + 4 * (tree_layer_off(ni, u) + ((row >> u) ^ 1)));
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // query gathers
 // ---------------------------------------------------------------------------
 __global__ void k_gather_digest(const uint64_t *__restrict__ base, const uint64_t *__restrict__ idx,
@@ -1540,6 +1625,12 @@ int subtree_max_levels() { return kSubMaxLevels; }
 hipError_t gather_abs(hipStream_t st, const uint64_t *src, void *out, size_t n, int rb) {
     if (!n) return hipSuccess;
     k_gather_abs<<<grid_for(n, 256, 4096), 256, 0, st>>>(src, (uint8_t *)out, n, rb);
+    return hipGetLastError();
+}
+hipError_t proof_records(hipStream_t st, const ProofJob *jobs, uint32_t njobs, uint32_t records) {
+    if (!records) return hipSuccess;
+    if (!njobs) return hipErrorInvalidValue;
+    k_proof_records<<<grid_for(records, 1, 4096), kRecThreads, 0, st>>>(jobs, njobs, records);
     return hipGetLastError();
 }
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters) {

@@ -35,6 +35,28 @@ struct VerifyJob {
     uint64_t roots_off;     // byte offset of the pi-tree roots
 };
 
+// One opening's query records for k_proof_records (device copy; pointers
+// into device memory).  The records are BasefoldProof's query openings
+// (basefold.rs:205-222) in bincode layout, record t at out + t * rec_bytes:
+// u64 w, the w elements of codeword row idx[t], u64 L, the L main-tree
+// siblings leaf to root, u64 d, then per round k (level i = d-1-k, ni =
+// 2^(b+i)): the pi sibling value, u64 b+i and the b+i pi-tree siblings.
+// Round k's pi sits at cw (k = 0) or pis + M - (M >> (k-1)); its tree at
+// pitrees + 4 (2M - (M >> (k-1)) - k) digests (k >= 1; round 0's at
+// pitrees).  rec0 is the job's first record in the launch (jobs sorted by it).
+struct ProofJob {
+    const uint64_t *idx;       // q query indices
+    const gf *cw;              // round-0 pi (width 1: the codeword; else the scalar combination)
+    const gf *rows;            // width > 1: the codeword matrix, w elements per row; else nullptr
+    const uint64_t *tree;      // main tree, 2M-1 digests, leaf layer first
+    const gf *pis;             // pi of rounds 1..d
+    const uint64_t *pitrees;   // pi trees of rounds 0..d-1
+    uint8_t *out;              // q records
+    uint64_t w;
+    uint64_t rec_bytes;
+    uint32_t b, d, q, rec0;
+};
+
 // Device-resident Fiat-Shamir state of one opening: the byte buffers of
 // GenericChallenger<_, Keccak256> (p3/src/challenger.rs:24-90 over plonky3's
 // HashChallenger<u8, _, 32>) and the Eval-sumcheck scalars that change from
@@ -129,6 +151,9 @@ hipError_t seg_launch(hipStream_t st, SegKernel which, const Seg *d_segs, int ns
 int seg_block_size(SegKernel which);
 // out[i] = rb (16 or 32) bytes from absolute device address src[i]
 hipError_t gather_abs(hipStream_t st, const uint64_t *src, void *out, size_t n, int rb);
+// every query record of njobs openings (records rec0 .. rec0 + q - 1 of job
+// a; records in total); jobs sorted by rec0
+hipError_t proof_records(hipStream_t st, const ProofJob *jobs, uint32_t njobs, uint32_t records);
 
 
 // ft[j] = XOR over set bits p of j of bt[p]

@@ -32,13 +32,6 @@ uint64_t rd64(const uint8_t *p) {
     return v;
 }
 
-// bytes of one query record: row, main path, then per round a value and a path
-size_t record_bytes(int b, int d, size_t w) {
-    size_t r = 8 + 16 * w + 8 + 32 * (size_t)(b + d) + 8;
-    for (int k = 0; k < d; k++) r += 16 + 8 + 32 * (size_t)(b + d - 1 - k);
-    return r;
-}
-
 // Decodes the bincode BasefoldProof for (b, d, q, w); false if its length
 // or any length prefix differs from the layout of such a proof.
 bool decode(const uint8_t *p, size_t len, int b, int d, size_t q, size_t w, Parsed &o) {
@@ -61,7 +54,7 @@ bool decode(const uint8_t *p, size_t len, int b, int d, size_t q, size_t w, Pars
     pos = o.roots_off + 32 * (size_t)d;
     if (rd64(p + pos) != q) return false;
     o.open_off = pos + 8;
-    o.rec_bytes = record_bytes(b, d, w);
+    o.rec_bytes = query_record_bytes(b, d, w);
     const size_t L = (size_t)(b + d);
     for (size_t t = 0; t < q; t++) {
         const uint8_t *r = p + o.open_off + t * o.rec_bytes;
