@@ -1297,26 +1297,6 @@ __global__ __launch_bounds__(kRecThreads) void k_proof_records(const ProofJob *_
 }
 
 // ---------------------------------------------------------------------------
-// query gathers
-// ---------------------------------------------------------------------------
-__global__ void k_gather_digest(const uint64_t *__restrict__ base, const uint64_t *__restrict__ idx,
-                                uint64_t *__restrict__ out, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(base + 4 * idx[i]);
-        ulonglong2 *o = reinterpret_cast<ulonglong2 *>(out + 4 * i);
-        o[0] = p[0];
-        o[1] = p[1];
-    }
-}
-__global__ void k_gather_elem(const gf *__restrict__ base, const uint64_t *__restrict__ idx, gf *__restrict__ out,
-                              size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        gf_st(out + i, gf_ld(base + idx[i]));
-}
-
-// ---------------------------------------------------------------------------
 // micro-benchmarks of the two multiply forms and of Keccak-f
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(CB) void k_bench_mul_comb(gf *io, size_t n, int iters) {
@@ -1578,14 +1558,6 @@ hipError_t verify_paths(hipStream_t st, const VerifyJob *jobs, uint32_t count, u
     const size_t n = (size_t)count * q * (size_t)(d + 1);
     if (!n) return hipSuccess;
     k_verify_paths<<<grid_for(n, KB, kCapHash), KB, 0, st>>>(jobs, count, q, b, d, w, vals, path_ok);
-    return hipGetLastError();
-}
-hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n) {
-    k_gather_digest<<<grid_for(n, 256, 4096), 256, 0, st>>>(base, idx, out, n);
-    return hipGetLastError();
-}
-hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *out, size_t n) {
-    k_gather_elem<<<grid_for(n, 256, 4096), 256, 0, st>>>(base, idx, out, n);
     return hipGetLastError();
 }
 

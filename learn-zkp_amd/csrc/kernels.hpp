@@ -248,8 +248,6 @@ hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, ui
                        const gf *ft, gf *vals, uint8_t *fold_ok);
 hipError_t verify_paths(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
                         const gf *vals, uint8_t *path_ok);
-hipError_t gather_digest(hipStream_t st, const uint64_t *base, const uint64_t *idx, uint64_t *out, size_t n);
-hipError_t gather_elem(hipStream_t st, const gf *base, const uint64_t *idx, gf *out, size_t n);
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters);
 hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n);
 hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters);
