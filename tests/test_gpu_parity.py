@@ -599,7 +599,8 @@ def test_batch_equals_plain_and_oracle(bfd, oracle, ctx):
     """Uneven batch: every proof equals the plain path's and the oracle's."""
     rng = np.random.default_rng(7)
     sizes = [int(v) for v in rng.integers(1, 11, size=12)] + [12, 3, 1]
-    for log2_c in (1, 2):
+    # rate 1 (log2_c = 0) has no queries: header-only proofs
+    for log2_c in (1, 2, 0, 3):
         _say(f"batch log2_c={log2_c} sizes={sizes}")
         evals = [bfd.random_vec(1000 + i, 1 << n) for i, n in enumerate(sizes)]
         points = [bfd.random_vec(2000 + i, n) for i, n in enumerate(sizes)]
