@@ -1,10 +1,11 @@
 // Host runtime and C ABI of the MI355X Basefold prover.
 //
 // The host drives one HIP stream per context.  Bulk work (encode, Merkle
-// hashing, folds, sumcheck reductions, query gathers) runs as kernels; the
-// rounds' Fiat-Shamir steps run on the device too (a one-wave kernel per
-// round), so an opening reads back once after its last round, samples the
-// queries on the host and gathers the openings.
+// hashing, folds, sumcheck reductions, the proof's query records) runs as
+// kernels; the rounds' Fiat-Shamir steps run on the device too (in whichever
+// block of a round's last tree launch finishes last), so an opening reads
+// back once after its last round, samples the queries on the host and has
+// the query records written on the device.
 #include "internal.hpp"
 
 using namespace bfd::rt;

@@ -9,7 +9,8 @@
 // jobs.  The rounds' transcripts live on the device, so an open tick needs
 // nothing from the host and the next one can be enqueued before it ends;
 // the host steps in once per job, to sample the queries after its last
-// round and to assemble its proof after the gather.
+// round and to copy its query records behind the proof header after the
+// gather.
 #pragma once
 #include "internal.hpp"
 
