@@ -268,17 +268,22 @@ def batched_vs_plain(bfd, ctx, torch, reps: int, tune_reps: int = 0, rates=None)
 
     p_ref, b_ref = plain(), batched()  # warm-up (also builds every code)
     assert p_ref == [b.tobytes() for b in b_ref], "batched proofs differ from the plain proofs"
+    # each leg's proofs are dropped after its clock stops: freeing 64 fresh
+    # output buffers (~100 MB, unmapped page by page) costs the caller
+    # ~2.5 ms and is not the prover's work
     tp, tb = [], []
     for _ in range(reps):
         ctx.synchronize()
         t0 = time.perf_counter()
-        plain()
+        out = plain()
         ctx.synchronize()
         tp.append(time.perf_counter() - t0)
+        del out
         t0 = time.perf_counter()
-        batched()
+        out = batched()
         ctx.synchronize()
         tb.append(time.perf_counter() - t0)
+        del out
     total = sum(1 << n for n in sizes)
     t_p, t_b = statistics.median(tp), statistics.median(tb)
     return {"workload": "64 polys, log2 sizes U{12..22} seed 0, rate 1/2", "total_evals": total,
