@@ -275,6 +275,13 @@ int bfd_ctx_create(int device, uint64_t hbm_budget_bytes, bfd_ctx **out) {
         ctx->budget = hbm_budget_bytes;
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        // the stream is destroyed again if set-up fails below
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() {
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } guard{ctx->stream};
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         ctx->cache.st = ctx->stream;
@@ -289,6 +296,7 @@ int bfd_ctx_create(int device, uint64_t hbm_budget_bytes, bfd_ctx **out) {
             uint64_t thr = UINT64_MAX;
             (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
         }
+        guard.s = nullptr;
         *out = ctx.release();
     });
 }

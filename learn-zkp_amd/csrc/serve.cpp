@@ -170,6 +170,15 @@ int bfd_server_create(bfd_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t max_acti
         S->ctx = ctx;
         S->budget = hbm_budget_bytes;
         S->max_active = max_active;
+        // streams made here are destroyed again if set-up fails below
+        struct StreamGuard {
+            bfd_server *s;
+            ~StreamGuard() {
+                if (!s) return;
+                if (s->s_commit) (void)hipStreamDestroy(s->s_commit);
+                if (s->s_open) (void)hipStreamDestroy(s->s_open);
+            }
+        } guard{S.get()};
         hip_check(hipStreamCreateWithFlags(&S->s_commit, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamCreateWithFlags(&S->s_open, hipStreamNonBlocking), "hipStreamCreate");
         S->commit_lane.reset(new Lane(S->s_commit, nullptr));
@@ -196,6 +205,7 @@ int bfd_server_create(bfd_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t max_acti
         if (!hbm_budget_bytes) S->budget = arena;
         bfd_server *raw = S.get();
         S->worker = std::thread(worker_loop, raw);
+        guard.s = nullptr;
         *out = S.release();
     });
 }
