@@ -21,6 +21,9 @@ def bfd():
 
 @pytest.fixture(scope="session")
 def oracle():
+    import os
     import oracle as O
     O.lib()
+    if not os.environ.get("BFO_THREADS"):
+        O.set_threads(min(16, os.cpu_count() or 1))  # exact arithmetic: results do not depend on it
     return O
