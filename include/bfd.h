@@ -102,6 +102,13 @@ int bfd_commitment_destroy(bfd_commitment *comm);
  * every column (width x 16 bytes), point has d elements */
 int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uint8_t *values);
 
+/* Basefold::eval (basefold.rs:126-131) of data that has no commitment: the
+ * trait's eval takes &Self::Data, a height x width row-major matrix (host,
+ * or device memory with evals_on_device; height = 2^n, n = the point's
+ * length).  values[col] = MLE(col)(point), width x 16 bytes. */
+int bfd_eval_data(bfd_ctx *ctx, const void *evals, int evals_on_device, size_t height, size_t width,
+                  const uint8_t *point, uint8_t *values);
+
 /* Basefold::open (basefold.rs:133-232) for one PolyEvals {point, values}
  * (values: width x 16 bytes).  Width > 1 first samples the random scalars by
  * eq from the challenger (basefold.rs:166-183).  The challenger is advanced
@@ -116,6 +123,37 @@ int bfd_open_into(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm
                   const uint8_t *values, bfd_challenger *challenger, uint8_t *proof, size_t capacity,
                   size_t *proof_len);
 void bfd_proof_free(uint8_t *proof);
+
+/* ---- Basefold::open driven by the caller's own transcript ----
+ * The trait's open (poly-commit/src/lib.rs:37-42) takes any
+ * `impl FieldChallenger<F> + CanObserve<Commitment>`, whose state need not
+ * be exportable (plonky3's HashChallenger keeps its buffers private).  These
+ * calls run the same opening with every transcript step on the caller's
+ * side, in the reference's order (basefold.rs:157-203, 335-369):
+ *   begin      width > 1: the caller first samples ceil(log2 width) scalar
+ *              challenges (sample_ext_vec, basefold.rs:398-408) and passes
+ *              them; width 1 passes none
+ *   d times:   bfd_open_round -> pi-tree root, compressed round poly
+ *              [c0, c2]; the caller observes the root, then c0 and c2, and
+ *              samples r; bfd_open_challenge(r)
+ *   bfd_open_final_poly -> final_poly; the caller observes it and samples
+ *              num_queries indices with sample_bits(log2_c + d)
+ *   bfd_open_finish(indices) -> the bincode proof (free with bfd_proof_free)
+ * The proof equals bfd_open's when the caller's transcript is a
+ * GenericChallenger<_, Keccak256>.  The opening borrows the commitment and
+ * the code until bfd_opening_destroy.  A call out of order is BFD_ERR_STATE.
+ * Each round synchronises once; bfd_open, with the transcript on the device,
+ * is the faster form for a challenger the library can take over. */
+typedef struct bfd_opening bfd_opening;
+int bfd_open_begin(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *comm, const uint8_t *point,
+                   const uint8_t *values, const uint8_t *scalar_challenges, size_t num_scalar_challenges,
+                   bfd_opening **out);
+int bfd_open_rounds_left(const bfd_opening *op, uint32_t *rounds);
+int bfd_open_round(bfd_opening *op, uint8_t root[32], uint8_t round_poly[32]);
+int bfd_open_challenge(bfd_opening *op, const uint8_t x[16]);
+int bfd_open_final_poly(bfd_opening *op, uint8_t out[16]);
+int bfd_open_finish(bfd_opening *op, const uint64_t *indices, size_t count, uint8_t **proof, size_t *proof_len);
+int bfd_opening_destroy(bfd_opening *op);
 /* Exact bincode size of the proof for (lambda, log2_c, d), width 1, and for
  * a commitment of `width` columns (each opening carries a whole row). */
 uint64_t bfd_proof_size(uint32_t lambda, uint32_t log2_c, uint32_t d);
@@ -237,6 +275,19 @@ int bfd_code_interpolate(const bfd_code *code, uint32_t i, uint64_t j, const uin
 typedef struct bfd_sc bfd_sc;
 int bfd_sc_quadratic_create(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys, int polys_on_device,
                             uint32_t npairs, const uint8_t *scalars, const uint32_t *pair_idx, bfd_sc **out);
+/* EvalProver::new(Eval::new(point), MultiPoly::base(poly))
+ * (sumcheck/src/function/eval.rs:27-47 over eval_impr.rs:13-207): the
+ * function eq(point, .) * poly over n variables, the Basefold sumcheck
+ * function.  poly: 2^n evaluations (host or device, copied); point: n
+ * coordinates, none 0 or 1 (r and 1 - r are inverted, eval_impr.rs:20-27,
+ * 69-79).  On this handle bfd_sc_round_poly returns the degree-2 round
+ * polynomial [c0, c1, c2] of eval.rs:90-104 (the subclaim argument is used
+ * in the first round; later rounds use the prover's inner subclaim, as the
+ * reference does), fix_last_var also reduces the inner subclaim, and
+ * evaluations gives the one table value.  compute_sum is eval.rs:86-88;
+ * once every variable is fixed (and n > 0) it is BFD_ERR_STATE. */
+int bfd_sc_eval_create(bfd_ctx *ctx, uint32_t n, const void *poly, int poly_on_device, const uint8_t *point,
+                       bfd_sc **out);
 int bfd_sc_num_vars(const bfd_sc *h, uint32_t *num_vars);  /* variables left */
 int bfd_sc_compute_sum(bfd_sc *h, uint8_t out[16]);
 int bfd_sc_round_poly(bfd_sc *h, const uint8_t subclaim[16], uint8_t coeffs[48]);
@@ -266,6 +317,12 @@ int bfd_verify(bfd_ctx *ctx, const bfd_code *code, size_t width, const uint8_t r
  * Montgomery-form product a*b*X^-128 (Mul) and inverse (invert_or_zero,
  * 0 for 0), little-endian 16-byte elements ---- */
 int bfd_field_mul(const uint8_t a[16], const uint8_t b[16], uint8_t out[16]);
+/* The same product on the device, elementwise: out[i] = a[i] * b[i], or
+ * a[i] * b[0] with shared_b set (b prepared once, the form the fold, fix and
+ * NTT kernels use for a challenge or twiddle).  a, out: n elements; b: n
+ * (1 with shared_b); host buffers, or device memory with on_device. */
+int bfd_field_mul_batch(bfd_ctx *ctx, const void *a, const void *b, int shared_b, void *out, size_t n,
+                        int on_device);
 int bfd_field_inv(const uint8_t a[16], uint8_t out[16]);
 
 /* ---- input generation: Polyval::random_vec(n, StdRng::seed_from_u64(seed))

@@ -145,6 +145,22 @@ def lib():
                                        ctypes.POINTER(ctypes.c_int)]
         L.bfd_commit_eval.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t,
                                       ctypes.c_size_t, u8p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, u8p]
+        L.bfd_eval_data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t,
+                                    u8p, u8p]
+        L.bfd_open_begin.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u8p, u8p, u8p,
+                                     ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
+        L.bfd_open_rounds_left.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+        L.bfd_open_round.argtypes = [ctypes.c_void_p, u8p, u8p]
+        L.bfd_open_challenge.argtypes = [ctypes.c_void_p, u8p]
+        L.bfd_open_final_poly.argtypes = [ctypes.c_void_p, u8p]
+        L.bfd_open_finish.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t,
+                                      ctypes.POINTER(u8p), ctypes.POINTER(ctypes.c_size_t)]
+        L.bfd_opening_destroy.argtypes = [ctypes.c_void_p]
+        L.bfd_sc_eval_create.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int, u8p,
+                                         ctypes.POINTER(ctypes.c_void_p)]
+        L.bfd_sc_destroy.argtypes = [ctypes.c_void_p]
+        L.bfd_field_mul_batch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         L.bfd_field_mul.argtypes = [u8p, u8p, u8p]
         L.bfd_field_inv.argtypes = [u8p, u8p]
         L.bfd_random_vec.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u8p]
@@ -549,10 +565,20 @@ class Basefold:
         return _verify_batch(self, width, comms, [_elems(e.point) for e in evals], [_elems(e.values) for e in evals],
                              proofs, challengers)
 
+    def eval_data(self, data, point, width: Optional[int] = None) -> np.ndarray:
+        """Basefold::eval (basefold.rs:126-131) on the data itself, as the
+        trait's eval(&Data, &Point) takes it: no commitment needed.  ``data``
+        as for commit; one value per column, shape (width, 2)."""
+        arr = np.ascontiguousarray(data, dtype=np.uint64)
+        if width is None:
+            width = arr.shape[1] if arr.ndim == 3 else 1
+        flat = arr.reshape(-1, 2)
+        return eval_data(flat, point, width, self.ctx)
+
     def open(self, comm_data: BasefoldCommitmentData, evals: Sequence[PolyEvals],
              challenger: GenericChallenger) -> bytes:
         if not evals:
-            return b""
+            return EMPTY_PROOF  # basefold.rs:139-141
         point, values = _elems(evals[0].point), _elems(evals[0].values)
         if len(values) != comm_data.width:
             raise BfdError(BFD_ERR_ARG, "open needs one value per committed column")
@@ -564,6 +590,90 @@ class Basefold:
             return ctypes.string_at(proof, plen.value)
         finally:
             lib().bfd_proof_free(proof)
+
+
+    def open_stepwise(self, comm_data: BasefoldCommitmentData, evals: Sequence[PolyEvals], challenger) -> bytes:
+        """Basefold::open (basefold.rs:133-232) with every transcript step on
+        the caller's challenger, which the library never sees: any object with
+        observe(bytes), observe_ext_slice(elems), sample_ext_element() and
+        sample_bits(bits), in the reference's order (bfd_open_begin ..
+        bfd_open_finish)."""
+        if not evals:
+            return EMPTY_PROOF  # basefold.rs:139-141
+        point, values = _elems(evals[0].point), _elems(evals[0].values)
+        w = comm_data.width
+        if len(values) != w:
+            raise BfdError(BFD_ERR_ARG, "open needs one value per committed column")
+        lw = (w - 1).bit_length() if w > 1 else 0
+        scal = np.zeros((max(1, lw), 2), dtype=np.uint64)
+        for i in range(lw):  # sample_random_scalars_by_eq (basefold.rs:398-408)
+            scal[i] = _elems(challenger.sample_ext_element())[0]
+        pt = point if len(point) else np.zeros((1, 2), dtype=np.uint64)
+        h = ctypes.c_void_p()
+        _check(lib().bfd_open_begin(self.ctx.h, self.code_h, comm_data.handle, _p(pt), _p(values), _p(scal), lw,
+                                    ctypes.byref(h)))
+        try:
+            left = ctypes.c_uint32()
+            _check(lib().bfd_open_rounds_left(h, ctypes.byref(left)))
+            root = (ctypes.c_uint8 * 32)()
+            rp = np.zeros((2, 2), dtype=np.uint64)
+            for _ in range(left.value):
+                _check(lib().bfd_open_round(h, root, _p(rp)))
+                challenger.observe(bytes(root))               # basefold.rs:356
+                challenger.observe_ext_slice(rp)              # sumcheck/src/lib.rs:94
+                x = np.ascontiguousarray(_elems(challenger.sample_ext_element()))
+                _check(lib().bfd_open_challenge(h, _p(x)))
+            fin = np.zeros((1, 2), dtype=np.uint64)
+            _check(lib().bfd_open_final_poly(h, _p(fin)))
+            challenger.observe_ext_slice(fin)                 # basefold.rs:194
+            q = self.num_queries
+            L = self.config.code.log2_c + self.config.code.d
+            idx = (ctypes.c_uint64 * max(1, q))(*[challenger.sample_bits(L) for _ in range(q)])
+            proof = u8p()
+            plen = ctypes.c_size_t()
+            _check(lib().bfd_open_finish(h, idx, q, ctypes.byref(proof), ctypes.byref(plen)))
+            try:
+                return ctypes.string_at(proof, plen.value)
+            finally:
+                lib().bfd_proof_free(proof)
+        finally:
+            lib().bfd_opening_destroy(h)
+
+
+# bincode of BasefoldProof::default(): four empty Vec length prefixes
+# (basefold.rs:49-66), what open returns for no evaluations (:139-141)
+EMPTY_PROOF = bytes(32)
+
+
+def eval_data(flat, point, width: int = 1, ctx: Optional[Context] = None, device_ptr: Optional[int] = None,
+              height: Optional[int] = None) -> np.ndarray:
+    """MLE value of every column of a row-major height x width matrix at
+    ``point`` (bfd_eval_data): host data as a flat (height*width, 2) array, or
+    device data at ``device_ptr`` with ``height`` given."""
+    ctx = ctx or default_context()
+    point = _elems(point)
+    pt = point if len(point) else np.zeros((1, 2), dtype=np.uint64)
+    out = np.zeros((width, 2), dtype=np.uint64)
+    if device_ptr is not None:
+        _check(lib().bfd_eval_data(ctx.h, ctypes.c_void_p(device_ptr), 1, height, width, _p(pt), _p(out)))
+    else:
+        flat = _elems(flat)
+        if width < 1 or len(flat) % width:
+            raise BfdError(BFD_ERR_ARG, "data does not hold whole rows of the given width")
+        _check(lib().bfd_eval_data(ctx.h, ctypes.c_void_p(flat.ctypes.data), 0, len(flat) // width, width, _p(pt),
+                                   _p(out)))
+    return out
+
+
+def field_mul_batch(a, b, shared_b: bool = False, ctx: Optional[Context] = None) -> np.ndarray:
+    """Elementwise POLYVAL product on the device (bfd_field_mul_batch): a[i] * b[i],
+    or a[i] * b[0] with shared_b."""
+    ctx = ctx or default_context()
+    a, b = np.ascontiguousarray(_elems(a)), np.ascontiguousarray(_elems(b))
+    out = np.zeros_like(a)
+    _check(lib().bfd_field_mul_batch(ctx.h, ctypes.c_void_p(a.ctypes.data), ctypes.c_void_p(b.ctypes.data),
+                                     int(shared_b), ctypes.c_void_p(out.ctypes.data), len(a), 0))
+    return out
 
 
 # results of Basefold::verify (basefold.rs:234-325) as bfd_verify_batch reports them
@@ -817,6 +927,71 @@ class QuadraticProver:
         if self.num_vars != 0:
             return None
         out = np.zeros((self.m, 2), dtype=np.uint64)
+        _check(lib().bfd_sc_evaluations(self.h, _p(out)))
+        return out
+
+
+class EvalProver:
+    """EvalProver (sumcheck/src/function/eval.rs:27-116 over eval_impr.rs) of
+    eq(point, .) * poly with the table resident in device memory (bfd_sc_*):
+    the Basefold sumcheck function, one SumcheckFunctionProver call per round.
+    compress keeps [c0, c2]; decompress restores c1 = subclaim + c2
+    (characteristic 2, eval.rs:68-79)."""
+
+    def __init__(self, point, poly, ctx: Optional[Context] = None, device_ptr: Optional[int] = None):
+        self.ctx = ctx or default_context()
+        point = _elems(point)
+        n = len(point)
+        pt = point if n else np.zeros((1, 2), dtype=np.uint64)
+        h = ctypes.c_void_p()
+        if device_ptr is not None:
+            _check(lib().bfd_sc_eval_create(self.ctx.h, n, ctypes.c_void_p(device_ptr), 1, _p(pt), ctypes.byref(h)))
+        else:
+            P = np.ascontiguousarray(_elems(poly))
+            if len(P) != 1 << n:
+                raise BfdError(BFD_ERR_ARG, "poly must hold 2^n evaluations")
+            _check(lib().bfd_sc_eval_create(self.ctx.h, n, ctypes.c_void_p(P.ctypes.data), 0, _p(pt),
+                                            ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bfd_sc_destroy(self.h)
+        self.h = None
+
+    @property
+    def num_vars(self) -> int:
+        v = ctypes.c_uint32()
+        _check(lib().bfd_sc_num_vars(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def compute_sum(self, round: int = 0) -> np.ndarray:
+        out = np.zeros((1, 2), dtype=np.uint64)
+        _check(lib().bfd_sc_compute_sum(self.h, _p(out)))
+        return out
+
+    def compute_round_poly(self, round: int, subclaim) -> np.ndarray:
+        out = np.zeros((3, 2), dtype=np.uint64)
+        _check(lib().bfd_sc_round_poly(self.h, _p(np.ascontiguousarray(_elems(subclaim))), _p(out)))
+        return out
+
+    @staticmethod
+    def compress_round_poly(round_poly) -> np.ndarray:
+        rp = _elems(round_poly)
+        return np.ascontiguousarray(rp[[0, 2]])
+
+    @staticmethod
+    def decompress_round_poly(subclaim, compressed) -> np.ndarray:
+        c = _elems(compressed)
+        return np.ascontiguousarray(np.stack([c[0], _elems(subclaim)[0] ^ c[1], c[1]]))
+
+    def fix_last_var(self, x) -> None:
+        _check(lib().bfd_sc_fix_last_var(self.h, _p(np.ascontiguousarray(_elems(x)))))
+
+    def evaluations(self) -> Optional[np.ndarray]:
+        if self.num_vars != 0:
+            return None
+        out = np.zeros((1, 2), dtype=np.uint64)
         _check(lib().bfd_sc_evaluations(self.h, _p(out)))
         return out
 

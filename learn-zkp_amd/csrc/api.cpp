@@ -32,24 +32,10 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     DevBuf comb;
     const gf *poly0 = cm->data.as<gf>(), *pi0 = cm->cw.as<gf>();
     if (w > 1) {
-        int lw = 0;
-        while (((size_t)1 << lw) < w) lw++;
-        std::vector<gf> rs(lw);
-        for (int i = 0; i < lw; i++) rs[i] = ch.sample();  // sample_ext_vec (challenger.rs:17-19)
-        std::vector<gf> sc = host_eq(rs.data(), lw);          // MultiPoly::eq, truncated to w
-        gf claim = gf_zero();
-        for (size_t t = 0; t < w; t++) claim = gf_add(claim, hmul(sc[t], load_elem(values_b + 16 * t)));
-        store_elem(claim_b, claim);
+        std::vector<gf> rs(scalar_challenges(w));
+        for (gf &v : rs) v = ch.sample();  // sample_ext_vec (challenger.rs:17-19)
+        combine_columns(ctx, cm, rs, values_b, claim_b, comb, poly0, pi0);
         claim_p = claim_b;
-        comb = ctx_buf(ctx, 16 * (cm->N + cm->M + w));
-        gf *p_poly = comb.as<gf>(), *p_pi = p_poly + cm->N, *p_sc = p_pi + cm->M;
-        ctx->scalar_pin.ensure(16 * w);
-        memcpy(ctx->scalar_pin.p, sc.data(), 16 * w);
-        hip_check(hipMemcpyAsync(p_sc, ctx->scalar_pin.p, 16 * w, hipMemcpyHostToDevice, st), "copy scalars");
-        ctx->timer.run(st, "k_rowdot", [&] { return launch::rowdot(st, cm->data.as<gf>(), w, p_sc, p_poly, cm->N); });
-        ctx->timer.run(st, "k_rowdot", [&] { return launch::rowdot(st, cm->cw.as<gf>(), w, p_sc, p_pi, cm->M); });
-        poly0 = p_poly;
-        pi0 = p_pi;
     }
     OpenRounds o;
     o.init((int)cm->n, (int)cm->b, code->q, point_b, claim_p);
@@ -161,33 +147,8 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     o.finish(load_elem(pin + ((rbb + 15) & ~(size_t)15)), ch);
     ctx->timer.host("round_transcript", now_ms() - th);
 
-    // openings: the query records are written on the device in their bincode
-    // layout (k_proof_records) and copied in behind the header
     double t0 = now_ms();
-    write_proof_header(o, dst);
-    const size_t hb = proof_header_bytes(o.d), body = sz - hb;
-    if (o.q) {
-        const size_t up = sizeof(ProofJob) + 8 * o.q;
-        ctx->pinned.ensure(up);
-        uint8_t *hu = (uint8_t *)ctx->pinned.p;
-        DevBuf dj = ctx_buf(ctx, up), dout = ctx_buf(ctx, body);
-        const ProofJob J =
-            proof_job(o, lay, reinterpret_cast<const uint64_t *>(dj.as<uint8_t>() + sizeof(ProofJob)), dout.as<uint8_t>(), 0);
-        memcpy(hu, &J, sizeof J);
-        memcpy(hu + sizeof J, o.idx.data(), 8 * o.q);
-        hip_check(hipMemcpyAsync(dj.p, hu, up, hipMemcpyHostToDevice, st), "copy query indices");
-        ctx->timer.run(st, "k_proof_records",
-                       [&] { return launch::proof_records(st, dj.as<ProofJob>(), 1, (uint32_t)o.q); });
-        ctx->gather_pin.ensure(body);
-        hip_check(hipMemcpyAsync(ctx->gather_pin.p, dout.p, body, hipMemcpyDeviceToHost, st), "copy query records");
-        hip_check(hipStreamSynchronize(st), "query records");
-        ctx->timer.collect();
-        ChunkCopy cc;
-        cc.add(dst + hb, (const uint8_t *)ctx->gather_pin.p, body);
-        cc.run();
-    } else {
-        ctx->timer.collect();
-    }
+    write_proof(ctx, o, lay, sz, dst);
     owned.release();
     *proof = dst;
     *proof_len = sz;
@@ -844,6 +805,34 @@ int bfd_field_mul(const uint8_t a[16], const uint8_t b[16], uint8_t out[16]) {
     return guarded([&] {
         if (!a || !b || !out) fail(BFD_ERR_ARG, "null argument");
         store_elem(out, hmul(load_elem(a), load_elem(b)));
+    });
+}
+
+int bfd_field_mul_batch(bfd_ctx *ctx, const void *a, const void *b, int shared_b, void *out, size_t n,
+                        int on_device) {
+    return guarded([&] {
+        if (!ctx || ((!a || !b || !out) && n)) fail(BFD_ERR_ARG, "null argument");
+        if (!n) return;
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        set_device(ctx);
+        hipStream_t st = ctx->stream;
+        const size_t nb = shared_b ? 1 : n;
+        DevBuf buf;
+        const gf *da = (const gf *)a, *db = (const gf *)b;
+        gf *dout = (gf *)out;
+        if (!on_device) {
+            buf = ctx_buf(ctx, 16 * (2 * n + nb));
+            gf *p = buf.as<gf>();
+            hip_check(hipMemcpyAsync(p, a, 16 * n, hipMemcpyHostToDevice, st), "copy a");
+            hip_check(hipMemcpyAsync(p + n, b, 16 * nb, hipMemcpyHostToDevice, st), "copy b");
+            da = p;
+            db = p + n;
+            dout = p + n + nb;
+        }
+        ctx->timer.run(st, "k_mul", [&] { return launch::mul(st, da, db, dout, n, shared_b != 0); });
+        if (!on_device) hip_check(hipMemcpyAsync(out, dout, 16 * n, hipMemcpyDeviceToHost, st), "copy product");
+        hip_check(hipStreamSynchronize(st), "mul");
+        ctx->timer.collect();
     });
 }
 

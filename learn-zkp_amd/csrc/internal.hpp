@@ -680,6 +680,38 @@ inline void do_commit(bfd_ctx *ctx, const bfd_code *code, const void *evals, boo
     *out = cm.release();
 }
 
+// challenges the random scalars of a width-w opening take: log2 of w rounded
+// up (sample_random_scalars_by_eq, basefold.rs:398-408)
+inline size_t scalar_challenges(size_t w) {
+    size_t lw = 0;
+    while (((size_t)1 << lw) < w) lw++;
+    return w > 1 ? lw : 0;
+}
+
+// Width > 1 (basefold.rs:166-183): the random scalars eq(rs, .) truncated to
+// w combine the claims (into claim_b), the evaluation rows (N) and the
+// codeword rows (M) into one polynomial and pi_0, held in comb; poly0 and
+// pi0 point at them afterwards.  Enqueued on the context stream.
+inline void combine_columns(bfd_ctx *ctx, const bfd_commitment *cm, const std::vector<gf> &rs,
+                            const uint8_t *values_b, uint8_t claim_b[16], DevBuf &comb, const gf *&poly0,
+                            const gf *&pi0) {
+    hipStream_t st = ctx->stream;
+    const size_t w = cm->w;
+    std::vector<gf> sc = host_eq(rs.data(), (int)rs.size());  // MultiPoly::eq, truncated to w
+    gf claim = gf_zero();
+    for (size_t t = 0; t < w; t++) claim = gf_add(claim, hmul(sc[t], load_elem(values_b + 16 * t)));
+    store_elem(claim_b, claim);
+    comb = ctx_buf(ctx, 16 * (cm->N + cm->M + w));
+    gf *p_poly = comb.as<gf>(), *p_pi = p_poly + cm->N, *p_sc = p_pi + cm->M;
+    ctx->scalar_pin.ensure(16 * w);
+    memcpy(ctx->scalar_pin.p, sc.data(), 16 * w);
+    hip_check(hipMemcpyAsync(p_sc, ctx->scalar_pin.p, 16 * w, hipMemcpyHostToDevice, st), "copy scalars");
+    ctx->timer.run(st, "k_rowdot", [&] { return launch::rowdot(st, cm->data.as<gf>(), w, p_sc, p_poly, cm->N); });
+    ctx->timer.run(st, "k_rowdot", [&] { return launch::rowdot(st, cm->cw.as<gf>(), w, p_sc, p_pi, cm->M); });
+    poly0 = p_poly;
+    pi0 = p_pi;
+}
+
 inline void put_u64(std::vector<uint8_t> &v, uint64_t x) {
     uint8_t b[8];
     memcpy(b, &x, 8);
@@ -750,29 +782,49 @@ struct OpenRounds {
     }
     int half_eq_vars() const { return n > 0 ? n - 1 : 0; }
 
-    // Round k (level i = d-1-k): observe the pi-tree root, turn the inner
-    // product S = sum half_eq[b] * poly_lo[b] into the round polynomial,
-    // observe it, sample the challenge and advance both subclaims.
-    gf round(int k, const uint8_t root[32], gf S, Challenger &ch) {
+    // The Eval prover's round polynomial of round k (level i = d-1-k) from
+    // the inner product S = sum half_eq[b] * poly_lo[b]: the inner linear
+    // polynomial [coeff0, coeff1] and the outer [rp0, rp1, rp2].
+    struct RoundPoly {
+        gf coeff0, coeff1, rp0, rp1, rp2;
+    };
+    RoundPoly round_poly(int k, gf S) const {
         const int i = d - 1 - k;
-        memcpy(pi_roots[k].data(), root, 32);
-        ch.observe(root, 32);                                           // basefold.rs:356
         const gf sub = (i == n - 1) ? subclaim : inner_sub;             // eval.rs:90-94
-        const gf coeff0 = hmul(cf[i], S);
-        const gf eval1 = hmul(gf_add(sub, hmul(gf_add(gf_one(), r[i]), coeff0)), rinv[i]);  // eval_impr.rs:41-43
-        const gf coeff1 = gf_add(eval1, coeff0);
+        RoundPoly p;
+        p.coeff0 = hmul(cf[i], S);
+        const gf eval1 = hmul(gf_add(sub, hmul(gf_add(gf_one(), r[i]), p.coeff0)), rinv[i]);  // eval_impr.rs:41-43
+        p.coeff1 = gf_add(eval1, p.coeff0);
         // delta (eval_impr.rs:45-58): scalar = eq(r[i+1..], x[i+1..]); in
         // characteristic 2 the two factors are s (1 + r_i) and s
         const gf d0 = hmul(s_delta, gf_add(gf_one(), r[i]));
         const gf d1 = s_delta;
-        const gf rp0 = hmul(coeff0, d0), rp1 = gf_add(hmul(coeff0, d1), hmul(coeff1, d0)), rp2 = hmul(coeff1, d1);
-        comp[k] = {rp0, rp2};                                           // eval.rs:64-66
-        ch.observe_elem(rp0);                                           // sumcheck/src/lib.rs:94
-        ch.observe_elem(rp2);
-        const gf x = ch.sample();                                       // lib.rs:96
-        inner_sub = gf_add(coeff0, hmul(coeff1, x));                    // eval.rs:110
-        subclaim = gf_add(rp0, hmul(x, gf_add(rp1, hmul(x, rp2))));     // lib.rs:100 (horner)
+        p.rp0 = hmul(p.coeff0, d0);
+        p.rp1 = gf_add(hmul(p.coeff0, d1), hmul(p.coeff1, d0));
+        p.rp2 = hmul(p.coeff1, d1);
+        return p;
+    }
+    // After the challenge x of round k: both subclaims and the delta scalar
+    // advance (fix_last_var and SumcheckSubclaim::reduce).
+    void advance(int k, const RoundPoly &p, gf x) {
+        const int i = d - 1 - k;
+        inner_sub = gf_add(p.coeff0, hmul(p.coeff1, x));                // eval.rs:110
+        subclaim = gf_add(p.rp0, hmul(x, gf_add(p.rp1, hmul(x, p.rp2))));  // lib.rs:100 (horner)
         s_delta = hmul(s_delta, gf_add(gf_one(), gf_add(r[i], x)));
+    }
+
+    // Round k with the transcript on the host: observe the pi-tree root,
+    // observe the compressed round polynomial, sample the challenge and
+    // advance both subclaims.
+    gf round(int k, const uint8_t root[32], gf S, Challenger &ch) {
+        memcpy(pi_roots[k].data(), root, 32);
+        ch.observe(root, 32);                                           // basefold.rs:356
+        const RoundPoly p = round_poly(k, S);
+        comp[k] = {p.rp0, p.rp2};                                       // eval.rs:64-66
+        ch.observe_elem(p.rp0);                                         // sumcheck/src/lib.rs:94
+        ch.observe_elem(p.rp2);
+        const gf x = ch.sample();                                       // lib.rs:96
+        advance(k, p, x);
         return x;
     }
 
@@ -1073,6 +1125,39 @@ struct ChunkCopy {
         parallel_for((n + kChunk - 1) / kChunk, [&](size_t c) { memset(dst + c * kChunk, 0, std::min(kChunk, n - c * kChunk)); });
     }
 };
+
+// One opening's proof into dst (sz = proof_size bytes, pages already
+// touched): the header from o on the host, then the query records at o.idx,
+// written on the device in their bincode layout (k_proof_records) and copied
+// in behind the header.  Stages through ctx->pinned and ctx->gather_pin and
+// synchronises the context stream.
+inline void write_proof(bfd_ctx *ctx, const OpenRounds &o, const OpenLayout &lay, size_t sz, uint8_t *dst) {
+    hipStream_t st = ctx->stream;
+    write_proof_header(o, dst);
+    const size_t hb = proof_header_bytes(o.d), body = sz - hb;
+    if (!o.q) {
+        hip_check(hipStreamSynchronize(st), "opening");
+        ctx->timer.collect();
+        return;
+    }
+    const size_t up = sizeof(ProofJob) + 8 * o.q;
+    ctx->pinned.ensure(up);
+    uint8_t *hu = (uint8_t *)ctx->pinned.p;
+    DevBuf dj = ctx_buf(ctx, up), dout = ctx_buf(ctx, body);
+    const ProofJob J =
+        proof_job(o, lay, reinterpret_cast<const uint64_t *>(dj.as<uint8_t>() + sizeof(ProofJob)), dout.as<uint8_t>(), 0);
+    memcpy(hu, &J, sizeof J);
+    memcpy(hu + sizeof J, o.idx.data(), 8 * o.q);
+    hip_check(hipMemcpyAsync(dj.p, hu, up, hipMemcpyHostToDevice, st), "copy query indices");
+    ctx->timer.run(st, "k_proof_records", [&] { return launch::proof_records(st, dj.as<ProofJob>(), 1, (uint32_t)o.q); });
+    ctx->gather_pin.ensure(body);
+    hip_check(hipMemcpyAsync(ctx->gather_pin.p, dout.p, body, hipMemcpyDeviceToHost, st), "copy query records");
+    hip_check(hipStreamSynchronize(st), "query records");
+    ctx->timer.collect();
+    ChunkCopy cc;
+    cc.add(dst + hb, (const uint8_t *)ctx->gather_pin.p, body);
+    cc.run();
+}
 
 }  // namespace rt
 }  // namespace bfd

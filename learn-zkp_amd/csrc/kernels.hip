@@ -1346,6 +1346,21 @@ __global__ __launch_bounds__(KB) void k_bench_keccak(uint64_t *io, size_t n, int
     }
 }
 
+// out[i] = a[i] * b[i] (gf_mul_reg), or with shared_b a[i] * b[0] through
+// b's masked pieces prepared once (gf_prepare / gf_mul_pre), the form the
+// fold, fix and NTT kernels use for a challenge or a twiddle
+__global__ __launch_bounds__(CB) void k_mul(const gf *__restrict__ a, const gf *__restrict__ b, gf *__restrict__ out,
+                                            size_t n, int shared_b) {
+    if (shared_b) {
+        const gf_pre pb = gf_prepare(gf_ld(b));
+        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+            gf_st(out + i, gf_mul_pre(gf_ld(a + i), pb));
+        return;
+    }
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        gf_st(out + i, gf_mul_reg(gf_ld(a + i), gf_ld(b + i)));
+}
+
 // plain 16-byte-per-thread copy: the reference point for HBM byte counters
 // p[i] = splitmix64(seed + i): synthetic input with the bit statistics of
 // random field elements (timing runs; the switching activity of the data
@@ -1610,6 +1625,11 @@ hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters) {
         k_bench_mul_comb<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
     else
         k_bench_mul_reg<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
+    return hipGetLastError();
+}
+hipError_t mul(hipStream_t st, const gf *a, const gf *b, gf *out, size_t n, bool shared_b) {
+    if (!n) return hipSuccess;
+    k_mul<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(a, b, out, n, shared_b ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n) {

@@ -301,6 +301,20 @@ def product_sumcheck(f, g, r):
     return claim, polys.reshape(n, 3, 2), chal, fin, sub
 
 
+def eval_sumcheck(poly, point, claim):
+    """prove_sumcheck over EvalProver (eq(point, .) * poly) with a fresh
+    challenger. Returns (round polys (n x 3 x 2, uncompressed), challenges,
+    final table value, subclaim)."""
+    P, p, c = as_elems(poly), as_elems(point), as_elems(claim)
+    n = len(p)
+    rp, ch, fin, sub = elems(3 * max(n, 1)), elems(max(n, 1)), elems(1), elems(1)
+    pp = p if n else elems(1)
+    rc = lib().bfo_sumcheck_eval(n, _p(P), _p(pp), _p(c), _p(rp), _p(ch), _p(fin), _p(sub))
+    if rc != 0:
+        raise RuntimeError(f"oracle eval sumcheck failed ({rc})")
+    return rp[:3 * n].reshape(n, 3, 2), ch[:n], fin, sub
+
+
 def quadratic_sumcheck(polys, scalars, pairs):
     """polys: m tables of 2^n elements (m x 2^n x 2), scalars: one per pair,
     pairs: [(a, b), ...]. Returns (claim, round polys (n x 2), challenges,

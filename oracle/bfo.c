@@ -751,6 +751,31 @@ static u128 inner_product(const u128 *a, const u128 *b, size_t n) {
 }
 
 
+/* EvalProver::compute_round_poly (eval.rs:89-106) over EvalImprProver
+ * (eval_impr.rs:187-197) for round variable i of n: sub is the subclaim the
+ * round uses (the outer one in the first round, the inner one after,
+ * eval.rs:90-94), chal[j] the challenge of variable j > i.  Writes the
+ * degree-2 round polynomial rp and the inner linear one [coeff0, coeff1]. */
+static void eval_round_poly(int n, int i, const u128 *r, const u128 *rinv, const u128 *omr_inv,
+                            const u128 *half_eq, const u128 *poly, const u128 *chal, u128 sub, u128 rp[3],
+                            u128 *coeff0_out, u128 *coeff1_out) {
+    const int hn = n > 0 ? n - 1 : 0;
+    u128 cf = ONE;                                /* correcting_factor(i), eval_impr.rs:81-88 */
+    for (int j = i; j < hn; j++) cf = fmul(cf, omr_inv[j]);
+    u128 coeff0 = fmul(cf, inner_product(half_eq, poly, (size_t)1 << i));
+    u128 eval1 = fmul(sub ^ fmul(ONE ^ r[i], coeff0), rinv[i]);  /* eval_impr.rs:41-43 */
+    u128 coeff1 = eval1 ^ coeff0;
+    u128 s = ONE;                                 /* delta_scalar, eval_impr.rs:45-51 */
+    if (i != n - 1) s = eq_eval(&r[i + 1], &chal[i + 1], (size_t)(n - 1 - i));
+    u128 d0 = fmul(s, ONE ^ r[i]);                /* delta, eval_impr.rs:53-58 */
+    u128 d1 = fmul(s, r[i] ^ (ONE ^ r[i]));
+    rp[0] = fmul(coeff0, d0);
+    rp[1] = fmul(coeff0, d1) ^ fmul(coeff1, d0);
+    rp[2] = fmul(coeff1, d1);
+    *coeff0_out = coeff0;
+    *coeff1_out = coeff1;
+}
+
 typedef struct {
     int n, b, d, L;
     size_t N, M, q;
@@ -891,18 +916,9 @@ int bfo_basefold_prove_px(int lambda, int log2_c, int n, size_t width, const uin
         memcpy(P.pi_roots[k], merkle_root_ptr(&P.pi_trees[k]), 32);
         bfo_challenger_observe(ch, P.pi_roots[k], 32);
 
-        /* EvalProver::compute_round_poly (eval.rs:89-106) over EvalImprProver (eval_impr.rs:187-197) */
-        u128 sub = (i == n - 1) ? subclaim : inner_sub;
-        u128 cf = ONE;                                /* correcting_factor(i), eval_impr.rs:81-88 */
-        for (int j = i; j < hn; j++) cf = fmul(cf, omr_inv[j]);
-        u128 coeff0 = fmul(cf, inner_product(half_eq, poly, (size_t)1 << i));
-        u128 eval1 = fmul(sub ^ fmul(ONE ^ r[i], coeff0), rinv[i]);  /* eval_impr.rs:41-43 */
-        u128 coeff1 = eval1 ^ coeff0;
-        u128 s = ONE;                                 /* delta_scalar, eval_impr.rs:45-51 */
-        if (i != n - 1) s = eq_eval(&r[i + 1], &chal[i + 1], (size_t)(n - 1 - i));
-        u128 d0 = fmul(s, ONE ^ r[i]);                /* delta, eval_impr.rs:53-58 */
-        u128 d1 = fmul(s, r[i] ^ (ONE ^ r[i]));
-        u128 rp[3] = {fmul(coeff0, d0), fmul(coeff0, d1) ^ fmul(coeff1, d0), fmul(coeff1, d1)};
+        u128 rp[3], coeff0, coeff1;
+        eval_round_poly(n, i, r, rinv, omr_inv, half_eq, poly, chal, (i == n - 1) ? subclaim : inner_sub, rp,
+                        &coeff0, &coeff1);
         P.comp[k][0] = rp[0];                         /* compress: eval.rs:64-66 */
         P.comp[k][1] = rp[2];
         ch_observe_elem(ch, rp[0]);                   /* lib.rs:94 */
@@ -955,6 +971,55 @@ int bfo_basefold_prove_px(int lambda, int log2_c, int n, size_t width, const uin
 }
 
 void bfo_free(void *p) { free(p); }
+
+/* prove_sumcheck (sumcheck/src/lib.rs:56-103) over EvalProver::new(Eval::new(
+ * point), MultiPoly::base(poly)) (eval.rs:27-116) with a fresh keccak
+ * challenger: observe the claim, then per round observe the compressed
+ * [rp0, rp2], sample, fix_last_var and reduce both subclaims.  Outputs the
+ * full round polynomials [rp0, rp1, rp2] (n x 3), the challenges in round
+ * order, the final table value and the final subclaim. */
+int bfo_sumcheck_eval(int n, const uint8_t *poly_b, const uint8_t *point_b, const uint8_t claim_b[16],
+                      uint8_t *round_polys, uint8_t *challenges, uint8_t final_eval[16], uint8_t subclaim_out[16]) {
+    if (n < 0 || n > 40) return -1;
+    const size_t N = (size_t)1 << n;
+    u128 r[64], rinv[64], omr_inv[64], chal[64];
+    for (int i = 0; i < n; i++) {
+        r[i] = ld(point_b + 16 * i);
+        if (r[i] == 0 || r[i] == ONE) return -1;
+        rinv[i] = finv(r[i]);                         /* EvalImpr::new, eval_impr.rs:20-27 */
+    }
+    const int hn = n > 0 ? n - 1 : 0;                 /* HalfEq::new, eval_impr.rs:69-79 */
+    for (int j = 0; j < hn; j++) omr_inv[j] = finv(ONE ^ r[j]);
+    u128 *half_eq = malloc(sizeof(u128) * ((size_t)1 << hn));
+    u128 *poly = malloc(sizeof(u128) * N);
+    eq_table(r, hn, ONE, half_eq);
+    memcpy(poly, poly_b, 16 * N);
+    u128 subclaim = ld(claim_b), inner_sub = 0;
+    bfo_challenger *ch = bfo_challenger_new();
+    if (n > 0) ch_observe_elem(ch, subclaim);         /* lib.rs:63-67 */
+    for (int i = n - 1; i >= 0; i--) {
+        const int k = n - 1 - i;
+        u128 rp[3], coeff0, coeff1;
+        eval_round_poly(n, i, r, rinv, omr_inv, half_eq, poly, chal, (i == n - 1) ? subclaim : inner_sub, rp,
+                        &coeff0, &coeff1);
+        for (int t = 0; t < 3; t++) st(round_polys + 48 * k + 16 * t, rp[t]);
+        ch_observe_elem(ch, rp[0]);                   /* compress (eval.rs:64-66), lib.rs:94 */
+        ch_observe_elem(ch, rp[2]);
+        u128 x = ch_sample(ch);                       /* lib.rs:96 */
+        chal[i] = x;
+        st(challenges + 16 * k, x);
+        size_t ph = (size_t)1 << i;                   /* fix_last_var, multilinear.rs:183-198 */
+        for (size_t j = 0; j < ph; j++) poly[j] = fmul(x, poly[j + ph] ^ poly[j]) ^ poly[j];
+        inner_sub = coeff0 ^ fmul(coeff1, x);         /* eval.rs:108-111 */
+        subclaim = horner(rp, 3, x);                  /* lib.rs:100 */
+    }
+    st(final_eval, poly[0]);
+    st(subclaim_out, subclaim);
+    free(half_eq);
+    free(poly);
+    bfo_challenger_free(ch);
+    return 0;
+}
 
 /* basefold.rs:234-325 with verify_interleaved_folding_sumcheck (:371-396) and
  * Eval::decompress_round_poly (eval.rs:68-79). */

@@ -107,6 +107,17 @@ int bfo_basefold_verify_px(int lambda, int log2_c, int n, size_t width, const ui
 void bfo_free(void *p);
 
 /*
+ * prove_sumcheck (sumcheck/src/lib.rs:56-103) over the Basefold sumcheck
+ * function EvalProver (eq(point, .) * poly, sumcheck/src/function/eval.rs
+ * over eval_impr.rs) with a fresh keccak challenger; claim is observed first.
+ * Outputs the full round polynomials [c0, c1, c2] (n x 3), the challenges in
+ * round order, the final table value and the final subclaim.  Returns -1 for
+ * a point coordinate 0 or 1.
+ */
+int bfo_sumcheck_eval(int n, const uint8_t *poly, const uint8_t *point, const uint8_t claim[16],
+                      uint8_t *round_polys, uint8_t *challenges, uint8_t final_eval[16], uint8_t subclaim[16]);
+
+/*
  * Degree-3 product sumcheck  sum_b f(b) g(b) eq(r, b)  (no reference
  * counterpart; follows the prove_sumcheck driver of sumcheck/src/lib.rs:56-103).
  * Fresh keccak challenger; observes claim first.  Outputs n x 3 compressed

@@ -218,6 +218,35 @@ def test_product_sumcheck_invariants(oracle):
         assert (O.mul(O.mul(fx, gx), eq) == sub).all()
 
 
+def test_eval_sumcheck_invariants(oracle):
+    """The Basefold sumcheck function (EvalProver, eq(point, .) * poly) driven by
+    prove_sumcheck: every round polynomial satisfies p(0) + p(1) = subclaim (char 2:
+    p(1) = c0 + c1 + c2), the challenges replay from the transcript (observe the
+    claim, then [c0, c2] per round), the final table value is poly's MLE at the
+    challenges and the final subclaim equals it times eq(point, x), each evaluated
+    independently of the prover's formulas."""
+    O = oracle
+    for n in range(1, 10):
+        poly, point = rnd(O, 1 << n, 500 + n), rnd(O, n, 600 + n)
+        claim = O.mle_evaluate(poly, point)
+        rps, chal, fin, sub = O.eval_sumcheck(poly, point, claim)
+        ch = O.Challenger()
+        ch.observe(claim.tobytes())
+        s = claim
+        for k in range(n):
+            c0, c1, c2 = rps[k, 0:1], rps[k, 1:2], rps[k, 2:3]
+            assert (c0 ^ (c0 ^ c1 ^ c2) == s).all(), (n, k)  # p(0) + p(1)
+            ch.observe(c0.tobytes() + c2.tobytes())
+            x = ch.sample()
+            assert (x == chal[k:k + 1]).all(), (n, k)
+            s = O.mul(O.mul(c2, x) ^ c1, x) ^ c0
+        assert (s == sub).all()
+        x = chal[::-1].copy()  # the challenge of round k fixes variable n-1-k
+        px = O.mle_evaluate(poly, x)
+        assert (fin == px).all()
+        assert (O.mul(px, O.mle_evaluate(O.eq_table(point), x)) == sub).all(), n
+
+
 # ---- width > 1 (one polynomial per column of a row-major matrix) ----
 def one(O):
     return O.from_canonical(O.from_int(1))
