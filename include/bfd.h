@@ -109,6 +109,17 @@ int bfd_eval(bfd_ctx *ctx, const bfd_commitment *comm, const uint8_t *point, uin
 int bfd_eval_data(bfd_ctx *ctx, const void *evals, int evals_on_device, size_t height, size_t width,
                   const uint8_t *point, uint8_t *values);
 
+/* bfd_eval_data on device-resident evaluations that allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its working memory is a fixed 4 MiB
+ * device array per GPU, held until the process ends, so height is at most
+ * 2^32 rows (width 1..65535 as for bfd_eval_data; larger is BFD_ERR_ARG).
+ * Calls that overlap on one GPU share that array and race; calls on one
+ * context serialise.  values: width x 16 bytes in host memory. */
+int bfd_eval_data_noalloc(bfd_ctx *ctx, const void *d_evals, size_t height, size_t width, const uint8_t *point,
+                          uint8_t *values);
+
 /* Basefold::open (basefold.rs:133-232) for one PolyEvals {point, values}
  * (values: width x 16 bytes).  Width > 1 first samples the random scalars by
  * eq from the challenger (basefold.rs:166-183).  The challenger is advanced

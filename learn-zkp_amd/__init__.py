@@ -147,6 +147,8 @@ def lib():
                                       ctypes.c_size_t, u8p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, u8p]
         L.bfd_eval_data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t,
                                     u8p, u8p]
+        L.bfd_eval_data_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, u8p,
+                                            u8p]
         L.bfd_open_begin.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, u8p, u8p, u8p,
                                      ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]
         L.bfd_open_rounds_left.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]

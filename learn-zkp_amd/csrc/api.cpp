@@ -249,6 +249,14 @@ int bfd_ctx_create(int device, uint64_t hbm_budget_bytes, bfd_ctx **out) {
         ctx->tuning.fill(default_tuning());
         ctx->batch_tuning = default_tuning();
         hip_check(hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+        // the allocation-free entry points' scratch address on this device; a
+        // failure here is reported by those entry points, not by the context
+        try {
+            ctx->scratch = noalloc_scratch(device);
+        } catch (const Error &e) {
+            (void)hipGetLastError();
+            ctx->scratch_error = e.msg;
+        }
         if (ctx->cus < 1) ctx->cus = 1;
         ctx->cache.cap = total_b / 4;
         if (hbm_budget_bytes && hbm_budget_bytes < ctx->cache.cap) ctx->cache.cap = hbm_budget_bytes;

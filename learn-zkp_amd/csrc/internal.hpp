@@ -330,6 +330,10 @@ struct LaneDeleter {
 
 struct bfd_ctx {
     int device = 0;
+    // device address of the allocation-free entry points' scratch array on
+    // this device (noalloc.cpp), taken when the context is created
+    gf *scratch = nullptr;
+    std::string scratch_error;
     // codes made on demand by the batched and serving paths, keyed by
     // (lambda, log2_c, d); owned by the context
     std::map<uint64_t, bfd_code *> code_cache;
@@ -390,6 +394,10 @@ struct bfd_challenger {
 
 namespace bfd {
 namespace rt {
+
+// device address of the allocation-free scratch array on `device`, the
+// current device; taken once per device (noalloc.cpp)
+gf *noalloc_scratch(int device);
 
 // BinaryReedSolomonCode::new with its device twiddle table (api.cpp)
 bfd_code *create_code(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, uint32_t d);
