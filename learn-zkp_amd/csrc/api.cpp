@@ -469,10 +469,14 @@ void bfd_proof_free(uint8_t *proof) { free(proof); }
 namespace {
 
 // The degree-3 product sumcheck with its transcript on the device: one
-// launch per round (fold, pair sums and, in the block that finishes last,
-// the round's Fiat-Shamir step), one upload of the transcript image before
-// the rounds and one read-back after them.  f and g are copied into working
-// tables first (host or device source).
+// launch per round while a round has more than prod_tail_max_pairs() pairs
+// (fold, pair sums and, in the block that finishes last, the round's
+// Fiat-Shamir step), then one single-block launch for the remaining rounds
+// with the tables in LDS; one upload of the transcript image before the
+// rounds and one read-back after them.  Round 0 reads the caller's tables
+// (device memory, or one upload of host tables) and round 1 folds them into
+// working tables of half their size, so the caller's tables are never
+// copied or modified.
 void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g_src, bool on_device,
                       const uint8_t *r_b, Challenger &ch, uint8_t claim_out[16], uint8_t *round_polys,
                       uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim_out[16]) {
@@ -480,7 +484,6 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
     set_device(ctx);
     hipStream_t st = ctx->stream;
     const size_t N = (size_t)1 << n;
-    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     if (n == 0) {  // prove_sumcheck returns before observing (sumcheck/src/lib.rs:63-65)
         uint8_t fg[32];
         if (on_device) {
@@ -500,72 +503,116 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
     std::vector<gf> r(n);
     for (uint32_t i = 0; i < n; i++) r[i] = load_elem(r_b + 16 * i);
     const int hn = (int)n - 1;
-    // correcting factors of the halved eq table (eval_impr.rs:81-111)
-    std::vector<gf> cf(n, gf_one());
-    for (int j = hn - 1; j >= 0; j--) {
-        gf t = gf_add(gf_one(), r[j]);
-        if (gf_eq(t, gf_zero())) fail(BFD_ERR_ARG, "r_j = 1 makes the halved eq table singular");
-        cf[j] = hmul(cf[j + 1], hinv(t));
+    // correcting factors of the halved eq table (eval_impr.rs:81-111), with
+    // one field inversion for all (1 + r_j)
+    std::vector<gf> inv(hn);
+    for (int j = 0; j < hn; j++) {
+        inv[j] = gf_add(gf_one(), r[j]);
+        if (gf_eq(inv[j], gf_zero())) fail(BFD_ERR_ARG, "r_j = 1 makes the halved eq table singular");
     }
+    batch_inverse(inv);
+    std::vector<gf> cf(n, gf_one());
+    for (int j = hn - 1; j >= 0; j--) cf[j] = hmul(cf[j + 1], inv[j]);
     if (ch.output.size() > 32) fail(BFD_ERR_ARG, "transcript state too large");
-    DevBuf df = ctx_buf(ctx, 16 * N), dg = ctx_buf(ctx, 16 * N), T = ctx_buf(ctx, 16 * ((size_t)1 << hn));
-    hip_check(hipMemcpyAsync(df.p, f_src, 16 * N, kind, st), "copy f");
-    hip_check(hipMemcpyAsync(dg.p, g_src, 16 * N, kind, st), "copy g");
-    device_eq(ctx, r.data(), hn, T.as<gf>());
-    // transcript image: ProdFs | cf | e0 | polys | chal | input | n counters
+    DevBuf up, W = ctx_buf(ctx, 2 * 16 * std::max<size_t>(2, N / 2)), T = ctx_buf(ctx, 16 * ((size_t)1 << hn));
+    const gf *fs0 = (const gf *)f_src, *gs0 = (const gf *)g_src;
+    if (!on_device) {
+        up = ctx_buf(ctx, 2 * 16 * N);
+        hip_check(hipMemcpyAsync(up.p, f_src, 16 * N, hipMemcpyHostToDevice, st), "copy f");
+        hip_check(hipMemcpyAsync(up.as<gf>() + N, g_src, 16 * N, hipMemcpyHostToDevice, st), "copy g");
+        fs0 = up.as<gf>();
+        gs0 = fs0 + N;
+    }
+    gf *wf = W.as<gf>(), *wg = wf + std::max<size_t>(2, N / 2);
+    // one upload: the two factors of the halved eq table, then the transcript
+    // image (kernels.hpp ProdFs) up to its accumulation slots, which the
+    // set-up kernel clears on the device
+    const int lo = (hn + 1) / 2, hi = hn - lo;
+    const std::vector<gf> A = host_eq(r.data(), lo), B = host_eq(r.data() + lo, hi);
+    const size_t fb = (16 * (A.size() + B.size()) + 255) & ~(size_t)255;
     const uint32_t in_cap = (uint32_t)(((ch.input.size() > 32 ? ch.input.size() : 32) + 64 + 7) & ~(size_t)7);
-    const size_t in_off = prod_input_off(n), cnt_off = (in_off + in_cap + 7) & ~(size_t)7;
-    const size_t img = cnt_off + 4 * (size_t)n;
-    DevBuf fs = ctx_buf(ctx, img), partials = ctx_buf(ctx, 16 * 3 * 512);
-    ctx->pinned.ensure(img + 64);
+    const size_t in_off = prod_input_off(n), img = prod_bytes(n, in_cap), acc_off = prod_acc_off(n, in_cap);
+    const size_t fin_off = prod_finals_off(n, in_cap);
+    DevBuf buf = ctx_buf(ctx, fb + img + 64);
+    ctx->pinned.ensure(fb + acc_off + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
-    memset(pin, 0, img);
+    memcpy(pin, A.data(), 16 * A.size());
+    memcpy(pin + 16 * A.size(), B.data(), 16 * B.size());
+    uint8_t *hreg = pin + fb;
+    memset(hreg, 0, acc_off);
     ProdFs h;
     memset(&h, 0, sizeof h);
     h.in_len = (uint32_t)ch.input.size();
     h.out_len = (uint32_t)ch.output.size();
     h.in_cap = in_cap;
     h.n = n;
+    // round 0's eq scalars: s = 1, K = e0 cf, L = cf at variable n-1
     h.s = gf_one();
+    h.K = hmul(gf_add(gf_one(), r[n - 1]), cf[n - 1]);
+    h.L = cf[n - 1];
     if (!ch.output.empty()) memcpy(h.out, ch.output.data(), ch.output.size());
-    memcpy(pin, &h, sizeof h);
-    gf *c = (gf *)(pin + prod_cf_off());
+    memcpy(hreg, &h, sizeof h);
+    gf *c = (gf *)(hreg + prod_cf_off());
     for (uint32_t i = 0; i < n; i++) {
+        const gf e0 = gf_add(gf_one(), r[i]);  // eq(r_i, X) = (1 + r_i) + X
         c[i] = cf[i];
-        c[n + i] = gf_add(gf_one(), r[i]);  // eq(r_i, X) = (1 + r_i) + X
+        c[n + i] = e0;
+        c[2 * n + i] = hmul(e0, cf[i]);
     }
-    if (!ch.input.empty()) memcpy(pin + in_off, ch.input.data(), ch.input.size());
-    hip_check(hipMemcpyAsync(fs.p, pin, img, hipMemcpyHostToDevice, st), "copy transcript");
-    ProdFs *dfs = (ProdFs *)fs.p;
-    unsigned *counters = (unsigned *)(fs.as<uint8_t>() + cnt_off);
-    for (uint32_t k = 0; k < n; k++) {
+    if (!ch.input.empty()) memcpy(hreg + in_off, ch.input.data(), ch.input.size());
+    hip_check(hipMemcpyAsync(buf.p, pin, fb + acc_off, hipMemcpyHostToDevice, st), "copy transcript");
+    const gf *dA = buf.as<gf>(), *dB = dA + A.size();
+    ProdFs *dfs = (ProdFs *)(buf.as<uint8_t>() + fb);
+    ctx->timer.run(st, "k_prod_prep", [&] {
+        return launch::prod_prep(st, dA, lo, dB, T.as<gf>(), (size_t)1 << hn, dfs);
+    });
+    uint32_t k = 0;
+    for (; k < n; k++) {
         const size_t half = (size_t)1 << (n - 1 - k);
-        ctx->timer.run(st, "k_prod_step", [&] {
-            return launch::prod_step(st, df.as<gf>(), dg.as<gf>(), T.as<gf>(), half, (int)k, dfs, partials.as<gf>(),
-                                     counters + k);
+        if (half <= launch::prod_tail_max_pairs()) break;
+        const gf *fsrc = k <= 1 ? fs0 : wf, *gsrc = k <= 1 ? gs0 : wg;
+        gf *fdst = k == 0 ? (gf *)fs0 : wf, *gdst = k == 0 ? (gf *)gs0 : wg;  // round 0 writes nothing
+        ctx->timer.run(st, "k_prod_round", [&] {
+            return launch::prod_step(st, fsrc, gsrc, fdst, gdst, T.as<gf>(), half, (int)k, dfs, ctx->cus);
         });
     }
-    // read back the region up to the digest and the last two entries of f, g
-    const size_t rb = in_off + 32, fo = (rb + 15) & ~(size_t)15;
-    hip_check(hipMemcpyAsync(pin, fs.p, rb, hipMemcpyDeviceToHost, st), "copy transcript back");
-    hip_check(hipMemcpyAsync(pin + fo, df.p, 32, hipMemcpyDeviceToHost, st), "copy f");
-    hip_check(hipMemcpyAsync(pin + fo + 32, dg.p, 32, hipMemcpyDeviceToHost, st), "copy g");
+    gf *fins = (gf *)((uint8_t *)dfs + fin_off);
+    {
+        const size_t half = (size_t)1 << (n - 1 - k);
+        const gf *fsrc = k <= 1 ? fs0 : wf, *gsrc = k <= 1 ? gs0 : wg;
+        ctx->timer.run(st, "k_prod_tail",
+                       [&] { return launch::prod_tail(st, fsrc, gsrc, fins, fins + 2, T.as<gf>(), half, (int)k, dfs); });
+    }
+    // one read-back: the region up to the last round's table entries
+    hip_check(hipMemcpyAsync(hreg, dfs, fin_off + 64, hipMemcpyDeviceToHost, st), "copy transcript back");
     hip_check(hipStreamSynchronize(st), "sumcheck");
     ctx->timer.collect();
-    memcpy(&h, pin, sizeof h);
+    memcpy(&h, hreg, sizeof h);
     if (h.n != n || h.in_len != 32 || h.out_len != 16) fail(BFD_ERR_HIP, "device transcript state is corrupt");
     store_elem(claim_out, h.claim);
-    memcpy(round_polys, pin + prod_polys_off(n), 48 * (size_t)n);
-    memcpy(challenges, pin + prod_chal_off(n), 16 * (size_t)n);
-    ch.input.assign(pin + in_off, pin + in_off + 32);
+    memcpy(round_polys, hreg + prod_polys_off(n), 48 * (size_t)n);
+    memcpy(challenges, hreg + prod_chal_off(n), 16 * (size_t)n);
+    ch.input.assign(hreg + in_off, hreg + in_off + 32);
     ch.output.assign(h.out, h.out + h.out_len);
+    pin = hreg;
+    const size_t fo = fin_off;
+    // the subclaim through the rounds (SumcheckSubclaim::reduce, lib.rs:98-100):
+    // c1 = subclaim + c2 + c3 in characteristic 2, then Horner at x
+    gf sub = h.claim;
+    for (uint32_t t = 0; t < n; t++) {
+        const uint8_t *p = pin + prod_polys_off(n) + 48 * (size_t)t;
+        const gf c0 = load_elem(p), c2 = load_elem(p + 16), c3 = load_elem(p + 32);
+        const gf c1 = gf_add(sub, gf_add(c2, c3));
+        const gf x = load_elem(pin + prod_chal_off(n) + 16 * (size_t)t);
+        sub = gf_add(c0, hmul(x, gf_add(c1, hmul(x, gf_add(c2, hmul(x, c3))))));
+    }
     // the last fold of the two remaining entries (fix_last_var with x)
     const gf x = h.x;
     const gf f0 = load_elem(pin + fo), f1 = load_elem(pin + fo + 16);
     const gf g0 = load_elem(pin + fo + 32), g1 = load_elem(pin + fo + 48);
     store_elem(final_fg, gf_add(f0, hmul(x, gf_add(f1, f0))));
     store_elem(final_fg + 16, gf_add(g0, hmul(x, gf_add(g1, g0))));
-    store_elem(subclaim_out, h.sub);
+    store_elem(subclaim_out, sub);
 }
 
 }  // namespace
@@ -640,7 +687,8 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
     hip_check(hipMemcpyAsync(dpairs.p, pair_idx, 8 * (size_t)npairs, hipMemcpyHostToDevice, st), "copy pairs");
     // blocks per pair: enough for the device at the first round
     const unsigned max_blocks = std::max(1u, std::min(512u, 4096u / npairs));
-    DevBuf partials = ctx_buf(ctx, 48 * (size_t)max_blocks * npairs);
+    DevBuf slots = ctx_buf(ctx, 48 * (size_t)launch::quad_slots(max_blocks) * npairs);
+    hip_check(hipMemsetAsync(slots.p, 0, slots.bytes, st), "clear slots");
     // transcript image: QuadFs | scalars | polys | chal | input | n counters
     const uint32_t in_cap = (uint32_t)(((ch.input.size() > 32 ? ch.input.size() : 32) + 64 + 7) & ~(size_t)7);
     const size_t in_off = quad_input_off(npairs, n), cnt_off = (in_off + in_cap + 7) & ~(size_t)7;
@@ -668,7 +716,7 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
         const size_t half = (size_t)1 << (n - 1 - k);
         ctx->timer.run(st, "k_quad_step", [&] {
             return launch::quad_step(st, dP.as<gf>(), N, dpairs.as<uint32_t>(), npairs, half, (int)k, dfs,
-                                     partials.as<gf>(), max_blocks, counters + k);
+                                     slots.as<gf>(), max_blocks, counters + k);
         });
         ctx->timer.run(st, "k_fix_multi", [&] { return launch::fix_multi_dev(st, dP.as<gf>(), N, m, half, x_dev); });
     }

@@ -473,142 +473,410 @@ __device__ __forceinline__ void block_xor3(gf a0, gf a1, gf a2, gf *partial) {
     }
 }
 
+// Counts the block once the values it published with device-scope atomics
+// are performed (done[0..nd) set after those atomics returned, so no release
+// fence, which would write back the whole L2, is needed); true in the block
+// that counts last, which then reads the values with device-scope atomic
+// loads (gf_ld_published).
+__device__ __forceinline__ bool block_counted(unsigned *counter, unsigned total, const unsigned *done, int nd) {
+    __shared__ unsigned last;
+    if (threadIdx.x == 0) {
+        unsigned inc = 1u;
+        for (int t = 0; t < nd; t++) inc &= done[t];
+        last = __hip_atomic_fetch_add(counter, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == total - 1;
+    }
+    __syncthreads();
+    return last;
+}
+
+// a gf published with device-scope atomics, read in the last block
+__device__ __forceinline__ gf gf_ld_published(const gf *p) {
+    const unsigned long long *q = (const unsigned long long *)p;
+    return gf{(uint64_t)__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+              (uint64_t)__hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+}
+
 // lane src's value of v, on every lane of the wave
 __device__ __forceinline__ gf lane_gf(gf v, int src) { return gf{(uint64_t)__shfl((long long)v.lo, src), (uint64_t)__shfl((long long)v.hi, src)}; }
 
+// Pair sums of the product sumcheck with T multiplied into f first:
+// tf0 = T f0, tf1 = T f1, then A0 += tf0 g0, A1 += tf1 g1 and
+// A2 += (tf0 + tf1)(g0 + g1), five products per pair.
 __device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf &a0, gf &a1, gf &a2) {
-    gf p0 = gf_mul_reg(f0, g0);
-    gf p1 = gf_mul_reg(f1, g1);
-    gf p2 = gf_mul_reg(gf_add(f0, f1), gf_add(g0, g1));
     const gf_pre pt = gf_prepare(t);
-    a0 = gf_add(a0, gf_mul_pre(p0, pt));
-    a1 = gf_add(a1, gf_mul_pre(p1, pt));
-    a2 = gf_add(a2, gf_mul_pre(p2, pt));
+    const gf tf0 = gf_mul_pre(f0, pt), tf1 = gf_mul_pre(f1, pt);
+    a0 = gf_add(a0, gf_mul_reg(tf0, g0));
+    a1 = gf_add(a1, gf_mul_reg(tf1, g1));
+    a2 = gf_add(a2, gf_mul_reg(gf_add(tf0, tf1), gf_add(g0, g1)));
 }
 
-// sums of the first round over tables of 2*half entries
-// One round's transcript step of the product sumcheck, run by the first
-// wave of the block that finishes the round (prove_sumcheck_round,
-// sumcheck/src/lib.rs:81-103): the round polynomial p(X) = s q(X) eq(r_i, X)
-// from the reduced sums, with q(X) = q0 + q1 X + q2 X^2 the pair sums scaled
-// by the correcting factor (q1 = A0 + A1 + A2 in characteristic 2) and
-// eq(r_i, X) = (1 + r_i) + X; then observe [claim,] c0, c2, c3, sample x and
-// advance the subclaim (Horner) and s.  Independent products run on
-// different lanes; lane 0 writes the results.
-__device__ void prod_fs_step(ProdFs *st, int k, gf A0, gf A1, gf A2) {
+// fold on the top variable: lo + x (hi - lo)  (multilinear.rs:183-198)
+__device__ __forceinline__ gf fold_top(gf lo, gf hi, const gf_pre &px) { return gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)); }
+
+// The eq scalars move from round to round with one product after each
+// challenge: with t = e0[i] + x_k (eq(r_i, x_k), variable i = n-1-k),
+// s_{k+1} = s_k t, K_{k+1} = P1 t and L_{k+1} = P2 t, where
+// P1 = s_k ecf[i-1] and P2 = s_k cf[i-1] are formed while round k runs.
+// prod_pre: {P1, P2} on two lanes (lane & 1 picks).
+__device__ __forceinline__ gf prod_pre(const gf *ecf, const gf *cf, int i, gf s, int lane) {
+    return gf_mul_reg(s, gf_ld(((lane & 1) ? cf : ecf) + i - 1));
+}
+// prod_next: lane 0 s_{k+1}, lane 1 K_{k+1}, lane 2 L_{k+1}; every lane of
+// the wave returns the three.
+__device__ __forceinline__ void prod_next(gf s, gf P1, gf P2, gf e0, gf x, gf out[3]) {
     const int lane = threadIdx.x & 63;
-    const uint32_t n = st->n;
-    const int i = (int)n - 1 - k;
-    uint8_t *base = (uint8_t *)st;
-    const gf *cf = (const gf *)(base + prod_cf_off()), *e0s = cf + n;
-    gf *polys = (gf *)(base + prod_polys_off(n)), *chal = (gf *)(base + prod_chal_off(n));
-    uint8_t *in = base + prod_input_off(n);
-    uint64_t *in64 = reinterpret_cast<uint64_t *>(in);
-    const gf c = gf_ld(cf + i), e0 = gf_ld(e0s + i), s = st->s;
-    gf p = gf_mul_reg(c, lane == 0 ? A0 : (lane == 1 ? A2 : gf_add(A0, gf_add(A1, A2))));
-    const gf q0 = lane_gf(p, 0), q2 = lane_gf(p, 1), q1 = lane_gf(p, 2);
-    p = gf_mul_reg(e0, lane == 0 ? q0 : (lane == 1 ? q1 : q2));
-    const gf t0 = lane_gf(p, 0), t1 = lane_gf(p, 1), t2 = lane_gf(p, 2);
-    p = gf_mul_reg(s, lane == 0 ? t0 : (lane == 1 ? gf_add(t1, q0) : (lane == 2 ? gf_add(t2, q1) : q2)));
-    const gf c0 = lane_gf(p, 0), c1 = lane_gf(p, 1), c2 = lane_gf(p, 2), c3 = lane_gf(p, 3);
-    const gf claim = gf_add(c1, gf_add(c2, c3));  // p(0) + p(1), used in round 0
+    const gf p = gf_mul_reg(lane == 0 ? s : (lane == 1 ? P1 : P2), gf_add(e0, x));
+    out[0] = lane_gf(p, 0);
+    out[1] = lane_gf(p, 1);
+    out[2] = lane_gf(p, 2);
+}
+
+// One round's transcript step of the product sumcheck, run by one whole wave
+// (prove_sumcheck_round, sumcheck/src/lib.rs:81-103).  With the pair sums
+// A0, A1, A2, Q1 = A0 + A1 + A2 and the eq scalars K = s e0 cf, L = s cf,
+// the round polynomial p(X) = s (e0 + X) cf (A0 + Q1 X + A2 X^2) has
+// c0 = K A0, c1 = L A0 + K Q1, c2 = L Q1 + K A2, c3 = L A2: six independent
+// products on six lanes.  Then observe [claim,] c0, c2, c3 (the claim
+// p(0) + p(1) = c1 + c2 + c3 in round 0, sumcheck/src/lib.rs:67) and sample
+// x.  With have_prev the challenger's input is the previous round's 32-byte
+// digest prev (every round after the first); otherwise the input bytes come
+// from the region (the first round, after whatever the caller observed; lane
+// 0 appends to them there).  Every lane gets the results; the caller stores
+// them.
+struct ProdRound {
+    gf c0, c2, c3, claim, x;
     uint64_t d[4];
-    if (k > 0 && st->in_len == 32) {
-        const uint64_t m[10] = {in64[0], in64[1], in64[2], in64[3], c0.lo, c0.hi, c2.lo, c2.hi, c3.lo, c3.hi};
-        kk::keccak256_words_pair<10>(m, d, lane & 1);
+};
+struct Digest {
+    uint64_t w[4];
+};
+__device__ ProdRound prod_transcript(ProdFs *st, int k, gf A0, gf A1, gf A2, gf K, gf L, bool have_prev,
+                                     Digest prev) {
+    const int lane = threadIdx.x & 63;
+    uint8_t *in = (uint8_t *)st + prod_input_off(st->n);
+    const gf Q1 = gf_add(A0, gf_add(A1, A2));
+    const gf u = (lane & 1) ? L : K;
+    const gf v = lane < 2 ? A0 : (lane < 4 ? Q1 : A2);
+    const gf p = gf_mul_reg(u, v);  // lane 0: K A0, 1: L A0, 2: K Q1, 3: L Q1, 4: K A2, 5: L A2
+    ProdRound R;
+    R.c0 = lane_gf(p, 0);
+    const gf c1 = gf_add(lane_gf(p, 1), lane_gf(p, 2));
+    R.c2 = gf_add(lane_gf(p, 3), lane_gf(p, 4));
+    R.c3 = lane_gf(p, 5);
+    R.claim = gf_add(c1, gf_add(R.c2, R.c3));
+    if (have_prev) {
+        const uint64_t m[10] = {prev.w[0], prev.w[1], prev.w[2], prev.w[3], R.c0.lo, R.c0.hi,
+                                R.c2.lo,   R.c2.hi,   R.c3.lo,   R.c3.hi};
+        kk::keccak256_words_pair<10>(m, R.d, lane & 1);
     } else {
         if (lane == 0) {
             uint32_t len = st->in_len;
-            const auto put = [&](uint64_t v) {
+            const auto put = [&](uint64_t w) {
 #pragma unroll 1
-                for (int t = 0; t < 8; t++) in[len + t] = (uint8_t)(v >> (8 * t));
+                for (int t = 0; t < 8; t++) in[len + t] = (uint8_t)(w >> (8 * t));
                 len += 8;
             };
             if (k == 0) {  // the claim is observed first (sumcheck/src/lib.rs:67)
-                put(claim.lo);
-                put(claim.hi);
+                put(R.claim.lo);
+                put(R.claim.hi);
             }
-            put(c0.lo);
-            put(c0.hi);
-            put(c2.lo);
-            put(c2.hi);
-            put(c3.lo);
-            put(c3.hi);
-            kk::keccak256_bytes(in, len, d);
+            put(R.c0.lo);
+            put(R.c0.hi);
+            put(R.c2.lo);
+            put(R.c2.hi);
+            put(R.c3.lo);
+            put(R.c3.hi);
+            kk::keccak256_bytes(in, len, R.d);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) d[q] = (uint64_t)__shfl((long long)d[q], 0);
+        for (int q = 0; q < 4; q++) R.d[q] = (uint64_t)__shfl((long long)R.d[q], 0);
     }
     // the challenge is the last 16 digest bytes, last byte first
-    const gf x{__builtin_bswap64(d[3]), __builtin_bswap64(d[2])};
-    p = gf_mul_reg(lane == 0 ? x : s, lane == 0 ? c3 : gf_add(e0, x));
-    const gf h1 = lane_gf(p, 0), s_next = lane_gf(p, 1);
-    const gf h2 = gf_mul_reg(x, gf_add(c2, h1));
-    const gf sub_next = gf_add(c0, gf_mul_reg(x, gf_add(c1, h2)));
-    if (lane != 0) return;
+    R.x = gf{__builtin_bswap64(R.d[3]), __builtin_bswap64(R.d[2])};
+    return R;
+}
+
+// The challenger state and the next round's eq scalars after round k into
+// the region: input = output = the digest, 16 output bytes left unread.
+__device__ __forceinline__ void prod_store_state(ProdFs *st, const ProdRound &R, const gf next[3]) {
+    uint64_t *in64 = reinterpret_cast<uint64_t *>((uint8_t *)st + prod_input_off(st->n));
     uint64_t *out64 = reinterpret_cast<uint64_t *>(st->out);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        in64[q] = d[q];
-        out64[q] = d[q];
+        in64[q] = R.d[q];
+        out64[q] = R.d[q];
     }
     st->in_len = 32;
     st->out_len = 16;
-    if (k == 0) st->claim = claim;
-    st->sub = sub_next;
-    st->s = s_next;
-    st->x = x;
-    polys[3 * k] = c0;
-    polys[3 * k + 1] = c2;
-    polys[3 * k + 2] = c3;
-    chal[k] = x;
+    st->s = next[0];
+    st->K = next[1];
+    st->L = next[2];
+    st->x = R.x;
 }
 
-// Round k of the product sumcheck (see launch::prod_step).  Rounds after the
-// first fold f and g on their top variable with the last challenge while
-// they form the pair sums of prod_terms (one pass over the tables).
-template <bool FOLD>
-__global__ __launch_bounds__(CB) void k_prod_step(gf *__restrict__ f, gf *__restrict__ g, const gf *__restrict__ T,
-                                                  size_t half, int k, ProdFs *fs, gf *partial, unsigned *counter) {
+// T[i] = A[i mod 2^m] B[i >> m] for i < n, and the region's accumulation
+// slots and counters cleared, in one grid-stride pass
+__global__ __launch_bounds__(CB) void k_prod_prep(const gf *__restrict__ A, int m, const gf *__restrict__ B,
+                                                  gf *__restrict__ T, size_t n, ProdFs *fs) {
+    const size_t mask = ((size_t)1 << m) - 1;
+    const size_t stride = (size_t)gridDim.x * blockDim.x, t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (size_t i = t0; i < n; i += stride) gf_st(T + i, gf_mul_reg(gf_ld(A + (i & mask)), gf_ld(B + (i >> m))));
+    const uint32_t nv = fs->n, in_cap = fs->in_cap;
+    uint64_t *z = (uint64_t *)((uint8_t *)fs + prod_acc_off(nv, in_cap));
+    const size_t words = (prod_bytes(nv, in_cap) - prod_acc_off(nv, in_cap) + 7) / 8;
+    for (size_t i = t0; i < words; i += stride) z[i] = 0;
+}
+
+// Round k of the product sumcheck (see launch::prod_step).  MODE 0: round 0,
+// sums over the caller's tables; 1: fold the caller's tables into the working
+// ones, then the sums; 2: fold the working tables in place, then the sums.
+template <int MODE>
+__global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst,
+                                                   const gf *__restrict__ T, size_t half, int k, ProdFs *fs) {
+    __shared__ gf pre[2];  // P1, P2 of this round
+    __shared__ unsigned last;
+    const uint32_t nv = fs->n;
+    const int iv = (int)nv - 1 - k;
+    const gf *cfs = (const gf *)((const uint8_t *)fs + prod_cf_off()), *e0s = cfs + nv, *ecfs = e0s + nv;
+    // P1, P2 for the next round's scalars, on two lanes of the second wave
+    // (the transcript runs on the first)
+    if (iv > 0 && (threadIdx.x == 64 || threadIdx.x == 65)) pre[threadIdx.x & 1] = prod_pre(ecfs, cfs, iv, fs->s, threadIdx.x);
     gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
-    if constexpr (FOLD) {
-        const size_t q = half, h = 2 * q;
+    if constexpr (MODE == 0) {
+        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half; j += (size_t)gridDim.x * blockDim.x)
+            prod_terms(gf_ld(fsrc + j), gf_ld(fsrc + j + half), gf_ld(gsrc + j), gf_ld(gsrc + j + half),
+                       gf_ld(T + j), a0, a1, a2);
+    } else {
+        // the previous tables hold 4 q entries; entries j and j + q of the
+        // folded ones come from j, j + 2q and j + q, j + 3q: each thread
+        // reads its four before writing its two, so in place is safe
+        const size_t q = half;
         const gf_pre px = gf_prepare(fs->x);
-        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < q;
-             j += (size_t)gridDim.x * blockDim.x) {
-            gf fa = gf_ld(f + j), fb = gf_ld(f + j + q), fc = gf_ld(f + j + h), fd = gf_ld(f + j + h + q);
-            gf ga = gf_ld(g + j), gb = gf_ld(g + j + q), gc = gf_ld(g + j + h), gd = gf_ld(g + j + h + q);
-            gf f0 = gf_add(fa, gf_mul_pre(gf_add(fc, fa), px));
-            gf f1 = gf_add(fb, gf_mul_pre(gf_add(fd, fb), px));
-            gf g0 = gf_add(ga, gf_mul_pre(gf_add(gc, ga), px));
-            gf g1 = gf_add(gb, gf_mul_pre(gf_add(gd, gb), px));
-            gf_st(f + j, f0);
-            gf_st(f + j + q, f1);
-            gf_st(g + j, g0);
-            gf_st(g + j + q, g1);
+        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < q; j += (size_t)gridDim.x * blockDim.x) {
+            const gf fa = gf_ld(fsrc + j), fb = gf_ld(fsrc + j + q), fc = gf_ld(fsrc + j + 2 * q),
+                     fd = gf_ld(fsrc + j + 3 * q);
+            const gf ga = gf_ld(gsrc + j), gb = gf_ld(gsrc + j + q), gc = gf_ld(gsrc + j + 2 * q),
+                     gd = gf_ld(gsrc + j + 3 * q);
+            const gf f0 = fold_top(fa, fc, px), f1 = fold_top(fb, fd, px);
+            const gf g0 = fold_top(ga, gc, px), g1 = fold_top(gb, gd, px);
+            gf_st(fdst + j, f0);
+            gf_st(fdst + j + q, f1);
+            gf_st(gdst + j, g0);
+            gf_st(gdst + j + q, g1);
             prod_terms(f0, f1, g0, g1, gf_ld(T + j), a0, a1, a2);
         }
-    } else {
-        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
-             j += (size_t)gridDim.x * blockDim.x)
-            prod_terms(gf_ld(f + j), gf_ld(f + j + half), gf_ld(g + j), gf_ld(g + j + half), gf_ld(T + j), a0, a1,
-                       a2);
     }
-    block_xor3(a0, a1, a2, partial);
-    __shared__ unsigned last;
-    __threadfence();
+    // block sums, XORed into one of the round's accumulation slots
+    __shared__ gf red[3][CB / 64];
+    a0 = wave_xor(a0);
+    a1 = wave_xor(a1);
+    a2 = wave_xor(a2);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a0;
+        red[1][threadIdx.x >> 6] = a1;
+        red[2][threadIdx.x >> 6] = a2;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    uint8_t *base = (uint8_t *)fs;
+    const uint32_t n = fs->n, in_cap = fs->in_cap;
+    unsigned long long *acc = (unsigned long long *)(base + prod_acc_off(n, in_cap)) + (size_t)k * 6 * kProdSlots;
+    unsigned *counter = (unsigned *)(base + prod_counters_off(n, in_cap)) + k;
+    // The block's sums go out as device-scope atomics whose results it waits
+    // for, so they are performed before its count below is issued; every
+    // block's sums are therefore in the slots before the last count returns.
+    // No block needs a release fence (which would write back the whole L2):
+    // the folded tables are read only by the next launch.
+    __shared__ unsigned done[3];
+    if (threadIdx.x < 3) {
+        gf v = gf_zero();
+        for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[threadIdx.x][w]);
+        unsigned long long *slot = acc + 6 * (blockIdx.x % kProdSlots) + 2 * threadIdx.x;
+        const unsigned long long o0 =
+            __hip_atomic_fetch_xor(slot, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long o1 =
+            __hip_atomic_fetch_xor(slot + 1, (unsigned long long)v.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        done[threadIdx.x] = (unsigned)(o0 ^ o1) | 1u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned inc = done[0] & done[1] & done[2] & 1u;  // 1, after all three threads' atomics returned
+        last = __hip_atomic_fetch_add(counter, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    }
     __syncthreads();
     if (!last || threadIdx.x >= 64) return;
-    __threadfence();
-    gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
-    for (unsigned t = threadIdx.x; t < gridDim.x; t += 64) {
-        A0 = gf_add(A0, gf_ld(partial + 3 * t));
-        A1 = gf_add(A1, gf_ld(partial + 3 * t + 1));
-        A2 = gf_add(A2, gf_ld(partial + 3 * t + 2));
+    // lane l XORs slots l, l + 64, ... (the slots a smaller grid used), all
+    // loads issued before the first use; then a wave reduction
+    const int lane = threadIdx.x;
+    unsigned long long v[6 * (kProdSlots / 64)];
+#pragma unroll
+    for (int t = 0; t < kProdSlots / 64; t++)
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+            v[6 * t + c] = (size_t)(lane + 64 * t) < gridDim.x
+                               ? __hip_atomic_load(acc + 6 * (lane + 64 * t) + c, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                               : 0ull;
+    gf A[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        A[c] = gf_zero();
+#pragma unroll
+        for (int t = 0; t < kProdSlots / 64; t++) A[c] = gf_add(A[c], gf{v[6 * t + 2 * c], v[6 * t + 2 * c + 1]});
+        A[c] = wave_xor(A[c]);
     }
-    prod_fs_step(fs, k, wave_xor(A0), wave_xor(A1), wave_xor(A2));
+    // after the first round the challenger's input is the last digest
+    Digest prev;
+    const uint64_t *in64 = reinterpret_cast<const uint64_t *>(base + prod_input_off(n));
+#pragma unroll
+    for (int w = 0; w < 4; w++) prev.w[w] = k > 0 ? in64[w] : 0;
+    const ProdRound R = prod_transcript(fs, k, A[0], A[1], A[2], fs->K, fs->L, k > 0, prev);
+    gf next[3] = {gf_zero(), gf_zero(), gf_zero()};
+    if (iv > 0) prod_next(fs->s, pre[0], pre[1], gf_ld(e0s + iv), R.x, next);
+    if (lane == 0) {
+        gf *polys = (gf *)(base + prod_polys_off(n)), *chal = (gf *)(base + prod_chal_off(n));
+        polys[3 * k] = R.c0;
+        polys[3 * k + 1] = R.c2;
+        polys[3 * k + 2] = R.c3;
+        chal[k] = R.x;
+        if (k == 0) fs->claim = R.claim;
+        prod_store_state(fs, R, next);
+    }
+}
+
+// The rounds from k0 on in one block of kTailThreads threads, tables in LDS.
+// A round of q pairs runs its products in three steps, one product per task
+// and at most two tasks per thread, with a barrier after each step, so a
+// round of few pairs costs three product latencies.  Task indices are padded
+// to Q = max(q, 64) per kind, so every wave's tasks of one pass have one kind:
+//   fold   4Q tasks: kind u (f low, f high, g low, g high) of pair j folds
+//          entries j + uq and j + (u+2)q of its table into j + uq (k > 0);
+//          in place, as no task reads what another writes;
+//   scale  2Q tasks: TF[uq + j] = T[j] F[j + uq], u = 0, 1;
+//   sum    3Q tasks: TF[j] G[j], TF[q+j] G[q+j], (TF[j] + TF[q+j])(G[j] + G[q+j]),
+//          reduced per kind by wave sums.
+// The running scalars (s, x, the digest) stay in LDS between rounds; the
+// last wave forms P1 and P2 during the scale step.
+constexpr int kTailThreads = 512;
+constexpr size_t kTailPairs = 256;  // pairs of the first tail round (4 * kTailPairs entries per table)
+__global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, const gf *gsrc, gf *fout, gf *gout,
+                                                            const gf *__restrict__ T, size_t half, int k0, ProdFs *fs) {
+    __shared__ gf F[4 * kTailPairs], G[4 * kTailPairs], Ts[kTailPairs], TF[2 * kTailPairs];
+    __shared__ gf red[3][kTailThreads / 64];
+    __shared__ gf sc[3];   // s, K, L of the round being run
+    __shared__ gf pre[2];  // P1, P2 of the round being run
+    __shared__ gf cst[3][32];  // cf, e0, ecf
+    __shared__ gf xs;
+    __shared__ uint64_t dig[4];
+    __shared__ gf outs[32][4];  // per round: c0, c2, c3, x (stored to the region at the end)
+    __shared__ gf claim0;
+    const uint32_t n = fs->n;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    // the previous round's tables (4 half entries; the caller's 2 half in
+    // round 0) and the halved eq table's first half entries
+    const size_t load = (k0 == 0 ? 2 : 4) * half;
+    for (size_t e = tid; e < load; e += kTailThreads) {
+        F[e] = gf_ld(fsrc + e);
+        G[e] = gf_ld(gsrc + e);
+    }
+    for (size_t e = tid; e < half; e += kTailThreads) Ts[e] = gf_ld(T + e);
+    if (tid < 3 * (int)n) cst[tid / n][tid % n] = gf_ld((const gf *)((const uint8_t *)fs + prod_cf_off()) + tid);
+    if (tid == 0) {
+        sc[0] = fs->s;
+        sc[1] = fs->K;
+        sc[2] = fs->L;
+        xs = fs->x;
+        const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)fs + prod_input_off(n));
+        for (int w = 0; w < 4; w++) dig[w] = in64[w];
+    }
+    __syncthreads();
+    size_t q = half;
+    for (int k = k0; k < (int)n; k++, q >>= 1) {
+        const int iv = (int)n - 1 - k;
+        const size_t Q = q < 64 ? 64 : q;
+        if (k > 0) {
+            const gf_pre px = gf_prepare(xs);
+            for (size_t t = tid; t < 4 * Q; t += kTailThreads) {
+                const size_t u = t / Q, j = t % Q;
+                if (j >= q) continue;
+                gf *Tab = u < 2 ? F : G;
+                const size_t lo = j + (u & 1) * q;
+                Tab[lo] = fold_top(Tab[lo], Tab[lo + 2 * q], px);
+            }
+        }
+        __syncthreads();
+        for (size_t t = tid; t < 2 * Q; t += kTailThreads) {
+            const size_t u = t / Q, j = t % Q;
+            if (j < q) TF[u * q + j] = gf_mul_reg(Ts[j], F[j + u * q]);
+        }
+        // P1, P2 of this round on the last wave
+        if (iv > 0 && (tid == kTailThreads - 64 || tid == kTailThreads - 63))
+            pre[tid & 1] = prod_pre(cst[2], cst[0], iv, sc[0], tid);
+        __syncthreads();
+        gf a[3] = {gf_zero(), gf_zero(), gf_zero()};
+        for (size_t t = tid; t < 3 * Q; t += kTailThreads) {
+            const size_t u = t / Q, j = t % Q;
+            if (j >= q) continue;
+            if (u == 0) a[0] = gf_mul_reg(TF[j], G[j]);
+            else if (u == 1) a[1] = gf_mul_reg(TF[q + j], G[q + j]);
+            else a[2] = gf_mul_reg(gf_add(TF[j], TF[q + j]), gf_add(G[j], G[q + j]));
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a[c] = wave_xor(a[c]);
+            if ((tid & 63) == 0) red[c][wave] = a[c];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            gf A[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                A[c] = gf_zero();
+                for (int w = 0; w < kTailThreads / 64; w++) A[c] = gf_add(A[c], red[c][w]);
+            }
+            Digest prev;
+#pragma unroll
+            for (int w = 0; w < 4; w++) prev.w[w] = dig[w];
+            const ProdRound R = prod_transcript(fs, k, A[0], A[1], A[2], sc[1], sc[2], k > 0, prev);
+            gf next[3] = {sc[0], sc[1], sc[2]};
+            if (iv > 0) prod_next(sc[0], pre[0], pre[1], cst[1][iv], R.x, next);
+            // lanes 0..2 store s, K, L (constant indices: no private array in scratch)
+            if (tid == 0) sc[0] = next[0];
+            if (tid == 1) sc[1] = next[1];
+            if (tid == 2) sc[2] = next[2];
+            if (tid == 0) {
+                xs = R.x;
+                for (int w = 0; w < 4; w++) dig[w] = R.d[w];
+                outs[k][0] = R.c0;
+                outs[k][1] = R.c2;
+                outs[k][2] = R.c3;
+                outs[k][3] = R.x;
+                if (k == 0) claim0 = R.claim;
+            }
+        }
+        __syncthreads();
+    }
+    // the last round's two entries of each table, for the host's final fold,
+    // and the rounds' outputs and the challenger state into the region
+    if (tid < 2) {
+        gf_st(fout + tid, F[tid]);
+        gf_st(gout + tid, G[tid]);
+    }
+    uint8_t *base = (uint8_t *)fs;
+    gf *polys = (gf *)(base + prod_polys_off(n)), *chal = (gf *)(base + prod_chal_off(n));
+    for (int k = k0 + tid; k < (int)n; k += kTailThreads) {
+        polys[3 * k] = outs[k][0];
+        polys[3 * k + 1] = outs[k][1];
+        polys[3 * k + 2] = outs[k][2];
+        chal[k] = outs[k][3];
+    }
+    if (tid == 0) {
+        if (k0 == 0) fs->claim = claim0;
+        ProdRound R;
+        R.x = xs;
+        for (int w = 0; w < 4; w++) R.d[w] = dig[w];
+        const gf next[3] = {sc[0], sc[1], sc[2]};
+        prod_store_state(fs, R, next);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -643,7 +911,7 @@ __global__ __launch_bounds__(CB) void k_quad_sums(const gf *__restrict__ P, size
 // [claim,] c0, c2 (the compressed polynomial, quadratic.rs:62-64), sample x
 // and advance the subclaim with c1 = subclaim + c2 (characteristic 2,
 // quadratic.rs:70-80).  Pairs are spread over the wave's lanes.
-__device__ void quad_fs_step(QuadFs *st, int k, const gf *partial, unsigned nb) {
+__device__ void quad_fs_step(QuadFs *st, int k, gf *slots, unsigned S) {
     const int lane = threadIdx.x & 63;
     const uint32_t n = st->n, np = st->npairs;
     uint8_t *base = (uint8_t *)st;
@@ -654,12 +922,15 @@ __device__ void quad_fs_step(QuadFs *st, int k, const gf *partial, unsigned nb) 
     gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
     for (uint32_t t = lane; t < np; t += 64) {
         gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
-        const gf *q = partial + 3 * (size_t)nb * t;
-        for (unsigned u = 0; u < nb; u++) {
-            A0 = gf_add(A0, gf_ld(q + 3 * u));
-            A1 = gf_add(A1, gf_ld(q + 3 * u + 1));
-            A2 = gf_add(A2, gf_ld(q + 3 * u + 2));
+        gf *q = slots + 3 * (size_t)S * t;
+#pragma unroll 4
+        for (unsigned u = 0; u < S; u++) {
+            A0 = gf_add(A0, gf_ld_published(q + 3 * u));
+            A1 = gf_add(A1, gf_ld_published(q + 3 * u + 1));
+            A2 = gf_add(A2, gf_ld_published(q + 3 * u + 2));
         }
+        // the next round XORs into cleared slots (it starts after this launch)
+        for (unsigned u = 0; u < 3 * S; u++) gf_st(q + u, gf_zero());
         const gf_pre s = gf_prepare(gf_ld(sc + t));
         c0 = gf_add(c0, gf_mul_pre(A0, s));
         c2 = gf_add(c2, gf_mul_pre(A2, s));
@@ -718,7 +989,7 @@ __device__ void quad_fs_step(QuadFs *st, int k, const gf *partial, unsigned nb) 
 // k_quad_sums with the round's transcript step in the last block to finish
 __global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size_t stride,
                                                   const uint32_t *__restrict__ pairs, size_t h, int k, QuadFs *fs,
-                                                  gf *partial, unsigned *counter) {
+                                                  gf *slots, unsigned S, unsigned *counter) {
     const size_t a = pairs[2 * blockIdx.y], b = pairs[2 * blockIdx.y + 1];
     const gf *f = P + a * stride, *g = P + b * stride;
     gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
@@ -728,15 +999,33 @@ __global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size
         a1 = gf_add(a1, gf_mul_reg(f1, g1));
         a2 = gf_add(a2, gf_mul_reg(gf_add(f0, f1), gf_add(g0, g1)));
     }
-    block_xor3(a0, a1, a2, partial + 3 * (size_t)gridDim.x * blockIdx.y);
-    __shared__ unsigned last;
-    __threadfence();
+    // block sums XORed (device-scope atomics the block waits for) into slot
+    // blockIdx.x mod S of the pair; counted after they are performed
+    __shared__ gf red[3][CB / 64];
+    __shared__ unsigned done[3];
+    a0 = wave_xor(a0);
+    a1 = wave_xor(a1);
+    a2 = wave_xor(a2);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a0;
+        red[1][threadIdx.x >> 6] = a1;
+        red[2][threadIdx.x >> 6] = a2;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1;
+    if (threadIdx.x < 3) {
+        gf v = gf_zero();
+        for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[threadIdx.x][w]);
+        unsigned long long *p =
+            (unsigned long long *)(slots + 3 * ((size_t)S * blockIdx.y + blockIdx.x % S) + threadIdx.x);
+        const unsigned long long o0 =
+            __hip_atomic_fetch_xor(p, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long o1 =
+            __hip_atomic_fetch_xor(p + 1, (unsigned long long)v.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        done[threadIdx.x] = (unsigned)(o0 ^ o1) | 1u;
+    }
     __syncthreads();
-    if (!last || threadIdx.x >= 64) return;
-    __threadfence();
-    quad_fs_step(fs, k, partial, gridDim.x);
+    if (!block_counted(counter, gridDim.x * gridDim.y, done, 3) || threadIdx.x >= 64) return;
+    quad_fs_step(fs, k, slots, S);
 }
 
 __global__ __launch_bounds__(CB) void k_fix_multi_dev(gf *__restrict__ P, size_t stride, size_t half,
@@ -1153,18 +1442,27 @@ __global__ __launch_bounds__(64) void ks_round(const Seg *segs, int nseg, const 
 // on the root and the reduced sums.  Each block publishes its writes with a
 // fence before it counts, and the last one fences before it reads them; no
 // block waits for another.  *counter is zero at launch.
-__global__ __launch_bounds__(kSubThreads) void k_top_round(uint64_t *__restrict__ prev, size_t h, int lv,
+// prev is not __restrict__: the last block reads the root written through it
+// by the subtree block (round->in points into the same tree).
+__global__ __launch_bounds__(kSubThreads) void k_top_round(uint64_t *prev, size_t h, int lv,
                                                             unsigned nsub, const gf *__restrict__ a,
                                                             const gf *__restrict__ b, size_t n, gf *partial,
                                                             unsigned ndot, const Seg *round, unsigned *counter,
                                                             int pair_w) {
     __shared__ ulonglong2 buf[kSubLds][2];
     __shared__ gf red[kSubThreads / 64];
-    __shared__ unsigned last;
+    __shared__ unsigned done[1];
     const unsigned blk = blockIdx.x;
     if (blk < nsub) {
+        // the tree levels (the root among them) are plain stores: this block
+        // releases them with a fence before it is counted
         subtree(prev, h, lv, blk, buf, pair_w);
+        __threadfence();
+        if (threadIdx.x == 0) done[0] = 1u;
+        __syncthreads();
     } else {
+        // the dot blocks publish their sums with device-scope atomics they
+        // wait for, so they need no release fence (a write-back of the L2)
         const unsigned db = blk - nsub;
         gf acc = gf_zero();
         for (size_t k = db * (size_t)kSubThreads + threadIdx.x; k < n; k += (size_t)ndot * kSubThreads)
@@ -1175,18 +1473,20 @@ __global__ __launch_bounds__(kSubThreads) void k_top_round(uint64_t *__restrict_
         if (threadIdx.x == 0) {
             gf sum = gf_zero();
             for (int w = 0; w < kSubThreads / 64; w++) sum = gf_add(sum, red[w]);
-            gf_st(partial + db, sum);
+            unsigned long long *p = (unsigned long long *)(partial + db);
+            const unsigned long long o0 =
+                __hip_atomic_exchange(p, (unsigned long long)sum.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long o1 =
+                __hip_atomic_exchange(p + 1, (unsigned long long)sum.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            done[0] = (unsigned)(o0 ^ o1) | 1u;
         }
+        __syncthreads();
     }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == nsub + ndot - 1;
-    __syncthreads();
-    if (!last || threadIdx.x >= 64) return;
-    __threadfence();
+    if (!block_counted(counter, nsub + ndot, done, 1) || threadIdx.x >= 64) return;
+    __threadfence();  // acquire: the subtree block's levels, the root among them
     const Seg &g = *round;
     gf acc = gf_zero();
-    for (unsigned t = threadIdx.x; t < ndot; t += 64) acc = gf_add(acc, gf_ld(partial + t));
+    for (unsigned t = threadIdx.x; t < ndot; t += 64) acc = gf_add(acc, gf_ld_published(partial + t));
     acc = wave_xor(acc);
     fs_round((FsState *)g.out, (int)g.count, acc, (const uint64_t *)g.in);
 }
@@ -1521,11 +1821,34 @@ hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_
     k_outer<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(A, m, B, out, n);
     return hipGetLastError();
 }
-hipError_t prod_step(hipStream_t st, gf *f, gf *g, const gf *T, size_t half, int k, ProdFs *fs, gf *partial,
-                     unsigned *counter) {
-    const unsigned gsz = grid_for(half, CB, 512);
-    if (k == 0) k_prod_step<false><<<gsz, CB, 0, st>>>(f, g, T, half, k, fs, partial, counter);
-    else k_prod_step<true><<<gsz, CB, 0, st>>>(f, g, T, half, k, fs, partial, counter);
+// blocks of k_prod_round<M> resident per compute unit (occupancy query, once)
+static unsigned prod_blocks_per_cu() {
+    static unsigned cached = 0;
+    if (!cached) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_prod_round<2>, CB, 0) != hipSuccess || nb < 1) nb = 4;
+        cached = (unsigned)nb;
+    }
+    return cached;
+}
+hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, const gf *T, size_t half,
+                     int k, ProdFs *fs, int cus) {
+    if (!half) return hipErrorInvalidValue;
+    const unsigned g = grid_for(half, CB, (unsigned)std::max(1, cus) * prod_blocks_per_cu());
+    if (k == 0) k_prod_round<0><<<g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, T, half, k, fs);
+    else if (fsrc != fdst) k_prod_round<1><<<g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, T, half, k, fs);
+    else k_prod_round<2><<<g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, T, half, k, fs);
+    return hipGetLastError();
+}
+hipError_t prod_prep(hipStream_t st, const gf *A, int m, const gf *B, gf *T, size_t n, ProdFs *fs) {
+    k_prod_prep<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(A, m, B, T, n, fs);
+    return hipGetLastError();
+}
+size_t prod_tail_max_pairs() { return kTailPairs; }
+hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, const gf *T, size_t half,
+                     int k0, ProdFs *fs) {
+    if (!half || half > kTailPairs || (half & (half - 1))) return hipErrorInvalidValue;
+    k_prod_tail<<<1, kTailThreads, 0, st>>>(fsrc, gsrc, fout, gout, T, half, k0, fs);
     return hipGetLastError();
 }
 hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,
@@ -1555,12 +1878,13 @@ hipError_t fix_multi_dev(hipStream_t st, gf *P, size_t stride, uint32_t m, size_
     return hipGetLastError();
 }
 hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
-                     int k, QuadFs *fs, gf *partial, unsigned max_blocks, unsigned *counter) {
+                     int k, QuadFs *fs, gf *slots, unsigned max_blocks, unsigned *counter) {
     if (npairs == 0 || npairs > 65535 || max_blocks == 0) return hipErrorInvalidValue;
     const unsigned gx = grid_for(h, CB, max_blocks);
-    k_quad_step<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, k, fs, partial, counter);
+    k_quad_step<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, k, fs, slots, quad_slots(max_blocks), counter);
     return hipGetLastError();
 }
+unsigned quad_slots(unsigned max_blocks) { return max_blocks < 16 ? max_blocks : 16; }
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
                        const gf *ft, gf *vals, uint8_t *fold_ok) {
     const size_t n = (size_t)count * q;

@@ -85,23 +85,43 @@ __host__ __device__ inline size_t fs_bytes(uint32_t n, uint32_t in_cap) { return
 
 // Device-resident transcript of a degree-3 product sumcheck (the
 // prove_sumcheck driver, sumcheck/src/lib.rs:56-103, over f g eq(r, .)): the
-// challenger's byte buffers and the running scalars.  One region laid out as
-//   ProdFs | gf cf[n] | gf e0[n] | gf polys[3n] | gf chal[n] | uint8_t input[in_cap]
-// cf[i] is the correcting factor of the halved eq table at round variable i
-// and e0[i] = 1 + r_i; polys and chal receive each round's compressed
-// polynomial [c0, c2, c3] and challenge.
+// challenger's byte buffers and the running eq product.  One region laid out
+// as
+//   ProdFs | gf cf[n] | gf e0[n] | gf ecf[n] | gf polys[3n] | gf chal[n] |
+//   uint8_t input[in_cap] | gf finals[4] | uint64_t acc[n][6 kProdSlots] |
+//   unsigned counters[n]
+// cf[i] is the correcting factor of the halved eq table at round variable i,
+// e0[i] = 1 + r_i and ecf[i] = e0[i] * cf[i]; polys and chal receive each
+// round's compressed polynomial [c0, c2, c3] and challenge, finals the last
+// round's two entries of f and of g.  acc[k] holds
+// round k's pair sums in 16 slots of [A0, A1, A2] that the blocks XOR into
+// (zero at upload); counters[k] counts round k's finished blocks.
 struct ProdFs {
     uint32_t in_len, out_len, in_cap, n;
-    gf s;      // product of eq(r_j, x_j) over the rounds so far
-    gf sub;    // subclaim
+    // eq scalars of the next round to run (variable i): s = the product of
+    // eq(r_j, x_j) over the rounds done, K = s e0[i] cf[i], L = s cf[i]
+    gf s, K, L;
     gf x;      // last sampled challenge (read by the next round's fold)
     gf claim;  // sum over the cube, observed before the first round
     uint8_t out[32];
 };
+constexpr int kProdSlots = 256;  // accumulation slots per round (blocks spread their atomics over them)
 __host__ __device__ inline size_t prod_cf_off() { return sizeof(ProdFs); }
-__host__ __device__ inline size_t prod_polys_off(uint32_t n) { return sizeof(ProdFs) + 32 * (size_t)n; }
+__host__ __device__ inline size_t prod_polys_off(uint32_t n) { return sizeof(ProdFs) + 48 * (size_t)n; }
 __host__ __device__ inline size_t prod_chal_off(uint32_t n) { return prod_polys_off(n) + 48 * (size_t)n; }
 __host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_off(n) + 16 * (size_t)n; }
+__host__ __device__ inline size_t prod_finals_off(uint32_t n, uint32_t in_cap) {
+    return (prod_input_off(n) + in_cap + 15) & ~(size_t)15;
+}
+__host__ __device__ inline size_t prod_acc_off(uint32_t n, uint32_t in_cap) {
+    return (prod_finals_off(n, in_cap) + 64 + 255) & ~(size_t)255;
+}
+__host__ __device__ inline size_t prod_counters_off(uint32_t n, uint32_t in_cap) {
+    return prod_acc_off(n, in_cap) + 8 * 6 * kProdSlots * (size_t)n;
+}
+__host__ __device__ inline size_t prod_bytes(uint32_t n, uint32_t in_cap) {
+    return prod_counters_off(n, in_cap) + 4 * (size_t)n;
+}
 
 // Device-resident transcript of a Quadratic sumcheck driven by
 // prove_sumcheck (quadratic.rs:84-136, sumcheck/src/lib.rs:56-103): the
@@ -213,16 +233,29 @@ hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t 
 hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, gf r);
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x);
 hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_t n);
-// Round k of a product sumcheck with its transcript step on the device:
-// round 0 reads f, g over 2 * half entries; later rounds first fold both
-// tables on the top variable with the previous round's challenge (read from
-// the ProdFs region) down to 2 * half entries.  Each block leaves its sums
-// in partial[3 * block, +3); the block that finishes last (counted in
-// *counter, zero at launch) reduces them and runs the round's transcript
-// step, which writes the round's polynomial and challenge into the region.
-// Every block and partial sum fits: at most 512 blocks.
-hipError_t prod_step(hipStream_t st, gf *f, gf *g, const gf *T, size_t half, int k, ProdFs *fs, gf *partial,
-                     unsigned *counter);
+// Round k of a product sumcheck with its transcript step on the device.
+// Round 0 reads f, g (2 * half entries each) from fsrc, gsrc.  Round k >= 1
+// first folds the previous tables (4 * half entries at fsrc, gsrc) on their
+// top variable with the previous round's challenge, read from the ProdFs
+// region, into 2 * half entries at fdst, gdst (which may equal the sources).
+// Every block XORs its pair sums into the round's accumulation slots; the
+// block that finishes last (counted in the region) reduces them and runs the
+// round's transcript step, which writes the round's polynomial and
+// challenge into the region.  cus sizes the grid to fill the device.
+hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, const gf *T, size_t half,
+                     int k, ProdFs *fs, int cus);
+// Set-up of a product sumcheck: the halved eq table T[i] = A[i mod 2^m]
+// B[i >> m] (n entries, as outer()) and zeroed accumulation slots and
+// counters of the ProdFs region, in one launch.
+hipError_t prod_prep(hipStream_t st, const gf *A, int m, const gf *B, gf *T, size_t n, ProdFs *fs);
+// The product sumcheck's rounds from k0 on (round k0 has half pairs, at most
+// prod_tail_max_pairs(), a power of two), all in one single-block launch:
+// the tables (4 * half entries at fsrc, gsrc; 2 * half in round 0) are
+// loaded once into the block's LDS and folded there.  The last round's two
+// entries of each table are written to fout, gout.
+hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, const gf *T, size_t half,
+                     int k0, ProdFs *fs);
+size_t prod_tail_max_pairs();
 // MLE value of every column of a rows x w matrix at a point whose eq table is
 // A (2^m entries) x B: partial[col * nblocks + block], one pass over the data
 hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,
@@ -236,12 +269,14 @@ hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t ha
 // the same with the challenge read from device memory (*x)
 hipError_t fix_multi_dev(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, const gf *x);
 // Round k of a Quadratic sumcheck with its transcript step on the device:
-// the three sums of every pair over halves of h entries (as quad_sums), and
-// in the block that finishes last (counted in *counter, zero at launch) the
-// round's Fiat-Shamir step into the QuadFs region.  partial holds
-// 3 * npairs * max_blocks elements.
+// the three sums of every pair over halves of h entries (as quad_sums), XORed
+// by the blocks into quad_slots(max_blocks) slots of three elements per pair
+// at slots (zero at launch; the last block reads and clears them), and in
+// the block that finishes last (counted in *counter, zero at launch) the
+// round's Fiat-Shamir step into the QuadFs region.
 hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
-                     int k, QuadFs *fs, gf *partial, unsigned max_blocks, unsigned *counter);
+                     int k, QuadFs *fs, gf *slots, unsigned max_blocks, unsigned *counter);
+unsigned quad_slots(unsigned max_blocks);
 // verifier kernels: fold chains (thread per proof x query) and Merkle paths
 // (thread per proof x query x path, d + 1 paths)
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,

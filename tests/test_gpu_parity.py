@@ -465,7 +465,8 @@ def test_open_into_caller_buffer(bfd, oracle, ctx):
 
 
 def test_product_sumcheck_bit_exact(bfd, oracle, ctx):
-    for n in [1, 2, 3, 5, 8, 12]:
+    # n 9, 10, 11 and 12 start the single-block tail at rounds 0, 1, 2 and 3
+    for n in [1, 2, 3, 5, 8, 9, 10, 11, 12, 14]:
         _say(f"product sumcheck n={n}")
         f, g, r = bfd.random_vec(400 + n, 1 << n), bfd.random_vec(500 + n, 1 << n), bfd.random_vec(600 + n, n)
         got = bfd.prove_sumcheck_product(f, g, r, bfd.GenericChallenger.keccak256(), ctx)
