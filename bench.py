@@ -101,9 +101,14 @@ def valu_per_node(kernel: str):
 VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
-def cpu_baseline(args) -> dict:
-    """The CPU restatement (oracle/) with OpenMP on this host's cores, on a
-    bounded sample: commit+open of 2^SAMPLE evaluations at the same rate."""
+def cpu_baseline(args, gpu_proof=None) -> dict:
+    """The CPU restatement (oracle/) with OpenMP on this host's cores, timed at
+    the benchmark's own size (2^LOG_N evaluations at rate 1/4, same seeded
+    evaluations and point): the median of `cpu_full_samples` runs after the
+    smaller runs below have warmed the thread pool.  Its proof must equal the
+    GPU proof of the timed steps.  Also reported: commit+open at 2^cpu_log_n
+    (median of `cpu_samples` after a warm-up) and the restatement's verifier
+    on one proof of the batched-verification workload (one core)."""
     import oracle as O
     cores = min(16, os.cpu_count() or 1)
     O.set_threads(cores)
@@ -115,7 +120,24 @@ def cpu_baseline(args) -> dict:
         t0 = time.perf_counter()
         O.basefold_prove(LAMBDA, LOG2_C, ev, pt)
         times.append(time.perf_counter() - t0)
-    t = statistics.median(times)
+    t_small = statistics.median(times)
+    del ev, pt
+    full = None
+    if args.cpu_full_samples > 0:
+        nf = args.log_n
+        evf, ptf = O.random_vec(0, 1 << nf), O.random_vec(1, nf)
+        tf = []
+        proof = None
+        for _ in range(args.cpu_full_samples):
+            t0 = time.perf_counter()
+            _, _, proof = O.basefold_prove(LAMBDA, LOG2_C, evf, ptf)
+            tf.append(time.perf_counter() - t0)
+            print(f"cpu baseline sample 2^{nf}: {tf[-1]:.1f} s", file=sys.stderr, flush=True)
+        full = {"log_n": nf, "seconds": tf, "median_s": statistics.median(tf),
+                "proof_equals_gpu": (proof == gpu_proof) if gpu_proof is not None else None}
+        if gpu_proof is not None and proof != gpu_proof:
+            raise SystemExit("the CPU restatement's proof differs from the GPU proof at the benchmark size")
+        del evf, ptf
     # the restatement's verifier on one proof of the batched-verification workload (2^16, rate 1/2)
     O.set_threads(1)
     ev1, pt1 = O.random_vec(3000, 1 << 16), O.random_vec(4000, 16)
@@ -126,11 +148,24 @@ def cpu_baseline(args) -> dict:
         assert O.basefold_verify(LAMBDA, 1, root1, pt1, claim1, proof1) == 0
         vt.append(time.perf_counter() - t0)
     O.set_threads(cores)
-    return {"value": (1 << n) / t, "unit": "elems/s", "cores": cores, "kind": "port",
-            "verify_ms_per_proof": 1e3 * statistics.median(vt), "verify_cores": 1,
-            "sample": f"commit+open of 2^{n} evaluations at rate 1/4 (median of {len(times)} after a warm-up, "
-                      f"C restatement of the reference CPU path, OpenMP {cores} threads, PCLMUL; host "
-                      f"{cpu_model()}, nproc {os.cpu_count()})"}
+    host = f"C restatement of the reference CPU path, OpenMP {cores} threads, PCLMUL; host {cpu_model()}, nproc {os.cpu_count()}"
+    if full:
+        value = (1 << full["log_n"]) / full["median_s"]
+        sample = (f"commit+open of 2^{full['log_n']} evaluations at rate 1/4, the benchmark's own inputs "
+                  f"(median of {len(full['seconds'])} after warm-up runs at 2^{n}; proof byte-equal to the GPU "
+                  f"proof), {host}")
+    else:
+        value = (1 << n) / t_small
+        sample = f"commit+open of 2^{n} evaluations at rate 1/4 (median of {len(times)} after a warm-up), {host}"
+    return {"value": value, "unit": "elems/s", "cores": cores, "kind": "port", "sample": sample,
+            "full_size": full,
+            "small": {"log_n": n, "median_s": t_small, "samples": len(times), "elems_per_s": (1 << n) / t_small},
+            "verify_ms_per_proof": 1e3 * statistics.median(vt), "verify_cores": 1}
+
+
+def stage(msg: str) -> None:
+    """Progress on stderr (the JSON line is the only stdout output)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_model() -> str:
@@ -144,7 +179,7 @@ def cpu_model() -> str:
     return "unknown CPU"
 
 
-def product_sumcheck(bfd, ctx, torch, reps: int) -> dict:
+def product_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
     """Config 3: degree-3 product sumcheck of f g eq(r, .) over 22 variables
     (f, g uniform from seeds 0 and 1, r from seed 2), tables resident in HBM;
     one call = claim + 22 rounds with the transcript (fresh challenger)."""
@@ -168,8 +203,16 @@ def product_sumcheck(bfd, ctx, torch, reps: int) -> dict:
         ts.append(time.perf_counter() - t0)
     assert all((a == b).all() for a, b in zip(out, ref)), "product sumcheck not deterministic"
     t = statistics.median(ts)
+    # multiplications: the halved eq table (2^(n-1)), five per pair in round 0
+    # (T f0, T f1 and three products) and nine after (four folds and five)
+    G = (1 << (n - 1)) + 5 * (1 << (n - 1)) + sum(9 * (1 << (n - 1 - k)) for k in range(1, n))
+    pos = {"gf_muls": G, "keccak_perms": n}
+    if rates:
+        pos["valu_floor_ms"] = 1e3 * (G / rates[1] + n / rates[0])
+        pos["valu_frac"] = pos["valu_floor_ms"] / (1e3 * t)
+        pos["note"] = "the floor counts multiplications at the isolated rate; the n transcript steps are serial"
     return {"workload": "f g eq(r, .), 22 variables, tables resident, fresh challenger", "ms": 1e3 * t,
-            "evals_per_s": (1 << n) / t}
+            "evals_per_s": (1 << n) / t, "position": pos}
 
 
 def mix_position(jobs, seconds: float, rates) -> dict:
@@ -190,7 +233,7 @@ def mix_position(jobs, seconds: float, rates) -> dict:
     return out
 
 
-def quadratic_sumcheck(bfd, ctx, torch, reps: int) -> dict:
+def quadratic_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
     """prove_sumcheck of a Quadratic (the reference's general sumcheck
     function, quadratic.rs) over 8 tables of 2^20 evaluations with 8 random
     pairs and scalars (numpy seed 5), tables resident, fresh challenger."""
@@ -217,8 +260,14 @@ def quadratic_sumcheck(bfd, ctx, torch, reps: int) -> dict:
         ts.append(time.perf_counter() - t0)
     assert all((np.asarray(a) == np.asarray(b)).all() for a, b in zip(out, ref)), "quadratic sumcheck not deterministic"
     t = statistics.median(ts)
+    # three products per pair and pair position, one per table entry for the fix
+    G = sum(3 * npairs * (1 << (n - 1 - k)) + m * (1 << (n - 1 - k)) for k in range(n))
+    pos = {"gf_muls": G, "keccak_perms": n}
+    if rates:
+        pos["valu_floor_ms"] = 1e3 * (G / rates[1] + n / rates[0])
+        pos["valu_frac"] = pos["valu_floor_ms"] / (1e3 * t)
     return {"workload": f"{m} tables of 2^{n}, {npairs} pairs, tables resident, fresh challenger", "ms": 1e3 * t,
-            "table_evals_per_s": m * (1 << n) / t}
+            "table_evals_per_s": m * (1 << n) / t, "position": pos}
 
 
 def batched_vs_plain(bfd, ctx, torch, reps: int, tune_reps: int = 0, rates=None) -> dict:
@@ -408,6 +457,11 @@ def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int, rates_k
             n = int(sizes[i])
             tickets[i] = srv.submit(pool[(n, i % 2)].data_ptr(), n, points[i], int(rates[i]))
 
+    # served proofs kept for the check after the clock: every 16th request and
+    # the first eight of the largest size
+    largest = [i for i in range(n_req) if int(sizes[i]) == int(sizes.max())][:8]
+    check = sorted(set(range(0, n_req, 16)) | set(largest))
+    kept = {}
     th = threading.Thread(target=client)
     th.start()
     lat = []
@@ -415,7 +469,11 @@ def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int, rates_k
     for i in range(n_req):
         while tickets[i] is None:
             time.sleep(0.0005)
-        lat.append(srv.wait(tickets[i])[3])
+        root, claim, proof, ms = srv.wait(tickets[i])
+        lat.append(ms)
+        if i in check:
+            kept[i] = (root, claim, proof)
+        del proof
         peak_resident = max(peak_resident, srv.stats()["resident_bytes"])
     th.join()
     makespan = time.perf_counter() - t0
@@ -423,12 +481,59 @@ def serving_trace(bfd, ctx, torch, n_req: int, rate: float, budget: int, rates_k
     srv.close()
     total = int(sum(1 << int(n) for n in sizes))
     lat = np.array(lat)
-    return {"workload": f"{n_req} requests, Poisson {rate:g}/s, log2 n U{{14..24}}, rate 1/2|1/4 50/50",
-            "budget_bytes": budget, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "elems_per_s": total / makespan, "makespan_s": makespan,
-            "arrival_span_s": float(gaps.sum()), "requests_per_s_served": n_req / makespan,
-            "position": mix_position([(int(n), int(c)) for n, c in zip(sizes, rates)], makespan, rates_k),
-            "peak_resident_sampled_bytes": peak_resident, "server_timing": timing}
+    out = {"workload": f"{n_req} requests, Poisson {rate:g}/s, log2 n U{{14..24}}, rate 1/2|1/4 50/50",
+           "budget_bytes": budget, "latency_samples": int(len(lat)),
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "p99_is_worst_of": int(round(0.01 * len(lat))), "mean_ms": float(lat.mean()),
+           "elems_per_s": total / makespan, "makespan_s": makespan,
+           "arrival_span_s": float(gaps.sum()), "requests_per_s_served": n_req / makespan,
+           "position": mix_position([(int(n), int(c)) for n, c in zip(sizes, rates)], makespan, rates_k),
+           "peak_resident_sampled_bytes": peak_resident, "server_timing": timing}
+    out["served_proofs_checked"] = check_served(bfd, ctx, kept, sizes, rates, points, pool)
+    return out
+
+
+def check_served(bfd, ctx, kept, sizes, rates, points, pool) -> dict:
+    """After the clock: the kept served proofs are verified on the device
+    (Basefold::verify with the harness transcript, grouped by code), and the
+    first four of at most 2^22 evaluations are compared byte for byte with the
+    plain path's proof of the same polynomial."""
+    by_code = {}
+    for i in kept:
+        by_code.setdefault((int(sizes[i]), int(rates[i])), []).append(i)
+    verified, rejected = 0, []
+    for (n, c), idx in sorted(by_code.items()):
+        pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(LAMBDA, c, n)), ctx)
+        chs = []
+        for i in idx:
+            ch = bfd.GenericChallenger.keccak256()
+            ch.observe(kept[i][0])
+            ch.observe_ext_slice(kept[i][1])
+            chs.append(ch)
+        res = pcs.verify_batch([kept[i][0] for i in idx], [bfd.PolyEvals(kept[i][1], points[i]) for i in idx],
+                               [kept[i][2] for i in idx], chs)
+        verified += sum(1 for r in res if r == 0)
+        rejected += [(i, r) for i, r in zip(idx, res) if r != 0]
+        del pcs
+    plain = 0
+    for i in sorted(kept):
+        n, c = int(sizes[i]), int(rates[i])
+        if n > 22 or plain >= 4:
+            continue
+        pcs = bfd.Basefold.setup(bfd.BasefoldConfig(bfd.BinaryReedSolomonCode(LAMBDA, c, n)), ctx)
+        cd = pcs.commit_device(pool[(n, i % 2)].data_ptr(), 1 << n)
+        claim = pcs.eval(cd, points[i])
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(cd.comm)
+        ch.observe_ext_slice(claim)
+        if pcs.open(cd, [bfd.PolyEvals(claim, points[i])], ch) != kept[i][2] or cd.comm != kept[i][0]:
+            raise SystemExit(f"served proof {i} differs from the plain path's")
+        plain += 1
+        del cd, pcs
+    if rejected:
+        raise SystemExit(f"served proofs rejected by the verifier: {rejected[:8]}")
+    return {"verified": verified, "byte_equal_to_plain": plain, "indices": "every 16th request and the first 8 "
+            "of the largest size"}
 
 
 def main() -> None:
@@ -438,12 +543,14 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log-n", type=int, default=LOG_N)
     ap.add_argument("--cpu-log-n", type=int, default=20)
-    ap.add_argument("--cpu-samples", type=int, default=10)
+    ap.add_argument("--cpu-samples", type=int, default=10, help="runs at 2^cpu-log-n (median)")
+    ap.add_argument("--cpu-full-samples", type=int, default=3,
+                    help="CPU runs at the benchmark's own size (median is cpu_baseline.value; 0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain figure")
     ap.add_argument("--no-ref-shapes", action="store_true",
                     help="skip the reference benchmark's own shapes (2^16..2^19 rows x 256 columns)")
-    ap.add_argument("--serve-requests", type=int, default=400, help="serving trace length (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="serving trace length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-sustain-rate", type=float, default=120.0,
                     help="second serving trace below the saturation rate (0: skip)")
@@ -630,27 +737,35 @@ def main() -> None:
         "proof_bytes": len(proof),
         "tuning": tuning,
     }
+    stage(f"main workload done: {ms_per_step:.2f} ms per step")
     if not args.no_ref_shapes:
+        stage("reference shapes")
         result["reference_shapes"] = reference_shapes(bfd, ctx, torch, 3)
-    result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 5)
-    result["quadratic_sumcheck"] = quadratic_sumcheck(bfd, ctx, torch, 5)
+    stage("sumchecks")
+    result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 5, (r_k, r_g))
+    result["quadratic_sumcheck"] = quadratic_sumcheck(bfd, ctx, torch, 5, (r_k, r_g))
     if not args.no_batch:
+        stage("batched vs plain")
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps,
                                              0 if args.no_tune else args.tune_reps, (r_k, r_g))
     if args.verify_proofs > 0:
+        stage("verification")
         result["verify"] = verification(bfd, ctx, torch, args.verify_proofs)
     if args.serve_requests > 0:
+        stage("serving")
         result["serving"] = serving_trace(bfd, ctx, torch, args.serve_requests, args.serve_rate,
                                           int(args.serve_budget_gb * 1e9), (r_k, r_g))
         # at 200 req/s the offered work exceeds what one GPU serves, so
         # latency grows with the trace; `requests_per_s_served` there is the
         # saturation throughput, and this trace shows latency below it
         if args.serve_sustain_rate > 0:
+            stage("serving below saturation")
             result["serving_sustainable"] = serving_trace(bfd, ctx, torch, args.serve_requests,
                                                           args.serve_sustain_rate, int(args.serve_budget_gb * 1e9),
                                                           (r_k, r_g))
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args)
+        stage("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(args, proof if args.log_n == LOG_N else None)
     print(json.dumps(result))
 
 
