@@ -293,7 +293,11 @@ def verify_both(bfd, oracle, ctx, pcs, log2_c, w, root, point, claims, proof):
 @pytest.mark.parametrize("n,log2_c,w", [(10, 1, 1), (9, 2, 3), (6, 0, 9), (1, 1, 2), (0, 1, 4)])
 def test_verifier_matches_restatement(bfd, oracle, ctx, n, log2_c, w):
     """Basefold::verify on the device against the restatement's verifier: honest proofs accept,
-    and every tampered part of the wire format gets the same error kind."""
+    and every tampered part of the wire format gets the same error kind.  The query_count and
+    truncation cases pin the restatement's stricter decoding (a proof must carry exactly
+    num_queries openings and no trailing bytes, else malformed, 4); the reference zips the
+    sampled indices with whatever openings the proof carries (basefold.rs:285) and would
+    check only those."""
     _say(f"case verify n={n} log2_c={log2_c} w={w}")
     evals, point = bfd.random_vec(600 + n, (1 << n) * w), bfd.random_vec(601 + n, n)
     root, claims, proof = gpu_prove_w(bfd, ctx, 128, log2_c, evals, w, point)
