@@ -179,6 +179,36 @@ def cpu_model() -> str:
     return "unknown CPU"
 
 
+def stepwise_open(bfd, pcs, cd, claim, point, reps: int) -> dict:
+    """Basefold::open with the transcript on a host challenger the caller
+    drives (bfd_open_begin .. bfd_open_finish: one device round trip per
+    round), against the same opening with the transcript on the device, on
+    one commitment; the two proofs must be byte-equal."""
+    def fresh():
+        ch = bfd.GenericChallenger.keccak256()
+        ch.observe(cd.comm)
+        ch.observe_ext_slice(claim)
+        return ch
+
+    ev = [bfd.PolyEvals(claim, point)]
+    dev_ms, step_ms = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        p_dev = pcs.open(cd, ev, fresh())
+        t1 = time.perf_counter()
+        p_step = pcs.open_stepwise(cd, ev, fresh())
+        t2 = time.perf_counter()
+        if p_dev != p_step:
+            raise SystemExit("step-wise open gave a proof different from the device-transcript open")
+        dev_ms.append(1e3 * (t1 - t0))
+        step_ms.append(1e3 * (t2 - t1))
+    rounds = len(point)
+    d, s = float(np.median(dev_ms)), float(np.median(step_ms))
+    return {"workload": f"open of one 2^{rounds} commitment, host GenericChallenger driven from Python",
+            "rounds": rounds, "device_transcript_ms": d, "stepwise_ms": s,
+            "extra_ms_per_round": (s - d) / max(1, rounds), "reps": reps, "proofs_equal": True}
+
+
 def product_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
     """Config 3: degree-3 product sumcheck of f g eq(r, .) over 22 variables
     (f, g uniform from seeds 0 and 1, r from seed 2), tables resident in HBM;
@@ -203,9 +233,10 @@ def product_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
         ts.append(time.perf_counter() - t0)
     assert all((a == b).all() for a, b in zip(out, ref)), "product sumcheck not deterministic"
     t = statistics.median(ts)
-    # multiplications: the halved eq table (2^(n-1)), five per pair in round 0
-    # (T f0, T f1 and three products) and nine after (four folds and five)
-    G = (1 << (n - 1)) + 5 * (1 << (n - 1)) + sum(9 * (1 << (n - 1 - k)) for k in range(1, n))
+    # multiplications: five per pair in round 0 (A f0, A f1 with the eq
+    # factor A, and three products) and nine after (four folds and five);
+    # the eq factor B enters once per block, a few thousand products
+    G = 5 * (1 << (n - 1)) + sum(9 * (1 << (n - 1 - k)) for k in range(1, n))
     pos = {"gf_muls": G, "keccak_perms": n}
     if rates:
         pos["valu_floor_ms"] = 1e3 * (G / rates[1] + n / rates[0])
@@ -633,8 +664,10 @@ def main() -> None:
     ctx.synchronize()
     kall = ctx.kernel_times(enable=False)
     name = max((kv for kv in kall.items() if not kv[0].startswith("host:")), key=lambda kv: kv[1][0])[0]
-    # the timed steps bracket only the dominant kernel's launches with events
-    # (events between launches cost the stream a few microseconds each)
+    # the timed steps time only the dominant kernel: one pair of HIP events
+    # on the library's stream brackets each run of its consecutive launches
+    # (a tree's single-level launches), as every event costs the stream about
+    # 5 us; the average launch is the bracketed time over the launches in it
     ctx.kernel_filter(name)
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -652,6 +685,8 @@ def main() -> None:
     ms_per_step = 1e3 * t / args.steps
 
     # dominant kernel family and its roofline position
+    if f"untimed:{name}" in ktimes:
+        raise SystemExit(f"{ktimes[f'untimed:{name}'][1]} launches of {name} ended a run with no event behind them")
     kms, launches = ktimes[name]
     M = N << LOG2_C
     c = 1 << LOG2_C
@@ -738,12 +773,15 @@ def main() -> None:
         "tuning": tuning,
     }
     stage(f"main workload done: {ms_per_step:.2f} ms per step")
+    cd = pcs.commit_device(ptr, N)
+    result["stepwise_open"] = stepwise_open(bfd, pcs, cd, claim, point, 3)
+    del cd
     if not args.no_ref_shapes:
         stage("reference shapes")
         result["reference_shapes"] = reference_shapes(bfd, ctx, torch, 3)
     stage("sumchecks")
-    result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 5, (r_k, r_g))
-    result["quadratic_sumcheck"] = quadratic_sumcheck(bfd, ctx, torch, 5, (r_k, r_g))
+    result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 20, (r_k, r_g))
+    result["quadratic_sumcheck"] = quadratic_sumcheck(bfd, ctx, torch, 20, (r_k, r_g))
     if not args.no_batch:
         stage("batched vs plain")
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps,

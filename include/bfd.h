@@ -397,8 +397,10 @@ int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms);
 int bfd_ctx_kernel_times(bfd_ctx *ctx, int enable, char *names, size_t name_len, double *ms,
                          uint32_t *launches, size_t cap, size_t *count);
 /* Restricts that timing to launches of one kernel name (NULL or "": all).
- * The events around each timed launch cost the stream a few microseconds,
- * so a run timed as a whole brackets only the kernel it needs. */
+ * An event costs the stream about 5 microseconds, so with a name set one
+ * pair of events brackets each run of consecutive launches of that kernel
+ * (closed by the next launch of another kernel); a run still open at a
+ * synchronisation is not timed and is reported as "untimed:<name>". */
 int bfd_ctx_kernel_filter(bfd_ctx *ctx, const char *only);
 
 #ifdef __cplusplus

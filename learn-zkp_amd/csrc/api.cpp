@@ -472,11 +472,12 @@ namespace {
 // launch per round while a round has more than prod_tail_max_pairs() pairs
 // (fold, pair sums and, in the block that finishes last, the round's
 // Fiat-Shamir step), then one single-block launch for the remaining rounds
-// with the tables in LDS; one upload of the transcript image before the
-// rounds and one read-back after them.  Round 0 reads the caller's tables
-// (device memory, or one upload of host tables) and round 1 folds them into
-// working tables of half their size, so the caller's tables are never
-// copied or modified.
+// with the tables in LDS; one upload of the eq factors and the transcript
+// image before the rounds and one read-back after them.  Round 0 reads the
+// caller's tables (device memory, or one upload of host tables) and round 1
+// folds them into working tables of half their size, so the caller's tables
+// are never copied or modified.  The halved eq table stays in its two
+// factors (kernels.hpp EqFactors).
 void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g_src, bool on_device,
                       const uint8_t *r_b, Challenger &ch, uint8_t claim_out[16], uint8_t *round_polys,
                       uint8_t *challenges, uint8_t final_fg[32], uint8_t subclaim_out[16]) {
@@ -514,7 +515,7 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
     std::vector<gf> cf(n, gf_one());
     for (int j = hn - 1; j >= 0; j--) cf[j] = hmul(cf[j + 1], inv[j]);
     if (ch.output.size() > 32) fail(BFD_ERR_ARG, "transcript state too large");
-    DevBuf up, W = ctx_buf(ctx, 2 * 16 * std::max<size_t>(2, N / 2)), T = ctx_buf(ctx, 16 * ((size_t)1 << hn));
+    DevBuf up, W = ctx_buf(ctx, 2 * 16 * std::max<size_t>(2, N / 2));
     const gf *fs0 = (const gf *)f_src, *gs0 = (const gf *)g_src;
     if (!on_device) {
         up = ctx_buf(ctx, 2 * 16 * N);
@@ -525,21 +526,23 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
     }
     gf *wf = W.as<gf>(), *wg = wf + std::max<size_t>(2, N / 2);
     // one upload: the two factors of the halved eq table, then the transcript
-    // image (kernels.hpp ProdFs) up to its accumulation slots, which the
-    // set-up kernel clears on the device
-    const int lo = (hn + 1) / 2, hi = hn - lo;
+    // image (kernels.hpp ProdFs) with its counters and round 0's
+    // accumulation slots zeroed.  The factors split the hn variables evenly,
+    // with at least 7 in A so that a block of the round kernels (128 pairs
+    // or more) always lies in one B entry
+    const int lo = std::max((hn + 1) / 2, std::min(hn, 7)), hi = hn - lo;
     const std::vector<gf> A = host_eq(r.data(), lo), B = host_eq(r.data() + lo, hi);
     const size_t fb = (16 * (A.size() + B.size()) + 255) & ~(size_t)255;
     const uint32_t in_cap = (uint32_t)(((ch.input.size() > 32 ? ch.input.size() : 32) + 64 + 7) & ~(size_t)7);
-    const size_t in_off = prod_input_off(n), img = prod_bytes(n, in_cap), acc_off = prod_acc_off(n, in_cap);
+    const size_t in_off = prod_input_off(n), img = prod_bytes(n, in_cap), upl = prod_upload_bytes(n, in_cap);
     const size_t fin_off = prod_finals_off(n, in_cap);
     DevBuf buf = ctx_buf(ctx, fb + img + 64);
-    ctx->pinned.ensure(fb + acc_off + 64);
+    ctx->pinned.ensure(fb + upl + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
     memcpy(pin, A.data(), 16 * A.size());
     memcpy(pin + 16 * A.size(), B.data(), 16 * B.size());
     uint8_t *hreg = pin + fb;
-    memset(hreg, 0, acc_off);
+    memset(hreg, 0, upl);
     ProdFs h;
     memset(&h, 0, sizeof h);
     h.in_len = (uint32_t)ch.input.size();
@@ -560,12 +563,9 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
         c[2 * n + i] = hmul(e0, cf[i]);
     }
     if (!ch.input.empty()) memcpy(hreg + in_off, ch.input.data(), ch.input.size());
-    hip_check(hipMemcpyAsync(buf.p, pin, fb + acc_off, hipMemcpyHostToDevice, st), "copy transcript");
-    const gf *dA = buf.as<gf>(), *dB = dA + A.size();
+    hip_check(hipMemcpyAsync(buf.p, pin, fb + upl, hipMemcpyHostToDevice, st), "copy transcript");
+    const EqFactors eq{buf.as<gf>(), buf.as<gf>() + A.size(), lo};
     ProdFs *dfs = (ProdFs *)(buf.as<uint8_t>() + fb);
-    ctx->timer.run(st, "k_prod_prep", [&] {
-        return launch::prod_prep(st, dA, lo, dB, T.as<gf>(), (size_t)1 << hn, dfs);
-    });
     uint32_t k = 0;
     for (; k < n; k++) {
         const size_t half = (size_t)1 << (n - 1 - k);
@@ -573,7 +573,7 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
         const gf *fsrc = k <= 1 ? fs0 : wf, *gsrc = k <= 1 ? gs0 : wg;
         gf *fdst = k == 0 ? (gf *)fs0 : wf, *gdst = k == 0 ? (gf *)gs0 : wg;  // round 0 writes nothing
         ctx->timer.run(st, "k_prod_round", [&] {
-            return launch::prod_step(st, fsrc, gsrc, fdst, gdst, T.as<gf>(), half, (int)k, dfs, ctx->cus);
+            return launch::prod_step(st, fsrc, gsrc, fdst, gdst, eq, half, (int)k, dfs, ctx->cus);
         });
     }
     gf *fins = (gf *)((uint8_t *)dfs + fin_off);
@@ -581,7 +581,7 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
         const size_t half = (size_t)1 << (n - 1 - k);
         const gf *fsrc = k <= 1 ? fs0 : wf, *gsrc = k <= 1 ? gs0 : wg;
         ctx->timer.run(st, "k_prod_tail",
-                       [&] { return launch::prod_tail(st, fsrc, gsrc, fins, fins + 2, T.as<gf>(), half, (int)k, dfs); });
+                       [&] { return launch::prod_tail(st, fsrc, gsrc, fins, fins + 2, eq, half, (int)k, dfs); });
     }
     // one read-back: the region up to the last round's table entries
     hip_check(hipMemcpyAsync(hreg, dfs, fin_off + 64, hipMemcpyDeviceToHost, st), "copy transcript back");
@@ -973,6 +973,7 @@ int bfd_ctx_kernel_filter(bfd_ctx *ctx, const char *only) {
     return guarded([&] {
         if (!ctx) fail(BFD_ERR_ARG, "null ctx");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        ctx->timer.close_run();
         ctx->timer.only = only ? only : "";
     });
 }

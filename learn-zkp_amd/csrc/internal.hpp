@@ -174,17 +174,23 @@ inline double now_ms() {
 // Optional per-kernel timing with HIP events on the context's stream.
 struct KernelTimer {
     bool enabled = false;
-    // when set, only launches of this name are timed: events between
-    // launches cost the stream a few microseconds each, so a timed run that
-    // needs one kernel's duration brackets that kernel alone
+    // when set, only launches of this name are timed, and a run of
+    // consecutive launches of it is bracketed by one pair of events (closed
+    // by the next timed launch of another name): every event recorded
+    // between launches costs the stream about 5 us, so a timed run that
+    // needs one kernel's average duration brackets runs of that kernel alone
     std::string only;
     struct Rec {
         std::string name;
         hipEvent_t a, b;
+        uint32_t count;  // launches between a and b
     };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
     std::map<std::string, std::pair<double, uint32_t>> totals;
+    bool run_open = false;
+    Rec cur{};
+    hipStream_t run_st = nullptr;
     hipEvent_t get() {
         if (!pool.empty()) {
             hipEvent_t e = pool.back();
@@ -195,13 +201,37 @@ struct KernelTimer {
         hip_check(hipEventCreate(&e), "hipEventCreate");
         return e;
     }
+    void close_run() {
+        if (!run_open) return;
+        cur.b = get();
+        hip_check(hipEventRecord(cur.b, run_st), "hipEventRecord");
+        pending.push_back(cur);
+        run_open = false;
+    }
     template <class F>
     void run(hipStream_t st, const char *name, F &&launch) {
-        if (!enabled || (!only.empty() && only != name)) {
+        if (!enabled) {
             hip_check(launch(), name);
             return;
         }
-        Rec r{name, get(), get()};
+        if (!only.empty()) {
+            if (only != name) {
+                close_run();
+                hip_check(launch(), name);
+                return;
+            }
+            if (run_open && run_st != st) close_run();
+            if (!run_open) {
+                cur = Rec{name, get(), nullptr, 0};
+                hip_check(hipEventRecord(cur.a, st), "hipEventRecord");
+                run_open = true;
+                run_st = st;
+            }
+            hip_check(launch(), name);
+            cur.count++;
+            return;
+        }
+        Rec r{name, get(), get(), 1};
         hip_check(hipEventRecord(r.a, st), "hipEventRecord");
         hip_check(launch(), name);
         hip_check(hipEventRecord(r.b, st), "hipEventRecord");
@@ -217,6 +247,16 @@ struct KernelTimer {
     // adds the launches that have completed; launches still in flight (work
     // enqueued after the synchronisation the caller waited for) stay pending
     void collect() {
+        // a run still open here ended right before the caller's
+        // synchronisation, with no event behind it: it is not timed, and
+        // counted under "untimed:<name>" so the caller sees the launches it
+        // is missing
+        if (run_open) {
+            auto &t = totals[std::string("untimed:") + cur.name];
+            t.second += cur.count;
+            pool.push_back(cur.a);
+            run_open = false;
+        }
         std::vector<Rec> later;
         for (auto &r : pending) {
             if (hipEventQuery(r.b) != hipSuccess) {
@@ -227,7 +267,7 @@ struct KernelTimer {
             hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
             auto &t = totals[r.name];
             t.first += ms;
-            t.second += 1;
+            t.second += r.count;
             pool.push_back(r.a);
             pool.push_back(r.b);
         }

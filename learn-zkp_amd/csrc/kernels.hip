@@ -614,25 +614,15 @@ __device__ __forceinline__ void prod_store_state(ProdFs *st, const ProdRound &R,
     st->x = R.x;
 }
 
-// T[i] = A[i mod 2^m] B[i >> m] for i < n, and the region's accumulation
-// slots and counters cleared, in one grid-stride pass
-__global__ __launch_bounds__(CB) void k_prod_prep(const gf *__restrict__ A, int m, const gf *__restrict__ B,
-                                                  gf *__restrict__ T, size_t n, ProdFs *fs) {
-    const size_t mask = ((size_t)1 << m) - 1;
-    const size_t stride = (size_t)gridDim.x * blockDim.x, t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    for (size_t i = t0; i < n; i += stride) gf_st(T + i, gf_mul_reg(gf_ld(A + (i & mask)), gf_ld(B + (i >> m))));
-    const uint32_t nv = fs->n, in_cap = fs->in_cap;
-    uint64_t *z = (uint64_t *)((uint8_t *)fs + prod_acc_off(nv, in_cap));
-    const size_t words = (prod_bytes(nv, in_cap) - prod_acc_off(nv, in_cap) + 7) / 8;
-    for (size_t i = t0; i < words; i += stride) z[i] = 0;
-}
-
 // Round k of the product sumcheck (see launch::prod_step).  MODE 0: round 0,
 // sums over the caller's tables; 1: fold the caller's tables into the working
 // ones, then the sums; 2: fold the working tables in place, then the sums.
+// Block b takes pairs [b C, (b + 1) C), C = half / gridDim.x, which share one
+// entry of the eq factor B: pair j is weighted by A[j mod 2^lo] alone and the
+// block's sums by that B entry.
 template <int MODE>
-__global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst,
-                                                   const gf *__restrict__ T, size_t half, int k, ProdFs *fs) {
+__global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq,
+                                                   size_t half, int k, ProdFs *fs) {
     __shared__ gf pre[2];  // P1, P2 of this round
     __shared__ unsigned last;
     const uint32_t nv = fs->n;
@@ -641,18 +631,29 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
     // P1, P2 for the next round's scalars, on two lanes of the second wave
     // (the transcript runs on the first)
     if (iv > 0 && (threadIdx.x == 64 || threadIdx.x == 65)) pre[threadIdx.x & 1] = prod_pre(ecfs, cfs, iv, fs->s, threadIdx.x);
+    uint8_t *base = (uint8_t *)fs;
+    const uint32_t in_cap = fs->in_cap;
+    unsigned long long *acc0 = (unsigned long long *)(base + prod_acc_off(nv, in_cap));
+    // the next round's slots start at zero (this launch ends before it runs)
+    if (k + 1 < (int)nv) {
+        unsigned long long *nxt = acc0 + (size_t)(k + 1) * 6 * kProdSlots;
+        for (size_t w = blockIdx.x * (size_t)CB + threadIdx.x; w < 6 * (size_t)kProdSlots; w += (size_t)gridDim.x * CB)
+            nxt[w] = 0;
+    }
+    const size_t C = half / gridDim.x, j0 = blockIdx.x * C, j1 = j0 + C;
+    const size_t amask = ((size_t)1 << eq.lo) - 1;
     gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
     if constexpr (MODE == 0) {
-        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half; j += (size_t)gridDim.x * blockDim.x)
+        for (size_t j = j0 + threadIdx.x; j < j1; j += CB)
             prod_terms(gf_ld(fsrc + j), gf_ld(fsrc + j + half), gf_ld(gsrc + j), gf_ld(gsrc + j + half),
-                       gf_ld(T + j), a0, a1, a2);
+                       gf_ld(eq.A + (j & amask)), a0, a1, a2);
     } else {
         // the previous tables hold 4 q entries; entries j and j + q of the
         // folded ones come from j, j + 2q and j + q, j + 3q: each thread
         // reads its four before writing its two, so in place is safe
         const size_t q = half;
         const gf_pre px = gf_prepare(fs->x);
-        for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < q; j += (size_t)gridDim.x * blockDim.x) {
+        for (size_t j = j0 + threadIdx.x; j < j1; j += CB) {
             const gf fa = gf_ld(fsrc + j), fb = gf_ld(fsrc + j + q), fc = gf_ld(fsrc + j + 2 * q),
                      fd = gf_ld(fsrc + j + 3 * q);
             const gf ga = gf_ld(gsrc + j), gb = gf_ld(gsrc + j + q), gc = gf_ld(gsrc + j + 2 * q),
@@ -663,7 +664,7 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
             gf_st(fdst + j + q, f1);
             gf_st(gdst + j, g0);
             gf_st(gdst + j + q, g1);
-            prod_terms(f0, f1, g0, g1, gf_ld(T + j), a0, a1, a2);
+            prod_terms(f0, f1, g0, g1, gf_ld(eq.A + (j & amask)), a0, a1, a2);
         }
     }
     // block sums, XORed into one of the round's accumulation slots
@@ -677,9 +678,8 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
         red[2][threadIdx.x >> 6] = a2;
     }
     __syncthreads();
-    uint8_t *base = (uint8_t *)fs;
-    const uint32_t n = fs->n, in_cap = fs->in_cap;
-    unsigned long long *acc = (unsigned long long *)(base + prod_acc_off(n, in_cap)) + (size_t)k * 6 * kProdSlots;
+    const uint32_t n = nv;
+    unsigned long long *acc = acc0 + (size_t)k * 6 * kProdSlots;
     unsigned *counter = (unsigned *)(base + prod_counters_off(n, in_cap)) + k;
     // The block's sums go out as device-scope atomics whose results it waits
     // for, so they are performed before its count below is issued; every
@@ -690,6 +690,7 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
     if (threadIdx.x < 3) {
         gf v = gf_zero();
         for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[threadIdx.x][w]);
+        v = gf_mul_reg(v, gf_ld(eq.B + (j0 >> eq.lo)));
         unsigned long long *slot = acc + 6 * (blockIdx.x % kProdSlots) + 2 * threadIdx.x;
         const unsigned long long o0 =
             __hip_atomic_fetch_xor(slot, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -759,7 +760,7 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
 constexpr int kTailThreads = 512;
 constexpr size_t kTailPairs = 256;  // pairs of the first tail round (4 * kTailPairs entries per table)
 __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, const gf *gsrc, gf *fout, gf *gout,
-                                                            const gf *__restrict__ T, size_t half, int k0, ProdFs *fs) {
+                                                            EqFactors eq, size_t half, int k0, ProdFs *fs) {
     __shared__ gf F[4 * kTailPairs], G[4 * kTailPairs], Ts[kTailPairs], TF[2 * kTailPairs];
     __shared__ gf red[3][kTailThreads / 64];
     __shared__ gf sc[3];   // s, K, L of the round being run
@@ -778,7 +779,8 @@ __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, cons
         F[e] = gf_ld(fsrc + e);
         G[e] = gf_ld(gsrc + e);
     }
-    for (size_t e = tid; e < half; e += kTailThreads) Ts[e] = gf_ld(T + e);
+    for (size_t e = tid; e < half; e += kTailThreads)
+        Ts[e] = gf_mul_reg(gf_ld(eq.A + (e & (((size_t)1 << eq.lo) - 1))), gf_ld(eq.B + (e >> eq.lo)));
     if (tid < 3 * (int)n) cst[tid / n][tid % n] = gf_ld((const gf *)((const uint8_t *)fs + prod_cf_off()) + tid);
     if (tid == 0) {
         sc[0] = fs->s;
@@ -1831,24 +1833,26 @@ static unsigned prod_blocks_per_cu() {
     }
     return cached;
 }
-hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, const gf *T, size_t half,
+hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq, size_t half,
                      int k, ProdFs *fs, int cus) {
-    if (!half) return hipErrorInvalidValue;
-    const unsigned g = grid_for(half, CB, (unsigned)std::max(1, cus) * prod_blocks_per_cu());
-    if (k == 0) k_prod_round<0><<<g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, T, half, k, fs);
-    else if (fsrc != fdst) k_prod_round<1><<<g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, T, half, k, fs);
-    else k_prod_round<2><<<g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, T, half, k, fs);
-    return hipGetLastError();
-}
-hipError_t prod_prep(hipStream_t st, const gf *A, int m, const gf *B, gf *T, size_t n, ProdFs *fs) {
-    k_prod_prep<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(A, m, B, T, n, fs);
+    if (half <= kTailPairs || (half & (half - 1)) || eq.lo < 7 || eq.lo > 62) return hipErrorInvalidValue;
+    // pairs per block: a power of two from CB up to 2^lo (one B entry per
+    // block), the smallest that keeps the grid within the resident blocks
+    const size_t cap = (size_t)std::max(1, cus) * prod_blocks_per_cu();
+    size_t C = CB;
+    while (half / C > cap && C < ((size_t)1 << eq.lo)) C <<= 1;
+    const size_t g = half / C;
+    if (g > 0x7fffffffu) return hipErrorInvalidValue;
+    if (k == 0) k_prod_round<0><<<(unsigned)g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, eq, half, k, fs);
+    else if (fsrc != fdst) k_prod_round<1><<<(unsigned)g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, eq, half, k, fs);
+    else k_prod_round<2><<<(unsigned)g, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, eq, half, k, fs);
     return hipGetLastError();
 }
 size_t prod_tail_max_pairs() { return kTailPairs; }
-hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, const gf *T, size_t half,
+hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, EqFactors eq, size_t half,
                      int k0, ProdFs *fs) {
-    if (!half || half > kTailPairs || (half & (half - 1))) return hipErrorInvalidValue;
-    k_prod_tail<<<1, kTailThreads, 0, st>>>(fsrc, gsrc, fout, gout, T, half, k0, fs);
+    if (!half || half > kTailPairs || (half & (half - 1)) || eq.lo < 0 || eq.lo > 62) return hipErrorInvalidValue;
+    k_prod_tail<<<1, kTailThreads, 0, st>>>(fsrc, gsrc, fout, gout, eq, half, k0, fs);
     return hipGetLastError();
 }
 hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,

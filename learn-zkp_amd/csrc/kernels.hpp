@@ -88,14 +88,15 @@ __host__ __device__ inline size_t fs_bytes(uint32_t n, uint32_t in_cap) { return
 // challenger's byte buffers and the running eq product.  One region laid out
 // as
 //   ProdFs | gf cf[n] | gf e0[n] | gf ecf[n] | gf polys[3n] | gf chal[n] |
-//   uint8_t input[in_cap] | gf finals[4] | uint64_t acc[n][6 kProdSlots] |
-//   unsigned counters[n]
+//   uint8_t input[in_cap] | gf finals[4] | unsigned counters[n] |
+//   uint64_t acc[n][6 kProdSlots]
 // cf[i] is the correcting factor of the halved eq table at round variable i,
 // e0[i] = 1 + r_i and ecf[i] = e0[i] * cf[i]; polys and chal receive each
 // round's compressed polynomial [c0, c2, c3] and challenge, finals the last
-// round's two entries of f and of g.  acc[k] holds
-// round k's pair sums in 16 slots of [A0, A1, A2] that the blocks XOR into
-// (zero at upload); counters[k] counts round k's finished blocks.
+// round's two entries of f and of g.  counters[k] counts round k's finished
+// blocks.  acc[k] holds round k's pair sums in kProdSlots slots of
+// [A0, A1, A2] that the blocks XOR into.  The upload before the rounds
+// carries the counters and acc[0] as zeros; round k clears acc[k + 1].
 struct ProdFs {
     uint32_t in_len, out_len, in_cap, n;
     // eq scalars of the next round to run (variable i): s = the product of
@@ -113,15 +114,29 @@ __host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_
 __host__ __device__ inline size_t prod_finals_off(uint32_t n, uint32_t in_cap) {
     return (prod_input_off(n) + in_cap + 15) & ~(size_t)15;
 }
-__host__ __device__ inline size_t prod_acc_off(uint32_t n, uint32_t in_cap) {
-    return (prod_finals_off(n, in_cap) + 64 + 255) & ~(size_t)255;
-}
 __host__ __device__ inline size_t prod_counters_off(uint32_t n, uint32_t in_cap) {
-    return prod_acc_off(n, in_cap) + 8 * 6 * kProdSlots * (size_t)n;
+    return prod_finals_off(n, in_cap) + 64;
+}
+__host__ __device__ inline size_t prod_acc_off(uint32_t n, uint32_t in_cap) {
+    return (prod_counters_off(n, in_cap) + 4 * (size_t)n + 255) & ~(size_t)255;
+}
+constexpr size_t kProdAccRound = 8 * 6 * (size_t)kProdSlots;  // bytes of one round's slots
+// bytes uploaded before the rounds: everything up to and including acc[0]
+__host__ __device__ inline size_t prod_upload_bytes(uint32_t n, uint32_t in_cap) {
+    return prod_acc_off(n, in_cap) + kProdAccRound;
 }
 __host__ __device__ inline size_t prod_bytes(uint32_t n, uint32_t in_cap) {
-    return prod_counters_off(n, in_cap) + 4 * (size_t)n;
+    return prod_acc_off(n, in_cap) + kProdAccRound * (size_t)n;
 }
+// The halved eq table of a product sumcheck as two factors:
+// T[j] = A[j mod 2^lo] B[j >> lo].  Round kernels multiply f by A per pair
+// and each block's sums by the one B entry of its range, so T is never
+// stored.
+struct EqFactors {
+    const gf *A;
+    const gf *B;
+    int lo;
+};
 
 // Device-resident transcript of a Quadratic sumcheck driven by
 // prove_sumcheck (quadratic.rs:84-136, sumcheck/src/lib.rs:56-103): the
@@ -238,22 +253,20 @@ hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_
 // first folds the previous tables (4 * half entries at fsrc, gsrc) on their
 // top variable with the previous round's challenge, read from the ProdFs
 // region, into 2 * half entries at fdst, gdst (which may equal the sources).
-// Every block XORs its pair sums into the round's accumulation slots; the
-// block that finishes last (counted in the region) reduces them and runs the
+// Pair j is weighted by the halved eq table entry of eq (half > 256 and
+// 2^eq.lo >= 128, so a block's range of pairs lies in one B entry).  Every
+// block XORs its pair sums into the round's accumulation slots; the block
+// that finishes last (counted in the region) reduces them and runs the
 // round's transcript step, which writes the round's polynomial and
 // challenge into the region.  cus sizes the grid to fill the device.
-hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, const gf *T, size_t half,
+hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq, size_t half,
                      int k, ProdFs *fs, int cus);
-// Set-up of a product sumcheck: the halved eq table T[i] = A[i mod 2^m]
-// B[i >> m] (n entries, as outer()) and zeroed accumulation slots and
-// counters of the ProdFs region, in one launch.
-hipError_t prod_prep(hipStream_t st, const gf *A, int m, const gf *B, gf *T, size_t n, ProdFs *fs);
 // The product sumcheck's rounds from k0 on (round k0 has half pairs, at most
 // prod_tail_max_pairs(), a power of two), all in one single-block launch:
 // the tables (4 * half entries at fsrc, gsrc; 2 * half in round 0) are
 // loaded once into the block's LDS and folded there.  The last round's two
 // entries of each table are written to fout, gout.
-hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, const gf *T, size_t half,
+hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, EqFactors eq, size_t half,
                      int k0, ProdFs *fs);
 size_t prod_tail_max_pairs();
 // MLE value of every column of a rows x w matrix at a point whose eq table is

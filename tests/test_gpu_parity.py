@@ -469,8 +469,10 @@ def test_open_into_caller_buffer(bfd, oracle, ctx):
 
 
 def test_product_sumcheck_bit_exact(bfd, oracle, ctx):
-    # n 9, 10, 11 and 12 start the single-block tail at rounds 0, 1, 2 and 3
-    for n in [1, 2, 3, 5, 8, 9, 10, 11, 12, 14]:
+    # n 9, 10, 11 and 12 start the single-block tail at rounds 0, 1, 2 and 3;
+    # from n 10 on the round kernels' blocks span 2^(n-8) entries of the eq
+    # factor B, at n 17 two blocks share each entry
+    for n in [1, 2, 3, 5, 8, 9, 10, 11, 12, 14, 17]:
         _say(f"product sumcheck n={n}")
         f, g, r = bfd.random_vec(400 + n, 1 << n), bfd.random_vec(500 + n, 1 << n), bfd.random_vec(600 + n, n)
         got = bfd.prove_sumcheck_product(f, g, r, bfd.GenericChallenger.keccak256(), ctx)
