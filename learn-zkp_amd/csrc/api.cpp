@@ -652,8 +652,11 @@ namespace {
 // prove_sumcheck for a Quadratic (quadratic.rs:84-136) with its transcript on
 // the device: per round one launch for the pair sums (whose last block runs
 // the round's Fiat-Shamir step) and one for the fix of every table with the
-// challenge that step left in device memory; one upload of the transcript
-// image before the rounds and one read-back after them.
+// challenge that step left in device memory, until the tables fit one
+// block's LDS; then one single-block launch for the remaining rounds.  One
+// upload of the transcript image before the rounds and one read-back after
+// them.  Round 0 reads the caller's tables and its fix writes working tables
+// of half their size, so the caller's tables are never copied or modified.
 void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys, bool on_device, uint32_t npairs,
                         const uint8_t *scalars, const uint32_t *pair_idx, Challenger &ch, uint8_t claim_out[16],
                         uint8_t *round_polys, uint8_t *challenges, uint8_t *final_evals, uint8_t subclaim_out[16]) {
@@ -681,9 +684,15 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
         return;
     }
     if (ch.output.size() > 32) fail(BFD_ERR_ARG, "transcript state too large");
-    DevBuf dP = ctx_buf(ctx, 16 * N * m), dpairs = ctx_buf(ctx, 8 * (size_t)npairs);
-    hip_check(hipMemcpyAsync(dP.p, polys, 16 * N * m, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st),
-              "copy tables");
+    DevBuf up, dpairs = ctx_buf(ctx, 8 * (size_t)npairs);
+    const gf *src0 = (const gf *)polys;
+    if (!on_device) {
+        up = ctx_buf(ctx, 16 * N * m);
+        hip_check(hipMemcpyAsync(up.p, polys, 16 * N * m, hipMemcpyHostToDevice, st), "copy tables");
+        src0 = up.as<gf>();
+    }
+    const size_t NW = std::max<size_t>(1, N / 2);  // working table stride
+    DevBuf W = ctx_buf(ctx, 16 * NW * m);
     hip_check(hipMemcpyAsync(dpairs.p, pair_idx, 8 * (size_t)npairs, hipMemcpyHostToDevice, st), "copy pairs");
     // blocks per pair: enough for the device at the first round
     const unsigned max_blocks = std::max(1u, std::min(512u, 4096u / npairs));
@@ -712,18 +721,29 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
     QuadFs *dfs = (QuadFs *)fs.p;
     unsigned *counters = (unsigned *)(fs.as<uint8_t>() + cnt_off);
     const gf *x_dev = (const gf *)(fs.as<uint8_t>() + offsetof(QuadFs, x));
+    const size_t tail = launch::quad_tail_max_half(m, npairs);
     for (uint32_t k = 0; k < n; k++) {
         const size_t half = (size_t)1 << (n - 1 - k);
+        const gf *src = k == 0 ? src0 : W.as<gf>();
+        const size_t sstride = k == 0 ? N : NW;
+        if (half <= tail) {
+            ctx->timer.run(st, "k_quad_tail", [&] {
+                return launch::quad_tail(st, src, sstride, W.as<gf>(), NW, dpairs.as<uint32_t>(), npairs, m, half,
+                                         (int)k, dfs);
+            });
+            break;
+        }
         ctx->timer.run(st, "k_quad_step", [&] {
-            return launch::quad_step(st, dP.as<gf>(), N, dpairs.as<uint32_t>(), npairs, half, (int)k, dfs,
+            return launch::quad_step(st, src, sstride, dpairs.as<uint32_t>(), npairs, half, (int)k, dfs,
                                      slots.as<gf>(), max_blocks, counters + k);
         });
-        ctx->timer.run(st, "k_fix_multi", [&] { return launch::fix_multi_dev(st, dP.as<gf>(), N, m, half, x_dev); });
+        ctx->timer.run(st, "k_fix_multi",
+                       [&] { return launch::fix_multi_dev(st, src, sstride, W.as<gf>(), NW, m, half, x_dev); });
     }
     const size_t rb = in_off + 32;
     hip_check(hipMemcpyAsync(pin, fs.p, rb, hipMemcpyDeviceToHost, st), "copy transcript back");
     // the m final evaluations P_p[0]
-    hip_check(hipMemcpy2DAsync(final_evals, 16, dP.p, 16 * N, 16, m, hipMemcpyDeviceToHost, st), "copy finals");
+    hip_check(hipMemcpy2DAsync(final_evals, 16, W.p, 16 * NW, 16, m, hipMemcpyDeviceToHost, st), "copy finals");
     hip_check(hipStreamSynchronize(st), "sumcheck");
     ctx->timer.collect();
     memcpy(&h, pin, sizeof h);

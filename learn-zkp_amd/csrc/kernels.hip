@@ -904,49 +904,29 @@ __global__ __launch_bounds__(CB) void k_quad_sums(const gf *__restrict__ P, size
     block_xor3(a0, a1, a2, partial + 3 * (size_t)gridDim.x * blockIdx.y);
 }
 
-// fix_last_var of table blockIdx.y (2*half entries -> half), in place
-// One round's transcript step of a Quadratic sumcheck, run by the first wave
-// of the block that finishes the round: per pair the three sums A0 = sum
-// f0 g0, A1 = sum f1 g1, A2 = sum (f0 + f1)(g0 + g1) over the round's
-// blocks, combined with the pair scalars into c0 = sum s A0, c2 = sum s A2
-// and the claim sum s (A0 + A1) (quadratic.rs:100-122); then observe
-// [claim,] c0, c2 (the compressed polynomial, quadratic.rs:62-64), sample x
-// and advance the subclaim with c1 = subclaim + c2 (characteristic 2,
-// quadratic.rs:70-80).  Pairs are spread over the wave's lanes.
-__device__ void quad_fs_step(QuadFs *st, int k, gf *slots, unsigned S) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t n = st->n, np = st->npairs;
-    uint8_t *base = (uint8_t *)st;
-    const gf *sc = (const gf *)(base + quad_scalars_off());
-    gf *polys = (gf *)(base + quad_polys_off(np)), *chal = (gf *)(base + quad_chal_off(np, n));
-    uint8_t *in = base + quad_input_off(np, n);
-    uint64_t *in64 = reinterpret_cast<uint64_t *>(in);
-    gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
-    for (uint32_t t = lane; t < np; t += 64) {
-        gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
-        gf *q = slots + 3 * (size_t)S * t;
-#pragma unroll 4
-        for (unsigned u = 0; u < S; u++) {
-            A0 = gf_add(A0, gf_ld_published(q + 3 * u));
-            A1 = gf_add(A1, gf_ld_published(q + 3 * u + 1));
-            A2 = gf_add(A2, gf_ld_published(q + 3 * u + 2));
-        }
-        // the next round XORs into cleared slots (it starts after this launch)
-        for (unsigned u = 0; u < 3 * S; u++) gf_st(q + u, gf_zero());
-        const gf_pre s = gf_prepare(gf_ld(sc + t));
-        c0 = gf_add(c0, gf_mul_pre(A0, s));
-        c2 = gf_add(c2, gf_mul_pre(A2, s));
-        both = gf_add(both, gf_mul_pre(gf_add(A0, A1), s));
-    }
-    c0 = wave_xor(c0);
-    c2 = wave_xor(c2);
-    both = wave_xor(both);
-    const gf sub = k == 0 ? both : st->sub;
-    const gf c1 = gf_add(sub, c2);
+// One round's transcript step of a Quadratic sumcheck, run by one whole wave
+// (prove_sumcheck_round, sumcheck/src/lib.rs:81-103): observe [claim,] c0, c2
+// (the compressed polynomial, quadratic.rs:62-64), sample x and advance the
+// subclaim with c1 = subclaim + c2 (characteristic 2, quadratic.rs:70-80).
+// In round 0 the subclaim is the claim sum s (A0 + A1), observed first
+// (sumcheck/src/lib.rs:67).  With have_prev the challenger's input is the
+// previous round's 32-byte digest prev; otherwise the input bytes come from
+// the region (the call's first round, after whatever the caller observed;
+// lane 0 appends to them there).  Every lane gets the results.
+struct QuadRound {
+    gf x, sub;
     uint64_t d[4];
-    if (k > 0 && st->in_len == 32) {
-        const uint64_t m[8] = {in64[0], in64[1], in64[2], in64[3], c0.lo, c0.hi, c2.lo, c2.hi};
-        kk::keccak256_words_pair<8>(m, d, lane & 1);
+};
+__device__ QuadRound quad_transcript(QuadFs *st, int k, gf c0, gf c2, gf both, gf sub_prev, bool have_prev,
+                                     Digest prev) {
+    const int lane = threadIdx.x & 63;
+    uint8_t *in = (uint8_t *)st + quad_input_off(st->npairs, st->n);
+    const gf sub = k == 0 ? both : sub_prev;
+    const gf c1 = gf_add(sub, c2);
+    QuadRound R;
+    if (have_prev) {
+        const uint64_t m[8] = {prev.w[0], prev.w[1], prev.w[2], prev.w[3], c0.lo, c0.hi, c2.lo, c2.hi};
+        kk::keccak256_words_pair<8>(m, R.d, lane & 1);
     } else {
         if (lane == 0) {
             uint32_t len = st->in_len;
@@ -955,7 +935,7 @@ __device__ void quad_fs_step(QuadFs *st, int k, gf *slots, unsigned S) {
                 for (int t = 0; t < 8; t++) in[len + t] = (uint8_t)(v >> (8 * t));
                 len += 8;
             };
-            if (k == 0) {  // the claim is observed first (sumcheck/src/lib.rs:67)
+            if (k == 0) {
                 put(both.lo);
                 put(both.hi);
             }
@@ -963,29 +943,115 @@ __device__ void quad_fs_step(QuadFs *st, int k, gf *slots, unsigned S) {
             put(c0.hi);
             put(c2.lo);
             put(c2.hi);
-            kk::keccak256_bytes(in, len, d);
+            kk::keccak256_bytes(in, len, R.d);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) d[q] = (uint64_t)__shfl((long long)d[q], 0);
+        for (int q = 0; q < 4; q++) R.d[q] = (uint64_t)__shfl((long long)R.d[q], 0);
     }
     // the challenge is the last 16 digest bytes, last byte first
-    const gf x{__builtin_bswap64(d[3]), __builtin_bswap64(d[2])};
-    const gf sub_next = gf_add(c0, gf_mul_reg(x, gf_add(c1, gf_mul_reg(x, c2))));
-    if (lane != 0) return;
+    R.x = gf{__builtin_bswap64(R.d[3]), __builtin_bswap64(R.d[2])};
+    R.sub = gf_add(c0, gf_mul_reg(R.x, gf_add(c1, gf_mul_reg(R.x, c2))));
+    return R;
+}
+
+// Round k's outputs into the region (one lane): polynomial, challenge,
+// claim (round 0), subclaim and the challenger state after the sample
+// (input = output = the digest, 16 output bytes left unread).
+__device__ __forceinline__ void quad_store(QuadFs *st, int k, gf c0, gf c2, gf both, const QuadRound &R) {
+    uint8_t *base = (uint8_t *)st;
+    const uint32_t n = st->n, np = st->npairs;
+    gf *polys = (gf *)(base + quad_polys_off(np)), *chal = (gf *)(base + quad_chal_off(np, n));
+    uint64_t *in64 = reinterpret_cast<uint64_t *>(base + quad_input_off(np, n));
     uint64_t *out64 = reinterpret_cast<uint64_t *>(st->out);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        in64[q] = d[q];
-        out64[q] = d[q];
+        in64[q] = R.d[q];
+        out64[q] = R.d[q];
     }
     st->in_len = 32;
     st->out_len = 16;
     if (k == 0) st->claim = both;
-    st->sub = sub_next;
-    st->x = x;
+    st->sub = R.sub;
+    st->x = R.x;
     polys[2 * k] = c0;
     polys[2 * k + 1] = c2;
-    chal[k] = x;
+    chal[k] = R.x;
+}
+
+// Adds the pair scalars' products c0 += s A0, c2 += s A2, both += s (A0 + A1)
+// (quadratic.rs:100-122) for the cnt <= 64 pairs base.. whose sums
+// P0, P1, P2 sit on lanes 0..cnt-1, one product per lane: three lanes per
+// pair, 21 pairs per pass.  Each lane adds to one of the three; the caller
+// reduces them over the wave.
+__device__ void quad_scale(const gf *sc, uint32_t base, uint32_t cnt, gf P0, gf P1, gf P2, gf &c0, gf &c2, gf &both) {
+    const int lane = threadIdx.x & 63, comp = lane % 3;
+    for (uint32_t i = 0; i < cnt; i += 21) {
+        const uint32_t src = i + (uint32_t)lane / 3;
+        const int sl = (int)(src & 63);
+        const gf B0 = lane_gf(P0, sl), B1 = lane_gf(P1, sl), B2 = lane_gf(P2, sl);
+        if (lane < 63 && src < cnt) {
+            const gf v = comp == 0 ? B0 : (comp == 1 ? B2 : gf_add(B0, B1));
+            const gf p = gf_mul_reg(v, gf_ld(sc + base + src));
+            if (comp == 0) c0 = gf_add(c0, p);
+            else if (comp == 1) c2 = gf_add(c2, p);
+            else both = gf_add(both, p);
+        }
+    }
+}
+
+// The transcript step of round k in the block that finishes the round, on
+// its first wave: per pair the three sums over the round's S slots (S a
+// power of two, at most 16; lanes take the slots of 64 / S pairs at a time,
+// all loads of a pass in flight together, then a butterfly over the S lanes
+// of each pair, and pair c + L of a chunk of 64 gathered on lane L), the
+// slots cleared for the next round, the pair scalars applied (quad_scale),
+// then quad_transcript.
+__device__ void quad_fs_step(QuadFs *st, int k, gf *slots, unsigned S) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = st->n, np = st->npairs;
+    const gf *sc = (const gf *)((const uint8_t *)st + quad_scalars_off());
+    const unsigned per = 64 / S, u = lane & (S - 1), tl = lane / S;
+    gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
+    for (uint32_t c = 0; c < np; c += 64) {
+        gf P0 = gf_zero(), P1 = gf_zero(), P2 = gf_zero();  // pair c + lane
+        for (uint32_t t0 = c; t0 < np && t0 < c + 64; t0 += per) {
+            const uint32_t t = t0 + tl;
+            gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
+            if (t < np) {
+                gf *q = slots + 3 * ((size_t)S * t + u);
+                A0 = gf_ld_published(q);
+                A1 = gf_ld_published(q + 1);
+                A2 = gf_ld_published(q + 2);
+                // the next round XORs into cleared slots (it starts after this launch)
+                gf_st(q, gf_zero());
+                gf_st(q + 1, gf_zero());
+                gf_st(q + 2, gf_zero());
+            }
+            for (unsigned o = 1; o < S; o <<= 1) {
+                A0 = gf_add(A0, gf{(uint64_t)__shfl_xor((long long)A0.lo, o), (uint64_t)__shfl_xor((long long)A0.hi, o)});
+                A1 = gf_add(A1, gf{(uint64_t)__shfl_xor((long long)A1.lo, o), (uint64_t)__shfl_xor((long long)A1.hi, o)});
+                A2 = gf_add(A2, gf{(uint64_t)__shfl_xor((long long)A2.lo, o), (uint64_t)__shfl_xor((long long)A2.hi, o)});
+            }
+            // pair c + L was reduced in pass (L / per) on lane (L % per) * S
+            const int src = (int)((lane % per) * S);
+            const gf B0 = lane_gf(A0, src), B1 = lane_gf(A1, src), B2 = lane_gf(A2, src);
+            if ((uint32_t)lane / per == (t0 - c) / per) {
+                P0 = B0;
+                P1 = B1;
+                P2 = B2;
+            }
+        }
+        quad_scale(sc, c, np - c < 64 ? np - c : 64, P0, P1, P2, c0, c2, both);
+    }
+    c0 = wave_xor(c0);
+    c2 = wave_xor(c2);
+    both = wave_xor(both);
+    Digest prev;
+    const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)st + quad_input_off(np, n));
+#pragma unroll
+    for (int w = 0; w < 4; w++) prev.w[w] = k > 0 ? in64[w] : 0;
+    const QuadRound R = quad_transcript(st, k, c0, c2, both, st->sub, k > 0, prev);
+    if (lane == 0) quad_store(st, k, c0, c2, both, R);
 }
 
 // k_quad_sums with the round's transcript step in the last block to finish
@@ -1030,14 +1096,119 @@ __global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size
     quad_fs_step(fs, k, slots, S);
 }
 
-__global__ __launch_bounds__(CB) void k_fix_multi_dev(gf *__restrict__ P, size_t stride, size_t half,
-                                                      const gf *__restrict__ xp) {
-    gf *f = P + blockIdx.y * stride;
+// fix_last_var of table blockIdx.y with the challenge at *xp: 2 half entries
+// at src (tables src_stride apart) to half entries at dst (dst_stride apart;
+// in place when dst == src)
+__global__ __launch_bounds__(CB) void k_fix_multi_dev(const gf *src, size_t src_stride, gf *dst, size_t dst_stride,
+                                                      size_t half, const gf *__restrict__ xp) {
+    const gf *f = src + blockIdx.y * src_stride;
+    gf *o = dst + blockIdx.y * dst_stride;
     const gf_pre px = gf_prepare(gf_ld(xp));
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
          j += (size_t)gridDim.x * blockDim.x) {
         const gf lo = gf_ld(f + j), hi = gf_ld(f + j + half);
-        gf_st(f + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+        gf_st(o + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+    }
+}
+
+// The Quadratic sumcheck's rounds from k0 on in one block, the m tables in
+// LDS (2 h0 entries each, read from src once).  Per round: the pair sums,
+// one wave per pair (lanes over the halves), into LDS; the transcript step
+// on the first wave; the fix of every table in place in LDS.  The digest,
+// challenge and subclaim stay in LDS between rounds; each table's final
+// entry is written to fin (tables fin_stride apart) and the challenger state
+// to the region at the end.
+constexpr int kQuadTailThreads = 512;
+constexpr size_t kQuadTailElems = 3584;  // table entries in LDS (56 KiB)
+constexpr uint32_t kQuadTailPairs = 128;
+__global__ __launch_bounds__(kQuadTailThreads) void k_quad_tail(const gf *src, size_t src_stride, gf *fin,
+                                                                size_t fin_stride, const uint32_t *__restrict__ pairs,
+                                                                uint32_t m, size_t h0, int k0, QuadFs *fs) {
+    __shared__ gf F[kQuadTailElems];
+    __shared__ gf S3[3][kQuadTailPairs];
+    __shared__ gf xs, subs;
+    __shared__ uint64_t dig[4];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t n = fs->n, np = fs->npairs;
+    const size_t w = 2 * h0;
+    for (size_t e = tid; e < (size_t)m * w; e += kQuadTailThreads) F[e] = gf_ld(src + (e / w) * src_stride + e % w);
+    if (tid == 0) {
+        subs = fs->sub;
+        const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)fs + quad_input_off(np, n));
+        for (int q = 0; q < 4; q++) dig[q] = in64[q];
+    }
+    __syncthreads();
+    const gf *sc = (const gf *)((const uint8_t *)fs + quad_scalars_off());
+    size_t h = h0;
+    for (int k = k0; k < (int)n; k++, h >>= 1) {
+        for (uint32_t t = wave; t < np; t += kQuadTailThreads / 64) {
+            const size_t fa = pairs[2 * t] * w, gb = pairs[2 * t + 1] * w;
+            gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
+            for (size_t j = lane; j < h; j += 64) {
+                const gf f0 = F[fa + j], f1 = F[fa + j + h], g0 = F[gb + j], g1 = F[gb + j + h];
+                a0 = gf_add(a0, gf_mul_reg(f0, g0));
+                a1 = gf_add(a1, gf_mul_reg(f1, g1));
+                a2 = gf_add(a2, gf_mul_reg(gf_add(f0, f1), gf_add(g0, g1)));
+            }
+            a0 = wave_xor(a0);
+            a1 = wave_xor(a1);
+            a2 = wave_xor(a2);
+            if (lane == 0) {
+                S3[0][t] = a0;
+                S3[1][t] = a1;
+                S3[2][t] = a2;
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
+            for (uint32_t c = 0; c < np; c += 64) {
+                const uint32_t t = c + lane;
+                const gf P0 = t < np ? S3[0][t] : gf_zero(), P1 = t < np ? S3[1][t] : gf_zero(),
+                         P2 = t < np ? S3[2][t] : gf_zero();
+                quad_scale(sc, c, np - c < 64 ? np - c : 64, P0, P1, P2, c0, c2, both);
+            }
+            c0 = wave_xor(c0);
+            c2 = wave_xor(c2);
+            both = wave_xor(both);
+            const Digest prev{{dig[0], dig[1], dig[2], dig[3]}};
+            const QuadRound R = quad_transcript(fs, k, c0, c2, both, subs, k > 0, prev);
+            if (lane == 0) {
+                uint8_t *base = (uint8_t *)fs;
+                gf *polys = (gf *)(base + quad_polys_off(np)), *chal = (gf *)(base + quad_chal_off(np, n));
+                polys[2 * k] = c0;
+                polys[2 * k + 1] = c2;
+                chal[k] = R.x;
+                if (k == 0) fs->claim = both;
+                dig[0] = R.d[0];
+                dig[1] = R.d[1];
+                dig[2] = R.d[2];
+                dig[3] = R.d[3];
+                xs = R.x;
+                subs = R.sub;
+            }
+        }
+        __syncthreads();
+        const gf_pre px = gf_prepare(xs);
+        for (size_t idx = tid; idx < (size_t)m * h; idx += kQuadTailThreads) {
+            const size_t t = idx / h, j = idx % h;
+            const gf lo = F[t * w + j], hi = F[t * w + j + h];
+            F[t * w + j] = gf_add(lo, gf_mul_pre(gf_add(hi, lo), px));
+        }
+        __syncthreads();
+    }
+    for (size_t t = tid; t < m; t += kQuadTailThreads) gf_st(fin + t * fin_stride, F[t * w]);
+    if (tid == 0) {
+        uint64_t *in64 = reinterpret_cast<uint64_t *>((uint8_t *)fs + quad_input_off(np, n));
+        uint64_t *out64 = reinterpret_cast<uint64_t *>(fs->out);
+        for (int q = 0; q < 4; q++) {
+            in64[q] = dig[q];
+            out64[q] = dig[q];
+        }
+        fs->in_len = 32;
+        fs->out_len = 16;
+        fs->sub = subs;
+        fs->x = xs;
     }
 }
 
@@ -1876,9 +2047,10 @@ hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t ha
     k_fix_multi<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(P, stride, half, x);
     return hipGetLastError();
 }
-hipError_t fix_multi_dev(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, const gf *x) {
+hipError_t fix_multi_dev(hipStream_t st, const gf *src, size_t src_stride, gf *dst, size_t dst_stride, uint32_t m,
+                         size_t half, const gf *x) {
     if (m == 0 || m > 65535) return hipErrorInvalidValue;
-    k_fix_multi_dev<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(P, stride, half, x);
+    k_fix_multi_dev<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(src, src_stride, dst, dst_stride, half, x);
     return hipGetLastError();
 }
 hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
@@ -1888,7 +2060,23 @@ hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t 
     k_quad_step<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, k, fs, slots, quad_slots(max_blocks), counter);
     return hipGetLastError();
 }
-unsigned quad_slots(unsigned max_blocks) { return max_blocks < 16 ? max_blocks : 16; }
+unsigned quad_slots(unsigned max_blocks) {
+    unsigned s = 1;
+    while (s * 2 <= max_blocks && s * 2 <= 16) s *= 2;
+    return s;
+}
+size_t quad_tail_max_half(uint32_t m, uint32_t npairs) {
+    if (npairs == 0 || npairs > kQuadTailPairs || m == 0) return 0;
+    size_t h = 256;
+    while (h && (size_t)m * 2 * h > kQuadTailElems) h >>= 1;
+    return h;
+}
+hipError_t quad_tail(hipStream_t st, const gf *src, size_t src_stride, gf *fin, size_t fin_stride,
+                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t h0, int k0, QuadFs *fs) {
+    if (!h0 || (h0 & (h0 - 1)) || h0 > quad_tail_max_half(m, npairs)) return hipErrorInvalidValue;
+    k_quad_tail<<<1, kQuadTailThreads, 0, st>>>(src, src_stride, fin, fin_stride, pairs, m, h0, k0, fs);
+    return hipGetLastError();
+}
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
                        const gf *ft, gf *vals, uint8_t *fold_ok) {
     const size_t n = (size_t)count * q;

@@ -279,8 +279,11 @@ hipError_t quad_sums(hipStream_t st, const gf *P, size_t stride, const uint32_t 
                      gf *partial, unsigned max_blocks, unsigned *nblocks);
 // fix the top variable of m tables (stride apart) from 2*half to half entries
 hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, gf x);
-// the same with the challenge read from device memory (*x)
-hipError_t fix_multi_dev(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t half, const gf *x);
+// the same with the challenge read from device memory (*x), from 2 * half
+// entries at src (tables src_stride apart) to half entries at dst (dst_stride
+// apart; in place when dst == src)
+hipError_t fix_multi_dev(hipStream_t st, const gf *src, size_t src_stride, gf *dst, size_t dst_stride, uint32_t m,
+                         size_t half, const gf *x);
 // Round k of a Quadratic sumcheck with its transcript step on the device:
 // the three sums of every pair over halves of h entries (as quad_sums), XORed
 // by the blocks into quad_slots(max_blocks) slots of three elements per pair
@@ -289,7 +292,18 @@ hipError_t fix_multi_dev(hipStream_t st, gf *P, size_t stride, uint32_t m, size_
 // round's Fiat-Shamir step into the QuadFs region.
 hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
                      int k, QuadFs *fs, gf *slots, unsigned max_blocks, unsigned *counter);
+// accumulation slots per pair for a grid of max_blocks blocks per pair: a
+// power of two, at most 16
 unsigned quad_slots(unsigned max_blocks);
+// The Quadratic sumcheck's rounds from k0 on (round k0 has h0 pairs of
+// entries per table, a power of two) in one single-block launch with the m
+// tables (2 * h0 entries each at src, tables src_stride apart) in LDS; each
+// table's final entry goes to fin (tables fin_stride apart).  h0 is at most
+// quad_tail_max_half(m, npairs), which is 0 when the tables or the pairs do
+// not fit.
+size_t quad_tail_max_half(uint32_t m, uint32_t npairs);
+hipError_t quad_tail(hipStream_t st, const gf *src, size_t src_stride, gf *fin, size_t fin_stride,
+                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t h0, int k0, QuadFs *fs);
 // verifier kernels: fold chains (thread per proof x query) and Merkle paths
 // (thread per proof x query x path, d + 1 paths)
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,

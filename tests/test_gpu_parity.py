@@ -161,6 +161,23 @@ def test_quadratic_sumcheck_bit_exact(bfd, oracle, ctx):
             assert (np.asarray(g).reshape(-1, 2) == np.asarray(w).reshape(-1, 2)).all(), n
 
 
+def test_quadratic_sumcheck_many_pairs(bfd, oracle, ctx):
+    """More pairs than the single-block rounds take (every round runs on the
+    multi-block path): 130, 300 and 1000 pairs, which give 16, 8 and 4
+    accumulation slots per pair and more pairs than one wave's lanes."""
+    for n, m, npairs in [(6, 3, 130), (5, 6, 300), (3, 4, 1000)]:
+        _say(f"case quadratic n={n} m={m} pairs={npairs}")
+        rng = np.random.default_rng(n * 1000 + npairs)
+        polys = bfd.random_vec(n + npairs, m << n).reshape(m, 1 << n, 2)
+        scalars = bfd.random_vec(n + npairs + 1, npairs)
+        f = bfd.Quadratic(polys, [(scalars[k], int(rng.integers(0, m)), int(rng.integers(0, m)))
+                                  for k in range(npairs)])
+        got = bfd.prove_sumcheck_quadratic(f, bfd.GenericChallenger.keccak256(), ctx)
+        want = oracle.quadratic_sumcheck(f.polys, [p[0] for p in f.pairs], [(p[1], p[2]) for p in f.pairs])
+        for g, w in zip(got, want):
+            assert (np.asarray(g).reshape(-1, 2) == np.asarray(w).reshape(-1, 2)).all(), (n, npairs)
+
+
 def test_quadratic_prover_stepwise(bfd, oracle, ctx):
     """The step-wise QuadraticProver (bfd_sc_*) under a host prove_sumcheck driver
     gives the restatement's claim, round polys, challenges, evaluations and
@@ -226,12 +243,17 @@ def test_quadratic_sumcheck_prefixes_and_device_inputs(bfd, oracle, ctx):
     state; tables in device memory give the host path's outputs and stay
     unmodified."""
     import torch
-    n = 9
-    f = quad_case(bfd, n, 990)
+    # n 9 starts on the multi-block path, n 7 in the single-block rounds
+    for n in (9, 7):
+        _quadratic_prefixes_and_device_inputs(bfd, oracle, ctx, n, torch)
+
+
+def _quadratic_prefixes_and_device_inputs(bfd, oracle, ctx, n, torch):
+    f = quad_case(bfd, n, 990 + n)
     fresh = oracle.quadratic_sumcheck(f.polys, [p[0] for p in f.pairs], [(p[1], p[2]) for p in f.pairs])
     rng = np.random.default_rng(4)
     for plen, sample_after in [(5, False), (32, False), (200, False), (40, True)]:
-        _say(f"quadratic prefix {plen} sample={sample_after}")
+        _say(f"quadratic n={n} prefix {plen} sample={sample_after}")
         prefix = bytes(rng.integers(0, 256, plen, dtype=np.uint8))
         ch, och = bfd.GenericChallenger.keccak256(), oracle.Challenger()
         ch.observe(prefix)
