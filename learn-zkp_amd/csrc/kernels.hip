@@ -499,8 +499,8 @@ __device__ __forceinline__ gf gf_ld_published(const gf *p) {
 // lane src's value of v, on every lane of the wave
 __device__ __forceinline__ gf lane_gf(gf v, int src) { return gf{(uint64_t)__shfl((long long)v.lo, src), (uint64_t)__shfl((long long)v.hi, src)}; }
 
-// Pair sums of the product sumcheck with T multiplied into f first:
-// tf0 = T f0, tf1 = T f1, then A0 += tf0 g0, A1 += tf1 g1 and
+// Pair sums of the product sumcheck with the pair's eq weight t multiplied
+// into f first: tf0 = t f0, tf1 = t f1, then A0 += tf0 g0, A1 += tf1 g1 and
 // A2 += (tf0 + tf1)(g0 + g1), five products per pair.
 __device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf &a0, gf &a1, gf &a2) {
     const gf_pre pt = gf_prepare(t);
@@ -752,7 +752,8 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
 //   fold   4Q tasks: kind u (f low, f high, g low, g high) of pair j folds
 //          entries j + uq and j + (u+2)q of its table into j + uq (k > 0);
 //          in place, as no task reads what another writes;
-//   scale  2Q tasks: TF[uq + j] = T[j] F[j + uq], u = 0, 1;
+//   scale  2Q tasks: TF[uq + j] = T[j] F[j + uq], u = 0, 1, with the halved
+//          eq table's entries T[j] = A[j mod 2^lo] B[j >> lo] formed once;
 //   sum    3Q tasks: TF[j] G[j], TF[q+j] G[q+j], (TF[j] + TF[q+j])(G[j] + G[q+j]),
 //          reduced per kind by wave sums.
 // The running scalars (s, x, the digest) stay in LDS between rounds; the
