@@ -152,6 +152,24 @@ struct u64p {
     uint32_t lo, hi;
 };
 
+// m ? a : b per bit, and (a ^ b) & m, each one v_bitop3_b32 (truth-table
+// index 4a + 2b + m)
+BFD_D uint32_t bsel(uint32_t m, uint32_t a, uint32_t b) { return __builtin_amdgcn_bitop3_b32(a, b, m, 0xE4); }
+BFD_D uint32_t xand(uint32_t a, uint32_t b, uint32_t m) { return __builtin_amdgcn_bitop3_b32(a, b, m, 0x28); }
+
+// One 32-bit word of a 32x32 carry-less product from its class sums: class
+// c's terms (c0: 3, c1: 2, c2: 3, c3: 4 products) are XORed and bit 4i + c
+// taken from class c, in eight bitwise operations (the masks are disjoint
+// and cover the word)
+BFD_D uint32_t clmul_pick(uint32_t c0a, uint32_t c0b, uint32_t c0c, uint32_t c1a, uint32_t c1b, uint32_t c2a,
+                          uint32_t c2b, uint32_t c2c, uint32_t c3a, uint32_t c3b, uint32_t c3c, uint32_t c3d) {
+    const uint32_t m0 = 0x11111111u, m1 = m0 << 1, m2 = m0 << 2, m3 = m0 << 3;
+    const uint32_t t1 = xand(c1a, c1b, m1);
+    const uint32_t t0 = bsel(m0, x3(c0a, c0b, c0c), t1);
+    const uint32_t t2 = bsel(m2, x3(c2a, c2b, c2c), t0);
+    return bsel(m3, x3(c3a, c3b, c3c) ^ c3d, t2);
+}
+
 // Class c collects the products x.p[i] * y.p[j] with i + j = c (mod 4).  Two
 // products of one class whose i + j differ by 4 may be ADDED (one multiply-
 // add) instead of XORed: at position 4u + c the first contributes at most
@@ -168,24 +186,11 @@ BFD_D u64p clmul32p(const pieces &x, const pieces &y) {
     const uint64_t p11 = (uint64_t)x.p[1] * y.p[1], p20 = (uint64_t)x.p[2] * y.p[0];
     const uint64_t p03 = (uint64_t)x.p[0] * y.p[3], p12 = (uint64_t)x.p[1] * y.p[2];
     const uint64_t p21 = (uint64_t)x.p[2] * y.p[1], p30 = (uint64_t)x.p[3] * y.p[0];
-#define LO_(v) ((uint32_t)(v))
-#define HI_(v) ((uint32_t)((v) >> 32))
-    uint32_t zl[4], zh[4];
-    zl[0] = x3(LO_(a00), LO_(p22), LO_(p31));
-    zh[0] = x3(HI_(a00), HI_(p22), HI_(p31));
-    zl[1] = LO_(a01) ^ LO_(a10);
-    zh[1] = HI_(a01) ^ HI_(a10);
-    zl[2] = x3(LO_(a02), LO_(p11), LO_(p20));
-    zh[2] = x3(HI_(a02), HI_(p11), HI_(p20));
-    zl[3] = x3(x3(LO_(p03), LO_(p12), LO_(p21)), LO_(p30), 0u);
-    zh[3] = x3(x3(HI_(p03), HI_(p12), HI_(p21)), HI_(p30), 0u);
-#undef LO_
-#undef HI_
-    const uint32_t m0 = 0x11111111u, m1 = m0 << 1, m2 = m0 << 2;
-    // disjoint masks: bit-field inserts pick each class's positions
-    uint32_t lo = (zl[0] & m0) | (~m0 & ((zl[1] & m1) | (~m1 & ((zl[2] & m2) | (~m2 & zl[3])))));
-    uint32_t hi = (zh[0] & m0) | (~m0 & ((zh[1] & m1) | (~m1 & ((zh[2] & m2) | (~m2 & zh[3])))));
-    return u64p{lo, hi};
+    return u64p{clmul_pick((uint32_t)a00, (uint32_t)p22, (uint32_t)p31, (uint32_t)a01, (uint32_t)a10, (uint32_t)a02,
+                           (uint32_t)p11, (uint32_t)p20, (uint32_t)p03, (uint32_t)p12, (uint32_t)p21, (uint32_t)p30),
+                clmul_pick((uint32_t)(a00 >> 32), (uint32_t)(p22 >> 32), (uint32_t)(p31 >> 32), (uint32_t)(a01 >> 32),
+                           (uint32_t)(a10 >> 32), (uint32_t)(a02 >> 32), (uint32_t)(p11 >> 32), (uint32_t)(p20 >> 32),
+                           (uint32_t)(p03 >> 32), (uint32_t)(p12 >> 32), (uint32_t)(p21 >> 32), (uint32_t)(p30 >> 32))};
 }
 
 // b's masked pieces for the nine Karatsuba products, reusable across

@@ -17,9 +17,10 @@ SRC = ROOT / "learn-zkp_amd" / "csrc" / "kernels.hip"
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 # bytes of scratch per lane allowed; every other kernel must use none.  The
-# three-wave build of the radix-8 pass trades a small spill for occupancy and
-# is used only where the autotuner measures it faster.
-ALLOWED = {"k_ntt_multi3_w3": 96}
+# three-wave build of the radix-8 pass (at most 168 VGPRs) trades a spill of
+# 180 bytes for occupancy and is used only where the autotuner measures it
+# faster.
+ALLOWED = {"k_ntt_multi3_w3": 192}
 
 
 @pytest.fixture(scope="module")
