@@ -382,22 +382,33 @@ __global__ __launch_bounds__(CB) void k_fold(const gf *__restrict__ w, gf *__res
 // fold of pi and fix_last_var of the table in one launch (both take the
 // round's challenge): blocks [0, nfold) fold, the others fix; the fix may run
 // in place (out == in), each thread reading only the j and j + half it owns
+// TAB: the products by x go through an LDS window table of x built by each
+// block (worth it when every thread has a few outputs)
+template <bool TAB>
 __global__ __launch_bounds__(CB) void k_fold_fix(const gf *__restrict__ w, gf *__restrict__ wout,
                                                  const gf *__restrict__ ft, size_t mid, const gf *in, gf *out,
                                                  size_t half, const gf *__restrict__ x, unsigned nfold) {
-    const gf_pre px = gf_prepare(gf_ld(x));
+    __shared__ GfTab T;
+    __shared__ gf u[128];
+    gf_pre px;
+    if constexpr (TAB) gf_tab_build(T, u, gf_ld(x));
+    else px = gf_prepare(gf_ld(x));
+    const auto byx = [&](gf v) {
+        if constexpr (TAB) return gf_tab_mul(v, T);
+        else return gf_mul_pre(v, px);
+    };
     if (blockIdx.x < nfold) {
         for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < mid; j += (size_t)nfold * blockDim.x) {
             gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
             hi = gf_add(hi, lo);
             lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
-            gf_st(wout + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+            gf_st(wout + j, gf_add(lo, byx(gf_add(hi, lo))));
         }
     } else {
         const size_t nb = gridDim.x - nfold;
         for (size_t j = (blockIdx.x - nfold) * (size_t)blockDim.x + threadIdx.x; j < half; j += nb * blockDim.x) {
             const gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
-            gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+            gf_st(out + j, gf_add(lo, byx(gf_add(hi, lo))));
         }
     }
 }
@@ -1825,10 +1836,12 @@ __global__ __launch_bounds__(KB) void k_bench_keccak(uint64_t *io, size_t n, int
 // fold, fix and NTT kernels use for a challenge or a twiddle
 __global__ __launch_bounds__(CB) void k_mul(const gf *__restrict__ a, const gf *__restrict__ b, gf *__restrict__ out,
                                             size_t n, int shared_b) {
-    if (shared_b) {
-        const gf_pre pb = gf_prepare(gf_ld(b));
+    if (shared_b) {  // through the window table of b[0], as the kernels that multiply by a round challenge
+        __shared__ GfTab T;
+        __shared__ gf u[128];
+        gf_tab_build(T, u, gf_ld(b));
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-            gf_st(out + i, gf_mul_pre(gf_ld(a + i), pb));
+            gf_st(out + i, gf_tab_mul(gf_ld(a + i), T));
         return;
     }
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1984,7 +1997,11 @@ hipError_t fold(hipStream_t st, const gf *w, gf *out, const gf *ft, size_t mid, 
 hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t mid, const gf *in, gf *out,
                     size_t half, const gf *x) {
     const unsigned nfold = grid_for(mid, CB, kCapMul), nfix = grid_for(half, CB, kCapMul);
-    k_fold_fix<<<nfold + nfix, CB, 0, st>>>(w, wout, ft, mid, in, out, half, x, nfold);
+    // the window table pays once every fold thread has two outputs or more
+    if (mid >= 2 * (size_t)nfold * CB)
+        k_fold_fix<true><<<nfold + nfix, CB, 0, st>>>(w, wout, ft, mid, in, out, half, x, nfold);
+    else
+        k_fold_fix<false><<<nfold + nfix, CB, 0, st>>>(w, wout, ft, mid, in, out, half, x, nfold);
     return hipGetLastError();
 }
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x) {
