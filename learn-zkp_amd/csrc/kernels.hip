@@ -414,12 +414,19 @@ __global__ __launch_bounds__(CB) void k_fold_fix(const gf *__restrict__ w, gf *_
 }
 
 // out[j] = in[j] + x * (in[j+half] - in[j])  (multilinear.rs:183-198)
+template <bool TAB>
 __global__ __launch_bounds__(CB) void k_fix(const gf *__restrict__ in, gf *__restrict__ out, size_t half, gf x) {
-    const gf_pre px = gf_prepare(x);
+    __shared__ GfTab T;
+    __shared__ gf u[128];
+    gf_pre px;
+    if constexpr (TAB) gf_tab_build(T, u, x);
+    else px = gf_prepare(x);
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
          j += (size_t)gridDim.x * blockDim.x) {
         gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
-        gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+        const gf d = gf_add(hi, lo);
+        if constexpr (TAB) gf_st(out + j, gf_add(lo, gf_tab_mul(d, T)));
+        else gf_st(out + j, gf_add(lo, gf_mul_pre(d, px)));
     }
 }
 
@@ -625,6 +632,34 @@ __device__ __forceinline__ void prod_store_state(ProdFs *st, const ProdRound &R,
     st->x = R.x;
 }
 
+// Rounds after the first: fold the previous tables (4 q entries, q = half)
+// on their top variable with x and form the pair sums of pairs [j0, j1).
+// Entries j and j + q of the folded tables come from j, j + 2q and j + q,
+// j + 3q: each thread reads its four before writing its two, so in place is
+// safe.  TAB: the products by x through the window table T.
+template <bool TAB>
+__device__ __forceinline__ void prod_fold_sums(const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst,
+                                               const EqFactors &eq, size_t q, size_t j0, size_t j1, size_t amask,
+                                               const GfTab *T, const gf_pre &px, gf &a0, gf &a1, gf &a2) {
+    const auto fold = [&](gf lo, gf hi) {
+        if constexpr (TAB) return gf_add(lo, gf_tab_mul(gf_add(hi, lo), *T));
+        else return fold_top(lo, hi, px);
+    };
+    for (size_t j = j0 + threadIdx.x; j < j1; j += CB) {
+        const gf fa = gf_ld(fsrc + j), fb = gf_ld(fsrc + j + q), fc = gf_ld(fsrc + j + 2 * q),
+                 fd = gf_ld(fsrc + j + 3 * q);
+        const gf ga = gf_ld(gsrc + j), gb = gf_ld(gsrc + j + q), gc = gf_ld(gsrc + j + 2 * q),
+                 gd = gf_ld(gsrc + j + 3 * q);
+        const gf f0 = fold(fa, fc), f1 = fold(fb, fd);
+        const gf g0 = fold(ga, gc), g1 = fold(gb, gd);
+        gf_st(fdst + j, f0);
+        gf_st(fdst + j + q, f1);
+        gf_st(gdst + j, g0);
+        gf_st(gdst + j + q, g1);
+        prod_terms(f0, f1, g0, g1, gf_ld(eq.A + (j & amask)), a0, a1, a2);
+    }
+}
+
 // Round k of the product sumcheck (see launch::prod_step).  MODE 0: round 0,
 // sums over the caller's tables; 1: fold the caller's tables into the working
 // ones, then the sums; 2: fold the working tables in place, then the sums.
@@ -658,25 +693,16 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
         for (size_t j = j0 + threadIdx.x; j < j1; j += CB)
             prod_terms(gf_ld(fsrc + j), gf_ld(fsrc + j + half), gf_ld(gsrc + j), gf_ld(gsrc + j + half),
                        gf_ld(eq.A + (j & amask)), a0, a1, a2);
+    } else if (C >= 2 * (size_t)CB) {
+        // two pairs or more per thread: the folds multiply by x through a
+        // window table
+        __shared__ GfTab T;
+        __shared__ gf u[128];
+        gf_tab_build(T, u, fs->x);
+        prod_fold_sums<true>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, &T, gf_pre{}, a0, a1, a2);
     } else {
-        // the previous tables hold 4 q entries; entries j and j + q of the
-        // folded ones come from j, j + 2q and j + q, j + 3q: each thread
-        // reads its four before writing its two, so in place is safe
-        const size_t q = half;
-        const gf_pre px = gf_prepare(fs->x);
-        for (size_t j = j0 + threadIdx.x; j < j1; j += CB) {
-            const gf fa = gf_ld(fsrc + j), fb = gf_ld(fsrc + j + q), fc = gf_ld(fsrc + j + 2 * q),
-                     fd = gf_ld(fsrc + j + 3 * q);
-            const gf ga = gf_ld(gsrc + j), gb = gf_ld(gsrc + j + q), gc = gf_ld(gsrc + j + 2 * q),
-                     gd = gf_ld(gsrc + j + 3 * q);
-            const gf f0 = fold_top(fa, fc, px), f1 = fold_top(fb, fd, px);
-            const gf g0 = fold_top(ga, gc, px), g1 = fold_top(gb, gd, px);
-            gf_st(fdst + j, f0);
-            gf_st(fdst + j + q, f1);
-            gf_st(gdst + j, g0);
-            gf_st(gdst + j + q, g1);
-            prod_terms(f0, f1, g0, g1, gf_ld(eq.A + (j & amask)), a0, a1, a2);
-        }
+        prod_fold_sums<false>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, nullptr, gf_prepare(fs->x), a0, a1,
+                              a2);
     }
     // block sums, XORed into one of the round's accumulation slots
     __shared__ gf red[3][CB / 64];
@@ -1111,15 +1137,22 @@ __global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size
 // fix_last_var of table blockIdx.y with the challenge at *xp: 2 half entries
 // at src (tables src_stride apart) to half entries at dst (dst_stride apart;
 // in place when dst == src)
+template <bool TAB>
 __global__ __launch_bounds__(CB) void k_fix_multi_dev(const gf *src, size_t src_stride, gf *dst, size_t dst_stride,
                                                       size_t half, const gf *__restrict__ xp) {
+    __shared__ GfTab T;
+    __shared__ gf u[128];
     const gf *f = src + blockIdx.y * src_stride;
     gf *o = dst + blockIdx.y * dst_stride;
-    const gf_pre px = gf_prepare(gf_ld(xp));
+    gf_pre px;
+    if constexpr (TAB) gf_tab_build(T, u, gf_ld(xp));
+    else px = gf_prepare(gf_ld(xp));
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < half;
          j += (size_t)gridDim.x * blockDim.x) {
         const gf lo = gf_ld(f + j), hi = gf_ld(f + j + half);
-        gf_st(o + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+        const gf d = gf_add(hi, lo);
+        if constexpr (TAB) gf_st(o + j, gf_add(lo, gf_tab_mul(d, T)));
+        else gf_st(o + j, gf_add(lo, gf_mul_pre(d, px)));
     }
 }
 
@@ -1499,28 +1532,44 @@ __global__ __launch_bounds__(CB) void ks_fix(const Seg *segs, int nseg) {
 // twiddles) and fix segments (as ks_fix) in one launch: a segment with a
 // twiddle table (aux) folds, one without fixes.  A block serves one segment,
 // so the two kinds share the device instead of running one after the other.
-__global__ __launch_bounds__(CB) void ks_fold_fix(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+template <bool TAB>
+__device__ __forceinline__ void seg_fold_fix(const Seg &g, const GfTab &T, const gf_pre &px) {
+    const auto byx = [&](gf v) {
+        if constexpr (TAB) return gf_tab_mul(v, T);
+        else return gf_mul_pre(v, px);
+    };
     if (g.aux) {
         const gf *w = (const gf *)g.in, *ft = (const gf *)g.aux;
         gf *out = (gf *)g.out;
         const size_t mid = g.count;
-        const gf_pre pr = gf_prepare(seg_challenge(g));
         SEG_LOOP(g, j) {
             gf lo = gf_ld(w + j), hi = gf_ld(w + j + mid);
             hi = gf_add(hi, lo);
             lo = gf_add(lo, gf_mul_reg(hi, gf_ld(ft + j)));
-            gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), pr)));
+            gf_st(out + j, gf_add(lo, byx(gf_add(hi, lo))));
         }
     } else {
         const gf *in = (const gf *)g.in;
         gf *out = (gf *)g.out;
         const size_t half = g.count;
-        const gf_pre px = gf_prepare(seg_challenge(g));
         SEG_LOOP(g, j) {
             gf lo = gf_ld(in + j), hi = gf_ld(in + j + half);
-            gf_st(out + j, gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)));
+            gf_st(out + j, gf_add(lo, byx(gf_add(hi, lo))));
         }
+    }
+}
+// a segment whose threads have two outputs or more multiplies by its
+// challenge through a window table
+__global__ __launch_bounds__(CB) void ks_fold_fix(const Seg *segs, int nseg) {
+    __shared__ GfTab T;
+    __shared__ gf u[128];
+    const Seg &g = my_seg(segs, nseg);
+    const gf x = seg_challenge(g);
+    if (g.count >= 2 * (size_t)g.nblk * CB) {
+        gf_tab_build(T, u, x);
+        seg_fold_fix<true>(g, T, gf_pre{});
+    } else {
+        seg_fold_fix<false>(g, T, gf_prepare(x));
     }
 }
 
@@ -2005,7 +2054,9 @@ hipError_t fold_fix(hipStream_t st, const gf *w, gf *wout, const gf *ft, size_t 
     return hipGetLastError();
 }
 hipError_t fix(hipStream_t st, const gf *in, gf *out, size_t half, gf x) {
-    k_fix<<<grid_for(half, CB, kCapMul), CB, 0, st>>>(in, out, half, x);
+    const unsigned g = grid_for(half, CB, kCapMul);
+    if (half >= 2 * (size_t)g * CB) k_fix<true><<<g, CB, 0, st>>>(in, out, half, x);
+    else k_fix<false><<<g, CB, 0, st>>>(in, out, half, x);
     return hipGetLastError();
 }
 hipError_t outer(hipStream_t st, const gf *A, int m, const gf *B, gf *out, size_t n) {
@@ -2068,7 +2119,11 @@ hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t ha
 hipError_t fix_multi_dev(hipStream_t st, const gf *src, size_t src_stride, gf *dst, size_t dst_stride, uint32_t m,
                          size_t half, const gf *x) {
     if (m == 0 || m > 65535) return hipErrorInvalidValue;
-    k_fix_multi_dev<<<dim3(grid_for(half, CB, kCapMul / 4), m), CB, 0, st>>>(src, src_stride, dst, dst_stride, half, x);
+    const unsigned g = grid_for(half, CB, kCapMul / 4);
+    if (half >= 2 * (size_t)g * CB)
+        k_fix_multi_dev<true><<<dim3(g, m), CB, 0, st>>>(src, src_stride, dst, dst_stride, half, x);
+    else
+        k_fix_multi_dev<false><<<dim3(g, m), CB, 0, st>>>(src, src_stride, dst, dst_stride, half, x);
     return hipGetLastError();
 }
 hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,

@@ -164,8 +164,10 @@ def test_quadratic_sumcheck_bit_exact(bfd, oracle, ctx):
 def test_quadratic_sumcheck_many_pairs(bfd, oracle, ctx):
     """More pairs than the single-block rounds take (every round runs on the
     multi-block path): 130, 300 and 1000 pairs, which give 16, 8 and 4
-    accumulation slots per pair and more pairs than one wave's lanes."""
-    for n, m, npairs in [(6, 3, 130), (5, 6, 300), (3, 4, 1000)]:
+    accumulation slots per pair and more pairs than one wave's lanes; and
+    2^18 tables, whose first fixes multiply through the challenge's window
+    table."""
+    for n, m, npairs in [(6, 3, 130), (5, 6, 300), (3, 4, 1000), (18, 3, 3)]:
         _say(f"case quadratic n={n} m={m} pairs={npairs}")
         rng = np.random.default_rng(n * 1000 + npairs)
         polys = bfd.random_vec(n + npairs, m << n).reshape(m, 1 << n, 2)
@@ -493,8 +495,9 @@ def test_open_into_caller_buffer(bfd, oracle, ctx):
 def test_product_sumcheck_bit_exact(bfd, oracle, ctx):
     # n 9, 10, 11 and 12 start the single-block tail at rounds 0, 1, 2 and 3;
     # from n 10 on the round kernels' blocks span 2^(n-8) entries of the eq
-    # factor B, at n 17 two blocks share each entry
-    for n in [1, 2, 3, 5, 8, 9, 10, 11, 12, 14, 17]:
+    # factor B, at n 17 two blocks share each entry; at n 21 round 1 folds
+    # through the window table of its challenge
+    for n in [1, 2, 3, 5, 8, 9, 10, 11, 12, 14, 17, 21]:
         _say(f"product sumcheck n={n}")
         f, g, r = bfd.random_vec(400 + n, 1 << n), bfd.random_vec(500 + n, 1 << n), bfd.random_vec(600 + n, n)
         got = bfd.prove_sumcheck_product(f, g, r, bfd.GenericChallenger.keccak256(), ctx)
