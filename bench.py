@@ -47,13 +47,25 @@ def algorithmic_bytes(n: int, log2_c: int) -> dict:
 
 
 def valu_work(n: int, log2_c: int) -> tuple:
-    """Keccak-f permutations K and GF(2^128) multiplications G of one
+    """Keccak-f permutations K and GF(2^128) multiplications of one
     commit+open: K = (2M-1) + (2M-2c-n) (leaves and nodes of the main tree and
-    of every pi tree), G = n*M/2 (NTT) + (2M-2c) (two per fold output) +
-    (2N-2) (fix_last_var and the inner products) + N/2 (eq table)."""
+    of every pi tree); G = n*M/2 (NTT) + (2M-2c) (two per fold output) +
+    (2N-2) (fix_last_var and the inner products) + N/2 (eq table), returned
+    as (K, G_reg, G_tab): G_tab are the products by a round challenge that go
+    through its window table (the fold's second product and the fix, in the
+    rounds whose fold has 2^19 outputs or more, where every thread of the
+    capped grid has two or more), G_reg the rest."""
     N, c = 1 << n, 1 << log2_c
     M = N * c
-    return (2 * M - 1) + (2 * M - 2 * c - n), n * M // 2 + (2 * M - 2 * c) + (2 * N - 2) + N // 2
+    G = n * M // 2 + (2 * M - 2 * c) + (2 * N - 2) + N // 2
+    G_tab = sum((M >> (k + 1)) + (N >> (k + 1)) for k in range(n) if (M >> (k + 1)) >= 1 << 19)
+    return (2 * M - 1) + (2 * M - 2 * c - n), G - G_tab, G_tab
+
+
+def valu_floor_s(K: int, G_reg: int, G_tab: int, rates) -> float:
+    """Seconds the work would take at the isolated kernels' rates (Keccak-f,
+    register multiply, window-table multiply)."""
+    return K / rates[0] + G_reg / rates[1] + G_tab / rates[2]
 
 
 SUBTREE_FROM = 1 << 17
@@ -235,11 +247,14 @@ def product_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
     t = statistics.median(ts)
     # multiplications: five per pair in round 0 (A f0, A f1 with the eq
     # factor A, and three products) and nine after (four folds and five);
-    # the eq factor B enters once per block, a few thousand products
+    # the eq factor B enters once per block, a few thousand products.  The
+    # folds of rounds with more than 2^18 pairs (two or more per thread) go
+    # through the challenge's window table.
     G = 5 * (1 << (n - 1)) + sum(9 * (1 << (n - 1 - k)) for k in range(1, n))
-    pos = {"gf_muls": G, "keccak_perms": n}
+    Gt = sum(4 * (1 << (n - 1 - k)) for k in range(1, n) if (1 << (n - 1 - k)) > 1 << 18)
+    pos = {"gf_muls": G, "gf_muls_by_table": Gt, "keccak_perms": n}
     if rates:
-        pos["valu_floor_ms"] = 1e3 * (G / rates[1] + n / rates[0])
+        pos["valu_floor_ms"] = 1e3 * valu_floor_s(n, G - Gt, Gt, rates)
         pos["valu_frac"] = pos["valu_floor_ms"] / (1e3 * t)
         pos["note"] = "the floor counts multiplications at the isolated rate; the n transcript steps are serial"
     return {"workload": "f g eq(r, .), 22 variables, tables resident, fresh challenger", "ms": 1e3 * t,
@@ -249,18 +264,22 @@ def product_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
 def mix_position(jobs, seconds: float, rates) -> dict:
     """Roofline position of a set of commit+open jobs [(log2 n, log2 c)]
     completed in `seconds`: algorithmic HBM GB/s (and its fraction of 8 TB/s)
-    and the VALU fraction (K/R_K + G/R_G)/t at the isolated kernels' rates."""
+    and the VALU fraction (the time of its Keccak-f permutations and
+    multiplications at the isolated kernels' rates over t; the engine's
+    window-table products counted as on the plain path, an approximation)."""
     B = sum(algorithmic_bytes(n, c)["total"] for n, c in jobs)
-    K = G = 0
+    K = Gr = Gt = 0
     for n, c in jobs:
-        k, g = valu_work(n, c)
+        k, gr, gt = valu_work(n, c)
         K += k
-        G += g
+        Gr += gr
+        Gt += gt
     out = {"hbm_gbs_algorithmic": B / seconds / 1e9, "hbm_frac": B / seconds / 1e9 / HBM_PEAK_GBS,
-           "keccak_perms": K, "gf_muls": G}
+           "keccak_perms": K, "gf_muls": Gr + Gt, "gf_muls_by_table": Gt}
     if rates:
-        out["valu_floor_ms"] = 1e3 * (K / rates[0] + G / rates[1])
-        out["valu_frac"] = (K / rates[0] + G / rates[1]) / seconds
+        f = valu_floor_s(K, Gr, Gt, rates)
+        out["valu_floor_ms"] = 1e3 * f
+        out["valu_frac"] = f / seconds
     return out
 
 
@@ -291,11 +310,14 @@ def quadratic_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
         ts.append(time.perf_counter() - t0)
     assert all((np.asarray(a) == np.asarray(b)).all() for a, b in zip(out, ref)), "quadratic sumcheck not deterministic"
     t = statistics.median(ts)
-    # three products per pair and pair position, one per table entry for the fix
+    # three products per pair and pair position, one per table entry for the
+    # fix; the fixes of rounds of 2^17 pairs or more (two or more per thread)
+    # go through the challenge's window table
     G = sum(3 * npairs * (1 << (n - 1 - k)) + m * (1 << (n - 1 - k)) for k in range(n))
-    pos = {"gf_muls": G, "keccak_perms": n}
+    Gt = sum(m * (1 << (n - 1 - k)) for k in range(n) if (1 << (n - 1 - k)) >= 1 << 17)
+    pos = {"gf_muls": G, "gf_muls_by_table": Gt, "keccak_perms": n}
     if rates:
-        pos["valu_floor_ms"] = 1e3 * (G / rates[1] + n / rates[0])
+        pos["valu_floor_ms"] = 1e3 * valu_floor_s(n, G - Gt, Gt, rates)
         pos["valu_frac"] = pos["valu_floor_ms"] / (1e3 * t)
     return {"workload": f"{m} tables of 2^{n}, {npairs} pairs, tables resident, fresh challenger", "ms": 1e3 * t,
             "table_evals_per_s": m * (1 << n) / t, "position": pos}
@@ -728,6 +750,9 @@ def main() -> None:
     r_k = (1 << 20) * 64 / (ctx.microbench(2, 1 << 20, 64) * 1e-3)
     ctx.microbench(1, 1 << 22, 2)
     r_g = (1 << 22) * 64 / (ctx.microbench(1, 1 << 22, 64) * 1e-3)
+    ctx.microbench(4, 1 << 22, 2)
+    r_t = (1 << 22) * 64 / (ctx.microbench(4, 1 << 22, 64) * 1e-3)
+    rates = (r_k, r_g, r_t)
     valu = None
     if perms_per_step:
         rate = perms_per_step * args.steps / (kms * 1e-3)
@@ -740,10 +765,11 @@ def main() -> None:
     # whole-step VALU position: time the step's Keccak-f permutations and
     # GF(2^128) multiplications would take at the isolated kernels' rates,
     # over the measured step time
-    K, G = valu_work(n, LOG2_C)
+    K, Gr, Gt = valu_work(n, LOG2_C)
     t_step = t / args.steps
-    valu_step = {"keccak_perms": K, "gf_muls": G, "isolated_keccak_perm_per_s": r_k,
-                 "isolated_gf_mul_per_s": r_g, "frac": (K / r_k + G / r_g) / t_step}
+    valu_step = {"keccak_perms": K, "gf_muls": Gr + Gt, "gf_muls_by_table": Gt, "isolated_keccak_perm_per_s": r_k,
+                 "isolated_gf_mul_per_s": r_g, "isolated_gf_mul_by_table_per_s": r_t,
+                 "frac": valu_floor_s(K, Gr, Gt, rates) / t_step}
 
     result = {
         "metric": "Basefold commit+open field-elems/s (2^24 evals, rate 1/4)",
@@ -780,19 +806,19 @@ def main() -> None:
         stage("reference shapes")
         result["reference_shapes"] = reference_shapes(bfd, ctx, torch, 3)
     stage("sumchecks")
-    result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 20, (r_k, r_g))
-    result["quadratic_sumcheck"] = quadratic_sumcheck(bfd, ctx, torch, 20, (r_k, r_g))
+    result["product_sumcheck"] = product_sumcheck(bfd, ctx, torch, 20, rates)
+    result["quadratic_sumcheck"] = quadratic_sumcheck(bfd, ctx, torch, 20, rates)
     if not args.no_batch:
         stage("batched vs plain")
         result["batched"] = batched_vs_plain(bfd, ctx, torch, args.batch_reps,
-                                             0 if args.no_tune else args.tune_reps, (r_k, r_g))
+                                             0 if args.no_tune else args.tune_reps, rates)
     if args.verify_proofs > 0:
         stage("verification")
         result["verify"] = verification(bfd, ctx, torch, args.verify_proofs)
     if args.serve_requests > 0:
         stage("serving")
         result["serving"] = serving_trace(bfd, ctx, torch, args.serve_requests, args.serve_rate,
-                                          int(args.serve_budget_gb * 1e9), (r_k, r_g))
+                                          int(args.serve_budget_gb * 1e9), rates)
         # at 200 req/s the offered work exceeds what one GPU serves, so
         # latency grows with the trace; `requests_per_s_served` there is the
         # saturation throughput, and this trace shows latency below it
@@ -800,7 +826,7 @@ def main() -> None:
             stage("serving below saturation")
             result["serving_sustainable"] = serving_trace(bfd, ctx, torch, args.serve_requests,
                                                           args.serve_sustain_rate, int(args.serve_budget_gb * 1e9),
-                                                          (r_k, r_g))
+                                                          rates)
     if not args.no_cpu:
         stage("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(args, proof if args.log_n == LOG_N else None)

@@ -388,9 +388,10 @@ int bfd_tuning_load(bfd_ctx *ctx, const char *path);
 
 /* ---- measurement helpers ---- */
 /* Times `iters` dependent multiplies (variant 0: LDS comb, 1: register
- * Karatsuba), `iters` chained Keccak-256 hashes of 64-byte messages (2, the
- * Merkle node hash) on n lanes, or one copy of n 16-byte elements (3);
- * returns milliseconds.  No reference counterpart (measurement only). */
+ * Karatsuba, 4: by one shared factor through its LDS window table), `iters`
+ * chained Keccak-256 hashes of 64-byte messages (2, the Merkle node hash) on
+ * n lanes, or one copy of n 16-byte elements (3); returns milliseconds.  No
+ * reference counterpart (measurement only). */
 int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms);
 /* Per-kernel timing of the last commit/open on this context: fills up to
  * `cap` (name, total ms, launches) records; returns the count in *count. */

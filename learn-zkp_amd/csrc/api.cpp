@@ -937,7 +937,7 @@ int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out) {
 int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms) {
     return guarded([&] {
         if (!ctx || !ms) fail(BFD_ERR_ARG, "null argument");
-        if (what < 0 || what > 3 || iters < 1) fail(BFD_ERR_ARG, "unknown micro-benchmark");
+        if (what < 0 || what > 4 || iters < 1) fail(BFD_ERR_ARG, "unknown micro-benchmark");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);
         hipStream_t st = ctx->stream;

@@ -1858,6 +1858,18 @@ __global__ __launch_bounds__(CB) void k_bench_mul_reg(gf *io, size_t n, int iter
         gf_st(io + 2 * i, a);
     }
 }
+// `iters` chained products by one factor c = io[1] per thread through the
+// block's window table of c (the products by a round challenge)
+__global__ __launch_bounds__(CB) void k_bench_mul_tab(gf *io, size_t n, int iters) {
+    __shared__ GfTab T;
+    __shared__ gf u[128];
+    gf_tab_build(T, u, gf_ld(io + 1));
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        gf a = gf_ld(io + 2 * i);
+        for (int k = 0; k < iters; k++) a = gf_tab_mul(a, T);
+        gf_st(io + 2 * i, a);
+    }
+}
 // `iters` chained Keccak-256 hashes of 64-byte messages per thread (the node
 // hash of k_nodes: next message = digest || digest ^ previous digest), the
 // messages stored word-major (io[k * n + i]) so loads and stores are
@@ -2212,6 +2224,8 @@ hipError_t proof_records(hipStream_t st, const ProofJob *jobs, uint32_t njobs, u
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters) {
     if (variant == 0)
         k_bench_mul_comb<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
+    else if (variant == 4)
+        k_bench_mul_tab<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
     else
         k_bench_mul_reg<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
     return hipGetLastError();
