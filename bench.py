@@ -618,8 +618,9 @@ def main() -> None:
     ap.add_argument("--tune-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(each proof is independent; sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(proofs are independent, so batches and requests shard by polynomial, one context per GPU)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
