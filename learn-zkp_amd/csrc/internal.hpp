@@ -1176,9 +1176,9 @@ struct ChunkCopy {
 
 // One opening's proof into dst (sz = proof_size bytes, pages already
 // touched): the header from o on the host, then the query records at o.idx,
-// written on the device in their bincode layout (k_proof_records) and copied
-// in behind the header.  Stages through ctx->pinned and ctx->gather_pin and
-// synchronises the context stream.
+// written by the device in their bincode layout (k_proof_records) into
+// ctx->gather_pin and copied in behind the header.  Stages the indices
+// through ctx->pinned and synchronises the context stream.
 inline void write_proof(bfd_ctx *ctx, const OpenRounds &o, const OpenLayout &lay, size_t sz, uint8_t *dst) {
     hipStream_t st = ctx->stream;
     write_proof_header(o, dst);
@@ -1191,20 +1191,26 @@ inline void write_proof(bfd_ctx *ctx, const OpenRounds &o, const OpenLayout &lay
     const size_t up = sizeof(ProofJob) + 8 * o.q;
     ctx->pinned.ensure(up);
     uint8_t *hu = (uint8_t *)ctx->pinned.p;
-    DevBuf dj = ctx_buf(ctx, up), dout = ctx_buf(ctx, body);
-    const ProofJob J =
-        proof_job(o, lay, reinterpret_cast<const uint64_t *>(dj.as<uint8_t>() + sizeof(ProofJob)), dout.as<uint8_t>(), 0);
+    // the records go from the kernel straight into pinned host memory: one
+    // pass over PCIe, no device buffer and no copy-engine transfer behind it
+    ctx->gather_pin.ensure(body);
+    uint8_t *hout = nullptr;
+    hip_check(hipHostGetDevicePointer((void **)&hout, ctx->gather_pin.p, 0), "hipHostGetDevicePointer");
+    DevBuf dj = ctx_buf(ctx, up);
+    const ProofJob J = proof_job(o, lay, reinterpret_cast<const uint64_t *>(dj.as<uint8_t>() + sizeof(ProofJob)), hout, 0);
     memcpy(hu, &J, sizeof J);
     memcpy(hu + sizeof J, o.idx.data(), 8 * o.q);
     hip_check(hipMemcpyAsync(dj.p, hu, up, hipMemcpyHostToDevice, st), "copy query indices");
     ctx->timer.run(st, "k_proof_records", [&] { return launch::proof_records(st, dj.as<ProofJob>(), 1, (uint32_t)o.q); });
-    ctx->gather_pin.ensure(body);
-    hip_check(hipMemcpyAsync(ctx->gather_pin.p, dout.p, body, hipMemcpyDeviceToHost, st), "copy query records");
+    const double t_wait = now_ms();
     hip_check(hipStreamSynchronize(st), "query records");
+    ctx->timer.host("proof_records_wait", now_ms() - t_wait);
     ctx->timer.collect();
+    const double t_copy = now_ms();
     ChunkCopy cc;
     cc.add(dst + hb, (const uint8_t *)ctx->gather_pin.p, body);
     cc.run();
+    ctx->timer.host("proof_records_copy", now_ms() - t_copy);
 }
 
 }  // namespace rt
