@@ -44,10 +44,6 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     const size_t M = cm->M, c = (size_t)1 << o.b;
 
     // device scratch for the whole opening
-    const int hn = o.half_eq_vars();
-    DevBuf half_eq = ctx_buf(ctx, 16 * ((size_t)1 << hn));
-    if (o.n > 0) device_eq(ctx, o.r.data(), hn, half_eq.as<gf>());
-    DevBuf poly = ctx_buf(ctx, 16 * (cm->N > 1 ? cm->N / 2 : 1));
     OpenLayout lay;
     lay.plan(M, d);
     DevBuf pis = ctx_buf(ctx, 16 * lay.pis_elems(M, c)), trees = ctx_buf(ctx, 32 * lay.pitree_digests());
@@ -59,6 +55,30 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     lay.tree = cm->tree.as<uint64_t>();
     lay.pis = pis.as<gf>();
     lay.pitrees = trees.as<uint64_t>();
+    std::vector<TreeStep> steps;
+    const bfd_tuning tn = ctx->tuning_for(M);
+    const launch::Shape sh = shape_of(tn, ctx->cus);
+    // round k's pi tree over rows (pi[j], pi[j+half]) (basefold.rs:350-355),
+    // up to its last launch, which also takes the round's inner product and
+    // transcript step when it is a subtree launch (returns whether it is)
+    const auto tree_base = [&](int k) {
+        const size_t half = (M >> k) / 2;
+        const gf *pik = lay.pi(k);
+        uint64_t *tk = trees.as<uint64_t>() + 4 * lay.tree_off[k];
+        ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half, sh); });
+        tree_steps(half, steps, (size_t)1 << tn.subtree_from_log2, (int)tn.subtree_levels);
+        const bool fused = !steps.empty() && steps.back().lv > 0;
+        tree_step_launches(ctx, tk, steps, sh, fused ? steps.size() - 1 : steps.size());
+        return fused;
+    };
+    // round 0's tree hashes pi_0, which the commitment holds: it goes first,
+    // so the device hashes while the host builds the eq table and the
+    // transcript image
+    const bool fused0 = d > 0 && tree_base(0);
+    const int hn = o.half_eq_vars();
+    DevBuf half_eq = ctx_buf(ctx, 16 * ((size_t)1 << hn));
+    if (o.n > 0) device_eq(ctx, o.r.data(), hn, half_eq.as<gf>());
+    DevBuf poly = ctx_buf(ctx, 16 * (cm->N > 1 ? cm->N / 2 : 1));
     // inner-product block sums of the current round (dot_blocks() <= 512)
     DevBuf partials = ctx_buf(ctx, 16 * 512);
     gf *sums = partials.as<gf>();
@@ -90,23 +110,16 @@ void do_open(bfd_ctx *ctx, const bfd_code *code, const bfd_commitment *cm, const
     const gf *x_dev = fs_challenge(fs.p);
 
     const gf *poly_cur = poly0;
-    std::vector<TreeStep> steps;
-    const bfd_tuning tn = ctx->tuning_for(M);
-    const launch::Shape sh = shape_of(tn, ctx->cus);
     ctx->timer.host("open_setup", now_ms() - t_setup);
     for (int k = 0; k < d; k++) {
         const int i = d - 1 - k;
         const size_t half = (M >> k) / 2, ph = (size_t)1 << i;
         const gf *pik = lay.pi(k);
         uint64_t *tk = trees.as<uint64_t>() + 4 * lay.tree_off[k];
-        // pi tree over rows (pi[j], pi[j+half])  (basefold.rs:350-355)
-        ctx->timer.run(st, "k_leaves_pair", [&] { return launch::leaves_pair(st, pik, tk, half, sh); });
-        tree_steps(half, steps, (size_t)1 << tn.subtree_from_log2, (int)tn.subtree_levels);
         // the inner product of the halved eq table with the low half of poly
         // (eval_impr.rs:187-197) and the round's transcript step (observe
         // root and round poly, sample x) ride on the tree's last launch
-        const bool fused = !steps.empty() && steps.back().lv > 0;
-        tree_step_launches(ctx, tk, steps, sh, fused ? steps.size() - 1 : steps.size());
+        const bool fused = k == 0 ? fused0 : tree_base(k);
         if (fused) {
             const TreeStep &ts = steps.back();
             ctx->timer.run(st, "k_top_round", [&] {
