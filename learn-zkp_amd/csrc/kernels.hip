@@ -2034,10 +2034,26 @@ hipError_t subtree(hipStream_t st, uint64_t *prev, size_t h, int lv, int cus) {
 }
 
 unsigned top_round_dot_blocks(size_t n) { return grid_for(n, kSubThreads, 512); }
+// blocks of k_top_round resident per compute unit (occupancy query, once)
+static unsigned top_round_blocks_per_cu() {
+    static unsigned cached = 0;
+    if (!cached) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_top_round, kSubThreads, 0) != hipSuccess || nb < 1)
+            nb = 1;
+        cached = (unsigned)nb;
+    }
+    return cached;
+}
 hipError_t top_round(hipStream_t st, uint64_t *prev, size_t h, int lv, const gf *a, const gf *b, size_t n,
                      gf *partial, const Seg *round, unsigned *counter, int cus) {
     if (lv < 1 || lv > kSubMaxLevels || (h & (h - 1)) || (h >> lv) == 0 || (h >> lv) > 4096) return hipErrorInvalidValue;
-    const unsigned nsub = (unsigned)(h >> lv), ndot = top_round_dot_blocks(n);
+    // every block resident at once: a dot block left over for a second
+    // wave of blocks would start only when another finishes, doubling the
+    // launch (the kernel's registers allow few blocks per compute unit)
+    const unsigned nsub = (unsigned)(h >> lv);
+    const unsigned resident = cus > 0 ? (unsigned)cus * top_round_blocks_per_cu() : ~0u;
+    const unsigned ndot = std::max(1u, std::min(top_round_dot_blocks(n), resident > nsub ? resident - nsub : 1u));
     k_top_round<<<nsub + ndot, kSubThreads, 0, st>>>(prev, h, lv, nsub, a, b, n, partial, ndot, round, counter,
                                                      pair_width(nsub, cus));
     return hipGetLastError();

@@ -233,10 +233,12 @@ hipError_t dot(hipStream_t st, const gf *a, const gf *b, size_t n, gf *partial, 
 // the number of blocks (partial sums) dot() uses for n elements
 unsigned dot_blocks(size_t n);
 // The last subtree step of a round's pi tree (as subtree()) with the round's
-// inner product sum a[k] b[k], k < n (partial sums in partial[0,
-// top_round_dot_blocks(n))), and the round's transcript step (the
-// SegKernel::Round work of *round, on the reduced sum) by whichever block
-// finishes last.  *counter must be zero.  cus as for subtree().
+// inner product sum a[k] b[k], k < n (partial sums in partial[0, ndot) with
+// ndot at most top_round_dot_blocks(n), and with cus given at most the
+// blocks that stay resident beside the subtree blocks), and the round's
+// transcript step (the SegKernel::Round work of *round, on the reduced sum)
+// by whichever block finishes last.  *counter must be zero.  cus as for
+// subtree().
 hipError_t top_round(hipStream_t st, uint64_t *prev, size_t h, int lv, const gf *a, const gf *b, size_t n,
                      gf *partial, const Seg *round, unsigned *counter, int cus = 0);
 unsigned top_round_dot_blocks(size_t n);
