@@ -390,7 +390,10 @@ int bfd_tuning_load(bfd_ctx *ctx, const char *path);
 /* Times `iters` dependent multiplies (variant 0: LDS comb, 1: register
  * Karatsuba, 4: by one shared factor through its LDS window table), `iters`
  * chained Keccak-256 hashes of 64-byte messages (2, the Merkle node hash) on
- * n lanes, or one copy of n 16-byte elements (3); returns milliseconds.  No
+ * n lanes, one copy of n 16-byte elements (3), or `iters` chained hashes on
+ * one lane pair of one wave (5, the transcript step's form; n ignored);
+ * returns milliseconds.  With n = 64 variants 1 and 2 run on one wave, so
+ * the time per iteration is the latency of one product or hash.  No
  * reference counterpart (measurement only). */
 int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms);
 /* Per-kernel timing of the last commit/open on this context: fills up to

@@ -950,11 +950,11 @@ int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out) {
 int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms) {
     return guarded([&] {
         if (!ctx || !ms) fail(BFD_ERR_ARG, "null argument");
-        if (what < 0 || what > 4 || iters < 1) fail(BFD_ERR_ARG, "unknown micro-benchmark");
+        if (what < 0 || what > 5 || iters < 1) fail(BFD_ERR_ARG, "unknown micro-benchmark");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);
         hipStream_t st = ctx->stream;
-        size_t bytes = what == 2 ? 200 * n : 32 * n;
+        size_t bytes = what == 2 ? 200 * n : (what == 5 ? 64 : 32 * n);
         DevBuf io(bytes, st);
         std::vector<uint8_t> h(bytes);
         for (size_t i = 0; i < bytes; i++) h[i] = (uint8_t)(i * 2654435761u >> 13);
@@ -964,6 +964,7 @@ int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms) {
         hip_check(hipEventCreate(&e), "event");
         auto go = [&](int it) {
             if (what == 2) return launch::bench_keccak(st, io.as<uint64_t>(), n, it);
+            if (what == 5) return launch::bench_keccak_pair(st, io.as<uint64_t>(), it);
             if (what == 3) return launch::copy16(st, io.as<gf>(), io.as<gf>() + n, n);
             return launch::bench_mul(st, what, io.as<gf>(), n, it);
         };

@@ -507,6 +507,21 @@ __device__ __forceinline__ bool block_counted(unsigned *counter, unsigned total,
     return last;
 }
 
+// Counts the block in two levels (kCountGroups, kernels.hpp): block bid of
+// nblocks counts in its group's counter, and the block that counts last in
+// its group counts in the launch counter; true in the block that counts last
+// there.  Called by one thread with inc = 1 made data-dependent on the
+// atomics the count vouches for (so they are performed first); a group's
+// last count returns after every member's, and the launch count after it.
+__device__ __forceinline__ bool grid_last(unsigned *cnt, unsigned nblocks, unsigned bid, unsigned inc) {
+    const unsigned G = nblocks < (unsigned)kCountGroups ? nblocks : (unsigned)kCountGroups;
+    const unsigned g = bid % G, members = (nblocks - g + G - 1) / G;
+    if (__hip_atomic_fetch_add(cnt + kCountLine * (1 + g), inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+        members - 1)
+        return false;
+    return __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == G - 1;
+}
+
 // a gf published with device-scope atomics, read in the last block
 __device__ __forceinline__ gf gf_ld_published(const gf *p) {
     const unsigned long long *q = (const unsigned long long *)p;
@@ -516,6 +531,9 @@ __device__ __forceinline__ gf gf_ld_published(const gf *p) {
 
 // lane src's value of v, on every lane of the wave
 __device__ __forceinline__ gf lane_gf(gf v, int src) { return gf{(uint64_t)__shfl((long long)v.lo, src), (uint64_t)__shfl((long long)v.hi, src)}; }
+// c ? a : b word by word (a select of whole elements on a lane-dependent
+// condition can otherwise be lowered through a stack copy)
+__device__ __forceinline__ gf gf_sel(bool c, gf a, gf b) { return gf{c ? a.lo : b.lo, c ? a.hi : b.hi}; }
 
 // Pair sums of the product sumcheck with the pair's eq weight t multiplied
 // into f first: tf0 = t f0, tf1 = t f1, then A0 += tf0 g0, A1 += tf1 g1 and
@@ -531,35 +549,51 @@ __device__ __forceinline__ void prod_terms(gf f0, gf f1, gf g0, gf g1, gf t, gf 
 // fold on the top variable: lo + x (hi - lo)  (multilinear.rs:183-198)
 __device__ __forceinline__ gf fold_top(gf lo, gf hi, const gf_pre &px) { return gf_add(lo, gf_mul_pre(gf_add(hi, lo), px)); }
 
-// The eq scalars move from round to round with one product after each
-// challenge: with t = e0[i] + x_k (eq(r_i, x_k), variable i = n-1-k),
-// s_{k+1} = s_k t, K_{k+1} = P1 t and L_{k+1} = P2 t, where
-// P1 = s_k ecf[i-1] and P2 = s_k cf[i-1] are formed while round k runs.
-// prod_pre: {P1, P2} on two lanes (lane & 1 picks).
-__device__ __forceinline__ gf prod_pre(const gf *ecf, const gf *cf, int i, gf s, int lane) {
-    return gf_mul_reg(s, gf_ld(((lane & 1) ? cf : ecf) + i - 1));
-}
-// prod_next: lane 0 s_{k+1}, lane 1 K_{k+1}, lane 2 L_{k+1}; every lane of
-// the wave returns the three.
-__device__ __forceinline__ void prod_next(gf s, gf P1, gf P2, gf e0, gf x, gf out[3]) {
+// The eq scalars s_k, K_k, L_k of round k (see ProdFs): round 0 takes them as
+// uploaded; a later round moves s_{k-1}, P1, P2 on by t = e0[i + 1] +
+// x_{k-1} with one product on lanes 0..2 of the calling wave.  Every lane
+// gets all three.
+__device__ __forceinline__ void prod_scalars(const ProdFs *fs, const gf *e0s, int k, gf &s, gf &K, gf &L) {
+    if (k == 0) {
+        s = fs->s;
+        K = fs->K;
+        L = fs->L;
+        return;
+    }
     const int lane = threadIdx.x & 63;
-    const gf p = gf_mul_reg(lane == 0 ? s : (lane == 1 ? P1 : P2), gf_add(e0, x));
-    out[0] = lane_gf(p, 0);
-    out[1] = lane_gf(p, 1);
-    out[2] = lane_gf(p, 2);
+    const gf t = gf_add(gf_ld(e0s + (fs->n - k)), fs->x);
+    const gf p = gf_mul_reg(gf_sel(lane == 0, fs->s, gf_sel(lane == 1, fs->P1, fs->P2)), t);
+    s = lane_gf(p, 0);
+    K = lane_gf(p, 1);
+    L = lane_gf(p, 2);
+}
+
+// The round polynomial p(X) = s (e0 + X) cf (A0 + Q1 X + A2 X^2) from the
+// pair sums A0, A1, A2 (Q1 = A0 + A1 + A2) and the eq scalars K = s e0 cf,
+// L = s cf: c0 = K A0, c1 = L A0 + K Q1, c2 = L Q1 + K A2, c3 = L A2, six
+// independent products on lanes 0..5 of the calling wave; every lane gets
+// the four.
+struct ProdPoly {
+    gf c0, c1, c2, c3;
+};
+__device__ __forceinline__ ProdPoly prod_coeffs(gf A0, gf A1, gf A2, gf K, gf L) {
+    const int lane = threadIdx.x & 63;
+    const gf Q1 = gf_add(A0, gf_add(A1, A2));
+    const gf u = gf_sel(lane & 1, L, K);
+    const gf v = gf_sel(lane < 2, A0, gf_sel(lane < 4, Q1, A2));
+    const gf p = gf_mul_reg(u, v);  // lane 0: K A0, 1: L A0, 2: K Q1, 3: L Q1, 4: K A2, 5: L A2
+    return ProdPoly{lane_gf(p, 0), gf_add(lane_gf(p, 1), lane_gf(p, 2)), gf_add(lane_gf(p, 3), lane_gf(p, 4)),
+                    lane_gf(p, 5)};
 }
 
 // One round's transcript step of the product sumcheck, run by one whole wave
-// (prove_sumcheck_round, sumcheck/src/lib.rs:81-103).  With the pair sums
-// A0, A1, A2, Q1 = A0 + A1 + A2 and the eq scalars K = s e0 cf, L = s cf,
-// the round polynomial p(X) = s (e0 + X) cf (A0 + Q1 X + A2 X^2) has
-// c0 = K A0, c1 = L A0 + K Q1, c2 = L Q1 + K A2, c3 = L A2: six independent
-// products on six lanes.  Then observe [claim,] c0, c2, c3 (the claim
-// p(0) + p(1) = c1 + c2 + c3 in round 0, sumcheck/src/lib.rs:67) and sample
-// x.  With have_prev the challenger's input is the previous round's 32-byte
-// digest prev (every round after the first); otherwise the input bytes come
-// from the region (the first round, after whatever the caller observed; lane
-// 0 appends to them there).  Every lane gets the results; the caller stores
+// (prove_sumcheck_round, sumcheck/src/lib.rs:81-103), given the round
+// polynomial: observe [claim,] c0, c2, c3 (the claim p(0) + p(1) =
+// c1 + c2 + c3 in round 0, sumcheck/src/lib.rs:67) and sample x.  With
+// have_prev the challenger's input is the previous round's 32-byte digest
+// prev (every round after the first); otherwise the input bytes come from the
+// region (the first round, after whatever the caller observed; lane 0
+// appends to them there).  Every lane gets the results; the caller stores
 // them.
 struct ProdRound {
     gf c0, c2, c3, claim, x;
@@ -568,20 +602,14 @@ struct ProdRound {
 struct Digest {
     uint64_t w[4];
 };
-__device__ ProdRound prod_transcript(ProdFs *st, int k, gf A0, gf A1, gf A2, gf K, gf L, bool have_prev,
-                                     Digest prev) {
+__device__ __forceinline__ ProdRound prod_observe(ProdFs *st, int k, const ProdPoly &c, bool have_prev, Digest prev) {
     const int lane = threadIdx.x & 63;
     uint8_t *in = (uint8_t *)st + prod_input_off(st->n);
-    const gf Q1 = gf_add(A0, gf_add(A1, A2));
-    const gf u = (lane & 1) ? L : K;
-    const gf v = lane < 2 ? A0 : (lane < 4 ? Q1 : A2);
-    const gf p = gf_mul_reg(u, v);  // lane 0: K A0, 1: L A0, 2: K Q1, 3: L Q1, 4: K A2, 5: L A2
     ProdRound R;
-    R.c0 = lane_gf(p, 0);
-    const gf c1 = gf_add(lane_gf(p, 1), lane_gf(p, 2));
-    R.c2 = gf_add(lane_gf(p, 3), lane_gf(p, 4));
-    R.c3 = lane_gf(p, 5);
-    R.claim = gf_add(c1, gf_add(R.c2, R.c3));
+    R.c0 = c.c0;
+    R.c2 = c.c2;
+    R.c3 = c.c3;
+    R.claim = gf_add(c.c1, gf_add(c.c2, c.c3));
     if (have_prev) {
         const uint64_t m[10] = {prev.w[0], prev.w[1], prev.w[2], prev.w[3], R.c0.lo, R.c0.hi,
                                 R.c2.lo,   R.c2.hi,   R.c3.lo,   R.c3.hi};
@@ -613,10 +641,15 @@ __device__ ProdRound prod_transcript(ProdFs *st, int k, gf A0, gf A1, gf A2, gf 
     R.x = gf{__builtin_bswap64(R.d[3]), __builtin_bswap64(R.d[2])};
     return R;
 }
+// the polynomial from the pair sums and eq scalars, then the step
+__device__ __forceinline__ ProdRound prod_transcript(ProdFs *st, int k, gf A0, gf A1, gf A2, gf K, gf L, bool have_prev,
+                                     Digest prev) {
+    return prod_observe(st, k, prod_coeffs(A0, A1, A2, K, L), have_prev, prev);
+}
 
-// The challenger state and the next round's eq scalars after round k into
-// the region: input = output = the digest, 16 output bytes left unread.
-__device__ __forceinline__ void prod_store_state(ProdFs *st, const ProdRound &R, const gf next[3]) {
+// The challenger state after a round into the region (one lane): input =
+// output = the digest, 16 output bytes left unread, and the challenge.
+__device__ __forceinline__ void prod_store_round(ProdFs *st, const ProdRound &R) {
     uint64_t *in64 = reinterpret_cast<uint64_t *>((uint8_t *)st + prod_input_off(st->n));
     uint64_t *out64 = reinterpret_cast<uint64_t *>(st->out);
 #pragma unroll
@@ -626,9 +659,6 @@ __device__ __forceinline__ void prod_store_state(ProdFs *st, const ProdRound &R,
     }
     st->in_len = 32;
     st->out_len = 16;
-    st->s = next[0];
-    st->K = next[1];
-    st->L = next[2];
     st->x = R.x;
 }
 
@@ -662,114 +692,197 @@ __device__ __forceinline__ void prod_fold_sums(const gf *fsrc, const gf *gsrc, g
 
 // Round k of the product sumcheck (see launch::prod_step).  MODE 0: round 0,
 // sums over the caller's tables; 1: fold the caller's tables into the working
-// ones, then the sums; 2: fold the working tables in place, then the sums.
-// Block b takes pairs [b C, (b + 1) C), C = half / gridDim.x, which share one
-// entry of the eq factor B: pair j is weighted by A[j mod 2^lo] alone and the
-// block's sums by that B entry.
+// ones, then the sums; 2: fold the working tables in place, then the sums;
+// 3 (rounds of few pairs, k >= 1): four lanes per pair, kProdSplit pairs per
+// block, so a lane's chain is three products instead of nine: its entry's
+// fold (f0, f1, g0, g1), then t f0 and t f1 on the f lanes, then the three
+// pair products, with the block's eq-scalar products on idle lanes of
+// wave 0.  Block b takes pairs [b C, (b + 1) C), C = half / gridDim.x,
+// which share one entry of the eq factor B: pair j is weighted by
+// A[j mod 2^lo] alone and the block's sums by that B entry.  A block
+// publishes its share of the round polynomial, (K B, L B) times its sums
+// (prod_coeffs), so the block that finishes last only adds the shares
+// before its transcript step; meanwhile its second wave forms the eq
+// scalars for the next round (P1, P2, ProdFs).
+constexpr int kProdSplit = CB / 4;
+constexpr size_t kProdSplitMax = (size_t)1 << 15;  // rounds of at most this many pairs use MODE 3
 template <int MODE>
 __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq,
                                                    size_t half, int k, ProdFs *fs) {
-    __shared__ gf pre[2];  // P1, P2 of this round
+    __shared__ gf sc[3];  // K B, L B of this block, s_k
+    __shared__ gf red[3][CB / 64];
     __shared__ unsigned last;
     const uint32_t nv = fs->n;
     const int iv = (int)nv - 1 - k;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const gf *cfs = (const gf *)((const uint8_t *)fs + prod_cf_off()), *e0s = cfs + nv, *ecfs = e0s + nv;
-    // P1, P2 for the next round's scalars, on two lanes of the second wave
-    // (the transcript runs on the first)
-    if (iv > 0 && (threadIdx.x == 64 || threadIdx.x == 65)) pre[threadIdx.x & 1] = prod_pre(ecfs, cfs, iv, fs->s, threadIdx.x);
     uint8_t *base = (uint8_t *)fs;
     const uint32_t in_cap = fs->in_cap;
     unsigned long long *acc0 = (unsigned long long *)(base + prod_acc_off(nv, in_cap));
-    // the next round's slots start at zero (this launch ends before it runs)
+    // the next round's slots and counters start at zero (this launch ends
+    // before it runs)
     if (k + 1 < (int)nv) {
-        unsigned long long *nxt = acc0 + (size_t)(k + 1) * 6 * kProdSlots;
-        for (size_t w = blockIdx.x * (size_t)CB + threadIdx.x; w < 6 * (size_t)kProdSlots; w += (size_t)gridDim.x * CB)
+        unsigned long long *nxt = acc0 + (size_t)(k + 1) * (kProdAccRound / 8);
+        for (size_t w = blockIdx.x * (size_t)CB + tid; w < kProdAccRound / 8; w += (size_t)gridDim.x * CB)
             nxt[w] = 0;
     }
     const size_t C = half / gridDim.x, j0 = blockIdx.x * C, j1 = j0 + C;
     const size_t amask = ((size_t)1 << eq.lo) - 1;
-    gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
-    if constexpr (MODE == 0) {
-        for (size_t j = j0 + threadIdx.x; j < j1; j += CB)
-            prod_terms(gf_ld(fsrc + j), gf_ld(fsrc + j + half), gf_ld(gsrc + j), gf_ld(gsrc + j + half),
-                       gf_ld(eq.A + (j & amask)), a0, a1, a2);
-    } else if (C >= 2 * (size_t)CB) {
-        // two pairs or more per thread: the folds multiply by x through a
-        // window table
-        __shared__ GfTab T;
-        __shared__ gf u[128];
-        gf_tab_build(T, u, fs->x);
-        prod_fold_sums<true>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, &T, gf_pre{}, a0, a1, a2);
+    const gf Bv = gf_ld(eq.B + (j0 >> eq.lo));
+    if constexpr (MODE == 3) {
+        // pair j0 + tid / 4; lane u of its quad: table u >> 1, entry j + (u & 1) half
+        const int u = lane & 3, t = u >> 1, qb = lane & ~3;
+        const size_t j = j0 + (size_t)(tid >> 2), e = j + (size_t)(u & 1) * half;
+        const gf *src = t ? gsrc : fsrc;
+        gf *dst = t ? gdst : fdst;
+        const gf x = fs->x;
+        const gf lo = gf_ld(src + e);
+        const gf V = gf_add(lo, gf_mul_reg(gf_add(gf_ld(src + e + 2 * half), lo), x));
+        gf_st(dst + e, V);
+        // t f0, t f1 on the f lanes; wave 0's lanes 2, 3, 6: s_k, K_k, L_k
+        const bool sl = wave == 0 && (lane == 2 || lane == 3 || lane == 6);
+        const gf sv = gf_sel(lane == 2, fs->s, gf_sel(lane == 3, fs->P1, fs->P2));
+        const gf M = gf_mul_reg(gf_sel(sl, sv, V), gf_sel(sl, gf_add(gf_ld(e0s + (nv - k)), x), gf_ld(eq.A + (j & amask))));
+        const gf X = gf_sel(t == 0, M, V);
+        const gf Ks = lane_gf(M, 3), Ls = lane_gf(M, 6);
+        if (wave == 0) {
+            const gf sk = lane_gf(M, 2);
+            if (lane == 0) sc[2] = sk;
+        }
+        // the pair products on lanes 0..2 of each quad; wave 0's lanes 3, 7: K_k B, L_k B
+        const gf X0 = lane_gf(X, qb), X1 = lane_gf(X, qb + 1), X2 = lane_gf(X, qb + 2), X3 = lane_gf(X, qb + 3);
+        const bool bl = wave == 0 && (lane == 3 || lane == 7);
+        const gf o1 = gf_sel(bl, gf_sel(lane == 3, Ks, Ls), gf_sel(u == 0, X0, gf_sel(u == 1, X1, gf_add(X0, X1))));
+        const gf o2 = gf_sel(bl, Bv, gf_sel(u == 0, X2, gf_sel(u == 1, X3, gf_add(X2, X3))));
+        const gf p = gf_mul_reg(o1, o2);
+        if (wave == 0) {
+            const gf kb = lane_gf(p, 3), lb = lane_gf(p, 7);
+            if (lane == 0) {
+                sc[0] = kb;
+                sc[1] = lb;
+            }
+        }
+        gf a = gf_sel(u < 3, p, gf_zero());
+#pragma unroll
+        for (int off = 4; off < 64; off <<= 1) a = gf_add(a, gf{(uint64_t)__shfl_xor((long long)a.lo, off),
+                                                                (uint64_t)__shfl_xor((long long)a.hi, off)});
+        if (lane < 3) red[lane][wave] = a;
     } else {
-        prod_fold_sums<false>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, nullptr, gf_prepare(fs->x), a0, a1,
-                              a2);
-    }
-    // block sums, XORed into one of the round's accumulation slots
-    __shared__ gf red[3][CB / 64];
-    a0 = wave_xor(a0);
-    a1 = wave_xor(a1);
-    a2 = wave_xor(a2);
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = a0;
-        red[1][threadIdx.x >> 6] = a1;
-        red[2][threadIdx.x >> 6] = a2;
+        // the eq scalars on the second wave, before its pairs: s_k, K_k, L_k,
+        // then K_k B and L_k B
+        if (wave == 1) {
+            gf s, K, L;
+            prod_scalars(fs, e0s, k, s, K, L);
+            const gf p = gf_mul_reg(lane == 0 ? K : L, Bv);
+            const gf kb = lane_gf(p, 0), lb = lane_gf(p, 1);
+            if (lane == 0) {
+                sc[0] = kb;
+                sc[1] = lb;
+                sc[2] = s;
+            }
+        }
+        gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
+        if constexpr (MODE == 0) {
+            for (size_t j = j0 + tid; j < j1; j += CB)
+                prod_terms(gf_ld(fsrc + j), gf_ld(fsrc + j + half), gf_ld(gsrc + j), gf_ld(gsrc + j + half),
+                           gf_ld(eq.A + (j & amask)), a0, a1, a2);
+        } else if (C >= 2 * (size_t)CB) {
+            // two pairs or more per thread: the folds multiply by x through a
+            // window table
+            __shared__ GfTab T;
+            __shared__ gf u[128];
+            gf_tab_build(T, u, fs->x);
+            prod_fold_sums<true>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, &T, gf_pre{}, a0, a1, a2);
+        } else {
+            prod_fold_sums<false>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, nullptr, gf_prepare(fs->x), a0,
+                                  a1, a2);
+        }
+        a0 = wave_xor(a0);
+        a1 = wave_xor(a1);
+        a2 = wave_xor(a2);
+        if (lane == 0) {
+            red[0][wave] = a0;
+            red[1][wave] = a1;
+            red[2][wave] = a2;
+        }
     }
     __syncthreads();
     const uint32_t n = nv;
-    unsigned long long *acc = acc0 + (size_t)k * 6 * kProdSlots;
-    unsigned *counter = (unsigned *)(base + prod_counters_off(n, in_cap)) + k;
-    // The block's sums go out as device-scope atomics whose results it waits
-    // for, so they are performed before its count below is issued; every
-    // block's sums are therefore in the slots before the last count returns.
-    // No block needs a release fence (which would write back the whole L2):
-    // the folded tables are read only by the next launch.
-    __shared__ unsigned done[3];
-    if (threadIdx.x < 3) {
-        gf v = gf_zero();
-        for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[threadIdx.x][w]);
-        v = gf_mul_reg(v, gf_ld(eq.B + (j0 >> eq.lo)));
-        unsigned long long *slot = acc + 6 * (blockIdx.x % kProdSlots) + 2 * threadIdx.x;
-        const unsigned long long o0 =
-            __hip_atomic_fetch_xor(slot, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long o1 =
-            __hip_atomic_fetch_xor(slot + 1, (unsigned long long)v.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        done[threadIdx.x] = (unsigned)(o0 ^ o1) | 1u;
+    unsigned long long *acc = acc0 + (size_t)k * (kProdAccRound / 8);
+    unsigned *counter = (unsigned *)(acc + kProdSlotBytes / 8);
+    // The block's share of the polynomial goes out as device-scope atomics
+    // whose results it waits for, so they are performed before its count
+    // below is issued; every block's shares are therefore in the slots before
+    // the last count returns.  No block needs a release fence (which would
+    // write back the whole L2): the folded tables are read only by the next
+    // launch.
+    if (wave == 0) {
+        gf A[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            A[c] = gf_zero();
+            for (int w = 0; w < CB / 64; w++) A[c] = gf_add(A[c], red[c][w]);
+        }
+        const ProdPoly cs = prod_coeffs(A[0], A[1], A[2], sc[0], sc[1]);
+        // c0..c3 on lanes 0..3
+        const gf v = gf_sel(lane < 2, gf_sel(lane == 0, cs.c0, cs.c1), gf_sel(lane == 2, cs.c2, cs.c3));
+        unsigned done = 1u;
+        if (lane < 4) {
+            unsigned long long *slot = acc + 8 * (blockIdx.x % kProdSlots) + 2 * lane;
+            const unsigned long long o0 =
+                __hip_atomic_fetch_xor(slot, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long o1 =
+                __hip_atomic_fetch_xor(slot + 1, (unsigned long long)v.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            done = (unsigned)(o0 ^ o1) | 1u;
+        }
+        // 1, after lanes 0..3's atomics returned
+        const unsigned inc = done & (unsigned)__shfl((int)done, 1) & (unsigned)__shfl((int)done, 2) &
+                             (unsigned)__shfl((int)done, 3) & 1u;
+        if (lane == 0) last = grid_last(counter, gridDim.x, blockIdx.x, inc);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned inc = done[0] & done[1] & done[2] & 1u;  // 1, after all three threads' atomics returned
-        last = __hip_atomic_fetch_add(counter, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    if (!last) return;
+    if (wave == 1) {
+        // the eq scalars for the next round (independent of its challenge)
+        if (iv > 0) {
+            const gf sk = sc[2];
+            const gf p = gf_mul_reg(sk, (lane & 1) ? gf_ld(cfs + iv - 1) : gf_ld(ecfs + iv - 1));
+            if (lane == 0) {
+                fs->s = sk;
+                fs->P1 = p;
+            } else if (lane == 1) {
+                fs->P2 = p;
+            }
+        }
+        return;
     }
-    __syncthreads();
-    if (!last || threadIdx.x >= 64) return;
+    if (wave != 0) return;
     // lane l XORs slots l, l + 64, ... (the slots a smaller grid used), all
     // loads issued before the first use; then a wave reduction
-    const int lane = threadIdx.x;
-    unsigned long long v[6 * (kProdSlots / 64)];
+    unsigned long long v[8 * (kProdSlots / 64)];
 #pragma unroll
     for (int t = 0; t < kProdSlots / 64; t++)
 #pragma unroll
-        for (int c = 0; c < 6; c++)
-            v[6 * t + c] = (size_t)(lane + 64 * t) < gridDim.x
-                               ? __hip_atomic_load(acc + 6 * (lane + 64 * t) + c, __ATOMIC_RELAXED,
+        for (int c = 0; c < 8; c++)
+            v[8 * t + c] = (size_t)(lane + 64 * t) < gridDim.x
+                               ? __hip_atomic_load(acc + 8 * (lane + 64 * t) + c, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT)
                                : 0ull;
-    gf A[3];
+    gf cv[4];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        A[c] = gf_zero();
+    for (int c = 0; c < 4; c++) {
+        cv[c] = gf_zero();
 #pragma unroll
-        for (int t = 0; t < kProdSlots / 64; t++) A[c] = gf_add(A[c], gf{v[6 * t + 2 * c], v[6 * t + 2 * c + 1]});
-        A[c] = wave_xor(A[c]);
+        for (int t = 0; t < kProdSlots / 64; t++) cv[c] = gf_add(cv[c], gf{v[8 * t + 2 * c], v[8 * t + 2 * c + 1]});
+        cv[c] = wave_xor(cv[c]);
     }
+    const ProdPoly cs{cv[0], cv[1], cv[2], cv[3]};
     // after the first round the challenger's input is the last digest
     Digest prev;
     const uint64_t *in64 = reinterpret_cast<const uint64_t *>(base + prod_input_off(n));
 #pragma unroll
     for (int w = 0; w < 4; w++) prev.w[w] = k > 0 ? in64[w] : 0;
-    const ProdRound R = prod_transcript(fs, k, A[0], A[1], A[2], fs->K, fs->L, k > 0, prev);
-    gf next[3] = {gf_zero(), gf_zero(), gf_zero()};
-    if (iv > 0) prod_next(fs->s, pre[0], pre[1], gf_ld(e0s + iv), R.x, next);
+    const ProdRound R = prod_observe(fs, k, cs, k > 0, prev);
     if (lane == 0) {
         gf *polys = (gf *)(base + prod_polys_off(n)), *chal = (gf *)(base + prod_chal_off(n));
         polys[3 * k] = R.c0;
@@ -777,129 +890,278 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
         polys[3 * k + 2] = R.c3;
         chal[k] = R.x;
         if (k == 0) fs->claim = R.claim;
-        prod_store_state(fs, R, next);
+        prod_store_round(fs, R);
     }
 }
 
-// The rounds from k0 on in one block of kTailThreads threads, tables in LDS.
-// A round of q pairs runs its products in three steps, one product per task
-// and at most two tasks per thread, with a barrier after each step, so a
-// round of few pairs costs three product latencies.  Task indices are padded
-// to Q = max(q, 64) per kind, so every wave's tasks of one pass have one kind:
-//   fold   4Q tasks: kind u (f low, f high, g low, g high) of pair j folds
-//          entries j + uq and j + (u+2)q of its table into j + uq (k > 0);
-//          in place, as no task reads what another writes;
-//   scale  2Q tasks: TF[uq + j] = T[j] F[j + uq], u = 0, 1, with the halved
-//          eq table's entries T[j] = A[j mod 2^lo] B[j >> lo] formed once;
-//   sum    3Q tasks: TF[j] G[j], TF[q+j] G[q+j], (TF[j] + TF[q+j])(G[j] + G[q+j]),
-//          reduced per kind by wave sums.
-// The running scalars (s, x, the digest) stay in LDS between rounds; the
-// last wave forms P1 and P2 during the scale step.
+// The rounds from k0 on in one block of kTailThreads threads, tables in LDS,
+// with one round of lookahead so that a round's table work runs while the
+// round before it samples its challenge.  Round k+1's pair sums are
+// quadratics in x_k: with q the pairs of round k, j < q/2 a pair of round
+// k+1, V_i = F_k[j + i q/2] and W_i = G_k[j + i q/2] (i < 4), the folded
+// entries are f0' = V0 + x (V2 + V0), f1' = V1 + x (V3 + V1) (the same for g
+// with W), so with T = T[j] (the halved eq table)
+//   A0' = sum (a0 + x b0)(c0 + x d0),   A1' = sum (a1 + x b1)(c1 + x d1),
+//   A2' = sum ((a0 + a1) + x (b0 + b1))((c0 + c1) + x (d0 + d1)),
+//   a0 = T V0, b0 = T (V2 + V0), a1 = T V1, b1 = T (V3 + V1),
+//   c0 = W0, d0 = W2 + W0, c1 = W1, d1 = W3 + W1,
+// and three Karatsuba products per sum give its coefficients of 1, x, x^2.
+// Wave 0 runs the transcript: in round k it evaluates the three quadratics
+// at x_{k-1} and moves the eq scalars (two product steps on six lanes),
+// forms the round polynomial (one step) and samples x_k.  Meanwhile the
+// worker waves fold F_{k-1} into F_k with x_{k-1} and form round k+1's
+// coefficient sums from F_k, eight lanes per pair of round k+1: one fold
+// each, one product by T on the f lanes, then the nine products.  Wave 4,
+// which shares wave 0's SIMD, sits the rounds out.  Round k0's sums are
+// formed directly before the first round, four lanes per pair.  (The
+// transcript step needs about 160 VGPRs, so the block stays at 512 threads.)
 constexpr int kTailThreads = 512;
+constexpr int kTailWaves = kTailThreads / 64;
 constexpr size_t kTailPairs = 256;  // pairs of the first tail round (4 * kTailPairs entries per table)
+
+__device__ __forceinline__ gf gf_shfl_xor(gf v, int off) {
+    return gf{(uint64_t)__shfl_xor((long long)v.lo, off), (uint64_t)__shfl_xor((long long)v.hi, off)};
+}
+
+// prod_transcript behind a call: its Keccak-f keeps its own registers
+// instead of sharing an allocation with the tail kernel's table work
+__device__ __noinline__ ProdRound prod_transcript_call(ProdFs *st, int k, gf A0, gf A1, gf A2, gf K, gf L,
+                                                       bool have_prev, Digest prev) {
+    return prod_transcript(st, k, A0, A1, A2, K, L, have_prev, prev);
+}
+
 __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, const gf *gsrc, gf *fout, gf *gout,
                                                             EqFactors eq, size_t half, int k0, ProdFs *fs) {
-    __shared__ gf F[4 * kTailPairs], G[4 * kTailPairs], Ts[kTailPairs], TF[2 * kTailPairs];
-    __shared__ gf red[3][kTailThreads / 64];
-    __shared__ gf sc[3];   // s, K, L of the round being run
-    __shared__ gf pre[2];  // P1, P2 of the round being run
-    __shared__ gf cst[3][32];  // cf, e0, ecf
-    __shared__ gf xs;
-    __shared__ uint64_t dig[4];
-    __shared__ gf outs[32][4];  // per round: c0, c2, c3, x (stored to the region at the end)
-    __shared__ gf claim0;
+    // F_k of round k lives in Fb when k - k0 is even, in Fa when odd; Fa
+    // first holds the input tables when k0 > 0 (4 half entries)
+    __shared__ gf Fa[4 * kTailPairs], Ga[4 * kTailPairs], Fb[2 * kTailPairs], Gb[2 * kTailPairs];
+    __shared__ gf Ts[kTailPairs];
+    __shared__ gf redA[3][kTailWaves];     // round k0's pair sums per wave
+    __shared__ gf red[2][9][kTailWaves];   // the next round's coefficient sums per wave (by round parity)
+    __shared__ gf sx[2];                   // challenge of the round just sampled (by round parity)
+    __shared__ gf cst[3][32];              // cf, e0, ecf
+    __shared__ gf outs[32][4];             // per round: c0, c2, c3, x (stored to the region at the end)
     const uint32_t n = fs->n;
-    const int tid = threadIdx.x, wave = tid >> 6;
-    // the previous round's tables (4 half entries; the caller's 2 half in
-    // round 0) and the halved eq table's first half entries
-    const size_t load = (k0 == 0 ? 2 : 4) * half;
-    for (size_t e = tid; e < load; e += kTailThreads) {
-        F[e] = gf_ld(fsrc + e);
-        G[e] = gf_ld(gsrc + e);
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const size_t q0 = half;
+    if (k0 == 0) {
+        for (size_t e = tid; e < 2 * q0; e += kTailThreads) {
+            Fb[e] = gf_ld(fsrc + e);
+            Gb[e] = gf_ld(gsrc + e);
+        }
+    } else {
+        for (size_t e = tid; e < 4 * q0; e += kTailThreads) {
+            Fa[e] = gf_ld(fsrc + e);
+            Ga[e] = gf_ld(gsrc + e);
+        }
     }
-    for (size_t e = tid; e < half; e += kTailThreads)
+    for (size_t e = tid; e < q0; e += kTailThreads)
         Ts[e] = gf_mul_reg(gf_ld(eq.A + (e & (((size_t)1 << eq.lo) - 1))), gf_ld(eq.B + (e >> eq.lo)));
     if (tid < 3 * (int)n) cst[tid / n][tid % n] = gf_ld((const gf *)((const uint8_t *)fs + prod_cf_off()) + tid);
-    if (tid == 0) {
-        sc[0] = fs->s;
-        sc[1] = fs->K;
-        sc[2] = fs->L;
-        xs = fs->x;
-        const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)fs + prod_input_off(n));
-        for (int w = 0; w < 4; w++) dig[w] = in64[w];
+    __syncthreads();
+    // round k0's pair sums (after folding the input with x_{k0-1} into F_k0
+    // when k0 > 0), four lanes per pair: f0, f1, g0, g1, then t f0, t f1,
+    // then the three products
+    {
+        const gf_pre px = gf_prepare(fs->x);
+        const int v = lane & 3, t = v >> 1, i = v & 1, base = lane & ~3;
+        gf acc = gf_zero();
+#pragma unroll 1
+        for (size_t g0 = 0; g0 < q0; g0 += kTailThreads / 4) {
+            const size_t j = g0 + (size_t)(tid >> 2);
+            const bool valid = j < q0;
+            const size_t e = j + (size_t)i * q0;
+            gf V = gf_zero();
+            if (valid) {
+                if (k0 > 0) {
+                    const gf *P = t ? Ga : Fa;
+                    V = fold_top(P[e], P[e + 2 * q0], px);
+                    (t ? Gb : Fb)[e] = V;
+                } else {
+                    V = (t ? Gb : Fb)[e];
+                }
+            }
+            const gf M = gf_mul_reg(V, valid ? Ts[j] : gf_zero());
+            const gf X = gf_sel(t == 0, M, V);
+            const gf X0 = lane_gf(X, base), X1 = lane_gf(X, base + 1), X2 = lane_gf(X, base + 2),
+                     X3 = lane_gf(X, base + 3);
+            const gf o1 = gf_sel(v == 0, X0, gf_sel(v == 1, X1, gf_add(X0, X1)));
+            const gf o2 = gf_sel(v == 0, X2, gf_sel(v == 1, X3, gf_add(X2, X3)));
+            const gf p = gf_mul_reg(o1, o2);
+            if (valid && v < 3) acc = gf_add(acc, p);
+        }
+#pragma unroll
+        for (int off = 4; off < 64; off <<= 1) acc = gf_add(acc, gf_shfl_xor(acc, off));
+        if (lane < 3) redA[lane][wave] = acc;
+    }
+    // wave 0's running state, kept in LDS between rounds: eq scalars s, K, L
+    // of the round being run, P1 and P2 for moving them past it, the last
+    // challenge and digest, the claim
+    __shared__ gf ws[7];  // s, K, L, P1, P2, x, claim
+    __shared__ uint64_t wdig[4];
+    if (wave == 0) {
+        gf s0, K0, L0;
+        prod_scalars(fs, (const gf *)((const uint8_t *)fs + prod_cf_off()) + n, k0, s0, K0, L0);
+        if (lane == 0) {
+            ws[0] = s0;
+            ws[1] = K0;
+            ws[2] = L0;
+            ws[3] = gf_zero();
+            ws[4] = gf_zero();
+            ws[5] = fs->x;
+            ws[6] = gf_zero();
+            const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)fs + prod_input_off(n));
+            for (int w = 0; w < 4; w++) wdig[w] = in64[w];
+        }
     }
     __syncthreads();
-    size_t q = half;
-    for (int k = k0; k < (int)n; k++, q >>= 1) {
+#pragma unroll 1
+    for (int k = k0; k < (int)n; k++) {
+        const size_t q = q0 >> (k - k0);
         const int iv = (int)n - 1 - k;
-        const size_t Q = q < 64 ? 64 : q;
-        if (k > 0) {
-            const gf_pre px = gf_prepare(xs);
-            for (size_t t = tid; t < 4 * Q; t += kTailThreads) {
-                const size_t u = t / Q, j = t % Q;
-                if (j >= q) continue;
-                gf *Tab = u < 2 ? F : G;
-                const size_t lo = j + (u & 1) * q;
-                Tab[lo] = fold_top(Tab[lo], Tab[lo + 2 * q], px);
-            }
-        }
-        __syncthreads();
-        for (size_t t = tid; t < 2 * Q; t += kTailThreads) {
-            const size_t u = t / Q, j = t % Q;
-            if (j < q) TF[u * q + j] = gf_mul_reg(Ts[j], F[j + u * q]);
-        }
-        // P1, P2 of this round on the last wave
-        if (iv > 0 && (tid == kTailThreads - 64 || tid == kTailThreads - 63))
-            pre[tid & 1] = prod_pre(cst[2], cst[0], iv, sc[0], tid);
-        __syncthreads();
-        gf a[3] = {gf_zero(), gf_zero(), gf_zero()};
-        for (size_t t = tid; t < 3 * Q; t += kTailThreads) {
-            const size_t u = t / Q, j = t % Q;
-            if (j >= q) continue;
-            if (u == 0) a[0] = gf_mul_reg(TF[j], G[j]);
-            else if (u == 1) a[1] = gf_mul_reg(TF[q + j], G[q + j]);
-            else a[2] = gf_mul_reg(gf_add(TF[j], TF[q + j]), gf_add(G[j], G[q + j]));
-        }
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            a[c] = wave_xor(a[c]);
-            if ((tid & 63) == 0) red[c][wave] = a[c];
-        }
-        __syncthreads();
+        const bool odd = (k - k0) & 1;
+        gf *Fc = odd ? Fa : Fb, *Gc = odd ? Ga : Gb;
+        const gf *Fp = odd ? Fb : Fa, *Gp = odd ? Gb : Ga;
         if (wave == 0) {
-            gf A[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                A[c] = gf_zero();
-                for (int w = 0; w < kTailThreads / 64; w++) A[c] = gf_add(A[c], red[c][w]);
+            gf s = ws[0], K = ws[1], L = ws[2], P1 = ws[3], P2 = ws[4];
+            const gf xprev = ws[5];
+            gf A0, A1, A2;
+            if (k == k0) {
+                gf a = gf_zero();
+                if (lane < 3)
+                    for (int w = 0; w < kTailWaves; w++) a = gf_add(a, redA[lane][w]);
+                A0 = lane_gf(a, 0);
+                A1 = lane_gf(a, 1);
+                A2 = lane_gf(a, 2);
+                if (iv > 0) {  // P1, P2 for leaving round k0 (lanes 0, 1)
+                    const gf pp = gf_mul_reg(s, (lane & 1) ? cst[0][iv - 1] : cst[2][iv - 1]);
+                    P1 = lane_gf(pp, 0);
+                    P2 = lane_gf(pp, 1);
+                }
+            } else {
+                // round k's coefficient sums (9 on lanes 0..8), then
+                // alpha_c, gamma_c and the middle product on lane c < 3
+                gf P = gf_zero();
+                if (lane < 9)
+                    for (int w = 1; w < kTailWaves; w++) P = gf_add(P, red[k & 1][lane][w]);
+                const int c = lane < 3 ? lane : 0;
+                const gf al = lane_gf(P, 3 * c), ga = lane_gf(P, 3 * c + 1), mid = lane_gf(P, 3 * c + 2);
+                const gf be = gf_add(mid, gf_add(al, ga));
+                const gf t = gf_add(cst[1][iv + 1], xprev);  // eq(r_i, x_{k-1}) of the variable just fixed
+                // step 1: lanes 0..2 x gamma_c, lanes 3..5 {s, P1, P2} t
+                const gf u1 = gf_sel(lane < 3, xprev, gf_sel(lane == 3, s, gf_sel(lane == 4, P1, P2)));
+                const gf r1 = gf_mul_reg(u1, gf_sel(lane < 3, ga, t));
+                const gf tc = gf_add(be, r1);
+                s = lane_gf(r1, 3);
+                K = lane_gf(r1, 4);
+                L = lane_gf(r1, 5);
+                // step 2: lanes 0..2 x (beta_c + x gamma_c), lanes 3, 4 s {ecf, cf}[iv - 1]
+                const gf u2 = gf_sel(lane < 3, xprev, s);
+                const gf v2 = gf_sel(lane < 3, tc, iv > 0 ? gf_sel(lane == 3, cst[2][iv - 1], cst[0][iv - 1]) : gf_zero());
+                const gf r2 = gf_mul_reg(u2, v2);
+                const gf Ac = gf_add(al, r2);
+                A0 = lane_gf(Ac, 0);
+                A1 = lane_gf(Ac, 1);
+                A2 = lane_gf(Ac, 2);
+                if (iv > 0) {
+                    P1 = lane_gf(r2, 3);
+                    P2 = lane_gf(r2, 4);
+                }
             }
             Digest prev;
 #pragma unroll
-            for (int w = 0; w < 4; w++) prev.w[w] = dig[w];
-            const ProdRound R = prod_transcript(fs, k, A[0], A[1], A[2], sc[1], sc[2], k > 0, prev);
-            gf next[3] = {sc[0], sc[1], sc[2]};
-            if (iv > 0) prod_next(sc[0], pre[0], pre[1], cst[1][iv], R.x, next);
-            // lanes 0..2 store s, K, L (constant indices: no private array in scratch)
-            if (tid == 0) sc[0] = next[0];
-            if (tid == 1) sc[1] = next[1];
-            if (tid == 2) sc[2] = next[2];
-            if (tid == 0) {
-                xs = R.x;
-                for (int w = 0; w < 4; w++) dig[w] = R.d[w];
+            for (int w = 0; w < 4; w++) prev.w[w] = wdig[w];
+            const ProdRound R = prod_transcript_call(fs, k, A0, A1, A2, K, L, k > 0, prev);
+            if (lane == 0) {
+                ws[0] = s;
+                ws[1] = K;
+                ws[2] = L;
+                ws[3] = P1;
+                ws[4] = P2;
+                ws[5] = R.x;
+                if (k == 0) ws[6] = R.claim;
+                for (int w = 0; w < 4; w++) wdig[w] = R.d[w];
+                sx[k & 1] = R.x;
                 outs[k][0] = R.c0;
                 outs[k][1] = R.c2;
                 outs[k][2] = R.c3;
                 outs[k][3] = R.x;
-                if (k == 0) claim0 = R.claim;
             }
+        } else if (wave & 3) {
+            // worker lanes: wl over the waves off wave 0's SIMD
+            const int wl = 64 * (wave - 1 - (wave >> 2)) + lane;
+            constexpr int kWorkLanes = 64 * (3 * kTailWaves / 4);
+            const gf xw = k > k0 ? sx[(k - 1) & 1] : gf_zero();
+            if (q >= 2) {
+                const size_t qh = q / 2;
+                const int u = lane & 7, t = u >> 2, i = u & 3, base = lane & ~7;
+                gf acc = gf_zero(), acc8 = gf_zero();
+#pragma unroll 1
+                for (size_t g0 = 0; g0 < qh; g0 += kWorkLanes / 8) {
+                    const size_t j = g0 + (size_t)(wl >> 3);
+                    const bool valid = j < qh;
+                    const size_t e = j + (size_t)i * qh;
+                    gf V = gf_zero();
+                    if (valid) {
+                        if (k > k0) {
+                            const gf *P = t ? Gp : Fp;
+                            V = gf_add(P[e], gf_mul_reg(gf_add(P[e + 2 * q], P[e]), xw));
+                            (t ? Gc : Fc)[e] = V;
+                        } else {
+                            V = (t ? Gc : Fc)[e];
+                        }
+                    }
+                    const gf Vx = lane_gf(V, lane ^ 2);
+                    const gf X1 = gf_sel(i >= 2, gf_add(V, Vx), V);
+                    const gf M = gf_mul_reg(X1, valid ? Ts[j] : gf_zero());
+                    const gf X = gf_sel(t == 0, M, X1);
+                    // lane u: product u of a0c0, b0d0, (a0+b0)(c0+d0), a1c1, b1d1,
+                    // (a1+b1)(c1+d1), (a0+a1)(c0+c1), (b0+b1)(d0+d1) (X of group
+                    // lanes 0..7 = a0, a1, b0, b1, c0, c1, d0, d1); every lane also
+                    // forms the ninth, (a0+a1+b0+b1)(c0+c1+d0+d1)
+                    const int s1 = (u == 0 || u == 2 || u == 6) ? 0 : ((u == 1 || u == 7) ? 2 : ((u == 3 || u == 5) ? 1 : 3));
+                    const int s2 = u == 2 ? 2 : (u == 5 ? 3 : (u == 6 ? 1 : (u == 7 ? 3 : -1)));
+                    const gf xa = lane_gf(X, base + s1), xb = lane_gf(X, base + (s2 < 0 ? s1 : s2));
+                    const gf ya = lane_gf(X, base + 4 + s1), yb = lane_gf(X, base + 4 + (s2 < 0 ? s1 : s2));
+                    const gf o1 = gf_sel(s2 < 0, xa, gf_add(xa, xb)), o2 = gf_sel(s2 < 0, ya, gf_add(ya, yb));
+                    const gf p = gf_mul_reg(o1, o2);
+                    __builtin_amdgcn_sched_barrier(0);  // one product's temporaries live at a time
+                    // sums of group lanes 0..3 and 4..7
+                    gf Y = gf_add(X, gf_shfl_xor(X, 1));
+                    Y = gf_add(Y, gf_shfl_xor(Y, 2));
+                    const gf p8 = gf_mul_reg(lane_gf(Y, base), lane_gf(Y, base + 4));
+                    if (valid) {
+                        acc = gf_add(acc, p);
+                        if (u == 0) acc8 = gf_add(acc8, p8);
+                    }
+                }
+#pragma unroll
+                for (int off = 8; off < 64; off <<= 1) {
+                    acc = gf_add(acc, gf_shfl_xor(acc, off));
+                    acc8 = gf_add(acc8, gf_shfl_xor(acc8, off));
+                }
+                gf(*rd)[kTailWaves] = red[(k + 1) & 1];
+                if (lane < 8) rd[lane][wave] = acc;
+                if (lane == 0) rd[8][wave] = acc8;
+            } else if (k > k0 && wl < 4) {
+                // the last round: its two entries of each table
+                const int t = wl >> 1, e = wl & 1;
+                const gf *P = t ? Gp : Fp;
+                (t ? Gc : Fc)[e] = gf_add(P[e], gf_mul_reg(gf_add(P[e + 2], P[e]), xw));
+            }
+        } else if (q >= 2) {
+            // wave 4 contributes zero coefficient sums
+            gf(*rd)[kTailWaves] = red[(k + 1) & 1];
+            if (lane < 9) rd[lane][wave] = gf_zero();
         }
         __syncthreads();
     }
     // the last round's two entries of each table, for the host's final fold,
     // and the rounds' outputs and the challenger state into the region
+    const bool odd = ((int)n - 1 - k0) & 1;
     if (tid < 2) {
-        gf_st(fout + tid, F[tid]);
-        gf_st(gout + tid, G[tid]);
+        gf_st(fout + tid, (odd ? Fa : Fb)[tid]);
+        gf_st(gout + tid, (odd ? Ga : Gb)[tid]);
     }
     uint8_t *base = (uint8_t *)fs;
     gf *polys = (gf *)(base + prod_polys_off(n)), *chal = (gf *)(base + prod_chal_off(n));
@@ -910,12 +1172,11 @@ __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, cons
         chal[k] = outs[k][3];
     }
     if (tid == 0) {
-        if (k0 == 0) fs->claim = claim0;
+        if (k0 == 0) fs->claim = ws[6];
         ProdRound R;
-        R.x = xs;
-        for (int w = 0; w < 4; w++) R.d[w] = dig[w];
-        const gf next[3] = {sc[0], sc[1], sc[2]};
-        prod_store_state(fs, R, next);
+        R.x = ws[5];
+        for (int w = 0; w < 4; w++) R.d[w] = wdig[w];
+        prod_store_round(fs, R);
     }
 }
 
@@ -1892,6 +2153,28 @@ __global__ __launch_bounds__(KB) void k_bench_keccak(uint64_t *io, size_t n, int
     }
 }
 
+// `iters` chained Keccak-256 hashes of 64-byte messages on one lane pair
+// (keccak256_words_pair, the form of the sumchecks' and open rounds'
+// transcript steps): one block of one wave, io[0..8) the message; the time
+// per hash is the latency of one transcript step's permutation
+__global__ __launch_bounds__(64) void k_bench_keccak_pair(uint64_t *io, int iters) {
+    const bool odd = threadIdx.x & 1;
+    uint64_t m[8], d[4];
+#pragma unroll
+    for (int k = 0; k < 8; k++) m[k] = io[k];
+    for (int k = 0; k < iters; k++) {
+        kk::keccak256_words_pair<8>(m, d, odd);
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            m[4 + w] = m[w] ^ d[w];
+            m[w] = d[w];
+        }
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 8; k++) io[k] = m[k];
+}
+
 // out[i] = a[i] * b[i] (gf_mul_reg), or with shared_b a[i] * b[0] through
 // b's masked pieces prepared once (gf_prepare / gf_mul_pre), the form the
 // fold, fix and NTT kernels use for a challenge or a twiddle
@@ -2104,6 +2387,11 @@ static unsigned prod_blocks_per_cu() {
 hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq, size_t half,
                      int k, ProdFs *fs, int cus) {
     if (half <= kTailPairs || (half & (half - 1)) || eq.lo < 7 || eq.lo > 62) return hipErrorInvalidValue;
+    // rounds of few pairs: four lanes per pair (kProdSplit pairs per block)
+    if (k >= 1 && half <= kProdSplitMax) {
+        k_prod_round<3><<<(unsigned)(half / kProdSplit), CB, 0, st>>>(fsrc, gsrc, fdst, gdst, eq, half, k, fs);
+        return hipGetLastError();
+    }
     // pairs per block: a power of two from CB up to 2^lo (one B entry per
     // block), the smallest that keeps the grid within the resident blocks
     const size_t cap = (size_t)std::max(1, cus) * prod_blocks_per_cu();
@@ -2261,6 +2549,10 @@ hipError_t fill_mix(hipStream_t st, uint64_t *p, size_t n, uint64_t seed) {
 }
 hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters) {
     k_bench_keccak<<<grid_for(n, KB, 1 << 20), KB, 0, st>>>(io, n, iters);
+    return hipGetLastError();
+}
+hipError_t bench_keccak_pair(hipStream_t st, uint64_t *io, int iters) {
+    k_bench_keccak_pair<<<1, 64, 0, st>>>(io, iters);
     return hipGetLastError();
 }
 

@@ -88,20 +88,25 @@ __host__ __device__ inline size_t fs_bytes(uint32_t n, uint32_t in_cap) { return
 // challenger's byte buffers and the running eq product.  One region laid out
 // as
 //   ProdFs | gf cf[n] | gf e0[n] | gf ecf[n] | gf polys[3n] | gf chal[n] |
-//   uint8_t input[in_cap] | gf finals[4] | unsigned counters[n] |
-//   uint64_t acc[n][6 kProdSlots]
+//   uint8_t input[in_cap] | gf finals[4] | per round k: uint64_t
+//   acc[8 kProdSlots], then its block counters (kCountWords words)
 // cf[i] is the correcting factor of the halved eq table at round variable i,
 // e0[i] = 1 + r_i and ecf[i] = e0[i] * cf[i]; polys and chal receive each
 // round's compressed polynomial [c0, c2, c3] and challenge, finals the last
-// round's two entries of f and of g.  counters[k] counts round k's finished
-// blocks.  acc[k] holds round k's pair sums in kProdSlots slots of
-// [A0, A1, A2] that the blocks XOR into.  The upload before the rounds
-// carries the counters and acc[0] as zeros; round k clears acc[k + 1].
+// round's two entries of f and of g.  Round k's acc holds its polynomial in
+// kProdSlots slots of [c0, c1, c2, c3] that the blocks XOR their shares
+// into; its counters count the finished blocks in two levels (kernels.hip
+// grid_last).  The upload before the rounds carries round 0's slots and
+// counters as zeros; round k clears round k + 1's.
 struct ProdFs {
     uint32_t in_len, out_len, in_cap, n;
-    // eq scalars of the next round to run (variable i): s = the product of
-    // eq(r_j, x_j) over the rounds done, K = s e0[i] cf[i], L = s cf[i]
-    gf s, K, L;
+    // eq scalars.  Round k (variable i = n-1-k) needs s_k, the product of
+    // eq(r_j, x_j) over the rounds done, K_k = s_k e0[i] cf[i] and
+    // L_k = s_k cf[i].  Round 0 takes s = s_0, K = K_0, L = L_0 as uploaded;
+    // before round k >= 1 the region holds s = s_{k-1} and P1 = s_{k-1}
+    // ecf[i], P2 = s_{k-1} cf[i], and with t = e0[i + 1] + x_{k-1},
+    // s_k = s_{k-1} t, K_k = P1 t, L_k = P2 t.
+    gf s, K, L, P1, P2;
     gf x;      // last sampled challenge (read by the next round's fold)
     gf claim;  // sum over the cube, observed before the first round
     uint8_t out[32];
@@ -114,13 +119,18 @@ __host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_
 __host__ __device__ inline size_t prod_finals_off(uint32_t n, uint32_t in_cap) {
     return (prod_input_off(n) + in_cap + 15) & ~(size_t)15;
 }
-__host__ __device__ inline size_t prod_counters_off(uint32_t n, uint32_t in_cap) {
-    return prod_finals_off(n, in_cap) + 64;
-}
 __host__ __device__ inline size_t prod_acc_off(uint32_t n, uint32_t in_cap) {
-    return (prod_counters_off(n, in_cap) + 4 * (size_t)n + 255) & ~(size_t)255;
+    return (prod_finals_off(n, in_cap) + 64 + 255) & ~(size_t)255;
 }
-constexpr size_t kProdAccRound = 8 * 6 * (size_t)kProdSlots;  // bytes of one round's slots
+// Two-level block counters of one launch: word 0 counts the groups done,
+// word kCountLine (1 + g) the blocks of group g done; each on its own
+// 128-byte line, so the atomics of different groups do not queue behind
+// one another.
+constexpr int kCountGroups = 16;
+constexpr int kCountLine = 32;
+constexpr size_t kCountWords = (size_t)kCountLine * (1 + kCountGroups);
+constexpr size_t kProdSlotBytes = 8 * 8 * (size_t)kProdSlots;
+constexpr size_t kProdAccRound = kProdSlotBytes + 4 * kCountWords;  // bytes of one round's slots and counters
 // bytes uploaded before the rounds: everything up to and including acc[0]
 __host__ __device__ inline size_t prod_upload_bytes(uint32_t n, uint32_t in_cap) {
     return prod_acc_off(n, in_cap) + kProdAccRound;
@@ -317,6 +327,7 @@ hipError_t mul(hipStream_t st, const gf *a, const gf *b, gf *out, size_t n, bool
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters);
 hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n);
 hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters);
+hipError_t bench_keccak_pair(hipStream_t st, uint64_t *io, int iters);
 // p[i] = splitmix64 of (seed, i) for n 64-bit words
 hipError_t fill_mix(hipStream_t st, uint64_t *p, size_t n, uint64_t seed);
 

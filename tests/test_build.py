@@ -19,8 +19,11 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # bytes of scratch per lane allowed; every other kernel must use none.  The
 # three-wave build of the radix-8 pass (at most 168 VGPRs) trades a spill of
 # 180 bytes for occupancy and is used only where the autotuner measures it
-# faster.
-ALLOWED = {"k_ntt_multi3_w3": 192}
+# faster.  The product sumcheck's single-block rounds call their transcript
+# step out of line (prod_transcript_call): the scratch is that call's frame,
+# the registers it saves once per round; inlined, the step and the table
+# work share one allocation and spill instead, at the same measured speed.
+ALLOWED = {"k_ntt_multi3_w3": 192, "k_prod_tail": 224}
 
 
 @pytest.fixture(scope="module")
