@@ -697,7 +697,7 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
         return;
     }
     if (ch.output.size() > 32) fail(BFD_ERR_ARG, "transcript state too large");
-    DevBuf up, dpairs = ctx_buf(ctx, 8 * (size_t)npairs);
+    DevBuf up;
     const gf *src0 = (const gf *)polys;
     if (!on_device) {
         up = ctx_buf(ctx, 16 * N * m);
@@ -706,15 +706,13 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
     }
     const size_t NW = std::max<size_t>(1, N / 2);  // working table stride
     DevBuf W = ctx_buf(ctx, 16 * NW * m);
-    hip_check(hipMemcpyAsync(dpairs.p, pair_idx, 8 * (size_t)npairs, hipMemcpyHostToDevice, st), "copy pairs");
     // blocks per pair: enough for the device at the first round
     const unsigned max_blocks = std::max(1u, std::min(512u, 4096u / npairs));
-    DevBuf slots = ctx_buf(ctx, 48 * (size_t)launch::quad_slots(max_blocks) * npairs);
-    hip_check(hipMemsetAsync(slots.p, 0, slots.bytes, st), "clear slots");
-    // transcript image: QuadFs | scalars | polys | chal | input | n counters
+    // one upload: the transcript image (kernels.hpp QuadFs) with the scalars,
+    // the pair indices and zeroed slots and counters
     const uint32_t in_cap = (uint32_t)(((ch.input.size() > 32 ? ch.input.size() : 32) + 64 + 7) & ~(size_t)7);
-    const size_t in_off = quad_input_off(npairs, n), cnt_off = (in_off + in_cap + 7) & ~(size_t)7;
-    const size_t img = cnt_off + 4 * (size_t)n;
+    const size_t in_off = quad_input_off(npairs, n), fin_off = quad_finals_off(npairs, n, in_cap);
+    const size_t img = quad_bytes(npairs, n, in_cap, m);
     DevBuf fs = ctx_buf(ctx, img);
     ctx->pinned.ensure(img + 64);
     uint8_t *pin = (uint8_t *)ctx->pinned.p;
@@ -730,43 +728,60 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
     memcpy(pin, &h, sizeof h);
     memcpy(pin + quad_scalars_off(), scalars, 16 * (size_t)npairs);
     if (!ch.input.empty()) memcpy(pin + in_off, ch.input.data(), ch.input.size());
+    memcpy(pin + quad_pairs_off(npairs, n, in_cap, m), pair_idx, 8 * (size_t)npairs);
     hip_check(hipMemcpyAsync(fs.p, pin, img, hipMemcpyHostToDevice, st), "copy transcript");
     QuadFs *dfs = (QuadFs *)fs.p;
-    unsigned *counters = (unsigned *)(fs.as<uint8_t>() + cnt_off);
+    const uint32_t *dpairs = (const uint32_t *)(fs.as<uint8_t>() + quad_pairs_off(npairs, n, in_cap, m));
+    gf *slots = (gf *)(fs.as<uint8_t>() + quad_slots_off(npairs, n, in_cap, m));
+    unsigned *counters = (unsigned *)(fs.as<uint8_t>() + quad_counters_off(npairs, n, in_cap, m));
     const gf *x_dev = (const gf *)(fs.as<uint8_t>() + offsetof(QuadFs, x));
     const size_t tail = launch::quad_tail_max_half(m, npairs);
+    bool tail_ran = false;
     for (uint32_t k = 0; k < n; k++) {
         const size_t half = (size_t)1 << (n - 1 - k);
         const gf *src = k == 0 ? src0 : W.as<gf>();
         const size_t sstride = k == 0 ? N : NW;
         if (half <= tail) {
             ctx->timer.run(st, "k_quad_tail", [&] {
-                return launch::quad_tail(st, src, sstride, W.as<gf>(), NW, dpairs.as<uint32_t>(), npairs, m, half,
-                                         (int)k, dfs);
+                return launch::quad_tail(st, src, sstride, (gf *)(fs.as<uint8_t>() + fin_off), 1, dpairs, npairs, m,
+                                         half, (int)k, dfs);
             });
+            tail_ran = true;
             break;
         }
         ctx->timer.run(st, "k_quad_step", [&] {
-            return launch::quad_step(st, src, sstride, dpairs.as<uint32_t>(), npairs, half, (int)k, dfs,
-                                     slots.as<gf>(), max_blocks, counters + k);
+            return launch::quad_step(st, src, sstride, dpairs, npairs, half, (int)k, dfs, slots, max_blocks, counters);
         });
         ctx->timer.run(st, "k_fix_multi",
                        [&] { return launch::fix_multi_dev(st, src, sstride, W.as<gf>(), NW, m, half, x_dev); });
     }
-    const size_t rb = in_off + 32;
+    // one read-back: the region up to the finals (the finals too when the
+    // single-block rounds wrote them there)
+    const size_t rb = tail_ran ? fin_off + 16 * (size_t)m : in_off + 32;
     hip_check(hipMemcpyAsync(pin, fs.p, rb, hipMemcpyDeviceToHost, st), "copy transcript back");
-    // the m final evaluations P_p[0]
-    hip_check(hipMemcpy2DAsync(final_evals, 16, W.p, 16 * NW, 16, m, hipMemcpyDeviceToHost, st), "copy finals");
+    if (!tail_ran)  // the m final evaluations P_p[0] of the working tables
+        hip_check(hipMemcpy2DAsync(final_evals, 16, W.p, 16 * NW, 16, m, hipMemcpyDeviceToHost, st), "copy finals");
     hip_check(hipStreamSynchronize(st), "sumcheck");
     ctx->timer.collect();
     memcpy(&h, pin, sizeof h);
     if (h.n != n || h.in_len != 32 || h.out_len != 16) fail(BFD_ERR_HIP, "device transcript state is corrupt");
+    if (tail_ran) memcpy(final_evals, pin + fin_off, 16 * (size_t)m);
     store_elem(claim_out, h.claim);
     memcpy(round_polys, pin + quad_polys_off(npairs), 32 * (size_t)n);
     memcpy(challenges, pin + quad_chal_off(npairs, n), 16 * (size_t)n);
     ch.input.assign(pin + in_off, pin + in_off + 32);
     ch.output.assign(h.out, h.out + h.out_len);
-    store_elem(subclaim_out, h.sub);
+    // the subclaim through the rounds (decompress_round_poly, quadratic.rs:70-80,
+    // then SumcheckSubclaim::reduce): c1 = subclaim + c2 in characteristic 2,
+    // then Horner at x
+    gf sub = h.claim;
+    for (uint32_t t = 0; t < n; t++) {
+        const uint8_t *p = pin + quad_polys_off(npairs) + 32 * (size_t)t;
+        const gf c0 = load_elem(p), c2 = load_elem(p + 16), c1 = gf_add(sub, c2);
+        const gf x = load_elem(pin + quad_chal_off(npairs, n) + 16 * (size_t)t);
+        sub = gf_add(c0, hmul(x, gf_add(c1, hmul(x, c2))));
+    }
+    store_elem(subclaim_out, sub);
 }
 
 void quadratic_args(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys, uint32_t npairs, const uint8_t *scalars,

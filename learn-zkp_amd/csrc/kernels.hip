@@ -1205,23 +1205,21 @@ __global__ __launch_bounds__(CB) void k_quad_sums(const gf *__restrict__ P, size
 
 // One round's transcript step of a Quadratic sumcheck, run by one whole wave
 // (prove_sumcheck_round, sumcheck/src/lib.rs:81-103): observe [claim,] c0, c2
-// (the compressed polynomial, quadratic.rs:62-64), sample x and advance the
-// subclaim with c1 = subclaim + c2 (characteristic 2, quadratic.rs:70-80).
-// In round 0 the subclaim is the claim sum s (A0 + A1), observed first
+// (the compressed polynomial, quadratic.rs:62-64) and sample x.  In round 0
+// the claim, the sum s (A0 + A1) over the pairs, is observed first
 // (sumcheck/src/lib.rs:67).  With have_prev the challenger's input is the
 // previous round's 32-byte digest prev; otherwise the input bytes come from
 // the region (the call's first round, after whatever the caller observed;
-// lane 0 appends to them there).  Every lane gets the results.
+// lane 0 appends to them there).  Every lane gets the results.  The
+// subclaim (c1 = subclaim + c2 in characteristic 2, quadratic.rs:70-80) is
+// carried on the host from the polynomials and challenges.
 struct QuadRound {
-    gf x, sub;
+    gf x;
     uint64_t d[4];
 };
-__device__ QuadRound quad_transcript(QuadFs *st, int k, gf c0, gf c2, gf both, gf sub_prev, bool have_prev,
-                                     Digest prev) {
+__device__ QuadRound quad_observe(QuadFs *st, int k, gf c0, gf c2, gf both, bool have_prev, Digest prev) {
     const int lane = threadIdx.x & 63;
     uint8_t *in = (uint8_t *)st + quad_input_off(st->npairs, st->n);
-    const gf sub = k == 0 ? both : sub_prev;
-    const gf c1 = gf_add(sub, c2);
     QuadRound R;
     if (have_prev) {
         const uint64_t m[8] = {prev.w[0], prev.w[1], prev.w[2], prev.w[3], c0.lo, c0.hi, c2.lo, c2.hi};
@@ -1249,13 +1247,12 @@ __device__ QuadRound quad_transcript(QuadFs *st, int k, gf c0, gf c2, gf both, g
     }
     // the challenge is the last 16 digest bytes, last byte first
     R.x = gf{__builtin_bswap64(R.d[3]), __builtin_bswap64(R.d[2])};
-    R.sub = gf_add(c0, gf_mul_reg(R.x, gf_add(c1, gf_mul_reg(R.x, c2))));
     return R;
 }
 
-// Round k's outputs into the region (one lane): polynomial, challenge,
-// claim (round 0), subclaim and the challenger state after the sample
-// (input = output = the digest, 16 output bytes left unread).
+// Round k's outputs into the region (one lane): polynomial, challenge, claim
+// (round 0) and the challenger state after the sample (input = output = the
+// digest, 16 output bytes left unread).
 __device__ __forceinline__ void quad_store(QuadFs *st, int k, gf c0, gf c2, gf both, const QuadRound &R) {
     uint8_t *base = (uint8_t *)st;
     const uint32_t n = st->n, np = st->npairs;
@@ -1270,7 +1267,6 @@ __device__ __forceinline__ void quad_store(QuadFs *st, int k, gf c0, gf c2, gf b
     st->in_len = 32;
     st->out_len = 16;
     if (k == 0) st->claim = both;
-    st->sub = R.sub;
     st->x = R.x;
     polys[2 * k] = c0;
     polys[2 * k + 1] = c2;
@@ -1289,7 +1285,7 @@ __device__ void quad_scale(const gf *sc, uint32_t base, uint32_t cnt, gf P0, gf 
         const int sl = (int)(src & 63);
         const gf B0 = lane_gf(P0, sl), B1 = lane_gf(P1, sl), B2 = lane_gf(P2, sl);
         if (lane < 63 && src < cnt) {
-            const gf v = comp == 0 ? B0 : (comp == 1 ? B2 : gf_add(B0, B1));
+            const gf v = gf_sel(comp == 0, B0, gf_sel(comp == 1, B2, gf_add(B0, B1)));
             const gf p = gf_mul_reg(v, gf_ld(sc + base + src));
             if (comp == 0) c0 = gf_add(c0, p);
             else if (comp == 1) c2 = gf_add(c2, p);
@@ -1298,67 +1294,20 @@ __device__ void quad_scale(const gf *sc, uint32_t base, uint32_t cnt, gf P0, gf 
     }
 }
 
-// The transcript step of round k in the block that finishes the round, on
-// its first wave: per pair the three sums over the round's S slots (S a
-// power of two, at most 16; lanes take the slots of 64 / S pairs at a time,
-// all loads of a pass in flight together, then a butterfly over the S lanes
-// of each pair, and pair c + L of a chunk of 64 gathered on lane L), the
-// slots cleared for the next round, the pair scalars applied (quad_scale),
-// then quad_transcript.
-__device__ void quad_fs_step(QuadFs *st, int k, gf *slots, unsigned S) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t n = st->n, np = st->npairs;
-    const gf *sc = (const gf *)((const uint8_t *)st + quad_scalars_off());
-    const unsigned per = 64 / S, u = lane & (S - 1), tl = lane / S;
-    gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
-    for (uint32_t c = 0; c < np; c += 64) {
-        gf P0 = gf_zero(), P1 = gf_zero(), P2 = gf_zero();  // pair c + lane
-        for (uint32_t t0 = c; t0 < np && t0 < c + 64; t0 += per) {
-            const uint32_t t = t0 + tl;
-            gf A0 = gf_zero(), A1 = gf_zero(), A2 = gf_zero();
-            if (t < np) {
-                gf *q = slots + 3 * ((size_t)S * t + u);
-                A0 = gf_ld_published(q);
-                A1 = gf_ld_published(q + 1);
-                A2 = gf_ld_published(q + 2);
-                // the next round XORs into cleared slots (it starts after this launch)
-                gf_st(q, gf_zero());
-                gf_st(q + 1, gf_zero());
-                gf_st(q + 2, gf_zero());
-            }
-            for (unsigned o = 1; o < S; o <<= 1) {
-                A0 = gf_add(A0, gf{(uint64_t)__shfl_xor((long long)A0.lo, o), (uint64_t)__shfl_xor((long long)A0.hi, o)});
-                A1 = gf_add(A1, gf{(uint64_t)__shfl_xor((long long)A1.lo, o), (uint64_t)__shfl_xor((long long)A1.hi, o)});
-                A2 = gf_add(A2, gf{(uint64_t)__shfl_xor((long long)A2.lo, o), (uint64_t)__shfl_xor((long long)A2.hi, o)});
-            }
-            // pair c + L was reduced in pass (L / per) on lane (L % per) * S
-            const int src = (int)((lane % per) * S);
-            const gf B0 = lane_gf(A0, src), B1 = lane_gf(A1, src), B2 = lane_gf(A2, src);
-            if ((uint32_t)lane / per == (t0 - c) / per) {
-                P0 = B0;
-                P1 = B1;
-                P2 = B2;
-            }
-        }
-        quad_scale(sc, c, np - c < 64 ? np - c : 64, P0, P1, P2, c0, c2, both);
-    }
-    c0 = wave_xor(c0);
-    c2 = wave_xor(c2);
-    both = wave_xor(both);
-    Digest prev;
-    const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)st + quad_input_off(np, n));
-#pragma unroll
-    for (int w = 0; w < 4; w++) prev.w[w] = k > 0 ? in64[w] : 0;
-    const QuadRound R = quad_transcript(st, k, c0, c2, both, st->sub, k > 0, prev);
-    if (lane == 0) quad_store(st, k, c0, c2, both, R);
-}
-
-// k_quad_sums with the round's transcript step in the last block to finish
+// Pair sums with the round's transcript step in the block that finishes
+// last.  Pair blockIdx.y sums A0 = sum f0 g0, A1 = sum f1 g1 and
+// A2 = sum (f0+f1)(g0+g1) over its share of the halves; the block publishes
+// its share of the round's compressed polynomial, s A0, s A2 and s (A0 + A1)
+// with the pair's scalar s (quadratic.rs:100-122), XORed with device-scope
+// atomics into slot (block mod kQuadSlots); the last block (two-level count,
+// grid_last) adds the slots, clears them and the counters for the next round
+// and runs the transcript step.
 __global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size_t stride,
                                                   const uint32_t *__restrict__ pairs, size_t h, int k, QuadFs *fs,
-                                                  gf *slots, unsigned S, unsigned *counter) {
+                                                  gf *slots, unsigned *counter) {
     const size_t a = pairs[2 * blockIdx.y], b = pairs[2 * blockIdx.y + 1];
     const gf *f = P + a * stride, *g = P + b * stride;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
     for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < h; j += (size_t)gridDim.x * blockDim.x) {
         const gf f0 = gf_ld(f + j), f1 = gf_ld(f + j + h), g0 = gf_ld(g + j), g1 = gf_ld(g + j + h);
@@ -1366,33 +1315,72 @@ __global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size
         a1 = gf_add(a1, gf_mul_reg(f1, g1));
         a2 = gf_add(a2, gf_mul_reg(gf_add(f0, f1), gf_add(g0, g1)));
     }
-    // block sums XORed (device-scope atomics the block waits for) into slot
-    // blockIdx.x mod S of the pair; counted after they are performed
     __shared__ gf red[3][CB / 64];
-    __shared__ unsigned done[3];
+    __shared__ unsigned last;
     a0 = wave_xor(a0);
     a1 = wave_xor(a1);
     a2 = wave_xor(a2);
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = a0;
-        red[1][threadIdx.x >> 6] = a1;
-        red[2][threadIdx.x >> 6] = a2;
+    if (lane == 0) {
+        red[0][wave] = a0;
+        red[1][wave] = a1;
+        red[2][wave] = a2;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
-        gf v = gf_zero();
-        for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[threadIdx.x][w]);
-        unsigned long long *p =
-            (unsigned long long *)(slots + 3 * ((size_t)S * blockIdx.y + blockIdx.x % S) + threadIdx.x);
-        const unsigned long long o0 =
-            __hip_atomic_fetch_xor(p, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long o1 =
-            __hip_atomic_fetch_xor(p + 1, (unsigned long long)v.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        done[threadIdx.x] = (unsigned)(o0 ^ o1) | 1u;
+    const unsigned nblocks = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t np = fs->npairs;
+    if (wave == 0) {
+        gf A[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            A[c] = gf_zero();
+            for (int w = 0; w < CB / 64; w++) A[c] = gf_add(A[c], red[c][w]);
+        }
+        // lanes 0, 1, 2: s A0, s A2, s (A0 + A1)
+        const gf sp = gf_ld((const gf *)((const uint8_t *)fs + quad_scalars_off()) + blockIdx.y);
+        const gf v = gf_mul_reg(sp, gf_sel(lane == 0, A[0], gf_sel(lane == 1, A[2], gf_add(A[0], A[1]))));
+        unsigned done = 1u;
+        if (lane < 3) {
+            unsigned long long *slot = (unsigned long long *)(slots + 3 * (bid % kQuadSlots) + lane);
+            const unsigned long long o0 =
+                __hip_atomic_fetch_xor(slot, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long o1 =
+                __hip_atomic_fetch_xor(slot + 1, (unsigned long long)v.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            done = (unsigned)(o0 ^ o1) | 1u;
+        }
+        // 1, after lanes 0..2's atomics returned
+        const unsigned inc = done & (unsigned)__shfl((int)done, 1) & (unsigned)__shfl((int)done, 2) & 1u;
+        if (lane == 0) last = grid_last(counter, nblocks, bid, inc);
     }
     __syncthreads();
-    if (!block_counted(counter, gridDim.x * gridDim.y, done, 3) || threadIdx.x >= 64) return;
-    quad_fs_step(fs, k, slots, S);
+    if (!last || wave != 0) return;
+    // lane l adds slots l, l + 64, ... (those the grid used), all loads
+    // issued before the first use, clears them, then a wave reduction
+    const unsigned used = nblocks < (unsigned)kQuadSlots ? nblocks : (unsigned)kQuadSlots;
+    gf c0 = gf_zero(), c2 = gf_zero(), both = gf_zero();
+#pragma unroll
+    for (int t = 0; t < kQuadSlots / 64; t++) {
+        const unsigned sl = (unsigned)lane + 64 * t;
+        if (sl < used) {
+            gf *q = slots + 3 * sl;
+            c0 = gf_add(c0, gf_ld_published(q));
+            c2 = gf_add(c2, gf_ld_published(q + 1));
+            both = gf_add(both, gf_ld_published(q + 2));
+            gf_st(q, gf_zero());
+            gf_st(q + 1, gf_zero());
+            gf_st(q + 2, gf_zero());
+        }
+    }
+    c0 = wave_xor(c0);
+    c2 = wave_xor(c2);
+    both = wave_xor(both);
+    // every block has counted: the counters start the next round at zero
+    for (int w = lane; w < (int)kCountWords; w += 64) counter[w] = 0u;
+    Digest prev;
+    const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)fs + quad_input_off(np, fs->n));
+#pragma unroll
+    for (int w = 0; w < 4; w++) prev.w[w] = k > 0 ? in64[w] : 0;
+    const QuadRound R = quad_observe(fs, k, c0, c2, both, k > 0, prev);
+    if (lane == 0) quad_store(fs, k, c0, c2, both, R);
 }
 
 // fix_last_var of table blockIdx.y with the challenge at *xp: 2 half entries
@@ -1432,14 +1420,13 @@ __global__ __launch_bounds__(kQuadTailThreads) void k_quad_tail(const gf *src, s
                                                                 uint32_t m, size_t h0, int k0, QuadFs *fs) {
     __shared__ gf F[kQuadTailElems];
     __shared__ gf S3[3][kQuadTailPairs];
-    __shared__ gf xs, subs;
+    __shared__ gf xs;
     __shared__ uint64_t dig[4];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t n = fs->n, np = fs->npairs;
     const size_t w = 2 * h0;
     for (size_t e = tid; e < (size_t)m * w; e += kQuadTailThreads) F[e] = gf_ld(src + (e / w) * src_stride + e % w);
     if (tid == 0) {
-        subs = fs->sub;
         const uint64_t *in64 = reinterpret_cast<const uint64_t *>((const uint8_t *)fs + quad_input_off(np, n));
         for (int q = 0; q < 4; q++) dig[q] = in64[q];
     }
@@ -1478,7 +1465,7 @@ __global__ __launch_bounds__(kQuadTailThreads) void k_quad_tail(const gf *src, s
             c2 = wave_xor(c2);
             both = wave_xor(both);
             const Digest prev{{dig[0], dig[1], dig[2], dig[3]}};
-            const QuadRound R = quad_transcript(fs, k, c0, c2, both, subs, k > 0, prev);
+            const QuadRound R = quad_observe(fs, k, c0, c2, both, k > 0, prev);
             if (lane == 0) {
                 uint8_t *base = (uint8_t *)fs;
                 gf *polys = (gf *)(base + quad_polys_off(np)), *chal = (gf *)(base + quad_chal_off(np, n));
@@ -1491,7 +1478,6 @@ __global__ __launch_bounds__(kQuadTailThreads) void k_quad_tail(const gf *src, s
                 dig[2] = R.d[2];
                 dig[3] = R.d[3];
                 xs = R.x;
-                subs = R.sub;
             }
         }
         __syncthreads();
@@ -1513,7 +1499,6 @@ __global__ __launch_bounds__(kQuadTailThreads) void k_quad_tail(const gf *src, s
         }
         fs->in_len = 32;
         fs->out_len = 16;
-        fs->sub = subs;
         fs->x = xs;
     }
 }
@@ -2446,13 +2431,8 @@ hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t 
                      int k, QuadFs *fs, gf *slots, unsigned max_blocks, unsigned *counter) {
     if (npairs == 0 || npairs > 65535 || max_blocks == 0) return hipErrorInvalidValue;
     const unsigned gx = grid_for(h, CB, max_blocks);
-    k_quad_step<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, k, fs, slots, quad_slots(max_blocks), counter);
+    k_quad_step<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, k, fs, slots, counter);
     return hipGetLastError();
-}
-unsigned quad_slots(unsigned max_blocks) {
-    unsigned s = 1;
-    while (s * 2 <= max_blocks && s * 2 <= 16) s *= 2;
-    return s;
 }
 size_t quad_tail_max_half(uint32_t m, uint32_t npairs) {
     if (npairs == 0 || npairs > kQuadTailPairs || m == 0) return 0;

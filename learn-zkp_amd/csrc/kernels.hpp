@@ -150,18 +150,22 @@ struct EqFactors {
 
 // Device-resident transcript of a Quadratic sumcheck driven by
 // prove_sumcheck (quadratic.rs:84-136, sumcheck/src/lib.rs:56-103): the
-// challenger's byte buffers and the running subclaim.  One region laid out as
-//   QuadFs | gf scalars[npairs] | gf polys[2n] | gf chal[n] | uint8_t input[in_cap]
+// challenger's byte buffers.  One region laid out as
+//   QuadFs | gf scalars[npairs] | gf polys[2n] | gf chal[n] |
+//   uint8_t input[in_cap] | gf finals[m] | uint32_t pairs[2 npairs] |
+//   gf slots[kQuadSlots][3] | uint32_t counters[kCountWords]
 // polys and chal receive each round's compressed polynomial [c0, c2] and
-// challenge.
+// challenge, finals every table's last entry.  The round kernels XOR their
+// shares of [c0, c2, claim] into the slots and count their blocks in the
+// counters (both zero at upload; the last block of a round clears them).
 struct QuadFs {
     uint32_t in_len, out_len, in_cap, n;
     uint32_t npairs, pad_[3];
-    gf sub;    // subclaim
     gf x;      // last sampled challenge (read by the round's fix)
     gf claim;  // sum over the cube, observed before the first round
     uint8_t out[32];
 };
+constexpr int kQuadSlots = 256;
 __host__ __device__ inline size_t quad_scalars_off() { return sizeof(QuadFs); }
 __host__ __device__ inline size_t quad_polys_off(uint32_t npairs) { return sizeof(QuadFs) + 16 * (size_t)npairs; }
 __host__ __device__ inline size_t quad_chal_off(uint32_t npairs, uint32_t n) {
@@ -169,6 +173,21 @@ __host__ __device__ inline size_t quad_chal_off(uint32_t npairs, uint32_t n) {
 }
 __host__ __device__ inline size_t quad_input_off(uint32_t npairs, uint32_t n) {
     return quad_chal_off(npairs, n) + 16 * (size_t)n;
+}
+__host__ __device__ inline size_t quad_finals_off(uint32_t npairs, uint32_t n, uint32_t in_cap) {
+    return (quad_input_off(npairs, n) + in_cap + 15) & ~(size_t)15;
+}
+__host__ __device__ inline size_t quad_pairs_off(uint32_t npairs, uint32_t n, uint32_t in_cap, uint32_t m) {
+    return quad_finals_off(npairs, n, in_cap) + 16 * (size_t)m;
+}
+__host__ __device__ inline size_t quad_slots_off(uint32_t npairs, uint32_t n, uint32_t in_cap, uint32_t m) {
+    return (quad_pairs_off(npairs, n, in_cap, m) + 8 * (size_t)npairs + 255) & ~(size_t)255;
+}
+__host__ __device__ inline size_t quad_counters_off(uint32_t npairs, uint32_t n, uint32_t in_cap, uint32_t m) {
+    return quad_slots_off(npairs, n, in_cap, m) + 48 * (size_t)kQuadSlots;
+}
+__host__ __device__ inline size_t quad_bytes(uint32_t npairs, uint32_t n, uint32_t in_cap, uint32_t m) {
+    return quad_counters_off(npairs, n, in_cap, m) + 4 * kCountWords;
 }
 
 enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPairDot, Nodes, Subtree, Fix, FoldFix,
@@ -297,16 +316,13 @@ hipError_t fix_multi(hipStream_t st, gf *P, size_t stride, uint32_t m, size_t ha
 hipError_t fix_multi_dev(hipStream_t st, const gf *src, size_t src_stride, gf *dst, size_t dst_stride, uint32_t m,
                          size_t half, const gf *x);
 // Round k of a Quadratic sumcheck with its transcript step on the device:
-// the three sums of every pair over halves of h entries (as quad_sums), XORed
-// by the blocks into quad_slots(max_blocks) slots of three elements per pair
-// at slots (zero at launch; the last block reads and clears them), and in
-// the block that finishes last (counted in *counter, zero at launch) the
-// round's Fiat-Shamir step into the QuadFs region.
+// the three sums of every pair over halves of h entries (as quad_sums), each
+// block's share of the round polynomial XORed into the region's slots, and
+// in the block that finishes last (two-level count in the region's counters)
+// the round's Fiat-Shamir step into the QuadFs region.  max_blocks caps the
+// blocks per pair.
 hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t *pairs, uint32_t npairs, size_t h,
                      int k, QuadFs *fs, gf *slots, unsigned max_blocks, unsigned *counter);
-// accumulation slots per pair for a grid of max_blocks blocks per pair: a
-// power of two, at most 16
-unsigned quad_slots(unsigned max_blocks);
 // The Quadratic sumcheck's rounds from k0 on (round k0 has h0 pairs of
 // entries per table, a power of two) in one single-block launch with the m
 // tables (2 * h0 entries each at src, tables src_stride apart) in LDS; each
