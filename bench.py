@@ -68,6 +68,16 @@ def valu_floor_s(K: int, G_reg: int, G_tab: int, rates) -> float:
     return K / rates[0] + G_reg / rates[1] + G_tab / rates[2]
 
 
+def round_floor_s(rounds, rates) -> float:
+    """Lower bound for a sumcheck on this device whose transcript steps run on
+    it: a round's polynomial needs the previous round's challenge, and its
+    challenge one Keccak-f permutation of the transcript, so every round takes
+    at least the longer of its multiplications at the isolated rates and one
+    permutation at the single-wave latency of the lane-pair form the
+    transcript uses (rates[3]).  rounds = [(G_reg, G_tab)] per round."""
+    return sum(max(gr / rates[1] + gt / rates[2], rates[3]) for gr, gt in rounds)
+
+
 SUBTREE_FROM = 1 << 17
 
 
@@ -250,13 +260,22 @@ def product_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
     # the eq factor B enters once per block, a few thousand products.  The
     # folds of rounds with more than 2^18 pairs (two or more per thread) go
     # through the challenge's window table.
-    G = 5 * (1 << (n - 1)) + sum(9 * (1 << (n - 1 - k)) for k in range(1, n))
-    Gt = sum(4 * (1 << (n - 1 - k)) for k in range(1, n) if (1 << (n - 1 - k)) > 1 << 18)
+    per_round = [(5 * (1 << (n - 1)), 0)]
+    for k in range(1, n):
+        h = 1 << (n - 1 - k)
+        per_round.append((5 * h, 4 * h) if h > 1 << 18 else (9 * h, 0))
+    G = sum(a + b for a, b in per_round)
+    Gt = sum(b for _, b in per_round)
     pos = {"gf_muls": G, "gf_muls_by_table": Gt, "keccak_perms": n}
     if rates:
         pos["valu_floor_ms"] = 1e3 * valu_floor_s(n, G - Gt, Gt, rates)
         pos["valu_frac"] = pos["valu_floor_ms"] / (1e3 * t)
-        pos["note"] = "the floor counts multiplications at the isolated rate; the n transcript steps are serial"
+        pos["transcript_perm_latency_us"] = 1e6 * rates[3]
+        pos["round_floor_ms"] = 1e3 * round_floor_s(per_round, rates)
+        pos["round_floor_frac"] = pos["round_floor_ms"] / (1e3 * t)
+        pos["note"] = ("valu_floor counts the multiplications at the isolated rates only; round_floor also gives "
+                       "every round at least one transcript permutation at its single-wave latency, since the "
+                       "rounds are sequential")
     return {"workload": "f g eq(r, .), 22 variables, tables resident, fresh challenger", "ms": 1e3 * t,
             "evals_per_s": (1 << n) / t, "position": pos}
 
@@ -313,12 +332,19 @@ def quadratic_sumcheck(bfd, ctx, torch, reps: int, rates=None) -> dict:
     # three products per pair and pair position, one per table entry for the
     # fix; the fixes of rounds of 2^17 pairs or more (two or more per thread)
     # go through the challenge's window table
-    G = sum(3 * npairs * (1 << (n - 1 - k)) + m * (1 << (n - 1 - k)) for k in range(n))
-    Gt = sum(m * (1 << (n - 1 - k)) for k in range(n) if (1 << (n - 1 - k)) >= 1 << 17)
+    per_round = []
+    for k in range(n):
+        h = 1 << (n - 1 - k)
+        per_round.append((3 * npairs * h, m * h) if h >= 1 << 17 else (3 * npairs * h + m * h, 0))
+    G = sum(a + b for a, b in per_round)
+    Gt = sum(b for _, b in per_round)
     pos = {"gf_muls": G, "gf_muls_by_table": Gt, "keccak_perms": n}
     if rates:
         pos["valu_floor_ms"] = 1e3 * valu_floor_s(n, G - Gt, Gt, rates)
         pos["valu_frac"] = pos["valu_floor_ms"] / (1e3 * t)
+        pos["transcript_perm_latency_us"] = 1e6 * rates[3]
+        pos["round_floor_ms"] = 1e3 * round_floor_s(per_round, rates)
+        pos["round_floor_frac"] = pos["round_floor_ms"] / (1e3 * t)
     return {"workload": f"{m} tables of 2^{n}, {npairs} pairs, tables resident, fresh challenger", "ms": 1e3 * t,
             "table_evals_per_s": m * (1 << n) / t, "position": pos}
 
@@ -753,7 +779,11 @@ def main() -> None:
     r_g = (1 << 22) * 64 / (ctx.microbench(1, 1 << 22, 64) * 1e-3)
     ctx.microbench(4, 1 << 22, 2)
     r_t = (1 << 22) * 64 / (ctx.microbench(4, 1 << 22, 64) * 1e-3)
-    rates = (r_k, r_g, r_t)
+    # single-wave latency of one lane-pair Keccak-f (the transcript step's
+    # form): the difference of two chains on one wave
+    ctx.microbench(5, 64, 4)
+    tau_k = (ctx.microbench(5, 64, 1100) - ctx.microbench(5, 64, 100)) * 1e-3 / 1000
+    rates = (r_k, r_g, r_t, tau_k)
     valu = None
     if perms_per_step:
         rate = perms_per_step * args.steps / (kms * 1e-3)
