@@ -574,17 +574,27 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
         c[i] = cf[i];
         c[n + i] = e0;
         c[2 * n + i] = hmul(e0, cf[i]);
+        // halved eq table entries whose indices differ in bit i: ratio r_i / (1 + r_i)
+        c[3 * n + i] = (int)i < hn ? hmul(r[i], inv[i]) : gf_zero();
     }
     if (!ch.input.empty()) memcpy(hreg + in_off, ch.input.data(), ch.input.size());
     hip_check(hipMemcpyAsync(buf.p, pin, fb + upl, hipMemcpyHostToDevice, st), "copy transcript");
     const EqFactors eq{buf.as<gf>(), buf.as<gf>() + A.size(), lo};
     ProdFs *dfs = (ProdFs *)(buf.as<uint8_t>() + fb);
     uint32_t k = 0;
+    bool look = false;  // a lookahead round has run
     for (; k < n; k++) {
         const size_t half = (size_t)1 << (n - 1 - k);
         if (half <= launch::prod_tail_max_pairs()) break;
         const gf *fsrc = k <= 1 ? fs0 : wf, *gsrc = k <= 1 ? gs0 : wg;
         gf *fdst = k == 0 ? (gf *)fs0 : wf, *gdst = k == 0 ? (gf *)gs0 : wg;  // round 0 writes nothing
+        if (k >= 1 && half <= launch::prod_look_max_pairs()) {
+            ctx->timer.run(st, "k_prod_look", [&] {
+                return launch::prod_look(st, fsrc, gsrc, fdst, gdst, eq, half, (int)k, dfs, !look);
+            });
+            look = true;
+            continue;
+        }
         ctx->timer.run(st, "k_prod_round", [&] {
             return launch::prod_step(st, fsrc, gsrc, fdst, gdst, eq, half, (int)k, dfs, ctx->cus);
         });

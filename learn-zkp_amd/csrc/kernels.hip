@@ -531,6 +531,10 @@ __device__ __forceinline__ gf gf_ld_published(const gf *p) {
 
 // lane src's value of v, on every lane of the wave
 __device__ __forceinline__ gf lane_gf(gf v, int src) { return gf{(uint64_t)__shfl((long long)v.lo, src), (uint64_t)__shfl((long long)v.hi, src)}; }
+// v from lane (this lane ^ off)
+__device__ __forceinline__ gf gf_shfl_xor(gf v, int off) {
+    return gf{(uint64_t)__shfl_xor((long long)v.lo, off), (uint64_t)__shfl_xor((long long)v.hi, off)};
+}
 // c ? a : b word by word (a select of whole elements on a lane-dependent
 // condition can otherwise be lowered through a stack copy)
 __device__ __forceinline__ gf gf_sel(bool c, gf a, gf b) { return gf{c ? a.lo : b.lo, c ? a.hi : b.hi}; }
@@ -692,20 +696,14 @@ __device__ __forceinline__ void prod_fold_sums(const gf *fsrc, const gf *gsrc, g
 
 // Round k of the product sumcheck (see launch::prod_step).  MODE 0: round 0,
 // sums over the caller's tables; 1: fold the caller's tables into the working
-// ones, then the sums; 2: fold the working tables in place, then the sums;
-// 3 (rounds of few pairs, k >= 1): four lanes per pair, kProdSplit pairs per
-// block, so a lane's chain is three products instead of nine: its entry's
-// fold (f0, f1, g0, g1), then t f0 and t f1 on the f lanes, then the three
-// pair products, with the block's eq-scalar products on idle lanes of
-// wave 0.  Block b takes pairs [b C, (b + 1) C), C = half / gridDim.x,
+// ones, then the sums; 2: fold the working tables in place, then the sums.
+// Block b takes pairs [b C, (b + 1) C), C = half / gridDim.x,
 // which share one entry of the eq factor B: pair j is weighted by
 // A[j mod 2^lo] alone and the block's sums by that B entry.  A block
 // publishes its share of the round polynomial, (K B, L B) times its sums
 // (prod_coeffs), so the block that finishes last only adds the shares
 // before its transcript step; meanwhile its second wave forms the eq
 // scalars for the next round (P1, P2, ProdFs).
-constexpr int kProdSplit = CB / 4;
-constexpr size_t kProdSplitMax = (size_t)1 << 15;  // rounds of at most this many pairs use MODE 3
 template <int MODE>
 __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq,
                                                    size_t half, int k, ProdFs *fs) {
@@ -729,82 +727,42 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
     const size_t C = half / gridDim.x, j0 = blockIdx.x * C, j1 = j0 + C;
     const size_t amask = ((size_t)1 << eq.lo) - 1;
     const gf Bv = gf_ld(eq.B + (j0 >> eq.lo));
-    if constexpr (MODE == 3) {
-        // pair j0 + tid / 4; lane u of its quad: table u >> 1, entry j + (u & 1) half
-        const int u = lane & 3, t = u >> 1, qb = lane & ~3;
-        const size_t j = j0 + (size_t)(tid >> 2), e = j + (size_t)(u & 1) * half;
-        const gf *src = t ? gsrc : fsrc;
-        gf *dst = t ? gdst : fdst;
-        const gf x = fs->x;
-        const gf lo = gf_ld(src + e);
-        const gf V = gf_add(lo, gf_mul_reg(gf_add(gf_ld(src + e + 2 * half), lo), x));
-        gf_st(dst + e, V);
-        // t f0, t f1 on the f lanes; wave 0's lanes 2, 3, 6: s_k, K_k, L_k
-        const bool sl = wave == 0 && (lane == 2 || lane == 3 || lane == 6);
-        const gf sv = gf_sel(lane == 2, fs->s, gf_sel(lane == 3, fs->P1, fs->P2));
-        const gf M = gf_mul_reg(gf_sel(sl, sv, V), gf_sel(sl, gf_add(gf_ld(e0s + (nv - k)), x), gf_ld(eq.A + (j & amask))));
-        const gf X = gf_sel(t == 0, M, V);
-        const gf Ks = lane_gf(M, 3), Ls = lane_gf(M, 6);
-        if (wave == 0) {
-            const gf sk = lane_gf(M, 2);
-            if (lane == 0) sc[2] = sk;
-        }
-        // the pair products on lanes 0..2 of each quad; wave 0's lanes 3, 7: K_k B, L_k B
-        const gf X0 = lane_gf(X, qb), X1 = lane_gf(X, qb + 1), X2 = lane_gf(X, qb + 2), X3 = lane_gf(X, qb + 3);
-        const bool bl = wave == 0 && (lane == 3 || lane == 7);
-        const gf o1 = gf_sel(bl, gf_sel(lane == 3, Ks, Ls), gf_sel(u == 0, X0, gf_sel(u == 1, X1, gf_add(X0, X1))));
-        const gf o2 = gf_sel(bl, Bv, gf_sel(u == 0, X2, gf_sel(u == 1, X3, gf_add(X2, X3))));
-        const gf p = gf_mul_reg(o1, o2);
-        if (wave == 0) {
-            const gf kb = lane_gf(p, 3), lb = lane_gf(p, 7);
-            if (lane == 0) {
-                sc[0] = kb;
-                sc[1] = lb;
-            }
-        }
-        gf a = gf_sel(u < 3, p, gf_zero());
-#pragma unroll
-        for (int off = 4; off < 64; off <<= 1) a = gf_add(a, gf{(uint64_t)__shfl_xor((long long)a.lo, off),
-                                                                (uint64_t)__shfl_xor((long long)a.hi, off)});
-        if (lane < 3) red[lane][wave] = a;
-    } else {
-        // the eq scalars on the second wave, before its pairs: s_k, K_k, L_k,
-        // then K_k B and L_k B
-        if (wave == 1) {
-            gf s, K, L;
-            prod_scalars(fs, e0s, k, s, K, L);
-            const gf p = gf_mul_reg(lane == 0 ? K : L, Bv);
-            const gf kb = lane_gf(p, 0), lb = lane_gf(p, 1);
-            if (lane == 0) {
-                sc[0] = kb;
-                sc[1] = lb;
-                sc[2] = s;
-            }
-        }
-        gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
-        if constexpr (MODE == 0) {
-            for (size_t j = j0 + tid; j < j1; j += CB)
-                prod_terms(gf_ld(fsrc + j), gf_ld(fsrc + j + half), gf_ld(gsrc + j), gf_ld(gsrc + j + half),
-                           gf_ld(eq.A + (j & amask)), a0, a1, a2);
-        } else if (C >= 2 * (size_t)CB) {
-            // two pairs or more per thread: the folds multiply by x through a
-            // window table
-            __shared__ GfTab T;
-            __shared__ gf u[128];
-            gf_tab_build(T, u, fs->x);
-            prod_fold_sums<true>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, &T, gf_pre{}, a0, a1, a2);
-        } else {
-            prod_fold_sums<false>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, nullptr, gf_prepare(fs->x), a0,
-                                  a1, a2);
-        }
-        a0 = wave_xor(a0);
-        a1 = wave_xor(a1);
-        a2 = wave_xor(a2);
+    // the eq scalars on the second wave, before its pairs: s_k, K_k, L_k,
+    // then K_k B and L_k B
+    if (wave == 1) {
+        gf s, K, L;
+        prod_scalars(fs, e0s, k, s, K, L);
+        const gf p = gf_mul_reg(gf_sel(lane == 0, K, L), Bv);
+        const gf kb = lane_gf(p, 0), lb = lane_gf(p, 1);
         if (lane == 0) {
-            red[0][wave] = a0;
-            red[1][wave] = a1;
-            red[2][wave] = a2;
+            sc[0] = kb;
+            sc[1] = lb;
+            sc[2] = s;
         }
+    }
+    gf a0 = gf_zero(), a1 = gf_zero(), a2 = gf_zero();
+    if constexpr (MODE == 0) {
+        for (size_t j = j0 + tid; j < j1; j += CB)
+            prod_terms(gf_ld(fsrc + j), gf_ld(fsrc + j + half), gf_ld(gsrc + j), gf_ld(gsrc + j + half),
+                       gf_ld(eq.A + (j & amask)), a0, a1, a2);
+    } else if (C >= 2 * (size_t)CB) {
+        // two pairs or more per thread: the folds multiply by x through a
+        // window table
+        __shared__ GfTab T;
+        __shared__ gf u[128];
+        gf_tab_build(T, u, fs->x);
+        prod_fold_sums<true>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, &T, gf_pre{}, a0, a1, a2);
+    } else {
+        prod_fold_sums<false>(fsrc, gsrc, fdst, gdst, eq, half, j0, j1, amask, nullptr, gf_prepare(fs->x), a0,
+                              a1, a2);
+    }
+    a0 = wave_xor(a0);
+    a1 = wave_xor(a1);
+    a2 = wave_xor(a2);
+    if (lane == 0) {
+        red[0][wave] = a0;
+        red[1][wave] = a1;
+        red[2][wave] = a2;
     }
     __syncthreads();
     const uint32_t n = nv;
@@ -894,6 +852,208 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
     }
 }
 
+// Rounds k >= 1 of at most kProdLookMax pairs with one round of lookahead
+// (see launch::prod_look).  With q the pairs of round k, eight lanes take pair
+// j < q/2 of round k+1: lane u folds entry j + (u & 3) q/2 of f (u < 4) or g
+// into round k's tables, then the f lanes multiply by A[j] (the block's B
+// entry enters its sums once), giving a0, a1, b0, b1 and c0, c1, d0, d1 as in
+// k_prod_tail, and the nine Karatsuba products p0..p8 of round k+1's
+// quadratics (p0 = a0 c0, p1 = b0 d0, p2 = (a0 + b0)(c0 + d0), p3..p5 the
+// same with index 1, p6..p8 for the sums of both).  The same products are
+// round k's pair sums: pair j has A0 = p0, A1 = p2, A2 = p1 (a0 + b0 = T V2
+// and c0 + d0 = W2), and pair j + q/2, weighted by rho T[j] (rho the ratio of
+// eq entries whose indices differ in the bit of q/2), A0 = p3, A1 = p5,
+// A2 = p4.  Each worker block XORs B times its nine sums into slot (block mod
+// kLookSlots) of the set for round k+1.  FIRST: every block works; the last
+// (two-level count) reads round k+1's set, forms round k's sums with rho and
+// runs round k's transcript step.  Otherwise block 0 runs round k's step at
+// once, from the set the previous launch left and x_{k-1} (Horner, as in
+// k_prod_tail), and clears that set, while blocks 1.. work.
+constexpr size_t kProdLookMax = (size_t)1 << 17;
+constexpr int kLookGroups = CB / 8;  // pairs of round k+1 per worker block
+
+// the nine lookahead sums (lane u of wave 0 holds sum u) into their slots
+__device__ __forceinline__ void look_publish(gf *set, unsigned wb, gf v, bool wait, unsigned &done) {
+    const int lane = threadIdx.x & 63;
+    done = 1u;
+    if (lane < 9) {
+        unsigned long long *q = (unsigned long long *)(set + 9 * (wb % kLookSlots) + lane);
+        const unsigned long long o0 =
+            __hip_atomic_fetch_xor(q, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long o1 =
+            __hip_atomic_fetch_xor(q + 1, (unsigned long long)v.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (wait) done = (unsigned)(o0 ^ o1) | 1u;
+    }
+}
+
+// the nine sums of a set on lanes 0..8 of the calling wave: lane l reads
+// sum l % 9 of slots l / 9, l / 9 + 7, ... (all loads issued before the
+// first use), then lanes 0..8 add the values of their sum; with clear the
+// slots are zeroed
+constexpr int kLookPasses = (kLookSlots + 6) / 7;
+__device__ __forceinline__ gf look_gather(gf *set, bool clear) {
+    const int lane = threadIdx.x & 63, c = lane % 9;
+    gf v[kLookPasses];
+#pragma unroll
+    for (int r = 0; r < kLookPasses; r++) {
+        const int sl = 7 * r + lane / 9;
+        v[r] = gf_zero();
+        if (lane < 63 && sl < kLookSlots) v[r] = gf_ld_published(set + 9 * sl + c);
+    }
+    if (clear) {
+#pragma unroll
+        for (int r = 0; r < kLookPasses; r++) {
+            const int sl = 7 * r + lane / 9;
+            if (lane < 63 && sl < kLookSlots) gf_st(set + 9 * sl + c, gf_zero());
+        }
+    }
+    gf P = gf_zero();
+    const int src = lane < 9 ? lane : 0;
+#pragma unroll
+    for (int r = 0; r < kLookPasses; r++)
+#pragma unroll
+        for (int g = 0; g < 7; g++) P = gf_add(P, lane_gf(v[r], src + 9 * g));
+    return gf_sel(lane < 9, P, gf_zero());
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(CB) void k_prod_look(const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq,
+                                                  size_t half, int k, ProdFs *fs) {
+    __shared__ gf red[9][CB / 64];
+    __shared__ unsigned last;
+    const uint32_t n = fs->n;
+    const int iv = (int)n - 1 - k;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    uint8_t *base = (uint8_t *)fs;
+    const uint32_t in_cap = fs->in_cap;
+    const gf *cfs = (const gf *)(base + prod_cf_off()), *e0s = cfs + n, *ecfs = e0s + n, *rhos = ecfs + n;
+    gf *chal = (gf *)(base + prod_chal_off(n)), *polys = (gf *)(base + prod_polys_off(n));
+    gf *look = (gf *)(base + prod_look_off(n, in_cap));
+    gf *set_in = look + (size_t)9 * kLookSlots * (k & 1), *set_out = look + (size_t)9 * kLookSlots * ((k + 1) & 1);
+    const gf xp = gf_ld(chal + k - 1);  // x_{k-1}
+    if (!FIRST && blockIdx.x == 0) {
+        // round k's transcript step from the lookahead sums and x_{k-1}
+        if (wave != 0) return;
+        const gf P = look_gather(set_in, true);
+        const int c = lane < 3 ? lane : 0;
+        const gf al = lane_gf(P, 3 * c), ga = lane_gf(P, 3 * c + 1), mid = lane_gf(P, 3 * c + 2);
+        const gf be = gf_add(mid, gf_add(al, ga));
+        const gf t = gf_add(gf_ld(e0s + iv + 1), xp);
+        const gf s0 = fs->s, P1 = fs->P1, P2 = fs->P2;
+        // lanes 0..2 x gamma_c, lanes 3..5 {s, P1, P2} t
+        const gf r1 = gf_mul_reg(gf_sel(lane < 3, xp, gf_sel(lane == 3, s0, gf_sel(lane == 4, P1, P2))),
+                                 gf_sel(lane < 3, ga, t));
+        const gf tc = gf_add(be, r1);
+        const gf s = lane_gf(r1, 3), K = lane_gf(r1, 4), L = lane_gf(r1, 5);
+        // lanes 0..2 x (beta_c + x gamma_c), lanes 3, 4 s {ecf, cf}[iv - 1]
+        const gf r2 = gf_mul_reg(gf_sel(lane < 3, xp, s),
+                                 gf_sel(lane < 3, tc, iv > 0 ? gf_sel(lane == 3, gf_ld(ecfs + iv - 1), gf_ld(cfs + iv - 1))
+                                                            : gf_zero()));
+        const gf Ac = gf_add(al, r2);
+        const gf P1n = lane_gf(r2, 3), P2n = lane_gf(r2, 4);
+        Digest prev;
+        const uint64_t *in64 = reinterpret_cast<const uint64_t *>(base + prod_input_off(n));
+#pragma unroll
+        for (int w = 0; w < 4; w++) prev.w[w] = in64[w];
+        const ProdRound R =
+            prod_observe(fs, k, prod_coeffs(lane_gf(Ac, 0), lane_gf(Ac, 1), lane_gf(Ac, 2), K, L), true, prev);
+        if (lane == 0) {
+            polys[3 * k] = R.c0;
+            polys[3 * k + 1] = R.c2;
+            polys[3 * k + 2] = R.c3;
+            chal[k] = R.x;
+            prod_store_round(fs, R);
+            fs->s = s;
+            if (iv > 0) {
+                fs->P1 = P1n;
+                fs->P2 = P2n;
+            }
+        }
+        return;
+    }
+    // worker: pair j of round k+1
+    const unsigned wb = FIRST ? blockIdx.x : blockIdx.x - 1;
+    const size_t q = half, qh = q / 2;
+    const size_t j0 = (size_t)wb * kLookGroups, j = j0 + (size_t)(tid >> 3);
+    const int u = lane & 7, tb = u >> 2, i = u & 3, gb = lane & ~7;
+    const size_t e = j + (size_t)i * qh;
+    const gf *src = tb ? gsrc : fsrc;
+    gf *dst = tb ? gdst : fdst;
+    const gf lo = gf_ld(src + e);
+    const gf V = gf_add(lo, gf_mul_reg(gf_add(gf_ld(src + e + 2 * q), lo), xp));
+    gf_st(dst + e, V);
+    const gf X1 = gf_sel(i >= 2, gf_add(V, lane_gf(V, lane ^ 2)), V);
+    const gf M = gf_mul_reg(X1, gf_ld(eq.A + (j & (((size_t)1 << eq.lo) - 1))));
+    const gf X = gf_sel(tb == 0, M, X1);
+    const int s1 = (u == 0 || u == 2 || u == 6) ? 0 : ((u == 1 || u == 7) ? 2 : ((u == 3 || u == 5) ? 1 : 3));
+    const int s2 = u == 2 ? 2 : (u == 5 ? 3 : (u == 6 ? 1 : (u == 7 ? 3 : -1)));
+    const gf xa = lane_gf(X, gb + s1), xb = lane_gf(X, gb + (s2 < 0 ? s1 : s2));
+    const gf ya = lane_gf(X, gb + 4 + s1), yb = lane_gf(X, gb + 4 + (s2 < 0 ? s1 : s2));
+    gf a = gf_mul_reg(gf_sel(s2 < 0, xa, gf_add(xa, xb)), gf_sel(s2 < 0, ya, gf_add(ya, yb)));
+    gf Y = gf_add(X, gf_shfl_xor(X, 1));
+    Y = gf_add(Y, gf_shfl_xor(Y, 2));
+    gf a8 = gf_mul_reg(lane_gf(Y, gb), lane_gf(Y, gb + 4));
+    a8 = gf_sel(u == 0, a8, gf_zero());
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) {
+        a = gf_add(a, gf_shfl_xor(a, off));
+        a8 = gf_add(a8, gf_shfl_xor(a8, off));
+    }
+    if (lane < 8) red[lane][wave] = a;
+    if (lane == 0) red[8][wave] = a8;
+    __syncthreads();
+    if (wave == 0) {
+        gf v = gf_zero();
+        if (lane < 9)
+            for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[lane][w]);
+        v = gf_mul_reg(v, gf_ld(eq.B + (j0 >> eq.lo)));
+        unsigned done;
+        look_publish(set_out, wb, v, FIRST, done);
+        if (FIRST) {
+            // 1, after lanes 0..8's atomics returned
+            unsigned inc = done;
+#pragma unroll
+            for (int l = 1; l < 9; l++) inc &= (unsigned)__shfl((int)done, l);
+            unsigned long long *acc = (unsigned long long *)(base + prod_acc_off(n, in_cap)) + (size_t)k * (kProdAccRound / 8);
+            if (lane == 0) last = grid_last((unsigned *)(acc + kProdSlotBytes / 8), gridDim.x, blockIdx.x, inc & 1u);
+        }
+    }
+    if (!FIRST) return;
+    __syncthreads();
+    if (!last || wave != 0) return;
+    // round k's transcript step: its pair sums from round k+1's set
+    const gf P = look_gather(set_out, false);
+    // lanes 0..2: A0 = p0 + rho p3, A1 = p2 + rho p5, A2 = p1 + rho p4;
+    // lanes 3..5: s, K, L
+    const int hi_i = lane == 0 ? 3 : (lane == 1 ? 5 : 4), lo_i = lane == 0 ? 0 : (lane == 1 ? 2 : 1);
+    const gf Ph = lane_gf(P, lane < 3 ? hi_i : 0), Pl = lane_gf(P, lane < 3 ? lo_i : 0);
+    const gf t = gf_add(gf_ld(e0s + iv + 1), xp);
+    const gf rho = gf_ld(rhos + (n - 2 - k));
+    const gf r1 = gf_mul_reg(gf_sel(lane < 3, Ph, gf_sel(lane == 3, fs->s, gf_sel(lane == 4, fs->P1, fs->P2))),
+                             gf_sel(lane < 3, rho, t));
+    const gf Ac = gf_add(Pl, r1);
+    const gf K = lane_gf(r1, 4), L = lane_gf(r1, 5);
+    Digest prev;
+    const uint64_t *in64 = reinterpret_cast<const uint64_t *>(base + prod_input_off(n));
+#pragma unroll
+    for (int w = 0; w < 4; w++) prev.w[w] = in64[w];
+    const ProdRound R = prod_observe(fs, k, prod_coeffs(lane_gf(Ac, 0), lane_gf(Ac, 1), lane_gf(Ac, 2), K, L), true, prev);
+    // the eq scalars for round k + 1 (s_k, then P1, P2)
+    const gf s = lane_gf(r1, 3);
+    const gf p = gf_mul_reg(s, iv > 0 ? gf_sel(lane & 1, gf_ld(cfs + iv - 1), gf_ld(ecfs + iv - 1)) : gf_zero());
+    if (lane == 0) {
+        polys[3 * k] = R.c0;
+        polys[3 * k + 1] = R.c2;
+        polys[3 * k + 2] = R.c3;
+        chal[k] = R.x;
+        prod_store_round(fs, R);
+        fs->s = s;
+        if (iv > 0) fs->P1 = p;
+    } else if (lane == 1 && iv > 0) {
+        fs->P2 = p;
+    }
+}
+
 // The rounds from k0 on in one block of kTailThreads threads, tables in LDS,
 // with one round of lookahead so that a round's table work runs while the
 // round before it samples its challenge.  Round k+1's pair sums are
@@ -919,9 +1079,6 @@ constexpr int kTailThreads = 512;
 constexpr int kTailWaves = kTailThreads / 64;
 constexpr size_t kTailPairs = 256;  // pairs of the first tail round (4 * kTailPairs entries per table)
 
-__device__ __forceinline__ gf gf_shfl_xor(gf v, int off) {
-    return gf{(uint64_t)__shfl_xor((long long)v.lo, off), (uint64_t)__shfl_xor((long long)v.hi, off)};
-}
 
 // prod_transcript behind a call: its Keccak-f keeps its own registers
 // instead of sharing an allocation with the tail kernel's table work
@@ -2372,11 +2529,6 @@ static unsigned prod_blocks_per_cu() {
 hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq, size_t half,
                      int k, ProdFs *fs, int cus) {
     if (half <= kTailPairs || (half & (half - 1)) || eq.lo < 7 || eq.lo > 62) return hipErrorInvalidValue;
-    // rounds of few pairs: four lanes per pair (kProdSplit pairs per block)
-    if (k >= 1 && half <= kProdSplitMax) {
-        k_prod_round<3><<<(unsigned)(half / kProdSplit), CB, 0, st>>>(fsrc, gsrc, fdst, gdst, eq, half, k, fs);
-        return hipGetLastError();
-    }
     // pairs per block: a power of two from CB up to 2^lo (one B entry per
     // block), the smallest that keeps the grid within the resident blocks
     const size_t cap = (size_t)std::max(1, cus) * prod_blocks_per_cu();
@@ -2390,6 +2542,18 @@ hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, g
     return hipGetLastError();
 }
 size_t prod_tail_max_pairs() { return kTailPairs; }
+size_t prod_look_max_pairs() { return kProdLookMax; }
+hipError_t prod_look(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq, size_t half,
+                     int k, ProdFs *fs, bool first) {
+    // q / 64 worker blocks of kLookGroups pairs of round k+1, within one
+    // entry of B
+    if (k < 1 || half <= kTailPairs || half > kProdLookMax || (half & (half - 1)) || eq.lo < 5 || eq.lo > 62)
+        return hipErrorInvalidValue;
+    const unsigned wblocks = (unsigned)(half / (2 * kLookGroups));
+    if (first) k_prod_look<true><<<wblocks, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, eq, half, k, fs);
+    else k_prod_look<false><<<wblocks + 1, CB, 0, st>>>(fsrc, gsrc, fdst, gdst, eq, half, k, fs);
+    return hipGetLastError();
+}
 hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, EqFactors eq, size_t half,
                      int k0, ProdFs *fs) {
     if (!half || half > kTailPairs || (half & (half - 1)) || eq.lo < 0 || eq.lo > 62) return hipErrorInvalidValue;

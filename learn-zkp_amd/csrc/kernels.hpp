@@ -87,11 +87,15 @@ __host__ __device__ inline size_t fs_bytes(uint32_t n, uint32_t in_cap) { return
 // prove_sumcheck driver, sumcheck/src/lib.rs:56-103, over f g eq(r, .)): the
 // challenger's byte buffers and the running eq product.  One region laid out
 // as
-//   ProdFs | gf cf[n] | gf e0[n] | gf ecf[n] | gf polys[3n] | gf chal[n] |
-//   uint8_t input[in_cap] | gf finals[4] | per round k: uint64_t
-//   acc[8 kProdSlots], then its block counters (kCountWords words)
+//   ProdFs | gf cf[n] | gf e0[n] | gf ecf[n] | gf rho[n] | gf polys[3n] |
+//   gf chal[n] | uint8_t input[in_cap] | gf finals[4] |
+//   gf look[2][kLookSlots][9] | per round k: uint64_t acc[8 kProdSlots],
+//   then its block counters (kCountWords words)
 // cf[i] is the correcting factor of the halved eq table at round variable i,
-// e0[i] = 1 + r_i and ecf[i] = e0[i] * cf[i]; polys and chal receive each
+// e0[i] = 1 + r_i, ecf[i] = e0[i] * cf[i] and rho[i] = r_i / (1 + r_i), the
+// ratio of halved eq table entries whose indices differ in bit i; look holds
+// the coefficient sums of the lookahead rounds (kernels.hip k_prod_look),
+// one set per round parity; polys and chal receive each
 // round's compressed polynomial [c0, c2, c3] and challenge, finals the last
 // round's two entries of f and of g.  Round k's acc holds its polynomial in
 // kProdSlots slots of [c0, c1, c2, c3] that the blocks XOR their shares
@@ -112,15 +116,19 @@ struct ProdFs {
     uint8_t out[32];
 };
 constexpr int kProdSlots = 256;  // accumulation slots per round (blocks spread their atomics over them)
+constexpr int kLookSlots = 64;   // slots of one set of lookahead coefficient sums
 __host__ __device__ inline size_t prod_cf_off() { return sizeof(ProdFs); }
-__host__ __device__ inline size_t prod_polys_off(uint32_t n) { return sizeof(ProdFs) + 48 * (size_t)n; }
+__host__ __device__ inline size_t prod_polys_off(uint32_t n) { return sizeof(ProdFs) + 64 * (size_t)n; }
 __host__ __device__ inline size_t prod_chal_off(uint32_t n) { return prod_polys_off(n) + 48 * (size_t)n; }
 __host__ __device__ inline size_t prod_input_off(uint32_t n) { return prod_chal_off(n) + 16 * (size_t)n; }
 __host__ __device__ inline size_t prod_finals_off(uint32_t n, uint32_t in_cap) {
     return (prod_input_off(n) + in_cap + 15) & ~(size_t)15;
 }
-__host__ __device__ inline size_t prod_acc_off(uint32_t n, uint32_t in_cap) {
+__host__ __device__ inline size_t prod_look_off(uint32_t n, uint32_t in_cap) {
     return (prod_finals_off(n, in_cap) + 64 + 255) & ~(size_t)255;
+}
+__host__ __device__ inline size_t prod_acc_off(uint32_t n, uint32_t in_cap) {
+    return prod_look_off(n, in_cap) + 2 * 9 * 16 * (size_t)kLookSlots;
 }
 // Two-level block counters of one launch: word 0 counts the groups done,
 // word kCountLine (1 + g) the blocks of group g done; each on its own
@@ -300,6 +308,17 @@ hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, g
 hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, EqFactors eq, size_t half,
                      int k0, ProdFs *fs);
 size_t prod_tail_max_pairs();
+// Rounds of at most prod_look_max_pairs() pairs (and more than
+// prod_tail_max_pairs()) with one round of lookahead: every block folds its
+// share of the previous tables into round k's and forms round k+1's
+// coefficient sums; with first set (the first such round) round k's own
+// sums come from the same products and the block that finishes last runs
+// its transcript step, otherwise one more block runs round k's transcript
+// step from the coefficient sums the previous launch left, while the others
+// work.
+hipError_t prod_look(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, gf *gdst, EqFactors eq, size_t half,
+                     int k, ProdFs *fs, bool first);
+size_t prod_look_max_pairs();
 // MLE value of every column of a rows x w matrix at a point whose eq table is
 // A (2^m entries) x B: partial[col * nblocks + block], one pass over the data
 hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,
