@@ -745,25 +745,72 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
     gf *slots = (gf *)(fs.as<uint8_t>() + quad_slots_off(npairs, n, in_cap, m));
     unsigned *counters = (unsigned *)(fs.as<uint8_t>() + quad_counters_off(npairs, n, in_cap, m));
     const gf *x_dev = (const gf *)(fs.as<uint8_t>() + offsetof(QuadFs, x));
-    const size_t tail = launch::quad_tail_max_half(m, npairs);
-    bool tail_ran = false;
+    const size_t tail = launch::quad_tail_max_half(m, npairs), HL = launch::quad_look_max_half();
+    // Rounds of at most HL pairs of entries (above the single-block rounds)
+    // run with one round of lookahead; they fold the previous round's tables
+    // themselves, into the other of two working buffers.  Every other round
+    // needs its own tables folded first: `pending` says cur holds the
+    // previous round's, still to be folded with that round's challenge.
+    const auto lookahead = [&](uint32_t k, size_t half) { return k >= 1 && half <= HL && half >= 64 && half > tail; };
+    DevBuf W2;
+    const size_t NW2 = 2 * std::min(HL, std::max<size_t>(1, N / 4));
+    const gf *cur = src0;
+    size_t cstride = N;
+    bool pending = false, look = false, tail_ran = false;
+    const auto other = [&](gf *&o, size_t &ostride) {
+        if (cur == W2.as<gf>()) {
+            o = W.as<gf>();
+            ostride = NW;
+        } else {
+            if (!W2.p) W2 = ctx_buf(ctx, 16 * NW2 * m);
+            o = W2.as<gf>();
+            ostride = NW2;
+        }
+    };
     for (uint32_t k = 0; k < n; k++) {
         const size_t half = (size_t)1 << (n - 1 - k);
-        const gf *src = k == 0 ? src0 : W.as<gf>();
-        const size_t sstride = k == 0 ? N : NW;
+        if (lookahead(k, half)) {
+            gf *o;
+            size_t os;
+            other(o, os);
+            ctx->timer.run(st, "k_quad_look", [&] {
+                return launch::quad_look(st, cur, cstride, o, os, dpairs, npairs, m, half, (int)k, dfs, !look);
+            });
+            cur = o;
+            cstride = os;
+            pending = true;
+            look = true;
+            continue;
+        }
+        if (pending) {  // this round's tables from the previous round's
+            gf *o;
+            size_t os;
+            other(o, os);
+            ctx->timer.run(st, "k_fix_multi",
+                           [&] { return launch::fix_multi_dev(st, cur, cstride, o, os, m, 2 * half, x_dev); });
+            cur = o;
+            cstride = os;
+            pending = false;
+        }
         if (half <= tail) {
             ctx->timer.run(st, "k_quad_tail", [&] {
-                return launch::quad_tail(st, src, sstride, (gf *)(fs.as<uint8_t>() + fin_off), 1, dpairs, npairs, m,
+                return launch::quad_tail(st, cur, cstride, (gf *)(fs.as<uint8_t>() + fin_off), 1, dpairs, npairs, m,
                                          half, (int)k, dfs);
             });
             tail_ran = true;
             break;
         }
         ctx->timer.run(st, "k_quad_step", [&] {
-            return launch::quad_step(st, src, sstride, dpairs, npairs, half, (int)k, dfs, slots, max_blocks, counters);
+            return launch::quad_step(st, cur, cstride, dpairs, npairs, half, (int)k, dfs, slots, max_blocks, counters);
         });
-        ctx->timer.run(st, "k_fix_multi",
-                       [&] { return launch::fix_multi_dev(st, src, sstride, W.as<gf>(), NW, m, half, x_dev); });
+        if (lookahead(k + 1, half / 2)) {
+            pending = true;  // the lookahead round folds these tables itself
+        } else {
+            ctx->timer.run(st, "k_fix_multi",
+                           [&] { return launch::fix_multi_dev(st, cur, cstride, W.as<gf>(), NW, m, half, x_dev); });
+            cur = W.as<gf>();
+            cstride = NW;
+        }
     }
     // one read-back: the region up to the finals (the finals too when the
     // single-block rounds wrote them there)

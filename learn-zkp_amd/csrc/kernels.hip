@@ -872,12 +872,15 @@ __global__ __launch_bounds__(CB) void k_prod_round(const gf *fsrc, const gf *gsr
 constexpr size_t kProdLookMax = (size_t)1 << 17;
 constexpr int kLookGroups = CB / 8;  // pairs of round k+1 per worker block
 
-// the nine lookahead sums (lane u of wave 0 holds sum u) into their slots
+// NS lookahead sums (lane u of wave 0 holds sum u) into slot (wb mod
+// kLookSlots) of a set; with wait, done is made nonzero from the atomics'
+// results (so a count issued after it follows them)
+template <int NS>
 __device__ __forceinline__ void look_publish(gf *set, unsigned wb, gf v, bool wait, unsigned &done) {
     const int lane = threadIdx.x & 63;
     done = 1u;
-    if (lane < 9) {
-        unsigned long long *q = (unsigned long long *)(set + 9 * (wb % kLookSlots) + lane);
+    if (lane < NS) {
+        unsigned long long *q = (unsigned long long *)(set + NS * (wb % kLookSlots) + lane);
         const unsigned long long o0 =
             __hip_atomic_fetch_xor(q, (unsigned long long)v.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned long long o1 =
@@ -886,34 +889,35 @@ __device__ __forceinline__ void look_publish(gf *set, unsigned wb, gf v, bool wa
     }
 }
 
-// the nine sums of a set on lanes 0..8 of the calling wave: lane l reads
-// sum l % 9 of slots l / 9, l / 9 + 7, ... (all loads issued before the
-// first use), then lanes 0..8 add the values of their sum; with clear the
+// the NS sums of a set on lanes 0..NS-1 of the calling wave: lane l reads sum
+// l % NS of slots l / NS, l / NS + 64 / NS, ... (all loads issued before the
+// first use), then lanes 0..NS-1 add the values of their sum; with clear the
 // slots are zeroed
-constexpr int kLookPasses = (kLookSlots + 6) / 7;
+template <int NS>
 __device__ __forceinline__ gf look_gather(gf *set, bool clear) {
-    const int lane = threadIdx.x & 63, c = lane % 9;
-    gf v[kLookPasses];
+    constexpr int per = 64 / NS, passes = (kLookSlots + per - 1) / per;
+    const int lane = threadIdx.x & 63, c = lane % NS;
+    gf v[passes];
 #pragma unroll
-    for (int r = 0; r < kLookPasses; r++) {
-        const int sl = 7 * r + lane / 9;
+    for (int r = 0; r < passes; r++) {
+        const int sl = per * r + lane / NS;
         v[r] = gf_zero();
-        if (lane < 63 && sl < kLookSlots) v[r] = gf_ld_published(set + 9 * sl + c);
+        if (lane < per * NS && sl < kLookSlots) v[r] = gf_ld_published(set + NS * sl + c);
     }
     if (clear) {
 #pragma unroll
-        for (int r = 0; r < kLookPasses; r++) {
-            const int sl = 7 * r + lane / 9;
-            if (lane < 63 && sl < kLookSlots) gf_st(set + 9 * sl + c, gf_zero());
+        for (int r = 0; r < passes; r++) {
+            const int sl = per * r + lane / NS;
+            if (lane < per * NS && sl < kLookSlots) gf_st(set + NS * sl + c, gf_zero());
         }
     }
     gf P = gf_zero();
-    const int src = lane < 9 ? lane : 0;
+    const int src = lane < NS ? lane : 0;
 #pragma unroll
-    for (int r = 0; r < kLookPasses; r++)
+    for (int r = 0; r < passes; r++)
 #pragma unroll
-        for (int g = 0; g < 7; g++) P = gf_add(P, lane_gf(v[r], src + 9 * g));
-    return gf_sel(lane < 9, P, gf_zero());
+        for (int g = 0; g < per; g++) P = gf_add(P, lane_gf(v[r], src + NS * g));
+    return gf_sel(lane < NS, P, gf_zero());
 }
 
 template <bool FIRST>
@@ -934,7 +938,7 @@ __global__ __launch_bounds__(CB) void k_prod_look(const gf *fsrc, const gf *gsrc
     if (!FIRST && blockIdx.x == 0) {
         // round k's transcript step from the lookahead sums and x_{k-1}
         if (wave != 0) return;
-        const gf P = look_gather(set_in, true);
+        const gf P = look_gather<9>(set_in, true);
         const int c = lane < 3 ? lane : 0;
         const gf al = lane_gf(P, 3 * c), ga = lane_gf(P, 3 * c + 1), mid = lane_gf(P, 3 * c + 2);
         const gf be = gf_add(mid, gf_add(al, ga));
@@ -1008,7 +1012,7 @@ __global__ __launch_bounds__(CB) void k_prod_look(const gf *fsrc, const gf *gsrc
             for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[lane][w]);
         v = gf_mul_reg(v, gf_ld(eq.B + (j0 >> eq.lo)));
         unsigned done;
-        look_publish(set_out, wb, v, FIRST, done);
+        look_publish<9>(set_out, wb, v, FIRST, done);
         if (FIRST) {
             // 1, after lanes 0..8's atomics returned
             unsigned inc = done;
@@ -1022,7 +1026,7 @@ __global__ __launch_bounds__(CB) void k_prod_look(const gf *fsrc, const gf *gsrc
     __syncthreads();
     if (!last || wave != 0) return;
     // round k's transcript step: its pair sums from round k+1's set
-    const gf P = look_gather(set_out, false);
+    const gf P = look_gather<9>(set_out, false);
     // lanes 0..2: A0 = p0 + rho p3, A1 = p2 + rho p5, A2 = p1 + rho p4;
     // lanes 3..5: s, K, L
     const int hi_i = lane == 0 ? 3 : (lane == 1 ? 5 : 4), lo_i = lane == 0 ? 0 : (lane == 1 ? 2 : 1);
@@ -1560,6 +1564,118 @@ __global__ __launch_bounds__(CB) void k_fix_multi_dev(const gf *src, size_t src_
         if constexpr (TAB) gf_st(o + j, gf_add(lo, gf_tab_mul(d, T)));
         else gf_st(o + j, gf_add(lo, gf_mul_pre(d, px)));
     }
+}
+
+// Quadratic sumcheck rounds k >= 1 with one round of lookahead (see
+// launch::quad_look).  Rows 0..npairs-1 are the pairs.  Every thread first
+// folds its share of the m tables' entries with x_{k-1} from src into round
+// k's tables at dst.  Then eight lanes take pair p (blockIdx.y) at position
+// j < h/2 of round k+1: lane u folds entry j + (u & 3) h/2 of the pair's
+// first (u < 4) or second table again, in registers, giving a0, a1, b0, b1
+// and c0, c1, d0, d1 (a_i = V_i, b_i = V_{i+2} + V_i, likewise c, d) as in
+// k_prod_look without eq weights, and the products p0 = a0 c0, p1 = b0 d0,
+// p2 = (a0 + b0)(c0 + d0) (round k+1's A0 as a quadratic in its challenge),
+// p3 = a1 c1, p4 = b1 d1, and p5..p7, the same three for A2 with a0 + a1 and
+// so on.  p0 + p3 and p1 + p4 are round k's A0 and A2 of the pair at
+// positions j and j + h/2.  Each block XORs s_p times its eight sums into
+// slot (block mod kLookSlots) of the set for round k+1.  FIRST: the last
+// block (two-level count) reads that set and runs round k's transcript step
+// with c0 = sum (p0 + p3), c2 = sum (p1 + p4); otherwise row npairs' first
+// block runs it from the set the previous launch left (c0 and c2 as
+// quadratics in x_{k-1}) and clears that set.
+constexpr size_t kQuadLookMax = (size_t)1 << 12;
+template <bool FIRST>
+__global__ __launch_bounds__(CB) void k_quad_look(const gf *src, size_t sstride, gf *dst, size_t dstride,
+                                                  const uint32_t *__restrict__ pairs, uint32_t m, size_t h, int k,
+                                                  QuadFs *fs) {
+    __shared__ gf red[8][CB / 64];
+    __shared__ unsigned last;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    uint8_t *base = (uint8_t *)fs;
+    const uint32_t n = fs->n, np = fs->npairs, in_cap = fs->in_cap;
+    gf *chal = (gf *)(base + quad_chal_off(np, n));
+    gf *look = (gf *)(base + quad_look_off(np, n, in_cap, m));
+    gf *set_in = look + (size_t)8 * kLookSlots * (k & 1), *set_out = look + (size_t)8 * kLookSlots * ((k + 1) & 1);
+    const gf xp = gf_ld(chal + k - 1);  // x_{k-1}
+    Digest prev;
+    const uint64_t *in64 = reinterpret_cast<const uint64_t *>(base + quad_input_off(np, n));
+    if (!FIRST && blockIdx.y == np) {
+        if (blockIdx.x != 0 || wave != 0) return;
+        const gf P = look_gather<8>(set_in, true);
+        // lane 0: c0 from sums 0, 1, 2; lane 1: c2 from sums 5, 6, 7
+        const int b = lane == 1 ? 5 : 0;
+        const gf al = lane_gf(P, b), ga = lane_gf(P, b + 1), mid = lane_gf(P, b + 2);
+        const gf be = gf_add(mid, gf_add(al, ga));
+        const gf r2 = gf_mul_reg(xp, gf_add(be, gf_mul_reg(xp, ga)));
+        const gf cv = gf_add(al, r2);
+        const gf c0 = lane_gf(cv, 0), c2 = lane_gf(cv, 1);
+#pragma unroll
+        for (int w = 0; w < 4; w++) prev.w[w] = in64[w];
+        const QuadRound R = quad_observe(fs, k, c0, c2, gf_zero(), true, prev);
+        if (lane == 0) quad_store(fs, k, c0, c2, gf_zero(), R);
+        return;
+    }
+    // every entry of round k's tables, folded once
+    const unsigned nb = gridDim.x * np, bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const size_t w2 = 2 * h, total = (size_t)nb * CB;
+    for (size_t f = (size_t)bid * CB + tid; f < (size_t)m * w2; f += total) {
+        const size_t t = f / w2, e = f % w2;
+        const gf *s0 = src + t * sstride;
+        const gf lo = gf_ld(s0 + e);
+        gf_st(dst + t * dstride + e, gf_add(lo, gf_mul_reg(gf_add(gf_ld(s0 + e + w2), lo), xp)));
+    }
+    // pair p, position j of round k+1
+    const uint32_t p = blockIdx.y;
+    const size_t qh = h / 2, j = (size_t)blockIdx.x * kLookGroups + (size_t)(tid >> 3);
+    const int u = lane & 7, tb = u >> 2, i = u & 3, gb = lane & ~7;
+    const gf *s0 = src + (size_t)pairs[2 * p + tb] * sstride;
+    const size_t e = j + (size_t)i * qh;
+    const gf lo = gf_ld(s0 + e);
+    const gf V = gf_add(lo, gf_mul_reg(gf_add(gf_ld(s0 + e + w2), lo), xp));
+    const gf X = gf_sel(i >= 2, gf_add(V, lane_gf(V, lane ^ 2)), V);
+    // lane u: p_u with operands X[s1] (+ X[s2]) and the same of the second
+    // table; u = 7 takes the sums of all four
+    const int s1 = (u == 0 || u == 2 || u == 5) ? 0 : ((u == 1 || u == 6) ? 2 : (u == 3 ? 1 : 3));
+    const int s2 = u == 2 ? 2 : (u == 5 ? 1 : (u == 6 ? 3 : -1));
+    const gf xa = lane_gf(X, gb + s1), xb = lane_gf(X, gb + (s2 < 0 ? s1 : s2));
+    const gf ya = lane_gf(X, gb + 4 + s1), yb = lane_gf(X, gb + 4 + (s2 < 0 ? s1 : s2));
+    gf Y = gf_add(X, gf_shfl_xor(X, 1));
+    Y = gf_add(Y, gf_shfl_xor(Y, 2));
+    const gf ys = lane_gf(Y, gb), zs = lane_gf(Y, gb + 4);
+    const gf o1 = gf_sel(u == 7, ys, gf_sel(s2 < 0, xa, gf_add(xa, xb)));
+    const gf o2 = gf_sel(u == 7, zs, gf_sel(s2 < 0, ya, gf_add(ya, yb)));
+    gf a = gf_mul_reg(o1, o2);
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) a = gf_add(a, gf_shfl_xor(a, off));
+    if (lane < 8) red[lane][wave] = a;
+    __syncthreads();
+    if (wave == 0) {
+        gf v = gf_zero();
+        if (lane < 8)
+            for (int w = 0; w < CB / 64; w++) v = gf_add(v, red[lane][w]);
+        v = gf_mul_reg(v, gf_ld((const gf *)(base + quad_scalars_off()) + p));
+        unsigned done;
+        look_publish<8>(set_out, bid, v, FIRST, done);
+        if (FIRST) {
+            unsigned inc = done;
+#pragma unroll
+            for (int l = 1; l < 8; l++) inc &= (unsigned)__shfl((int)done, l);
+            unsigned *counter = (unsigned *)(base + quad_counters_off(np, n, in_cap, m));
+            if (lane == 0) last = grid_last(counter, nb, bid, inc & 1u);
+        }
+    }
+    if (!FIRST) return;
+    __syncthreads();
+    if (!last || wave != 0) return;
+    // round k's transcript step from its sums in round k+1's set
+    const gf P = look_gather<8>(set_out, false);
+    const gf c0 = gf_add(lane_gf(P, 0), lane_gf(P, 3)), c2 = gf_add(lane_gf(P, 1), lane_gf(P, 4));
+    unsigned *counter = (unsigned *)(base + quad_counters_off(np, n, in_cap, m));
+    for (int w = lane; w < (int)kCountWords; w += 64) counter[w] = 0u;
+#pragma unroll
+    for (int w = 0; w < 4; w++) prev.w[w] = in64[w];
+    const QuadRound R = quad_observe(fs, k, c0, c2, gf_zero(), true, prev);
+    if (lane == 0) quad_store(fs, k, c0, c2, gf_zero(), R);
 }
 
 // The Quadratic sumcheck's rounds from k0 on in one block, the m tables in
@@ -2596,6 +2712,21 @@ hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t 
     if (npairs == 0 || npairs > 65535 || max_blocks == 0) return hipErrorInvalidValue;
     const unsigned gx = grid_for(h, CB, max_blocks);
     k_quad_step<<<dim3(gx, npairs), CB, 0, st>>>(P, stride, pairs, h, k, fs, slots, counter);
+    return hipGetLastError();
+}
+size_t quad_look_max_half() { return kQuadLookMax; }
+hipError_t quad_look(hipStream_t st, const gf *src, size_t src_stride, gf *dst, size_t dst_stride,
+                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t half, int k, QuadFs *fs, bool first) {
+    // half / 64 blocks per pair of kLookGroups positions of round k+1
+    if (k < 1 || npairs == 0 || npairs > 65534 || m == 0 || half < 2 * kLookGroups || half > kQuadLookMax ||
+        (half & (half - 1)))
+        return hipErrorInvalidValue;
+    const unsigned gx = (unsigned)(half / (2 * kLookGroups));
+    if (first)
+        k_quad_look<true><<<dim3(gx, npairs), CB, 0, st>>>(src, src_stride, dst, dst_stride, pairs, m, half, k, fs);
+    else
+        k_quad_look<false><<<dim3(gx, npairs + 1), CB, 0, st>>>(src, src_stride, dst, dst_stride, pairs, m, half, k,
+                                                                 fs);
     return hipGetLastError();
 }
 size_t quad_tail_max_half(uint32_t m, uint32_t npairs) {

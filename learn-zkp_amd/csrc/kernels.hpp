@@ -161,11 +161,14 @@ struct EqFactors {
 // challenger's byte buffers.  One region laid out as
 //   QuadFs | gf scalars[npairs] | gf polys[2n] | gf chal[n] |
 //   uint8_t input[in_cap] | gf finals[m] | uint32_t pairs[2 npairs] |
-//   gf slots[kQuadSlots][3] | uint32_t counters[kCountWords]
+//   gf slots[kQuadSlots][3] | uint32_t counters[kCountWords] |
+//   gf look[2][kLookSlots][8]
 // polys and chal receive each round's compressed polynomial [c0, c2] and
 // challenge, finals every table's last entry.  The round kernels XOR their
 // shares of [c0, c2, claim] into the slots and count their blocks in the
-// counters (both zero at upload; the last block of a round clears them).
+// counters (both zero at upload; the last block of a round clears them);
+// look holds the coefficient sums of the lookahead rounds (kernels.hip
+// k_quad_look), one set per round parity.
 struct QuadFs {
     uint32_t in_len, out_len, in_cap, n;
     uint32_t npairs, pad_[3];
@@ -194,8 +197,11 @@ __host__ __device__ inline size_t quad_slots_off(uint32_t npairs, uint32_t n, ui
 __host__ __device__ inline size_t quad_counters_off(uint32_t npairs, uint32_t n, uint32_t in_cap, uint32_t m) {
     return quad_slots_off(npairs, n, in_cap, m) + 48 * (size_t)kQuadSlots;
 }
+__host__ __device__ inline size_t quad_look_off(uint32_t npairs, uint32_t n, uint32_t in_cap, uint32_t m) {
+    return (quad_counters_off(npairs, n, in_cap, m) + 4 * kCountWords + 255) & ~(size_t)255;
+}
 __host__ __device__ inline size_t quad_bytes(uint32_t npairs, uint32_t n, uint32_t in_cap, uint32_t m) {
-    return quad_counters_off(npairs, n, in_cap, m) + 4 * kCountWords;
+    return quad_look_off(npairs, n, in_cap, m) + 2 * 8 * 16 * (size_t)kLookSlots;
 }
 
 enum class SegKernel { NttFirst, NttLayer, NttMulti2, NttMulti3, LeavesElem, LeavesPairDot, Nodes, Subtree, Fix, FoldFix,
@@ -349,6 +355,18 @@ hipError_t quad_step(hipStream_t st, const gf *P, size_t stride, const uint32_t 
 // quad_tail_max_half(m, npairs), which is 0 when the tables or the pairs do
 // not fit.
 size_t quad_tail_max_half(uint32_t m, uint32_t npairs);
+// Rounds k >= 1 of at most quad_look_max_half() pairs of entries (and more
+// than the single-block rounds take) with one round of lookahead: the m
+// tables (4 half entries each at src, src_stride apart) are folded with
+// x_{k-1} into round k's (2 half entries at dst, dst_stride apart, a
+// different buffer), and every pair's coefficient sums of round k+1 are
+// formed from the same entries; with first set round k's own sums come from
+// the same products and the block that finishes last runs its transcript
+// step, otherwise one more block runs it from the sums the previous launch
+// left.  max_blocks caps the blocks per pair.
+hipError_t quad_look(hipStream_t st, const gf *src, size_t src_stride, gf *dst, size_t dst_stride,
+                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t half, int k, QuadFs *fs, bool first);
+size_t quad_look_max_half();
 hipError_t quad_tail(hipStream_t st, const gf *src, size_t src_stride, gf *fin, size_t fin_stride,
                      const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t h0, int k0, QuadFs *fs);
 // verifier kernels: fold chains (thread per proof x query) and Merkle paths
