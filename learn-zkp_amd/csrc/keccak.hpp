@@ -6,7 +6,8 @@
 // gfx950 specifics: every 64-bit lane is a pair of 32-bit halves; rotations
 // are two v_alignbit_b32; the five-way column parity of theta is two
 // three-input XORs and chi (a ^ (~b & c)) is one three-input LUT op, both via
-// v_bitop3_b32.  A round is ~190 VALU instructions; the 24 rounds are fully
+// v_bitop3_b32, and so is theta's A ^ C[x-1] ^ rotl(C[x+1], 1) (D is never
+// formed).  A round is ~174 VALU instructions; the 24 rounds are fully
 // unrolled so round constants become immediates and the zero padding lanes
 // of one-block messages fold away in the first round.
 #pragma once
@@ -68,36 +69,44 @@ __device__ __forceinline__ void keccak_f(lane s[25]) {
             c[x].lo = xor3(xor3(s[x].lo, s[x + 5].lo, s[x + 10].lo), s[x + 15].lo, s[x + 20].lo);
             c[x].hi = xor3(xor3(s[x].hi, s[x + 5].hi, s[x + 10].hi), s[x + 15].hi, s[x + 20].hi);
         }
-        lane d[5];
+        // D[x] = C[x-1] ^ rotl(C[x+1], 1) is never formed: each state lane
+        // takes both terms in one three-input XOR (one operation per half
+        // instead of two)
+        lane rc[5];
 #pragma unroll
-        for (int x = 0; x < 5; x++) d[x] = lx(c[(x + 4) % 5], rotl<1>(c[(x + 1) % 5]));
+        for (int x = 0; x < 5; x++) rc[x] = rotl<1>(c[x]);
+        const auto td = [&](int i) {
+            const int x = i % 5;
+            return lane{xor3(s[i].lo, c[(x + 4) % 5].lo, rc[(x + 1) % 5].lo),
+                        xor3(s[i].hi, c[(x + 4) % 5].hi, rc[(x + 1) % 5].hi)};
+        };
         // theta + rho + pi: B[y + 5*((2x+3y)%5)] = rotl(A[x+5y] ^ D[x], r[x][y])
         lane b[25];
-        b[0] = lx(s[0], d[0]);
-        b[10] = rotl<1>(lx(s[1], d[1]));
-        b[20] = rotl<62>(lx(s[2], d[2]));
-        b[5] = rotl<28>(lx(s[3], d[3]));
-        b[15] = rotl<27>(lx(s[4], d[4]));
-        b[16] = rotl<36>(lx(s[5], d[0]));
-        b[1] = rotl<44>(lx(s[6], d[1]));
-        b[11] = rotl<6>(lx(s[7], d[2]));
-        b[21] = rotl<55>(lx(s[8], d[3]));
-        b[6] = rotl<20>(lx(s[9], d[4]));
-        b[7] = rotl<3>(lx(s[10], d[0]));
-        b[17] = rotl<10>(lx(s[11], d[1]));
-        b[2] = rotl<43>(lx(s[12], d[2]));
-        b[12] = rotl<25>(lx(s[13], d[3]));
-        b[22] = rotl<39>(lx(s[14], d[4]));
-        b[23] = rotl<41>(lx(s[15], d[0]));
-        b[8] = rotl<45>(lx(s[16], d[1]));
-        b[18] = rotl<15>(lx(s[17], d[2]));
-        b[3] = rotl<21>(lx(s[18], d[3]));
-        b[13] = rotl<8>(lx(s[19], d[4]));
-        b[14] = rotl<18>(lx(s[20], d[0]));
-        b[24] = rotl<2>(lx(s[21], d[1]));
-        b[9] = rotl<61>(lx(s[22], d[2]));
-        b[19] = rotl<56>(lx(s[23], d[3]));
-        b[4] = rotl<14>(lx(s[24], d[4]));
+        b[0] = td(0);
+        b[10] = rotl<1>(td(1));
+        b[20] = rotl<62>(td(2));
+        b[5] = rotl<28>(td(3));
+        b[15] = rotl<27>(td(4));
+        b[16] = rotl<36>(td(5));
+        b[1] = rotl<44>(td(6));
+        b[11] = rotl<6>(td(7));
+        b[21] = rotl<55>(td(8));
+        b[6] = rotl<20>(td(9));
+        b[7] = rotl<3>(td(10));
+        b[17] = rotl<10>(td(11));
+        b[2] = rotl<43>(td(12));
+        b[12] = rotl<25>(td(13));
+        b[22] = rotl<39>(td(14));
+        b[23] = rotl<41>(td(15));
+        b[8] = rotl<45>(td(16));
+        b[18] = rotl<15>(td(17));
+        b[3] = rotl<21>(td(18));
+        b[13] = rotl<8>(td(19));
+        b[14] = rotl<18>(td(20));
+        b[24] = rotl<2>(td(21));
+        b[9] = rotl<61>(td(22));
+        b[19] = rotl<56>(td(23));
+        b[4] = rotl<14>(td(24));
         // chi: A[x+5y] = B[x+5y] ^ (~B[x+1+5y] & B[x+2+5y])
 #pragma unroll
         for (int y = 0; y < 25; y += 5)
@@ -123,10 +132,11 @@ __device__ __forceinline__ void keccak_f(lane s[25]) {
 // half with one v_alignbit_b32, the same instruction on both lanes
 // (R < 32: alignbit(own, partner, 32 - R), the low half of the even lane and
 // the high half of the odd one; R > 32: alignbit(partner, own, 64 - R)).
-// Per round and lane: 10 XOR3 (parities), 5 moves + 5 alignbits + 5 XORs (D),
-// 25 XORs (A ^ D), 24 moves + 24 alignbits (rho) and 25 chi, about 125
-// instructions against 183 for the one-lane permutation, so a pair finishes a
-// permutation in about two thirds of the time at 1.36 times the instructions.
+// Per round and lane: 10 XOR3 (parities), 5 moves + 5 alignbits (the rotated
+// parities), 25 XOR3 (A ^ D), 24 moves + 24 alignbits (rho) and 25 chi, about
+// 118 instructions against 174 for the one-lane permutation, so a pair
+// finishes a permutation in about two thirds of the time at 1.35 times the
+// instructions.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
     // bound_ctrl set: no lane of the pair is ever out of range, and the
@@ -155,36 +165,38 @@ __device__ __forceinline__ void keccak_f_pair(uint32_t s[25], uint32_t hm) {
         uint32_t c[5];
 #pragma unroll
         for (int x = 0; x < 5; x++) c[x] = xor3(xor3(s[x], s[x + 5], s[x + 10]), s[x + 15], s[x + 20]);
-        uint32_t d[5];
+        uint32_t rc[5];
 #pragma unroll
-        for (int x = 0; x < 5; x++) d[x] = c[(x + 4) % 5] ^ rotl_half<1>(c[(x + 1) % 5]);
+        for (int x = 0; x < 5; x++) rc[x] = rotl_half<1>(c[x]);
+        // A ^ D in one three-input XOR, as in the one-lane permutation
+        const auto td = [&](int i) { return xor3(s[i], c[(i % 5 + 4) % 5], rc[(i % 5 + 1) % 5]); };
         // theta + rho + pi, the one-lane permutation's table
         uint32_t b[25];
-        b[0] = s[0] ^ d[0];
-        b[10] = rotl_half<1>(s[1] ^ d[1]);
-        b[20] = rotl_half<62>(s[2] ^ d[2]);
-        b[5] = rotl_half<28>(s[3] ^ d[3]);
-        b[15] = rotl_half<27>(s[4] ^ d[4]);
-        b[16] = rotl_half<36>(s[5] ^ d[0]);
-        b[1] = rotl_half<44>(s[6] ^ d[1]);
-        b[11] = rotl_half<6>(s[7] ^ d[2]);
-        b[21] = rotl_half<55>(s[8] ^ d[3]);
-        b[6] = rotl_half<20>(s[9] ^ d[4]);
-        b[7] = rotl_half<3>(s[10] ^ d[0]);
-        b[17] = rotl_half<10>(s[11] ^ d[1]);
-        b[2] = rotl_half<43>(s[12] ^ d[2]);
-        b[12] = rotl_half<25>(s[13] ^ d[3]);
-        b[22] = rotl_half<39>(s[14] ^ d[4]);
-        b[23] = rotl_half<41>(s[15] ^ d[0]);
-        b[8] = rotl_half<45>(s[16] ^ d[1]);
-        b[18] = rotl_half<15>(s[17] ^ d[2]);
-        b[3] = rotl_half<21>(s[18] ^ d[3]);
-        b[13] = rotl_half<8>(s[19] ^ d[4]);
-        b[14] = rotl_half<18>(s[20] ^ d[0]);
-        b[24] = rotl_half<2>(s[21] ^ d[1]);
-        b[9] = rotl_half<61>(s[22] ^ d[2]);
-        b[19] = rotl_half<56>(s[23] ^ d[3]);
-        b[4] = rotl_half<14>(s[24] ^ d[4]);
+        b[0] = td(0);
+        b[10] = rotl_half<1>(td(1));
+        b[20] = rotl_half<62>(td(2));
+        b[5] = rotl_half<28>(td(3));
+        b[15] = rotl_half<27>(td(4));
+        b[16] = rotl_half<36>(td(5));
+        b[1] = rotl_half<44>(td(6));
+        b[11] = rotl_half<6>(td(7));
+        b[21] = rotl_half<55>(td(8));
+        b[6] = rotl_half<20>(td(9));
+        b[7] = rotl_half<3>(td(10));
+        b[17] = rotl_half<10>(td(11));
+        b[2] = rotl_half<43>(td(12));
+        b[12] = rotl_half<25>(td(13));
+        b[22] = rotl_half<39>(td(14));
+        b[23] = rotl_half<41>(td(15));
+        b[8] = rotl_half<45>(td(16));
+        b[18] = rotl_half<15>(td(17));
+        b[3] = rotl_half<21>(td(18));
+        b[13] = rotl_half<8>(td(19));
+        b[14] = rotl_half<18>(td(20));
+        b[24] = rotl_half<2>(td(21));
+        b[9] = rotl_half<61>(td(22));
+        b[19] = rotl_half<56>(td(23));
+        b[4] = rotl_half<14>(td(24));
 #pragma unroll
         for (int y = 0; y < 25; y += 5)
 #pragma unroll
