@@ -1892,7 +1892,9 @@ __global__ __launch_bounds__(KB) void k_verify_paths(const VerifyJob *__restrict
 // Segment g owns blocks [blk0, blk0 + nblk) and grid-strides over its own
 // `count` items; a block finds its segment by binary search on blk0.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ const Seg &my_seg(const Seg *segs, int nseg) {
+// The block's segment, by value: its fields stay in scalar registers instead
+// of being reloaded after every store of the grid-stride loop.
+__device__ __forceinline__ Seg my_seg(const Seg *segs, int nseg) {
     int lo = 0, hi = nseg - 1;
     const uint32_t bx = blockIdx.x;
     while (lo < hi) {
@@ -1908,7 +1910,7 @@ __device__ __forceinline__ const Seg &my_seg(const Seg *segs, int nseg) {
 
 // in = evaluations, out = codeword, aux = level-0 twiddles, arg = b
 __global__ __launch_bounds__(256) void ks_ntt_first(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const int b = (int)g.arg;
     const size_t cm = ((size_t)1 << b) - 1;
     const gf *data = (const gf *)g.in, *ft0 = (const gf *)g.aux;
@@ -1925,7 +1927,7 @@ __global__ __launch_bounds__(256) void ks_ntt_first(const Seg *segs, int nseg) {
 
 // out = codeword (in place), aux = level-s twiddles, arg = s + b
 __global__ __launch_bounds__(256) void ks_ntt_layer(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const int sb = (int)g.arg;
     const size_t tmask = ((size_t)1 << sb) - 1;
     gf *cw = (gf *)g.out;
@@ -1947,7 +1949,7 @@ __global__ __launch_bounds__(256) void ks_ntt_layer(const Seg *segs, int nseg) {
 template <int R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ks_ntt_multi(const Seg *segs,
                                                                                             int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const int K = (int)g.arg;
     const gf *__restrict__ t0 = (const gf *)g.aux;
     const gf *__restrict__ t1 = t0 + ((size_t)1 << K);
@@ -1958,27 +1960,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 __global__ __launch_bounds__(KB) void ks_leaves_elem(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const gf *rows = (const gf *)g.in;
     uint64_t *out = (uint64_t *)g.out;
     SEG_LOOP(g, i) {
         gf v = gf_ld(rows + i);
         uint64_t m[2] = {v.lo, v.hi}, d[4];
-        kk::keccak256_words<2>(m, d);
+        kk::keccak256_words<2, true>(m, d);
         st_digest(out + 4 * i, d);
     }
 }
 
 
 __global__ __launch_bounds__(KB) void ks_nodes(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const uint64_t *prev = (const uint64_t *)g.in;
     uint64_t *out = (uint64_t *)g.out;
     SEG_LOOP(g, i) {
         const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(prev + 8 * i);
         ulonglong2 a = p[0], b = p[1], c = p[2], e = p[3];
         uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
-        kk::keccak256_words<8>(m, d);
+        kk::keccak256_words<8, true>(m, d);
         st_digest(out + 4 * i, d);
     }
 }
@@ -1987,7 +1989,7 @@ __global__ __launch_bounds__(KB) void ks_nodes(const Seg *segs, int nseg) {
 // so lv = log2(h / count)), x.lo = pair_w; see subtree()
 __global__ __launch_bounds__(kSubThreads) void ks_subtree(const Seg *segs, int nseg) {
     __shared__ ulonglong2 buf[kSubLds][2];
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     uint64_t *prev = (uint64_t *)g.in;
     const int lv = __builtin_ctzll(g.arg) - __builtin_ctzll(g.count);
     const int pair_w = (int)g.x.lo;
@@ -2001,7 +2003,7 @@ __global__ __launch_bounds__(kSubThreads) void ks_subtree(const Seg *segs, int n
 // leaves its sum in partial[x.lo + b]) in one launch, so the small inner
 // products run beside the leaf hashing instead of as a launch of their own.
 __global__ __launch_bounds__(KB) void ks_leaves_pair_dot(const Seg *segs, int nseg, gf *__restrict__ partial) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     if (g.out) {
         const gf *pi = (const gf *)g.in;
         uint64_t *out = (uint64_t *)g.out;
@@ -2036,7 +2038,7 @@ __device__ __forceinline__ gf seg_challenge(const Seg &g) { return g.arg ? gf_ld
 // in = table (2*count), out = fixed table (count), challenge per
 // seg_challenge
 __global__ __launch_bounds__(CB) void ks_fix(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const gf *in = (const gf *)g.in;
     gf *out = (gf *)g.out;
     const size_t half = g.count;
@@ -2082,7 +2084,7 @@ __device__ __forceinline__ void seg_fold_fix(const Seg &g, const GfTab &T, const
 __global__ __launch_bounds__(CB) void ks_fold_fix(const Seg *segs, int nseg) {
     __shared__ GfTab T;
     __shared__ gf u[128];
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const gf x = seg_challenge(g);
     if (g.count >= 2 * (size_t)g.nblk * CB) {
         gf_tab_build(T, u, x);
@@ -2176,7 +2178,7 @@ __device__ void fs_round(FsState *st, int k, gf S, const uint64_t *root) {
 // FsState, count = round k, x = (first block, blocks) of the round's inner
 // product sums in `partial`
 __global__ __launch_bounds__(64) void ks_round(const Seg *segs, int nseg, const gf *__restrict__ partial) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const uint32_t b0 = (uint32_t)g.x.lo, nb = (uint32_t)g.x.hi;
     gf acc = gf_zero();
     for (uint32_t t = threadIdx.x; t < nb; t += 64) acc = gf_add(acc, gf_ld(partial + b0 + t));
@@ -2246,7 +2248,7 @@ __global__ __launch_bounds__(kSubThreads) void k_top_round(uint64_t *prev, size_
 
 // in = A (2^m), aux = B, out = table, arg = m
 __global__ __launch_bounds__(CB) void ks_outer(const Seg *segs, int nseg) {
-    const Seg &g = my_seg(segs, nseg);
+    const Seg g = my_seg(segs, nseg);
     const gf *A = (const gf *)g.in, *B = (const gf *)g.aux;
     gf *out = (gf *)g.out;
     const int m = (int)g.arg;
