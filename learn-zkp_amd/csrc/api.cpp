@@ -601,13 +601,18 @@ void product_sumcheck(bfd_ctx *ctx, uint32_t n, const void *f_src, const void *g
     }
     gf *fins = (gf *)((uint8_t *)dfs + fin_off);
     {
+        // the single-block launch ends by writing the region up to the last
+        // round's table entries into hreg (stream order puts it after the
+        // upload that read hreg)
         const size_t half = (size_t)1 << (n - 1 - k);
         const gf *fsrc = k <= 1 ? fs0 : wf, *gsrc = k <= 1 ? gs0 : wg;
-        ctx->timer.run(st, "k_prod_tail",
-                       [&] { return launch::prod_tail(st, fsrc, gsrc, fins, fins + 2, eq, half, (int)k, dfs); });
+        uint8_t *dpin = nullptr;
+        hip_check(hipHostGetDevicePointer((void **)&dpin, ctx->pinned.p, 0), "hipHostGetDevicePointer");
+        ctx->timer.run(st, "k_prod_tail", [&] {
+            return launch::prod_tail(st, fsrc, gsrc, fins, fins + 2, eq, half, (int)k, dfs, dpin + fb,
+                                     (uint32_t)(fin_off + 64));
+        });
     }
-    // one read-back: the region up to the last round's table entries
-    hip_check(hipMemcpyAsync(hreg, dfs, fin_off + 64, hipMemcpyDeviceToHost, st), "copy transcript back");
     hip_check(hipStreamSynchronize(st), "sumcheck");
     ctx->timer.collect();
     memcpy(&h, hreg, sizeof h);
@@ -741,6 +746,8 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
     memcpy(pin + quad_pairs_off(npairs, n, in_cap, m), pair_idx, 8 * (size_t)npairs);
     hip_check(hipMemcpyAsync(fs.p, pin, img, hipMemcpyHostToDevice, st), "copy transcript");
     QuadFs *dfs = (QuadFs *)fs.p;
+    uint8_t *dpin = nullptr;  // pin as the device addresses it
+    hip_check(hipHostGetDevicePointer((void **)&dpin, pin, 0), "hipHostGetDevicePointer");
     const uint32_t *dpairs = (const uint32_t *)(fs.as<uint8_t>() + quad_pairs_off(npairs, n, in_cap, m));
     gf *slots = (gf *)(fs.as<uint8_t>() + quad_slots_off(npairs, n, in_cap, m));
     unsigned *counters = (unsigned *)(fs.as<uint8_t>() + quad_counters_off(npairs, n, in_cap, m));
@@ -795,7 +802,7 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
         if (half <= tail) {
             ctx->timer.run(st, "k_quad_tail", [&] {
                 return launch::quad_tail(st, cur, cstride, (gf *)(fs.as<uint8_t>() + fin_off), 1, dpairs, npairs, m,
-                                         half, (int)k, dfs);
+                                         half, (int)k, dfs, dpin, (uint32_t)(fin_off + 16 * (size_t)m));
             });
             tail_ran = true;
             break;
@@ -812,12 +819,13 @@ void quadratic_sumcheck(bfd_ctx *ctx, uint32_t n, uint32_t m, const void *polys,
             cstride = NW;
         }
     }
-    // one read-back: the region up to the finals (the finals too when the
-    // single-block rounds wrote them there)
-    const size_t rb = tail_ran ? fin_off + 16 * (size_t)m : in_off + 32;
-    hip_check(hipMemcpyAsync(pin, fs.p, rb, hipMemcpyDeviceToHost, st), "copy transcript back");
-    if (!tail_ran)  // the m final evaluations P_p[0] of the working tables
+    // the single-block rounds end by writing the region up to the finals into
+    // pin; otherwise one read-back of the region and one of the finals
+    if (!tail_ran) {
+        hip_check(hipMemcpyAsync(pin, fs.p, in_off + 32, hipMemcpyDeviceToHost, st), "copy transcript back");
+        // the m final evaluations P_p[0] of the working tables
         hip_check(hipMemcpy2DAsync(final_evals, 16, W.p, 16 * NW, 16, m, hipMemcpyDeviceToHost, st), "copy finals");
+    }
     hip_check(hipStreamSynchronize(st), "sumcheck");
     ctx->timer.collect();
     memcpy(&h, pin, sizeof h);

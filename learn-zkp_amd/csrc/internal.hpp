@@ -68,14 +68,15 @@ int guarded(F &&f) {
 // ---------------------------------------------------------------------------
 // device memory: stream-ordered allocations from the device's default pool
 // ---------------------------------------------------------------------------
-// Freed blocks of one stream kept for reuse: a block of at least kMin bytes
-// waits here for the next request of exactly its size on the same stream, so
-// repeated commits and opens of one shape make no allocator calls.  Reuse is
-// safe by stream order (the block's last use and its next use are on the
-// same stream).  Holds at most `cap` bytes; a block that does not fit empties
-// the cache first.
+// Freed blocks of one stream kept for reuse: a block waits here for the next
+// request of exactly its size on the same stream, so
+// repeated commits, opens and sumchecks of one shape make no allocator calls
+// (a stream-ordered free costs the calling thread 15-30 us, even for the few
+// kilobytes of a transcript image).  Reuse is safe by stream order (the
+// block's last use and its next use are on the same stream).  Holds at most
+// `cap` bytes; a block that does not fit empties the cache first.
 struct BlockCache {
-    static constexpr size_t kMin = (size_t)1 << 20;
+    static constexpr size_t kMin = 1;
     std::mutex m;
     std::multimap<size_t, void *> blocks;
     size_t held = 0, cap = 0;

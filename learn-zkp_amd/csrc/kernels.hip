@@ -1091,8 +1091,22 @@ __device__ __noinline__ ProdRound prod_transcript_call(ProdFs *st, int k, gf A0,
     return prod_transcript(st, k, A0, A1, A2, K, L, have_prev, prev);
 }
 
+// The region's first `bytes` bytes (a multiple of 8) into pinned host memory,
+// by the whole block, after the block's own writes to the region: every
+// thread fences and meets the others, then reads through L2 (agent-scope
+// loads, so no stale line of this CU's L1 is used) and stores to the host.
+// Takes the place of a device-to-host copy after the launch.
+__device__ __forceinline__ void region_to_host(const uint8_t *region, uint8_t *host, uint32_t bytes) {
+    __threadfence();
+    __syncthreads();
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(region);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(host);
+    for (uint32_t w = threadIdx.x; w < bytes / 8; w += blockDim.x)
+        dst[w] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, const gf *gsrc, gf *fout, gf *gout,
-                                                            EqFactors eq, size_t half, int k0, ProdFs *fs) {
+                                                            EqFactors eq, size_t half, int k0, ProdFs *fs, uint8_t *host, uint32_t host_bytes) {
     // F_k of round k lives in Fb when k - k0 is even, in Fa when odd; Fa
     // first holds the input tables when k0 > 0 (4 half entries)
     __shared__ gf Fa[4 * kTailPairs], Ga[4 * kTailPairs], Fb[2 * kTailPairs], Gb[2 * kTailPairs];
@@ -1339,6 +1353,7 @@ __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, cons
         for (int w = 0; w < 4; w++) R.d[w] = wdig[w];
         prod_store_round(fs, R);
     }
+    if (host) region_to_host(base, host, host_bytes);
 }
 
 // ---------------------------------------------------------------------------
@@ -1690,7 +1705,8 @@ constexpr size_t kQuadTailElems = 3584;  // table entries in LDS (56 KiB)
 constexpr uint32_t kQuadTailPairs = 128;
 __global__ __launch_bounds__(kQuadTailThreads) void k_quad_tail(const gf *src, size_t src_stride, gf *fin,
                                                                 size_t fin_stride, const uint32_t *__restrict__ pairs,
-                                                                uint32_t m, size_t h0, int k0, QuadFs *fs) {
+                                                                uint32_t m, size_t h0, int k0, QuadFs *fs,
+                                                                uint8_t *host, uint32_t host_bytes) {
     __shared__ gf F[kQuadTailElems];
     __shared__ gf S3[3][kQuadTailPairs];
     __shared__ gf xs;
@@ -1774,6 +1790,7 @@ __global__ __launch_bounds__(kQuadTailThreads) void k_quad_tail(const gf *src, s
         fs->out_len = 16;
         fs->x = xs;
     }
+    if (host) region_to_host((const uint8_t *)fs, host, host_bytes);
 }
 
 __global__ __launch_bounds__(CB) void k_fix_multi(gf *__restrict__ P, size_t stride, size_t half, gf x) {
@@ -2673,9 +2690,10 @@ hipError_t prod_look(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, g
     return hipGetLastError();
 }
 hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, EqFactors eq, size_t half,
-                     int k0, ProdFs *fs) {
-    if (!half || half > kTailPairs || (half & (half - 1)) || eq.lo < 0 || eq.lo > 62) return hipErrorInvalidValue;
-    k_prod_tail<<<1, kTailThreads, 0, st>>>(fsrc, gsrc, fout, gout, eq, half, k0, fs);
+                     int k0, ProdFs *fs, uint8_t *host, uint32_t host_bytes) {
+    if (!half || half > kTailPairs || (half & (half - 1)) || eq.lo < 0 || eq.lo > 62 || (host_bytes & 7))
+        return hipErrorInvalidValue;
+    k_prod_tail<<<1, kTailThreads, 0, st>>>(fsrc, gsrc, fout, gout, eq, half, k0, fs, host, host_bytes);
     return hipGetLastError();
 }
 hipError_t mle_dot(hipStream_t st, const gf *data, size_t w, const gf *A, int m, const gf *B, size_t rows, gf *partial,
@@ -2738,9 +2756,11 @@ size_t quad_tail_max_half(uint32_t m, uint32_t npairs) {
     return h;
 }
 hipError_t quad_tail(hipStream_t st, const gf *src, size_t src_stride, gf *fin, size_t fin_stride,
-                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t h0, int k0, QuadFs *fs) {
-    if (!h0 || (h0 & (h0 - 1)) || h0 > quad_tail_max_half(m, npairs)) return hipErrorInvalidValue;
-    k_quad_tail<<<1, kQuadTailThreads, 0, st>>>(src, src_stride, fin, fin_stride, pairs, m, h0, k0, fs);
+                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t h0, int k0, QuadFs *fs,
+                     uint8_t *host, uint32_t host_bytes) {
+    if (!h0 || (h0 & (h0 - 1)) || h0 > quad_tail_max_half(m, npairs) || (host_bytes & 7)) return hipErrorInvalidValue;
+    k_quad_tail<<<1, kQuadTailThreads, 0, st>>>(src, src_stride, fin, fin_stride, pairs, m, h0, k0, fs, host,
+                                                host_bytes);
     return hipGetLastError();
 }
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,

@@ -310,9 +310,11 @@ hipError_t prod_step(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fdst, g
 // prod_tail_max_pairs(), a power of two), all in one single-block launch:
 // the tables (4 * half entries at fsrc, gsrc; 2 * half in round 0) are
 // loaded once into the block's LDS and folded there.  The last round's two
-// entries of each table are written to fout, gout.
+// entries of each table are written to fout, gout.  With host set, the
+// kernel ends by copying the region's first host_bytes bytes (a multiple of
+// 8) into that pinned host memory, in place of a copy after the launch.
 hipError_t prod_tail(hipStream_t st, const gf *fsrc, const gf *gsrc, gf *fout, gf *gout, EqFactors eq, size_t half,
-                     int k0, ProdFs *fs);
+                     int k0, ProdFs *fs, uint8_t *host = nullptr, uint32_t host_bytes = 0);
 size_t prod_tail_max_pairs();
 // Rounds of at most prod_look_max_pairs() pairs (and more than
 // prod_tail_max_pairs()) with one round of lookahead: every block folds its
@@ -367,8 +369,10 @@ size_t quad_tail_max_half(uint32_t m, uint32_t npairs);
 hipError_t quad_look(hipStream_t st, const gf *src, size_t src_stride, gf *dst, size_t dst_stride,
                      const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t half, int k, QuadFs *fs, bool first);
 size_t quad_look_max_half();
+// host, host_bytes: as for prod_tail
 hipError_t quad_tail(hipStream_t st, const gf *src, size_t src_stride, gf *fin, size_t fin_stride,
-                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t h0, int k0, QuadFs *fs);
+                     const uint32_t *pairs, uint32_t npairs, uint32_t m, size_t h0, int k0, QuadFs *fs,
+                     uint8_t *host = nullptr, uint32_t host_bytes = 0);
 // verifier kernels: fold chains (thread per proof x query) and Merkle paths
 // (thread per proof x query x path, d + 1 paths)
 hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
