@@ -140,11 +140,10 @@ __device__ __forceinline__ void keccak_f(lane s[25]) {
 // half with one v_alignbit_b32, the same instruction on both lanes
 // (R < 32: alignbit(own, partner, 32 - R), the low half of the even lane and
 // the high half of the odd one; R > 32: alignbit(partner, own, 64 - R)).
-// Per round and lane: 10 XOR3 (parities), 5 moves + 5 alignbits (the rotated
-// parities), 25 XOR3 (A ^ D), 24 moves + 24 alignbits (rho) and 25 chi, about
-// 118 instructions against 174 for the one-lane permutation, so a pair
-// finishes a permutation in about two thirds of the time at 1.35 times the
-// instructions.
+// Per round and lane: 10 XOR3 (parities), 5 moves + 5 alignbits + 5 XORs (D),
+// 25 XORs (A ^ D), 24 moves + 24 alignbits (rho) and 25 chi, about 123
+// instructions against 174 for the one-lane permutation, so a pair finishes a
+// permutation in about two thirds of the time at 1.4 times the instructions.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
     // bound_ctrl set: no lane of the pair is ever out of range, and the
@@ -166,18 +165,20 @@ __device__ __forceinline__ uint32_t rotl_half(uint32_t w) {
 }
 
 // s[x + 5y]: this lane's half of state lane (x, y); hm = ~0 on the odd lane
-// of the pair, 0 on the even lane
+// of the pair, 0 on the even lane.  D is formed first and XORed into each
+// lane with a second operation: a single wave finishes the permutation in
+// 5.2 us that way against 5.55 us with the one-lane permutation's merged
+// three-input form, which has five instructions fewer per round.
 __device__ __forceinline__ void keccak_f_pair(uint32_t s[25], uint32_t hm) {
 #pragma unroll
     for (int round = 0; round < 24; round++) {
         uint32_t c[5];
 #pragma unroll
         for (int x = 0; x < 5; x++) c[x] = xor3(xor3(s[x], s[x + 5], s[x + 10]), s[x + 15], s[x + 20]);
-        uint32_t rc[5];
+        uint32_t d[5];
 #pragma unroll
-        for (int x = 0; x < 5; x++) rc[x] = rotl_half<1>(c[x]);
-        // A ^ D in one three-input XOR, as in the one-lane permutation
-        const auto td = [&](int i) { return xor3(s[i], c[(i % 5 + 4) % 5], rc[(i % 5 + 1) % 5]); };
+        for (int x = 0; x < 5; x++) d[x] = c[(x + 4) % 5] ^ rotl_half<1>(c[(x + 1) % 5]);
+        const auto td = [&](int i) { return s[i] ^ d[i % 5]; };
         // theta + rho + pi, the one-lane permutation's table
         uint32_t b[25];
         b[0] = td(0);
