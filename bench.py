@@ -773,10 +773,14 @@ def main() -> None:
     # second inside it against an isolated, memory-free Keccak-f kernel
     perms_per_step = {"k_nodes": per_step_units["k_nodes"], "k_leaves_elem": M,
                       "k_leaves_pair": per_step_units["k_leaves_pair"]}.get(name)
-    ctx.microbench(2, 1 << 20, 2)
-    r_k = (1 << 20) * 64 / (ctx.microbench(2, 1 << 20, 64) * 1e-3)
-    ctx.microbench(1, 1 << 22, 2)
-    r_g = (1 << 22) * 64 / (ctx.microbench(1, 1 << 22, 64) * 1e-3)
+    # isolated rates at launch sizes past the point where they stop rising
+    # (a smaller launch pays its first and last waves' ramp: 9.1 G perm/s at
+    # 2^20 chains, 10.1 at 2^22, 10.6 at 2^23; 90, 92, 94 G mul/s at 2^22,
+    # 2^23, 2^24 products)
+    ctx.microbench(2, 1 << 23, 2)
+    r_k = (1 << 23) * 64 / (ctx.microbench(2, 1 << 23, 64) * 1e-3)
+    ctx.microbench(1, 1 << 24, 2)
+    r_g = (1 << 24) * 64 / (ctx.microbench(1, 1 << 24, 64) * 1e-3)
     ctx.microbench(4, 1 << 22, 2)
     r_t = (1 << 22) * 64 / (ctx.microbench(4, 1 << 22, 64) * 1e-3)
     # single-wave latency of one lane-pair Keccak-f (the transcript step's

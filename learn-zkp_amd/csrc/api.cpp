@@ -1034,11 +1034,13 @@ int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms) {
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);
         hipStream_t st = ctx->stream;
-        size_t bytes = what == 2 ? 200 * n : (what == 5 ? 64 : 32 * n);
+        // Keccak: 8 message words per chain; the rest: two elements per item
+        const size_t bytes = what == 2 ? 64 * n : (what == 5 ? 64 : 32 * n);
         DevBuf io(bytes, st);
-        std::vector<uint8_t> h(bytes);
-        for (size_t i = 0; i < bytes; i++) h[i] = (uint8_t)(i * 2654435761u >> 13);
+        std::vector<uint64_t> h(bytes / 8);
+        for (size_t i = 0; i < h.size(); i++) h[i] = (i + 1) * 0x9E3779B97F4A7C15ull ^ (i >> 5);
         hip_check(hipMemcpyAsync(io.p, h.data(), bytes, hipMemcpyHostToDevice, st), "copy");
+        hip_check(hipStreamSynchronize(st), "copy");  // h goes out of scope before the launches end
         hipEvent_t a, e;
         hip_check(hipEventCreate(&a), "event");
         hip_check(hipEventCreate(&e), "event");
