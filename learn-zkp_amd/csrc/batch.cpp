@@ -110,13 +110,13 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
     ctx->timer.collect();
     lane.commit_finish(all);
     phase("batch_finish");
-    // open ticks, two in flight: while one runs, the next is enqueued, so
-    // the host's share of a tick (sampling queries, assembling proofs)
-    // overlaps the device's
+    // open ticks, up to Lane::kSets in flight: while the oldest runs, the
+    // next ones are enqueued, so the host's share of a tick (sampling
+    // queries, assembling proofs) overlaps the device's
     std::deque<OpenTick> inflight;
     std::vector<Job *> work;
     for (;;) {
-        if (inflight.size() < 2) {
+        if (inflight.size() < (size_t)Lane::kSets) {
             work.clear();
             for (Job *j : all)
                 if (j->stage == Job::Open || j->stage == Job::Gather) work.push_back(j);

@@ -367,7 +367,7 @@ void Lane::commit_finish(std::vector<Job *> &jobs) {
 // ---------------------------------------------------------------------------
 OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
     OpenTick t;
-    cur_ ^= 1;
+    cur_ = (cur_ + 1) % kSets;
     const int c = cur_;
     t.par = c;
     t.done = ev_[c];

@@ -106,6 +106,11 @@ struct OpenTick {
 // One execution lane: a stream with its own staging buffers.
 class Lane {
   public:
+    // open ticks that may be in flight at once (each stages through its own
+    // set of buffers): the host's share of a tick (sampling the queries of
+    // jobs past their last round, copying finished proofs) overlaps the
+    // device's next ticks, which at the end of a batch are short
+    static constexpr int kSets = 3;
     Lane(hipStream_t st, KernelTimer *timer);
     ~Lane();
     hipStream_t stream() const { return st_; }
@@ -114,7 +119,7 @@ class Lane {
     void commit_tick(std::vector<Job *> &jobs);
     void commit_finish(std::vector<Job *> &jobs);
     // enqueue an open tick for the jobs with device work (stage Open or
-    // Gather); at most two ticks may be in flight, finished in order with
+    // Gather); at most kSets ticks may be in flight, finished in order with
     // open_finish once t.done has completed
     OpenTick open_tick(std::vector<Job *> &jobs);
     void open_finish(OpenTick &t);
@@ -136,15 +141,15 @@ class Lane {
     KernelTimer *timer_;
     bfd_tuning tune_ = default_tuning();
     int cus_ = 256;
-    // staging in two halves, alternating per tick: a tick may be enqueued
-    // while the previous one still runs, so it must not overwrite what that
-    // one's copies and kernels have yet to read
+    // staging in kSets sets, one per tick in turn: up to kSets open ticks
+    // may be in flight, so a tick must not overwrite what the ones before it
+    // have yet to read
     int cur_ = 0;
-    hipEvent_t ev_[2] = {nullptr, nullptr};
-    PinnedBuf seg_pin_[2], rb_pin_[2], gather_pin_[2], stage_pin_[2], plan_pin_[2];
-    DevBuf seg_dev_[2], rb_dev_[2], addr_dev_[2], gather_dev_[2], plan_dev_[2], eq_dev_[2];
+    hipEvent_t ev_[kSets] = {};
+    PinnedBuf seg_pin_[kSets], rb_pin_[kSets], gather_pin_[kSets], stage_pin_[kSets], plan_pin_[kSets];
+    DevBuf seg_dev_[kSets], rb_dev_[kSets], addr_dev_[kSets], gather_dev_[kSets], plan_dev_[kSets], eq_dev_[kSets];
     DevBuf partials_;  // inner-product sums (stream-ordered: one tick at a time uses them)
-    size_t seg_cap_[2] = {0, 0}, seg_used_[2] = {0, 0}, flushed_[2] = {0, 0};
+    size_t seg_cap_[kSets] = {}, seg_used_[kSets] = {}, flushed_[kSets] = {};
     struct Pending {  // a recorded segmented launch (ring entries at off)
         SegKernel k;
         size_t off;

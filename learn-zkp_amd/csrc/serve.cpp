@@ -123,7 +123,7 @@ void worker_loop(bfd_server *S) {
                     commit_busy = true;
                 }
             }
-            if (open_q.size() < 2) {
+            if (open_q.size() < (size_t)Lane::kSets) {
                 work.clear();
                 for (Job *j : opening)
                     if (j->stage == Job::Open || j->stage == Job::Gather) work.push_back(j);
