@@ -328,11 +328,13 @@ int bfd_verify(bfd_ctx *ctx, const bfd_code *code, size_t width, const uint8_t r
  * Montgomery-form product a*b*X^-128 (Mul) and inverse (invert_or_zero,
  * 0 for 0), little-endian 16-byte elements ---- */
 int bfd_field_mul(const uint8_t a[16], const uint8_t b[16], uint8_t out[16]);
-/* The same product on the device, elementwise: out[i] = a[i] * b[i], or
- * a[i] * b[0] with shared_b set (b prepared once, the form the fold, fix and
- * NTT kernels use for a challenge or twiddle).  a, out: n elements; b: n
- * (1 with shared_b); host buffers, or device memory with on_device. */
-int bfd_field_mul_batch(bfd_ctx *ctx, const void *a, const void *b, int shared_b, void *out, size_t n,
+/* The same product on the device, in each of the forms the kernels use:
+ * form 0, out[i] = a[i] * b[i] in registers; form 1, out[i] = a[i] * b[0]
+ * through b[0]'s window table (the products by a round challenge); form 2,
+ * out[i] = a[i] * b[i] with four lanes per product (the transcript steps'
+ * serial products).  a, out: n elements; b: n (1 with form 1); host
+ * buffers, or device memory with on_device.  Other forms: BFD_ERR_ARG. */
+int bfd_field_mul_batch(bfd_ctx *ctx, const void *a, const void *b, int form, void *out, size_t n,
                         int on_device);
 int bfd_field_inv(const uint8_t a[16], uint8_t out[16]);
 
@@ -388,12 +390,13 @@ int bfd_tuning_load(bfd_ctx *ctx, const char *path);
 
 /* ---- measurement helpers ---- */
 /* Times `iters` dependent multiplies (variant 0: LDS comb, 1: register
- * Karatsuba, 4: by one shared factor through its LDS window table), `iters`
- * chained Keccak-256 hashes of 64-byte messages (2, the Merkle node hash) on
- * n lanes, one copy of n 16-byte elements (3), or `iters` chained hashes on
+ * Karatsuba, 4: by one shared factor through its LDS window table, 6: four
+ * lanes per product, lanes 0..15 of each wave chaining), `iters` chained
+ * Keccak-256 hashes of 64-byte messages (2, the Merkle node hash) on n
+ * lanes, one copy of n 16-byte elements (3), or `iters` chained hashes on
  * one lane pair of one wave (5, the transcript step's form; n ignored);
- * returns milliseconds.  With n = 64 variants 1 and 2 run on one wave, so
- * the time per iteration is the latency of one product or hash.  No
+ * returns milliseconds.  With n = 64 variants 1, 2 and 6 run on one wave,
+ * so the time per iteration is the latency of one product or hash.  No
  * reference counterpart (measurement only). */
 int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms);
 /* Per-kernel timing of the last commit/open on this context: fills up to

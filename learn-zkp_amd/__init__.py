@@ -667,14 +667,16 @@ def eval_data(flat, point, width: int = 1, ctx: Optional[Context] = None, device
     return out
 
 
-def field_mul_batch(a, b, shared_b: bool = False, ctx: Optional[Context] = None) -> np.ndarray:
+def field_mul_batch(a, b, shared_b: bool = False, ctx: Optional[Context] = None, wide: bool = False) -> np.ndarray:
     """Elementwise POLYVAL product on the device (bfd_field_mul_batch): a[i] * b[i],
-    or a[i] * b[0] with shared_b."""
+    or a[i] * b[0] with shared_b (through b[0]'s window table); wide takes the
+    four-lanes-per-product form of the transcript steps."""
     ctx = ctx or default_context()
     a, b = np.ascontiguousarray(_elems(a)), np.ascontiguousarray(_elems(b))
     out = np.zeros_like(a)
+    form = 1 if shared_b else (2 if wide else 0)
     _check(lib().bfd_field_mul_batch(ctx.h, ctypes.c_void_p(a.ctypes.data), ctypes.c_void_p(b.ctypes.data),
-                                     int(shared_b), ctypes.c_void_p(out.ctypes.data), len(a), 0))
+                                     form, ctypes.c_void_p(out.ctypes.data), len(a), 0))
     return out
 
 

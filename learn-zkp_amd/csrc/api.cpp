@@ -977,14 +977,16 @@ int bfd_field_mul(const uint8_t a[16], const uint8_t b[16], uint8_t out[16]) {
     });
 }
 
-int bfd_field_mul_batch(bfd_ctx *ctx, const void *a, const void *b, int shared_b, void *out, size_t n,
+int bfd_field_mul_batch(bfd_ctx *ctx, const void *a, const void *b, int form, void *out, size_t n,
                         int on_device) {
     return guarded([&] {
         if (!ctx || ((!a || !b || !out) && n)) fail(BFD_ERR_ARG, "null argument");
+        if (form < 0 || form > 2) fail(BFD_ERR_ARG, "unknown product form");
         if (!n) return;
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);
         hipStream_t st = ctx->stream;
+        const bool shared_b = form == 1;
         const size_t nb = shared_b ? 1 : n;
         DevBuf buf;
         const gf *da = (const gf *)a, *db = (const gf *)b;
@@ -998,7 +1000,7 @@ int bfd_field_mul_batch(bfd_ctx *ctx, const void *a, const void *b, int shared_b
             db = p + n;
             dout = p + n + nb;
         }
-        ctx->timer.run(st, "k_mul", [&] { return launch::mul(st, da, db, dout, n, shared_b != 0); });
+        ctx->timer.run(st, "k_mul", [&] { return launch::mul(st, da, db, dout, n, form); });
         if (!on_device) hip_check(hipMemcpyAsync(out, dout, 16 * n, hipMemcpyDeviceToHost, st), "copy product");
         hip_check(hipStreamSynchronize(st), "mul");
         ctx->timer.collect();
@@ -1030,7 +1032,7 @@ int bfd_random_vec(uint64_t seed, size_t n, uint8_t *out) {
 int bfd_microbench(bfd_ctx *ctx, int what, size_t n, int iters, double *ms) {
     return guarded([&] {
         if (!ctx || !ms) fail(BFD_ERR_ARG, "null argument");
-        if (what < 0 || what > 5 || iters < 1) fail(BFD_ERR_ARG, "unknown micro-benchmark");
+        if (what < 0 || what > 6 || iters < 1) fail(BFD_ERR_ARG, "unknown micro-benchmark");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         set_device(ctx);
         hipStream_t st = ctx->stream;

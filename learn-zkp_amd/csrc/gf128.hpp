@@ -216,14 +216,9 @@ BFD_D void kara64(const pieces &a0, const pieces &a1, const pieces &a01, const p
     r[3] = h.hi;
 }
 
-BFD_D gf gf_mul_pre(gf a, const gf_pre &pb) {
-    pieces a0 = split4((uint32_t)a.lo), a1 = split4((uint32_t)(a.lo >> 32)), a2 = split4((uint32_t)a.hi),
-           a3 = split4((uint32_t)(a.hi >> 32));
-    pieces a01 = pxor(a0, a1), a23 = pxor(a2, a3), a02 = pxor(a0, a2), a13 = pxor(a1, a3), ax = pxor(a02, a13);
-    uint32_t L[4], H[4], Mm[4];
-    kara64(a0, a1, a01, pb.q[0], pb.q[1], pb.q[2], L);
-    kara64(a2, a3, a23, pb.q[3], pb.q[4], pb.q[5], H);
-    kara64(a02, a13, ax, pb.q[6], pb.q[7], pb.q[8], Mm);
+// The product from its three 64x64 Karatsuba products (L of the low halves,
+// H of the high halves, Mm of their sums), then the Montgomery fold.
+BFD_D gf gf_combine(const uint32_t L[4], const uint32_t H[4], const uint32_t Mm[4]) {
     // 256-bit product w0..w7
     uint32_t w[8];
     w[0] = L[0];
@@ -248,6 +243,17 @@ BFD_D gf gf_mul_pre(gf a, const gf_pre &pb) {
     w[7] = x3(x3(w[7], w[3], w[3] >> 1), w[3] >> 2, w[3] >> 7);
     w[5] = x3(x3(w[5], w[2] << 31, w[2] << 30), w[2] << 25, 0u);
     return gf{(uint64_t)w[4] | ((uint64_t)w[5] << 32), (uint64_t)w[6] | ((uint64_t)w[7] << 32)};
+}
+
+BFD_D gf gf_mul_pre(gf a, const gf_pre &pb) {
+    pieces a0 = split4((uint32_t)a.lo), a1 = split4((uint32_t)(a.lo >> 32)), a2 = split4((uint32_t)a.hi),
+           a3 = split4((uint32_t)(a.hi >> 32));
+    pieces a01 = pxor(a0, a1), a23 = pxor(a2, a3), a02 = pxor(a0, a2), a13 = pxor(a1, a3), ax = pxor(a02, a13);
+    uint32_t L[4], H[4], Mm[4];
+    kara64(a0, a1, a01, pb.q[0], pb.q[1], pb.q[2], L);
+    kara64(a2, a3, a23, pb.q[3], pb.q[4], pb.q[5], H);
+    kara64(a02, a13, ax, pb.q[6], pb.q[7], pb.q[8], Mm);
+    return gf_combine(L, H, Mm);
 }
 
 BFD_D gf gf_mul_reg(gf a, gf b) { return gf_mul_pre(a, gf_prepare(b)); }

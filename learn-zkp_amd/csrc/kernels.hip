@@ -299,10 +299,12 @@ __device__ __forceinline__ void subtree(uint64_t *__restrict__ prev, size_t h, i
                 buf[out + t][odd] = odd ? ulonglong2{d[2], d[3]} : ulonglong2{d[0], d[1]};
             }
         } else if (tid < w) {
+            // a latency-bound level: the two-step theta, whose single-wave
+            // latency is lower (7.9 against 9.1 us)
             const ulonglong2 a = buf[in + 2 * tid][0], b = buf[in + 2 * tid][1];
             const ulonglong2 c = buf[in + 2 * tid + 1][0], e = buf[in + 2 * tid + 1][1];
             uint64_t m[8] = {a.x, a.y, b.x, b.y, c.x, c.y, e.x, e.y}, d[4];
-            kk::keccak256_words<8>(m, d);
+            kk::keccak256_words<8, true>(m, d);
             buf[out + tid][0] = ulonglong2{d[0], d[1]};
             buf[out + tid][1] = ulonglong2{d[2], d[3]};
         }
@@ -539,6 +541,38 @@ __device__ __forceinline__ gf gf_shfl_xor(gf v, int off) {
 // condition can otherwise be lowered through a stack copy)
 __device__ __forceinline__ gf gf_sel(bool c, gf a, gf b) { return gf{c ? a.lo : b.lo, c ? a.hi : b.hi}; }
 
+// word v of lane Q of this lane's quad (DPP quad_perm [Q, Q, Q, Q])
+template <int Q>
+__device__ __forceinline__ uint32_t quad_lane(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, Q * 0x55, 0xF, 0xF, true);
+}
+
+// a * b with four lanes per product, for the serial chains of the
+// transcript steps: lanes base + l, l < 16, get a * b for the operands they
+// hold (the other lanes' results are unspecified; every lane of the wave
+// must take part).  Lane 4l + u forms one 64x64 Karatsuba product of lane
+// base + l's operands (u = 0 the low halves, 1 the high halves, 2 and 3
+// their sums), the quad exchanges them with DPP, and lane base + l takes the
+// folded result back: about 170 instructions on the chain instead of about
+// 380.
+__device__ __forceinline__ gf gf_mul_wide(gf a, gf b, int base = 0) {
+    const int lane = threadIdx.x & 63, u = lane & 3;
+    const gf A = lane_gf(a, base + (lane >> 2)), B = lane_gf(b, base + (lane >> 2));
+    const uint64_t x = u == 0 ? A.lo : (u == 1 ? A.hi : A.lo ^ A.hi);
+    const uint64_t y = u == 0 ? B.lo : (u == 1 ? B.hi : B.lo ^ B.hi);
+    const pieces x0 = split4((uint32_t)x), x1 = split4((uint32_t)(x >> 32));
+    const pieces y0 = split4((uint32_t)y), y1 = split4((uint32_t)(y >> 32));
+    uint32_t r[4], L[4], H[4], Mm[4];
+    kara64(x0, x1, pxor(x0, x1), y0, y1, pxor(y0, y1), r);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        L[q] = quad_lane<0>(r[q]);
+        H[q] = quad_lane<1>(r[q]);
+        Mm[q] = quad_lane<2>(r[q]);
+    }
+    return lane_gf(gf_combine(L, H, Mm), 4 * (lane - base));
+}
+
 // Pair sums of the product sumcheck with the pair's eq weight t multiplied
 // into f first: tf0 = t f0, tf1 = t f1, then A0 += tf0 g0, A1 += tf1 g1 and
 // A2 += (tf0 + tf1)(g0 + g1), five products per pair.
@@ -566,7 +600,7 @@ __device__ __forceinline__ void prod_scalars(const ProdFs *fs, const gf *e0s, in
     }
     const int lane = threadIdx.x & 63;
     const gf t = gf_add(gf_ld(e0s + (fs->n - k)), fs->x);
-    const gf p = gf_mul_reg(gf_sel(lane == 0, fs->s, gf_sel(lane == 1, fs->P1, fs->P2)), t);
+    const gf p = gf_mul_wide(gf_sel(lane == 0, fs->s, gf_sel(lane == 1, fs->P1, fs->P2)), t);
     s = lane_gf(p, 0);
     K = lane_gf(p, 1);
     L = lane_gf(p, 2);
@@ -585,7 +619,7 @@ __device__ __forceinline__ ProdPoly prod_coeffs(gf A0, gf A1, gf A2, gf K, gf L)
     const gf Q1 = gf_add(A0, gf_add(A1, A2));
     const gf u = gf_sel(lane & 1, L, K);
     const gf v = gf_sel(lane < 2, A0, gf_sel(lane < 4, Q1, A2));
-    const gf p = gf_mul_reg(u, v);  // lane 0: K A0, 1: L A0, 2: K Q1, 3: L Q1, 4: K A2, 5: L A2
+    const gf p = gf_mul_wide(u, v);  // lane 0: K A0, 1: L A0, 2: K Q1, 3: L Q1, 4: K A2, 5: L A2
     return ProdPoly{lane_gf(p, 0), gf_add(lane_gf(p, 1), lane_gf(p, 2)), gf_add(lane_gf(p, 3), lane_gf(p, 4)),
                     lane_gf(p, 5)};
 }
@@ -945,12 +979,12 @@ __global__ __launch_bounds__(CB) void k_prod_look(const gf *fsrc, const gf *gsrc
         const gf t = gf_add(gf_ld(e0s + iv + 1), xp);
         const gf s0 = fs->s, P1 = fs->P1, P2 = fs->P2;
         // lanes 0..2 x gamma_c, lanes 3..5 {s, P1, P2} t
-        const gf r1 = gf_mul_reg(gf_sel(lane < 3, xp, gf_sel(lane == 3, s0, gf_sel(lane == 4, P1, P2))),
+        const gf r1 = gf_mul_wide(gf_sel(lane < 3, xp, gf_sel(lane == 3, s0, gf_sel(lane == 4, P1, P2))),
                                  gf_sel(lane < 3, ga, t));
         const gf tc = gf_add(be, r1);
         const gf s = lane_gf(r1, 3), K = lane_gf(r1, 4), L = lane_gf(r1, 5);
         // lanes 0..2 x (beta_c + x gamma_c), lanes 3, 4 s {ecf, cf}[iv - 1]
-        const gf r2 = gf_mul_reg(gf_sel(lane < 3, xp, s),
+        const gf r2 = gf_mul_wide(gf_sel(lane < 3, xp, s),
                                  gf_sel(lane < 3, tc, iv > 0 ? gf_sel(lane == 3, gf_ld(ecfs + iv - 1), gf_ld(cfs + iv - 1))
                                                             : gf_zero()));
         const gf Ac = gf_add(al, r2);
@@ -1033,7 +1067,7 @@ __global__ __launch_bounds__(CB) void k_prod_look(const gf *fsrc, const gf *gsrc
     const gf Ph = lane_gf(P, lane < 3 ? hi_i : 0), Pl = lane_gf(P, lane < 3 ? lo_i : 0);
     const gf t = gf_add(gf_ld(e0s + iv + 1), xp);
     const gf rho = gf_ld(rhos + (n - 2 - k));
-    const gf r1 = gf_mul_reg(gf_sel(lane < 3, Ph, gf_sel(lane == 3, fs->s, gf_sel(lane == 4, fs->P1, fs->P2))),
+    const gf r1 = gf_mul_wide(gf_sel(lane < 3, Ph, gf_sel(lane == 3, fs->s, gf_sel(lane == 4, fs->P1, fs->P2))),
                              gf_sel(lane < 3, rho, t));
     const gf Ac = gf_add(Pl, r1);
     const gf K = lane_gf(r1, 4), L = lane_gf(r1, 5);
@@ -1044,7 +1078,7 @@ __global__ __launch_bounds__(CB) void k_prod_look(const gf *fsrc, const gf *gsrc
     const ProdRound R = prod_observe(fs, k, prod_coeffs(lane_gf(Ac, 0), lane_gf(Ac, 1), lane_gf(Ac, 2), K, L), true, prev);
     // the eq scalars for round k + 1 (s_k, then P1, P2)
     const gf s = lane_gf(r1, 3);
-    const gf p = gf_mul_reg(s, iv > 0 ? gf_sel(lane & 1, gf_ld(cfs + iv - 1), gf_ld(ecfs + iv - 1)) : gf_zero());
+    const gf p = gf_mul_wide(s, iv > 0 ? gf_sel(lane & 1, gf_ld(cfs + iv - 1), gf_ld(ecfs + iv - 1)) : gf_zero());
     if (lane == 0) {
         polys[3 * k] = R.c0;
         polys[3 * k + 1] = R.c2;
@@ -1209,7 +1243,7 @@ __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, cons
                 A1 = lane_gf(a, 1);
                 A2 = lane_gf(a, 2);
                 if (iv > 0) {  // P1, P2 for leaving round k0 (lanes 0, 1)
-                    const gf pp = gf_mul_reg(s, (lane & 1) ? cst[0][iv - 1] : cst[2][iv - 1]);
+                    const gf pp = gf_mul_wide(s, (lane & 1) ? cst[0][iv - 1] : cst[2][iv - 1]);
                     P1 = lane_gf(pp, 0);
                     P2 = lane_gf(pp, 1);
                 }
@@ -1225,7 +1259,7 @@ __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, cons
                 const gf t = gf_add(cst[1][iv + 1], xprev);  // eq(r_i, x_{k-1}) of the variable just fixed
                 // step 1: lanes 0..2 x gamma_c, lanes 3..5 {s, P1, P2} t
                 const gf u1 = gf_sel(lane < 3, xprev, gf_sel(lane == 3, s, gf_sel(lane == 4, P1, P2)));
-                const gf r1 = gf_mul_reg(u1, gf_sel(lane < 3, ga, t));
+                const gf r1 = gf_mul_wide(u1, gf_sel(lane < 3, ga, t));
                 const gf tc = gf_add(be, r1);
                 s = lane_gf(r1, 3);
                 K = lane_gf(r1, 4);
@@ -1233,7 +1267,7 @@ __global__ __launch_bounds__(kTailThreads) void k_prod_tail(const gf *fsrc, cons
                 // step 2: lanes 0..2 x (beta_c + x gamma_c), lanes 3, 4 s {ecf, cf}[iv - 1]
                 const gf u2 = gf_sel(lane < 3, xprev, s);
                 const gf v2 = gf_sel(lane < 3, tc, iv > 0 ? gf_sel(lane == 3, cst[2][iv - 1], cst[0][iv - 1]) : gf_zero());
-                const gf r2 = gf_mul_reg(u2, v2);
+                const gf r2 = gf_mul_wide(u2, v2);
                 const gf Ac = gf_add(al, r2);
                 A0 = lane_gf(Ac, 0);
                 A1 = lane_gf(Ac, 1);
@@ -1513,7 +1547,7 @@ __global__ __launch_bounds__(CB) void k_quad_step(const gf *__restrict__ P, size
         }
         // lanes 0, 1, 2: s A0, s A2, s (A0 + A1)
         const gf sp = gf_ld((const gf *)((const uint8_t *)fs + quad_scalars_off()) + blockIdx.y);
-        const gf v = gf_mul_reg(sp, gf_sel(lane == 0, A[0], gf_sel(lane == 1, A[2], gf_add(A[0], A[1]))));
+        const gf v = gf_mul_wide(sp, gf_sel(lane == 0, A[0], gf_sel(lane == 1, A[2], gf_add(A[0], A[1]))));
         unsigned done = 1u;
         if (lane < 3) {
             unsigned long long *slot = (unsigned long long *)(slots + 3 * (bid % kQuadSlots) + lane);
@@ -1621,7 +1655,7 @@ __global__ __launch_bounds__(CB) void k_quad_look(const gf *src, size_t sstride,
         const int b = lane == 1 ? 5 : 0;
         const gf al = lane_gf(P, b), ga = lane_gf(P, b + 1), mid = lane_gf(P, b + 2);
         const gf be = gf_add(mid, gf_add(al, ga));
-        const gf r2 = gf_mul_reg(xp, gf_add(be, gf_mul_reg(xp, ga)));
+        const gf r2 = gf_mul_wide(xp, gf_add(be, gf_mul_wide(xp, ga)));
         const gf cv = gf_add(al, r2);
         const gf c0 = lane_gf(cv, 0), c2 = lane_gf(cv, 1);
 #pragma unroll
@@ -2133,13 +2167,13 @@ __device__ void fs_round(FsState *st, int k, gf S, const uint64_t *root) {
     const gf one_r = gf_add(gf_one(), gf_ld(r + i)), sd = st->s_delta;
     // coeff0 = cf_i S (eval_impr.rs:187-197 scaled by the correcting factor)
     // and d0 = s_delta (1 + r_i) at once
-    gf p = gf_mul_reg(lane == 1 ? sd : gf_ld(cf + i), lane == 1 ? one_r : S);
+    gf p = gf_mul_wide(lane == 1 ? sd : gf_ld(cf + i), lane == 1 ? one_r : S);
     const gf coeff0 = lane_gf(p, 0), d0 = lane_gf(p, 1);
     // eval_1 = (sub + (1 + r_i) coeff0) / r_i  (eval_impr.rs:41-43)
-    const gf eval1 = gf_mul_reg(gf_add(sub, gf_mul_reg(one_r, coeff0)), gf_ld(rinv + i));
+    const gf eval1 = gf_mul_wide(gf_add(sub, gf_mul_wide(one_r, coeff0)), gf_ld(rinv + i));
     const gf coeff1 = gf_add(eval1, coeff0);
     // the four products of (coeff0 + coeff1 X)(d0 + s_delta X)
-    p = gf_mul_reg((lane & 2) ? coeff1 : coeff0, (lane & 1) ? sd : d0);
+    p = gf_mul_wide((lane & 2) ? coeff1 : coeff0, (lane & 1) ? sd : d0);
     const gf rp0 = lane_gf(p, 0), rp1 = gf_add(lane_gf(p, 1), lane_gf(p, 2)), rp2 = lane_gf(p, 3);
     // observe root || rp0 || rp2 (little-endian words; observing drops the
     // unread output) and hash the whole input for the sample
@@ -2169,9 +2203,9 @@ __device__ void fs_round(FsState *st, int k, gf S, const uint64_t *root) {
     const gf x{__builtin_bswap64(d[3]), __builtin_bswap64(d[2])};
     // x rp2, coeff1 x and s_delta (1 + r_i + x) at once (eval.rs:110,
     // lib.rs:100, eval_impr.rs:45-58)
-    p = gf_mul_reg(lane == 0 ? x : (lane == 1 ? coeff1 : sd), lane == 0 ? rp2 : (lane == 1 ? x : gf_add(one_r, x)));
+    p = gf_mul_wide(lane == 0 ? x : (lane == 1 ? coeff1 : sd), lane == 0 ? rp2 : (lane == 1 ? x : gf_add(one_r, x)));
     const gf xrp2 = lane_gf(p, 0), cx = lane_gf(p, 1), sd_next = lane_gf(p, 2);
-    const gf sub_next = gf_add(rp0, gf_mul_reg(x, gf_add(rp1, xrp2)));
+    const gf sub_next = gf_add(rp0, gf_mul_wide(x, gf_add(rp1, xrp2)));
     if (lane != 0) return;
     uint64_t *out64 = reinterpret_cast<uint64_t *>(st->out);
 #pragma unroll
@@ -2396,6 +2430,23 @@ __global__ __launch_bounds__(CB) void k_bench_mul_reg(gf *io, size_t n, int iter
         gf_st(io + 2 * i, a);
     }
 }
+// `iters` chained products in the four-lane form (gf_mul_wide), whole waves:
+// lanes 0..15 of each wave chain their own operands (the single-wave latency
+// of a transcript step's product at n = 64)
+__global__ __launch_bounds__(CB) void k_bench_mul_wide(gf *io, size_t n, int iters) {
+    const int lane = threadIdx.x & 63;
+    for (size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x - lane; i0 < n;
+         i0 += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = i0 + lane;
+        gf a = i < n ? gf_ld(io + 2 * i) : gf_zero(), b = i < n ? gf_ld(io + 2 * i + 1) : gf_zero();
+        for (int k = 0; k < iters; k++) {
+            gf c = gf_mul_wide(a, b);
+            b = a;
+            a = c;
+        }
+        if (i < n && lane < 16) gf_st(io + 2 * i, a);
+    }
+}
 // `iters` chained products by one factor c = io[1] per thread through the
 // block's window table of c (the products by a round challenge)
 __global__ __launch_bounds__(CB) void k_bench_mul_tab(gf *io, size_t n, int iters) {
@@ -2456,8 +2507,21 @@ __global__ __launch_bounds__(64) void k_bench_keccak_pair(uint64_t *io, int iter
 // b's masked pieces prepared once (gf_prepare / gf_mul_pre), the form the
 // fold, fix and NTT kernels use for a challenge or a twiddle
 __global__ __launch_bounds__(CB) void k_mul(const gf *__restrict__ a, const gf *__restrict__ b, gf *__restrict__ out,
-                                            size_t n, int shared_b) {
-    if (shared_b) {  // through the window table of b[0], as the kernels that multiply by a round challenge
+                                            size_t n, int form) {
+    if (form == 2) {  // four lanes per product, as the transcript steps; whole waves take part
+        const int lane = threadIdx.x & 63;
+        for (size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x - lane; i0 < n;
+             i0 += (size_t)gridDim.x * blockDim.x) {
+            const size_t i = i0 + lane;
+            const gf x = i < n ? gf_ld(a + i) : gf_zero(), y = i < n ? gf_ld(b + i) : gf_zero();
+            gf r = gf_zero();
+#pragma unroll
+            for (int q = 0; q < 4; q++) r = gf_sel((lane >> 4) == q, gf_mul_wide(x, y, 16 * q), r);
+            if (i < n) gf_st(out + i, r);
+        }
+        return;
+    }
+    if (form == 1) {  // through the window table of b[0], as the kernels that multiply by a round challenge
         __shared__ GfTab T;
         __shared__ gf u[128];
         gf_tab_build(T, u, gf_ld(b));
@@ -2827,13 +2891,16 @@ hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters) {
         k_bench_mul_comb<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
     else if (variant == 4)
         k_bench_mul_tab<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
+    else if (variant == 6)
+        k_bench_mul_wide<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
     else
         k_bench_mul_reg<<<grid_for(n, CB, 1 << 20), CB, 0, st>>>(io, n, iters);
     return hipGetLastError();
 }
-hipError_t mul(hipStream_t st, const gf *a, const gf *b, gf *out, size_t n, bool shared_b) {
+hipError_t mul(hipStream_t st, const gf *a, const gf *b, gf *out, size_t n, int form) {
     if (!n) return hipSuccess;
-    k_mul<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(a, b, out, n, shared_b ? 1 : 0);
+    if (form < 0 || form > 2) return hipErrorInvalidValue;
+    k_mul<<<grid_for(n, CB, kCapMul), CB, 0, st>>>(a, b, out, n, form);
     return hipGetLastError();
 }
 hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n) {

@@ -379,8 +379,9 @@ hipError_t verify_fold(hipStream_t st, const VerifyJob *jobs, uint32_t count, ui
                        const gf *ft, gf *vals, uint8_t *fold_ok);
 hipError_t verify_paths(hipStream_t st, const VerifyJob *jobs, uint32_t count, uint32_t q, int b, int d, size_t w,
                         const gf *vals, uint8_t *path_ok);
-// out[i] = a[i] * b[i], or a[i] * b[0] with shared_b (b prepared once)
-hipError_t mul(hipStream_t st, const gf *a, const gf *b, gf *out, size_t n, bool shared_b);
+// out[i] = a[i] * b[i] (form 0), a[i] * b[0] through b[0]'s window table
+// (form 1), or a[i] * b[i] with four lanes per product (form 2, gf_mul_wide)
+hipError_t mul(hipStream_t st, const gf *a, const gf *b, gf *out, size_t n, int form);
 hipError_t bench_mul(hipStream_t st, int variant, gf *io, size_t n, int iters);
 hipError_t copy16(hipStream_t st, const gf *src, gf *dst, size_t n);
 hipError_t bench_keccak(hipStream_t st, uint64_t *io, size_t n, int iters);

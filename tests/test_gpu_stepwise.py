@@ -230,6 +230,7 @@ def test_device_multiply_carry_bound(bfd, oracle, ctx):
     B = np.array([[p & (2**64 - 1), p >> 64] for _ in pats for p in pats], dtype=np.uint64)
     exp = oracle.mul(A, B, impl=2)
     assert (bfd.field_mul_batch(A, B, ctx=ctx) == exp).all()
+    assert (bfd.field_mul_batch(A, B, ctx=ctx, wide=True) == exp).all()
     for i, p in enumerate(pats):
         b = np.array([[p & (2**64 - 1), p >> 64]], dtype=np.uint64)
         a = np.ascontiguousarray(A[:len(pats)])
@@ -240,7 +241,10 @@ def test_device_multiply_carry_bound(bfd, oracle, ctx):
     clear_b = rng.random((n, 128)) < 1 / 16
     a = _dense(clear_a)
     b = _dense(clear_b)
-    assert (bfd.field_mul_batch(a, b, ctx=ctx) == oracle.mul(a, b, impl=2)).all()
+    exp = oracle.mul(a, b, impl=2)
+    assert (bfd.field_mul_batch(a, b, ctx=ctx) == exp).all()
+    # the four-lane form on a ragged count (the last wave partly past n)
+    assert (bfd.field_mul_batch(a[:n - 37], b[:n - 37], ctx=ctx, wide=True) == exp[:n - 37]).all()
 
 
 def _dense(clear):
