@@ -115,22 +115,30 @@ void batch_commit_open(bfd_ctx *ctx, uint32_t lambda, uint32_t log2_c, size_t co
     // queries, assembling proofs) overlaps the device's
     std::deque<OpenTick> inflight;
     std::vector<Job *> work;
+    // enqueues the next tick when a buffer set is free (the oldest tick's
+    // set stays taken until its proofs are written)
+    const auto enqueue = [&] {
+        if (inflight.size() >= (size_t)Lane::kSets) return false;
+        work.clear();
+        for (Job *j : all)
+            if (j->stage == Job::Open || j->stage == Job::Gather) work.push_back(j);
+        if (work.empty()) return false;
+        inflight.push_back(lane.open_tick(work));
+        phase("batch_enqueue");
+        return true;
+    };
     for (;;) {
-        if (inflight.size() < (size_t)Lane::kSets) {
-            work.clear();
-            for (Job *j : all)
-                if (j->stage == Job::Open || j->stage == Job::Gather) work.push_back(j);
-            if (!work.empty()) {
-                inflight.push_back(lane.open_tick(work));
-                phase("batch_enqueue");
-                continue;
-            }
-        }
+        if (enqueue()) continue;
         if (inflight.empty()) break;
         hip_check(hipEventSynchronize(inflight.front().done), "open tick");
         phase("batch_wait");
         ctx->timer.collect();
-        lane.open_finish(inflight.front());
+        // sample the finished jobs' queries first, so that the tick that
+        // gathers their openings runs while the host writes proofs
+        lane.open_finals(inflight.front());
+        phase("batch_finish");
+        enqueue();
+        lane.open_proofs(inflight.front());
         inflight.pop_front();
         phase("batch_finish");
     }

@@ -392,12 +392,18 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
                 img += align_up(fs_bytes((uint32_t)jp->n, jp->fs_cap));
             }
         if (elems + img) {
-            ensure(eq_dev_[c], 16 * elems + 16, st_);
-            stage_pin_[c].ensure(16 * elems + img + 64);
+            // one upload for all of them: the eq halves, then each job's image
+            // (256-byte aligned), which the same ks_outer launch copies into
+            // the job's region as 16-byte units (one copy call per job cost
+            // the host 10-40 us each)
+            const size_t im0 = 16 * elems;
+            ensure(eq_dev_[c], im0 + img + 16, st_);
+            stage_pin_[c].ensure(im0 + img + 64);
             gf *h = (gf *)stage_pin_[c].p;
-            uint8_t *im = (uint8_t *)stage_pin_[c].p + 16 * elems;
-            size_t e = 0;
+            uint8_t *im = (uint8_t *)stage_pin_[c].p + im0;
+            size_t e = 0, io = 0;
             segs.clear();
+            std::vector<Seg> copies;
             for (Job *jp : jobs) {
                 Job &j = *jp;
                 if (j.stage != Job::Open || j.k != 0) continue;
@@ -408,12 +414,15 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
                 memcpy(h + e, j.eqB.data(), 16 * j.eqB.size());
                 e += j.eqB.size();
                 segs.push_back(make_seg(A, j.half_eq, B, (size_t)1 << j.o.half_eq_vars(), (uint64_t)j.eq_lo));
-                const size_t nb = fs_write(j.o, *j.ch, j.fs_cap, im);
-                hip_check(hipMemcpyAsync(j.fs, im, nb, hipMemcpyHostToDevice, st_), "copy transcript");
-                im += align_up(fs_bytes((uint32_t)j.n, j.fs_cap));
+                // the region holds align_up(fs_bytes) >= the image rounded up to 16
+                const size_t nb = fs_write(j.o, *j.ch, j.fs_cap, im + io), n16 = (nb + 15) / 16;
+                memset(im + io + nb, 0, 16 * n16 - nb);
+                copies.push_back(make_seg(eq_dev_[c].as<uint8_t>() + im0 + io, j.fs, nullptr, n16, 0));
+                io += align_up(fs_bytes((uint32_t)j.n, j.fs_cap));
             }
-            if (elems)
-                hip_check(hipMemcpyAsync(eq_dev_[c].p, h, 16 * elems, hipMemcpyHostToDevice, st_), "copy eq halves");
+            segs.insert(segs.end(), copies.begin(), copies.end());
+            hip_check(hipMemcpyAsync(eq_dev_[c].p, h, im0 + io, hipMemcpyHostToDevice, st_),
+                      "copy eq halves and transcripts");
             seg(SegKernel::Outer, segs, "ks_outer");
         }
     }
@@ -507,8 +516,8 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         j.k++;
     }
     // (E) openings of jobs whose queries are known: their query records,
-    // written on the device in bincode layout (k_proof_records) and read
-    // back for open_finish to copy behind each proof's header
+    // written by the device in bincode layout (k_proof_records) into pinned
+    // host memory, for open_proofs to copy behind each proof's header
     size_t body = 0, nidx = 0;
     for (Job *jp : jobs) {
         Job &j = *jp;
@@ -522,8 +531,11 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         const size_t G = t.gathered.size(), up = sizeof(ProofJob) * G + 8 * nidx;
         plan_pin_[c].ensure(up);
         ensure(plan_dev_[c], up, st_);
-        ensure(gather_dev_[c], body, st_);
+        // the records go from the kernel straight into pinned host memory
+        // (one pass over PCIe, no device buffer and no copy behind it)
         gather_pin_[c].ensure(body);
+        uint8_t *hout = nullptr;
+        hip_check(hipHostGetDevicePointer((void **)&hout, gather_pin_[c].p, 0), "hipHostGetDevicePointer");
         ProofJob *hj = (ProofJob *)plan_pin_[c].p;
         uint64_t *hidx = reinterpret_cast<uint64_t *>(hj + G);
         const uint64_t *didx = reinterpret_cast<const uint64_t *>(plan_dev_[c].as<uint8_t>() + sizeof(ProofJob) * G);
@@ -532,7 +544,7 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         for (size_t g = 0; g < G; g++) {
             const Job &j = *t.gathered[g];
             memcpy(hidx + io, j.o.idx.data(), 8 * j.o.q);
-            hj[g] = proof_job(j.o, j.lay, didx + io, gather_dev_[c].as<uint8_t>() + t.gather_at[g], rec);
+            hj[g] = proof_job(j.o, j.lay, didx + io, hout + t.gather_at[g], rec);
             io += j.o.q;
             rec += (uint32_t)j.o.q;
         }
@@ -540,8 +552,6 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
         auto go = [&] { return launch::proof_records(st_, plan_dev_[c].as<ProofJob>(), (uint32_t)G, rec); };
         if (timer_) timer_->run(st_, "k_proof_records", go);
         else hip_check(go(), "proof records");
-        hip_check(hipMemcpyAsync(gather_pin_[c].p, gather_dev_[c].p, body, hipMemcpyDeviceToHost, st_),
-                  "copy query records");
     }
     for (Job *jp : t.gathered) jp->stage = Job::Gathering;
     hip_check(hipEventRecord(t.done, st_), "hipEventRecord");
@@ -549,9 +559,14 @@ OpenTick Lane::open_tick(std::vector<Job *> &jobs) {
 }
 
 void Lane::open_finish(OpenTick &t) {
+    open_finals(t);
+    open_proofs(t);
+}
+
+void Lane::open_proofs(OpenTick &t) {
     const int c = t.par;
     // proofs of the jobs this tick gathered: header from the host's round
-    // data, query records from the read-back
+    // data, query records from the kernel's writes to pinned memory
     const uint8_t *recs = (const uint8_t *)gather_pin_[c].p;
     ChunkCopy cc;
     for (size_t g = 0; g < t.gathered.size(); g++) {
@@ -574,6 +589,10 @@ void Lane::open_finish(OpenTick &t) {
         jp->stage = Job::Done;
         jp->t_done = now;
     }
+}
+
+void Lane::open_finals(OpenTick &t) {
+    const int c = t.par;
     // jobs past their last round: the round data and the challenger state
     // come from the read-back; the queries are sampled here
     // (basefold.rs:194-203); every job owns a distinct challenger

@@ -123,6 +123,12 @@ class Lane {
     // open_finish once t.done has completed
     OpenTick open_tick(std::vector<Job *> &jobs);
     void open_finish(OpenTick &t);
+    // open_finish in two parts: open_finals samples the queries of the jobs
+    // past their last round (after it, the next tick can gather them);
+    // open_proofs writes the proofs of the jobs this tick gathered, and
+    // until it has run the tick's buffer set stays taken
+    void open_finals(OpenTick &t);
+    void open_proofs(OpenTick &t);
     // launch shape of the ticks enqueued from now on (its ntt_order,
     // subtree_from_log2, subtree_levels and seg_blocks_per_cu apply)
     void set_tuning(const bfd_tuning &t, int cus) {
@@ -147,7 +153,7 @@ class Lane {
     int cur_ = 0;
     hipEvent_t ev_[kSets] = {};
     PinnedBuf seg_pin_[kSets], rb_pin_[kSets], gather_pin_[kSets], stage_pin_[kSets], plan_pin_[kSets];
-    DevBuf seg_dev_[kSets], rb_dev_[kSets], addr_dev_[kSets], gather_dev_[kSets], plan_dev_[kSets], eq_dev_[kSets];
+    DevBuf seg_dev_[kSets], rb_dev_[kSets], addr_dev_[kSets], plan_dev_[kSets], eq_dev_[kSets];
     DevBuf partials_;  // inner-product sums (stream-ordered: one tick at a time uses them)
     size_t seg_cap_[kSets] = {}, seg_used_[kSets] = {}, flushed_[kSets] = {};
     struct Pending {  // a recorded segmented launch (ring entries at off)

@@ -2298,10 +2298,16 @@ __global__ __launch_bounds__(kSubThreads) void k_top_round(uint64_t *prev, size_
 }
 
 // in = A (2^m), aux = B, out = table, arg = m
+// A segment without aux is a copy of count 16-byte units from in to out (the
+// starting transcript images of the jobs entering their first round).
 __global__ __launch_bounds__(CB) void ks_outer(const Seg *segs, int nseg) {
     const Seg g = my_seg(segs, nseg);
     const gf *A = (const gf *)g.in, *B = (const gf *)g.aux;
     gf *out = (gf *)g.out;
+    if (!B) {
+        SEG_LOOP(g, i) gf_st(out + i, gf_ld(A + i));
+        return;
+    }
     const int m = (int)g.arg;
     const size_t mask = ((size_t)1 << m) - 1;
     SEG_LOOP(g, i) gf_st(out + i, gf_mul_reg(gf_ld(A + (i & mask)), gf_ld(B + (i >> m))));
