@@ -53,9 +53,9 @@ void worker_loop(bfd_server *S) {
         hipEvent_t ev_commit;
         hip_check(hipEventCreateWithFlags(&ev_commit, hipEventDisableTiming), "event");
         std::vector<Job *> committing, opening, work;
-        // open ticks in flight (at most two: the rounds need nothing from
-        // the host, so the next tick is enqueued while one runs) and the
-        // time each was enqueued
+        // open ticks in flight (at most Lane::kSets, one per buffer set: the
+        // rounds need nothing from the host, so the next ticks are enqueued
+        // while one runs) and the time each was enqueued
         std::deque<std::pair<OpenTick, double>> open_q;
         bool commit_busy = false;
         double commit_t0 = 0;
